@@ -1,0 +1,77 @@
+/*
+ * wrfdc -- C ABI of the MI355X (gfx950) RK3 split-explicit dycore.
+ *
+ * The library advances the state of one WRF-JAX dycore domain that the
+ * caller keeps resident in device memory.  Plain pointers and sizes only; no
+ * C++ exceptions cross this boundary.  Every entry point returns 0 on success
+ * and a negative WRFDC_E* code on failure (details in wrfdc_last_error()).
+ *
+ * Reference interfaces replaced (paths relative to the WRF-JAX source tree):
+ *   wrfdc_create     build_step_em_core_context  src/wrf_jax/core/step_em.py:40-65
+ *                    (+ build_direct_step_plan   src/wrf_jax/core/direct_step.py:2412-2460)
+ *   wrfdc_bind       model_state_to_fields       src/wrf_jax/core/conversion.py:62-67
+ *                    (caller-owned device arrays instead of a ModelState pytree)
+ *   wrfdc_step       step_em_core_once           src/wrf_jax/core/step_em.py:74-93
+ *                    = run_direct_step           src/wrf_jax/core/direct_step.py:3425-3531
+ *   wrfdc_stage      one StageFn callback of strict_solve_callbacks()
+ *                                                src/wrf_jax/runtime.py:16235-16301
+ *   wrfdc_checksums  assert_state_finite         src/wrf_jax/data_io.py:222-228
+ *                    (plus FP64 mass / theta / moisture checksums)
+ */
+#ifndef WRFDC_H
+#define WRFDC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WRFDC_OK 0
+#define WRFDC_EINVAL -1    /* bad argument / unsupported option (ValueError) */
+#define WRFDC_ENOTIMPL -2  /* option not implemented (NotImplementedError) */
+#define WRFDC_EUNBOUND -3  /* a required field has no device buffer bound */
+#define WRFDC_EHIP -4      /* HIP runtime error */
+#define WRFDC_ENONFINITE -5 /* NaN/Inf found by wrfdc_checksums (FloatingPointError) */
+
+/* Scalar configuration; field order and types match the Python mirror
+ * wrfdc.config.CConfig.  Dimensions are mass-point counts. */
+typedef struct wrfdc_config {
+  int nx, ny, nz;       /* mass points */
+  int ip;               /* i pitch (elements) of 3-D/2-D arrays, >= nx+1 */
+  int nmoist, nscalar;  /* moist species (index 0 = QV) and extra scalars */
+  int spec_bdy_width, spec_zone, relax_zone;
+  int time_step_sound;  /* acoustic steps per dt, 0 = WRF formula */
+  int pd_moist, pd_scalar; /* positive-definite advection on the last stage */
+  int top_lid;
+  int pad_;
+  double dx, dy, dt;
+  double ptop, p0, t0, g, r_d, r_v, cp, cpovcv, cvpm, rvovrd;
+  double epssm, smdiv, emdiv, c_s, prandtl;
+  double cf1, cf2, cf3, cfn, cfn1;
+  double dtbc;          /* seconds since the last lateral-boundary update */
+  double reradius;      /* 1 / earth radius (curvature terms) */
+} wrfdc_config;
+
+typedef struct wrfdc_ctx wrfdc_ctx;
+
+/* Field registry: names and kinds of every array the step reads or writes.
+ * kind: '3' [jde][kde][ip], '2' [jde][ip], '1' [kde], 'M' [nmoist][jde][kde][ip],
+ * 'S' [nscalar][jde][kde][ip], 'X' [bw][kde][jde], 'Y' [bw][kde][ip],
+ * 'x' [bw][jde], 'y' [bw][ip]; all float32. */
+int wrfdc_field_count(void);
+const char *wrfdc_field_name(int index);
+int wrfdc_field_kind(int index);
+
+int wrfdc_create(const wrfdc_config *cfg, wrfdc_ctx **out);
+void wrfdc_destroy(wrfdc_ctx *ctx);
+int wrfdc_bind(wrfdc_ctx *ctx, const char *name, void *device_ptr);
+int wrfdc_set_stream(wrfdc_ctx *ctx, void *hip_stream);
+int wrfdc_step(wrfdc_ctx *ctx, int nsteps, double dtbc0);
+int wrfdc_stage(wrfdc_ctx *ctx, const char *name, int rk_step, int iteration);
+int wrfdc_checksums(wrfdc_ctx *ctx, double out[4]);
+int wrfdc_sync(wrfdc_ctx *ctx);
+const char *wrfdc_last_error(wrfdc_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

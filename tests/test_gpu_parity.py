@@ -1,0 +1,139 @@
+"""GPU parity: HIP kernels (through the C ABI) vs the CPU oracle.
+
+Every test uploads the same seeded state to the GPU and to the oracle, runs
+the same stage or step on both and compares every registry field.  Bars
+(FP32 vs FP32 oracle): relative L2 <= 1e-5 per field after one stage or one
+RK3 step, NRMSE <= 1e-3 per prognostic field after 100 steps.
+"""
+
+import numpy as np
+import pytest
+
+from parity import NRMSE_LONG, REL_L2_STEP, compare, nrmse, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(nx, ny, nz, **kw):
+    from wrfdc.cases import make_case
+
+    return make_case(nx, ny, nz, **kw)
+
+
+@pytest.fixture(scope="module")
+def stepped_state():
+    """A state after one oracle step: every work array holds realistic values."""
+    from oracle import Oracle
+
+    case = _case(28, 24, 12, nmoist=6, nscalar=2, hydrometeors=True, hill_height=500.0)
+    o = Oracle(case.config, case.arrays)
+    o.step(1, 0.0)
+    o.set_dtbc(2 * case.config.dt)
+    state = o.copy()
+    o.close()
+    return case, state
+
+
+@pytest.fixture(scope="module")
+def domain_for(gpu):
+    from wrfdc.device import Domain
+
+    cache = {}
+
+    def get(config):
+        key = config
+        if key not in cache:
+            cache[key] = Domain(config, gpu)
+        return cache[key]
+
+    yield get
+    for d in cache.values():
+        d.close()
+
+
+STAGES = [
+    ("rk_step_prep", 1, 0), ("first_rk_step_part1", 1, 0), ("compute_diff_metrics", 1, 0),
+    ("cal_deform_and_div", 1, 0), ("smag2d_km", 1, 0), ("horizontal_diffusion_2", 1, 0),
+    ("first_rk_step_part2", 1, 0), ("advect_u", 2, 0), ("advect_v", 2, 0), ("advect_w", 2, 0),
+    ("advect_scalar", 2, 0), ("advect_scalar_pd", 3, 0), ("rhs_ph", 2, 0),
+    ("horizontal_pressure_gradient", 2, 0), ("pg_buoy_w", 2, 0), ("rk_tendency", 2, 0),
+    ("dry_boundary_tendency", 1, 0), ("dry_boundary_tendency", 2, 0), ("small_step_prep", 1, 0),
+    ("small_step_prep", 2, 0), ("calc_p_rho", 2, 0), ("calc_p_rho", 2, 1), ("calc_coef_w", 2, 0),
+    ("advance_uv", 2, 1), ("advance_mu_t", 2, 1), ("advance_w", 2, 1), ("sumflux", 2, 1),
+    ("sumflux", 2, 2), ("sumflux", 1, 1), ("small_step_finish", 2, 0), ("small_step_finish", 3, 0),
+    ("calc_p_rho_phi", 2, 0), ("rk_stage", 1, 0), ("rk_stage", 2, 0), ("rk_stage", 3, 0),
+]
+
+
+@pytest.mark.parametrize("name,rk,it", STAGES, ids=[f"{n}-rk{r}-it{i}" for n, r, i in STAGES])
+def test_stage_parity(stepped_state, domain_for, name, rk, it):
+    from oracle import Oracle
+
+    case, state = stepped_state
+    print(f"[stage parity] {name} rk={rk} it={it} shape={case.config.nx}x{case.config.ny}x{case.config.nz}", flush=True)
+    dom = domain_for(case.config.with_dtbc(2 * case.config.dt))
+    o = Oracle(case.config.with_dtbc(2 * case.config.dt), state)
+    o.stage(name, rk, it)
+    dom.upload(state)
+    dom.stage(name, rk, it)
+    got = dom.download()
+    bad = compare(got, o.arrays, tol=REL_L2_STEP)
+    o.close()
+    assert not bad, f"{name}: fields above rel-L2 {REL_L2_STEP}: {bad}"
+
+
+SHAPES = [(28, 24, 12), (13, 17, 7), (40, 33, 20)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=[f"{a}x{b}x{c}" for a, b, c in SHAPES])
+def test_one_step_parity(domain_for, shape):
+    from oracle import Oracle
+
+    nx, ny, nz = shape
+    print(f"[step parity] shape={shape}", flush=True)
+    case = _case(nx, ny, nz, nmoist=6, nscalar=2, hydrometeors=True, hill_height=500.0)
+    dom = domain_for(case.config)
+    o = Oracle(case.config, case.arrays)
+    o.step(1, 0.0)
+    dom.upload(case.arrays)
+    dom.step(1, 0.0)
+    got = dom.download()
+    bad = compare(got, o.arrays, tol=REL_L2_STEP)
+    o.close()
+    assert not bad, f"fields above rel-L2 {REL_L2_STEP} after one RK3 step: {bad}"
+
+
+def test_hundred_steps_nrmse(domain_for):
+    from oracle import Oracle
+
+    print("[100-step NRMSE] shape=(32, 28, 12)", flush=True)
+    case = _case(32, 28, 12, nmoist=6, nscalar=2, hydrometeors=True, hill_height=500.0)
+    dom = domain_for(case.config)
+    o = Oracle(case.config, case.arrays)
+    o.step(100, 0.0)
+    dom.upload(case.arrays)
+    dom.step(100, 0.0)
+    got = dom.download()
+    errs = {}
+    for n in ("u_2", "v_2", "w_2", "t_2", "ph_2", "mu_2", "p", "al", "moist", "scalar"):
+        errs[n] = nrmse(got[n], o.arrays[n])
+    o.close()
+    print("NRMSE after 100 steps:", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert all(np.isfinite(v) and v <= NRMSE_LONG for v in errs.values()), errs
+
+
+def test_checksums_and_nonfinite(domain_for):
+    case = _case(28, 24, 12, nmoist=6, nscalar=2, hydrometeors=True)
+    dom = domain_for(case.config)
+    dom.upload(case.arrays)
+    ck = dom.checksums()
+    A = case.arrays
+    mass = (A["mu_2"][:-1, :case.config.nx].astype(np.float64) + A["mub"][:-1, :case.config.nx]).sum()
+    theta = A["t_2"][:-1, :-1, :case.config.nx].astype(np.float64).sum()
+    assert rel_l2(ck[0], mass) < 1e-12 and rel_l2(ck[1], theta) < 1e-12 and ck[3] == 0
+    bad = dict(A)
+    bad["t_2"] = A["t_2"].copy()
+    bad["t_2"][3, 2, 5] = np.nan
+    dom.upload({"t_2": bad["t_2"]})
+    with pytest.raises(FloatingPointError):
+        dom.checksums()

@@ -1,0 +1,334 @@
+// Flux-form advection kernels: 5th-order horizontal / 3rd-order vertical with
+// the reference's boundary-degraded orders for specified boundaries
+// (src/wrf_jax/dyn_em/module_advect_em.py: advect_u/v/w/scalar/scalar_pd).
+// Each thread owns one output point and evaluates the fluxes of its faces.
+#pragma once
+#include "wrfdc_common.h"
+
+__device__ __forceinline__ float flux3(float qm2, float qm1, float q0, float qp1, float ua) {
+  return (7.f * (q0 + qm1) - (qp1 + qm2)) / 12.f + sgn1(ua) * ((qp1 - qm2) - 3.f * (q0 - qm1)) / 12.f;
+}
+__device__ __forceinline__ float flux5(float qm3, float qm2, float qm1, float q0, float qp1, float qp2, float ua) {
+  return (37.f * (q0 + qm1) - 8.f * (qp1 + qm2) + (qp2 + qm3)) / 60.f -
+         sgn1(ua) * ((qp2 - qm3) - 5.f * (qp1 - qm2) + 10.f * (q0 - qm1)) / 60.f;
+}
+__device__ __forceinline__ float flux3pd(float qm2, float qm1, float q0, float qp1, float ua) {
+  float f4 = (7.f / 12.f) * (q0 + qm1) - (1.f / 12.f) * (qp1 + qm2);
+  return f4 + sgn1(ua) * (1.f / 12.f) * ((qp1 - qm2) - 3.f * (q0 - qm1));
+}
+__device__ __forceinline__ float flux5pd(float qm3, float qm2, float qm1, float q0, float qp1, float qp2, float ua) {
+  float f6 = (37.f / 60.f) * (q0 + qm1) - (2.f / 15.f) * (qp1 + qm2) + (1.f / 60.f) * (qp2 + qm3);
+  return f6 - sgn1(ua) * (1.f / 60.f) * ((qp2 - qm3) - 5.f * (qp1 - qm2) + 10.f * (q0 - qm1));
+}
+__device__ __forceinline__ float flux_up(float qm1, float q0, float cr) {
+  return 0.5f * fminf(1.f, cr + fabsf(cr)) * qm1 + 0.5f * fmaxf(-1.f, cr - fabsf(cr)) * q0;
+}
+
+// along-j / along-i stencils of a field at fixed (i,k) / (k,j)
+#define QJ(q, i, k, j) A3(q, i, k, j)
+#define QI(q, i, k, j) A3(q, i, k, j)
+
+// ---------------- scalar (theta, moist, scalars) ----------------------------
+__device__ __forceinline__ float fy_s(const DevCtx *__restrict__ x, const float *q, const float *rv, int i, int k, int j) {
+  const int jde = x->jde;
+  float vel = A3(rv, i, k, j);
+  if (j >= 4 && j <= jde - 3)
+    return vel * flux5(A3(q, i, k, j - 3), A3(q, i, k, j - 2), A3(q, i, k, j - 1), A3(q, i, k, j), A3(q, i, k, j + 1), A3(q, i, k, j + 2), vel);
+  if (j == 2 || j == jde - 1) return 0.5f * vel * (A3(q, i, k, j) + A3(q, i, k, j - 1));
+  return vel * flux3(A3(q, i, k, j - 2), A3(q, i, k, j - 1), A3(q, i, k, j), A3(q, i, k, j + 1), vel);
+}
+__device__ __forceinline__ float fx_s(const DevCtx *__restrict__ x, const float *q, const float *ru, int i, int k, int j) {
+  const int ide = x->ide;
+  float vel = A3(ru, i, k, j);
+  if (i >= 4 && i <= ide - 3)
+    return vel * flux5(A3(q, i - 3, k, j), A3(q, i - 2, k, j), A3(q, i - 1, k, j), A3(q, i, k, j), A3(q, i + 1, k, j), A3(q, i + 2, k, j), vel);
+  if (i == 2 || i == ide - 1) return 0.5f * vel * (A3(q, i, k, j) + A3(q, i - 1, k, j));
+  return vel * flux3(A3(q, i - 2, k, j), A3(q, i - 1, k, j), A3(q, i, k, j), A3(q, i + 1, k, j), vel);
+}
+__device__ __forceinline__ float fz_s(const DevCtx *__restrict__ x, const float *q, const float *rom, int i, int k, int j) {
+  const int ktf = x->kde - 1;
+  if (k <= 1 || k >= x->kde) return 0.f;
+  const float *fzm = FL(fnm), *fzp = FL(fnp);
+  float vel = A3(rom, i, k, j);
+  if (k == 2 || k == ktf) return vel * (A1(fzm, k) * A3(q, i, k, j) + A1(fzp, k) * A3(q, i, k - 1, j));
+  return vel * flux3(A3(q, i, k - 2, j), A3(q, i, k - 1, j), A3(q, i, k, j), A3(q, i, k + 1, j), -vel);
+}
+
+// advect_scalar (module_advect_em.py:90-369): tend -= div(flux), mass points
+__global__ void k_advect_scalar(const DevCtx *__restrict__ x, const float *q, float *tend, const float *ru, const float *rv,
+                                const float *rom, R3 r) {
+  GET_IKJ(r);
+  const float *msftx = FL(msftx), *rdzw = FL(rdnw);
+  float t = A3(tend, i, k, j);
+  if (j >= 2 && j <= x->jde - 2) t -= A2(msftx, i, j) * x->rdy * (fy_s(x, q, rv, i, k, j + 1) - fy_s(x, q, rv, i, k, j));
+  if (i >= 2 && i <= x->ide - 2) t -= A2(msftx, i, j) * x->rdx * (fx_s(x, q, ru, i + 1, k, j) - fx_s(x, q, ru, i, k, j));
+  t -= A1(rdzw, k) * (fz_s(x, q, rom, i, k + 1, j) - fz_s(x, q, rom, i, k, j));
+  A3(tend, i, k, j) = t;
+}
+
+// ---------------- u momentum (u points i in [2, ide-1]) ---------------------
+__device__ __forceinline__ float fy_u(const DevCtx *__restrict__ x, const float *u, const float *rv, int i, int k, int j) {
+  const int jde = x->jde;
+  if (j >= 4 && j <= jde - 3) {
+    float vel = 0.5f * (A3(rv, i, k, j) + A3(rv, i - 1, k, j));
+    return vel * flux5(A3(u, i, k, j - 3), A3(u, i, k, j - 2), A3(u, i, k, j - 1), A3(u, i, k, j), A3(u, i, k, j + 1), A3(u, i, k, j + 2), vel);
+  }
+  if (j == 2 || j == jde - 1) return 0.25f * (A3(rv, i, k, j) + A3(rv, i - 1, k, j)) * (A3(u, i, k, j) + A3(u, i, k, j - 1));
+  float vel = 0.5f * (A3(rv, i, k, j) + A3(rv, i - 1, k, j));
+  return vel * flux3(A3(u, i, k, j - 2), A3(u, i, k, j - 1), A3(u, i, k, j), A3(u, i, k, j + 1), vel);
+}
+__device__ __forceinline__ float fx_u(const DevCtx *__restrict__ x, const float *u, const float *ru, int i, int k, int j) {
+  const int ide = x->ide;
+  if (i >= 4 && i <= ide - 2) {
+    float vel = 0.5f * (A3(ru, i, k, j) + A3(ru, i - 1, k, j));
+    return vel * flux5(A3(u, i - 3, k, j), A3(u, i - 2, k, j), A3(u, i - 1, k, j), A3(u, i, k, j), A3(u, i + 1, k, j), A3(u, i + 2, k, j), vel);
+  }
+  if (i == 2) {
+    float ub = A3(u, i - 1, k, j);
+    if (A3(u, i, k, j) < 0.f) ub = A3(u, i, k, j);
+    return 0.25f * (A3(ru, i, k, j) + A3(ru, i - 1, k, j)) * (A3(u, i, k, j) + ub);
+  }
+  if (i == ide) {
+    float ub = A3(u, i, k, j);
+    if (A3(u, i - 1, k, j) > 0.f) ub = A3(u, i - 1, k, j);
+    return 0.25f * (A3(ru, i, k, j) + A3(ru, i - 1, k, j)) * (A3(u, i - 1, k, j) + ub);
+  }
+  float vel = 0.5f * (A3(ru, i, k, j) + A3(ru, i - 1, k, j));  // i == 3 or ide-1
+  return vel * flux3(A3(u, i - 2, k, j), A3(u, i - 1, k, j), A3(u, i, k, j), A3(u, i + 1, k, j), vel);
+}
+__device__ __forceinline__ float fz_u(const DevCtx *__restrict__ x, const float *u, const float *rom, int i, int k, int j) {
+  const int ktf = x->kde - 1;
+  if (k <= 1 || k >= x->kde) return 0.f;
+  const float *fzm = FL(fnm), *fzp = FL(fnp);
+  if (k == 2 || k == ktf)
+    return 0.5f * (A3(rom, i, k, j) + A3(rom, i - 1, k, j)) * (A1(fzm, k) * A3(u, i, k, j) + A1(fzp, k) * A3(u, i, k - 1, j));
+  float vel = 0.5f * (A3(rom, i - 1, k, j) + A3(rom, i, k, j));
+  return vel * flux3(A3(u, i, k - 2, j), A3(u, i, k - 1, j), A3(u, i, k, j), A3(u, i, k + 1, j), -vel);
+}
+// advect_u (module_advect_em.py:762-1034); r covers i [2, ide-1], j [1, jde-1]
+__global__ void k_advect_u(const DevCtx *__restrict__ x, const float *u, float *tend, const float *ru, const float *rv,
+                           const float *rom, R3 r) {
+  GET_IKJ(r);
+  const float *msfux = FL(msfux), *rdzw = FL(rdnw);
+  float t = A3(tend, i, k, j);
+  if (j >= 2 && j <= x->jde - 2) t -= A2(msfux, i, j) * x->rdy * (fy_u(x, u, rv, i, k, j + 1) - fy_u(x, u, rv, i, k, j));
+  t -= A2(msfux, i, j) * x->rdx * (fx_u(x, u, ru, i + 1, k, j) - fx_u(x, u, ru, i, k, j));
+  t -= A1(rdzw, k) * (fz_u(x, u, rom, i, k + 1, j) - fz_u(x, u, rom, i, k, j));
+  A3(tend, i, k, j) = t;
+}
+
+// ---------------- v momentum (v rows j in [2, jde-1]) -----------------------
+__device__ __forceinline__ float fy_v(const DevCtx *__restrict__ x, const float *v, const float *rv, int i, int k, int j) {
+  const int jde = x->jde;
+  if (j >= 4 && j <= jde - 2) {
+    float vel = 0.5f * (A3(rv, i, k, j) + A3(rv, i, k, j - 1));
+    return vel * flux5(A3(v, i, k, j - 3), A3(v, i, k, j - 2), A3(v, i, k, j - 1), A3(v, i, k, j), A3(v, i, k, j + 1), A3(v, i, k, j + 2), vel);
+  }
+  if (j == 2) {
+    float vb = A3(v, i, k, j - 1);
+    if (A3(v, i, k, j) < 0.f) vb = A3(v, i, k, j);
+    return 0.25f * (A3(rv, i, k, j) + A3(rv, i, k, j - 1)) * (A3(v, i, k, j) + vb);
+  }
+  if (j == jde) {
+    float vb = A3(v, i, k, j);
+    if (A3(v, i, k, j - 1) > 0.f) vb = A3(v, i, k, j - 1);
+    return 0.25f * (A3(rv, i, k, j) + A3(rv, i, k, j - 1)) * (vb + A3(v, i, k, j - 1));
+  }
+  float vel = 0.5f * (A3(rv, i, k, j) + A3(rv, i, k, j - 1));  // j == 3 or jde-1
+  return vel * flux3(A3(v, i, k, j - 2), A3(v, i, k, j - 1), A3(v, i, k, j), A3(v, i, k, j + 1), vel);
+}
+__device__ __forceinline__ float fx_v(const DevCtx *__restrict__ x, const float *v, const float *ru, int i, int k, int j) {
+  const int ide = x->ide;
+  if (i >= 4 && i <= ide - 3) {
+    float vel = 0.5f * (A3(ru, i, k, j) + A3(ru, i, k, j - 1));
+    return vel * flux5(A3(v, i - 3, k, j), A3(v, i - 2, k, j), A3(v, i - 1, k, j), A3(v, i, k, j), A3(v, i + 1, k, j), A3(v, i + 2, k, j), vel);
+  }
+  if (i == 2 || i == ide - 1) return 0.25f * (A3(ru, i, k, j) + A3(ru, i, k, j - 1)) * (A3(v, i, k, j) + A3(v, i - 1, k, j));
+  float vel = 0.5f * (A3(ru, i, k, j) + A3(ru, i, k, j - 1));  // i == 3 or ide-2
+  return vel * flux3(A3(v, i - 2, k, j), A3(v, i - 1, k, j), A3(v, i, k, j), A3(v, i + 1, k, j), vel);
+}
+__device__ __forceinline__ float fz_v(const DevCtx *__restrict__ x, const float *v, const float *rom, int i, int k, int j) {
+  const int ktf = x->kde - 1;
+  if (k <= 1 || k >= x->kde) return 0.f;
+  const float *fzm = FL(fnm), *fzp = FL(fnp);
+  if (k == 2 || k == ktf)
+    return 0.5f * (A3(rom, i, k, j) + A3(rom, i, k, j - 1)) * (A1(fzm, k) * A3(v, i, k, j) + A1(fzp, k) * A3(v, i, k - 1, j));
+  float vel = 0.5f * (A3(rom, i, k, j) + A3(rom, i, k, j - 1));
+  return vel * flux3(A3(v, i, k - 2, j), A3(v, i, k - 1, j), A3(v, i, k, j), A3(v, i, k + 1, j), -vel);
+}
+// advect_v (module_advect_em.py:1037-1354); r covers i [1, ide-1], j [2, jde-1]
+__global__ void k_advect_v(const DevCtx *__restrict__ x, const float *v, float *tend, const float *ru, const float *rv,
+                           const float *rom, R3 r) {
+  GET_IKJ(r);
+  const float *msfvy = FL(msfvy), *msfvx = FL(msfvx), *rdzw = FL(rdnw);
+  float t = A3(tend, i, k, j);
+  t -= A2(msfvy, i, j) * x->rdy * (fy_v(x, v, rv, i, k, j + 1) - fy_v(x, v, rv, i, k, j));
+  if (i >= 2 && i <= x->ide - 2) t -= A2(msfvy, i, j) * x->rdx * (fx_v(x, v, ru, i + 1, k, j) - fx_v(x, v, ru, i, k, j));
+  t -= (A2(msfvy, i, j) / A2(msfvx, i, j)) * A1(rdzw, k) * (fz_v(x, v, rom, i, k + 1, j) - fz_v(x, v, rom, i, k, j));
+  A3(tend, i, k, j) = t;
+}
+
+// ---------------- w momentum (levels k in [2, kde]) -------------------------
+__device__ __forceinline__ float rvw(const DevCtx *__restrict__ x, const float *rv, int i, int k, int j) {
+  const float *fzm = FL(fnm), *fzp = FL(fnp);
+  if (k == x->kde) return (2.f - A1(fzm, k - 1)) * A3(rv, i, k - 1, j) - A1(fzp, k - 1) * A3(rv, i, k - 2, j);
+  return A1(fzm, k) * A3(rv, i, k, j) + A1(fzp, k) * A3(rv, i, k - 1, j);
+}
+__device__ __forceinline__ float fy_w(const DevCtx *__restrict__ x, const float *w, const float *rv, int i, int k, int j) {
+  const int jde = x->jde;
+  float vel = rvw(x, rv, i, k, j);
+  if (j >= 4 && j <= jde - 3)
+    return vel * flux5(A3(w, i, k, j - 3), A3(w, i, k, j - 2), A3(w, i, k, j - 1), A3(w, i, k, j), A3(w, i, k, j + 1), A3(w, i, k, j + 2), vel);
+  if (j == 2 || j == jde - 1) return 0.5f * vel * (A3(w, i, k, j) + A3(w, i, k, j - 1));
+  return vel * flux3(A3(w, i, k, j - 2), A3(w, i, k, j - 1), A3(w, i, k, j), A3(w, i, k, j + 1), vel);
+}
+__device__ __forceinline__ float fx_w(const DevCtx *__restrict__ x, const float *w, const float *ru, int i, int k, int j) {
+  const int ide = x->ide;
+  float vel = rvw(x, ru, i, k, j);
+  if (i >= 4 && i <= ide - 3)
+    return vel * flux5(A3(w, i - 3, k, j), A3(w, i - 2, k, j), A3(w, i - 1, k, j), A3(w, i, k, j), A3(w, i + 1, k, j), A3(w, i + 2, k, j), vel);
+  if (i == 2 || i == ide - 1) return 0.5f * vel * (A3(w, i, k, j) + A3(w, i - 1, k, j));
+  return vel * flux3(A3(w, i - 2, k, j), A3(w, i - 1, k, j), A3(w, i, k, j), A3(w, i + 1, k, j), vel);
+}
+__device__ __forceinline__ float fz_w(const DevCtx *__restrict__ x, const float *w, const float *rom, int i, int k, int j) {
+  if (k <= 1 || k > x->kde) return 0.f;
+  if (k == 2 || k == x->kde) return 0.25f * (A3(rom, i, k, j) + A3(rom, i, k - 1, j)) * (A3(w, i, k, j) + A3(w, i, k - 1, j));
+  float vel = 0.5f * (A3(rom, i, k, j) + A3(rom, i, k - 1, j));
+  return vel * flux3(A3(w, i, k - 2, j), A3(w, i, k - 1, j), A3(w, i, k, j), A3(w, i, k + 1, j), -vel);
+}
+// advect_w (module_advect_em.py:1357-1640); r covers i [1, ide-1], k [2, kde], j [1, jde-1]
+__global__ void k_advect_w(const DevCtx *__restrict__ x, const float *w, float *tend, const float *ru, const float *rv,
+                           const float *rom, R3 r) {
+  GET_IKJ(r);
+  const float *msftx = FL(msftx), *rdzu = FL(rdn);
+  float t = A3(tend, i, k, j);
+  if (j >= 2 && j <= x->jde - 2) t -= A2(msftx, i, j) * x->rdy * (fy_w(x, w, rv, i, k, j + 1) - fy_w(x, w, rv, i, k, j));
+  if (i >= 2 && i <= x->ide - 2) t -= A2(msftx, i, j) * x->rdx * (fx_w(x, w, ru, i + 1, k, j) - fx_w(x, w, ru, i, k, j));
+  if (k <= x->kde - 1) t -= A1(rdzu, k) * (fz_w(x, w, rom, i, k + 1, j) - fz_w(x, w, rom, i, k, j));
+  else t += 2.f * A1(rdzu, k - 1) * fz_w(x, w, rom, i, k, j);
+  A3(tend, i, k, j) = t;
+}
+
+// ---------------- positive-definite scalar advection ------------------------
+// pass 1: high-order minus upwind (antidiffusive) and upwind fluxes
+// (module_advect_em.py:537-660) on all faces the limiter and tendency read.
+__global__ void k_pd_fluxes(const DevCtx *__restrict__ x, const float *q, const float *q_old, const float *ru, const float *rv,
+                            const float *rom, const float *mut, float dt, R3 r) {
+  GET_IKJ(r);
+  const float *c1 = FL(c1h), *c2 = FL(c2h), *msftx = FL(msftx), *msfty = FL(msfty), *rdzw = FL(rdnw);
+  const float *fzm = FL(fnm), *fzp = FL(fnp);
+  float *fqx = SCR(S_FQX), *fqy = SCR(S_FQY), *fqz = SCR(S_FQZ), *fqxl = SCR(S_FQXL), *fqyl = SCR(S_FQYL), *fqzl = SCR(S_FQZL);
+  const int ide = x->ide, jde = x->jde, ktf = x->kde - 1;
+  const float rdx = x->rdx, rdy = x->rdy;
+  // y face j (rows 2..jde-1)
+  if (j >= 2 && j <= jde - 1 && i <= ide - 1 && k <= ktf) {
+    float dy = 2.f / (A2(msftx, i, j) + A2(msftx, i, j - 1)) / rdy;
+    float mu = 0.5f * ((A1(c1, k) * A2(mut, i, j) + A1(c2, k)) + (A1(c1, k) * A2(mut, i, j - 1) + A1(c2, k)));
+    float vel = A3(rv, i, k, j);
+    float cr = vel * dt / dy / mu;
+    float low = mu * (dy / dt) * flux_up(A3(q_old, i, k, j - 1), A3(q_old, i, k, j), cr);
+    float high;
+    if (j >= 4 && j <= jde - 3)
+      high = vel * flux5pd(A3(q, i, k, j - 3), A3(q, i, k, j - 2), A3(q, i, k, j - 1), A3(q, i, k, j), A3(q, i, k, j + 1), A3(q, i, k, j + 2), vel);
+    else if (j == 2 || j == jde - 1)
+      high = 0.5f * vel * (A3(q, i, k, j) + A3(q, i, k, j - 1));
+    else
+      high = vel * flux3pd(A3(q, i, k, j - 2), A3(q, i, k, j - 1), A3(q, i, k, j), A3(q, i, k, j + 1), vel);
+    A3(fqyl, i, k, j) = low;
+    A3(fqy, i, k, j) = high - low;
+  } else {
+    A3(fqyl, i, k, j) = 0.f;
+    A3(fqy, i, k, j) = 0.f;
+  }
+  // x face i (2..ide-1)
+  if (i >= 2 && i <= ide - 1 && j <= jde - 1 && k <= ktf) {
+    float dx = 2.f / (A2(msfty, i, j) + A2(msfty, i - 1, j)) / rdx;
+    float mu = 0.5f * ((A1(c1, k) * A2(mut, i, j) + A1(c2, k)) + (A1(c1, k) * A2(mut, i - 1, j) + A1(c2, k)));
+    float vel = A3(ru, i, k, j);
+    float cr = vel * dt / dx / mu;
+    float low = mu * (dx / dt) * flux_up(A3(q_old, i - 1, k, j), A3(q_old, i, k, j), cr);
+    float high;
+    if (i >= 4 && i <= ide - 3)
+      high = vel * flux5pd(A3(q, i - 3, k, j), A3(q, i - 2, k, j), A3(q, i - 1, k, j), A3(q, i, k, j), A3(q, i + 1, k, j), A3(q, i + 2, k, j), vel);
+    else if (i == 2 || i == ide - 1)
+      high = 0.5f * vel * (A3(q, i, k, j) + A3(q, i - 1, k, j));
+    else
+      high = vel * flux3pd(A3(q, i - 2, k, j), A3(q, i - 1, k, j), A3(q, i, k, j), A3(q, i + 1, k, j), vel);
+    A3(fqxl, i, k, j) = low;
+    A3(fqx, i, k, j) = high - low;
+  } else {
+    A3(fqxl, i, k, j) = 0.f;
+    A3(fqx, i, k, j) = 0.f;
+  }
+  // z face k (2..ktf); zero at k = 1 and k = kde
+  if (k >= 2 && k <= ktf && i <= ide - 1 && j <= jde - 1) {
+    float dz = 2.f / (A1(rdzw, k) + A1(rdzw, k - 1));
+    float mu = A1(c1, k) * A2(mut, i, j) + A1(c2, k);
+    float vel = A3(rom, i, k, j);
+    float cr = vel * dt / dz / mu;
+    float low = mu * (dz / dt) * flux_up(A3(q_old, i, k - 1, j), A3(q_old, i, k, j), cr);
+    float high;
+    if (k == 2 || k == ktf) high = vel * (A1(fzm, k) * A3(q, i, k, j) + A1(fzp, k) * A3(q, i, k - 1, j));
+    else high = vel * flux3pd(A3(q, i, k - 2, j), A3(q, i, k - 1, j), A3(q, i, k, j), A3(q, i, k + 1, j), -vel);
+    A3(fqzl, i, k, j) = low;
+    A3(fqz, i, k, j) = high - low;
+  } else {
+    A3(fqzl, i, k, j) = 0.f;
+    A3(fqz, i, k, j) = 0.f;
+  }
+}
+// pass 2: renormalisation factor of each limited cell (1 = not limited)
+// (module_advect_em.py:662-720); cells i in [2, ide-2], j in [2, jde-2].
+__global__ void k_pd_scale(const DevCtx *__restrict__ x, const float *q_old, const float *mub, const float *mu_old, float dt, R3 r) {
+  GET_IKJ(r);
+  const float *c1 = FL(c1h), *c2 = FL(c2h), *msftx = FL(msftx), *msfty = FL(msfty), *rdzw = FL(rdnw);
+  const float *fqx = SCR(S_FQX), *fqy = SCR(S_FQY), *fqz = SCR(S_FQZ), *fqxl = SCR(S_FQXL), *fqyl = SCR(S_FQYL), *fqzl = SCR(S_FQZL);
+  float *sc = SCR(S_SCALE);
+  float scale = 1.f;
+  if (i >= 2 && i <= x->ide - 2 && j >= 2 && j <= x->jde - 2 && k <= x->kde - 1) {
+    const float rdx = x->rdx, rdy = x->rdy;
+    float mm = A2(msftx, i, j) * A2(msfty, i, j);
+    float ph_low = ((A1(c1, k) * A2(mub, i, j) + A1(c2, k)) + (A1(c1, k) * A2(mu_old, i, j))) * A3(q_old, i, k, j) -
+                   dt * (mm * (rdx * (A3(fqxl, i + 1, k, j) - A3(fqxl, i, k, j)) + rdy * (A3(fqyl, i, k, j + 1) - A3(fqyl, i, k, j))) +
+                         A2(msfty, i, j) * A1(rdzw, k) * (A3(fqzl, i, k + 1, j) - A3(fqzl, i, k, j)));
+    float flux_out = dt * (mm * (rdx * (fmaxf(0.f, A3(fqx, i + 1, k, j)) - fminf(0.f, A3(fqx, i, k, j))) +
+                                 rdy * (fmaxf(0.f, A3(fqy, i, k, j + 1)) - fminf(0.f, A3(fqy, i, k, j)))) +
+                           A2(msfty, i, j) * A1(rdzw, k) * (fminf(0.f, A3(fqz, i, k + 1, j)) - fmaxf(0.f, A3(fqz, i, k, j))));
+    if (flux_out > ph_low) scale = fmaxf(0.f, ph_low / (flux_out + 1.0e-20f));
+  }
+  A3(sc, i, k, j) = scale;
+}
+// pass 3: tendency from the renormalised faces; a face is scaled by the
+// cell its antidiffusive flux leaves (the only cell that may limit it).
+__global__ void k_pd_tend(const DevCtx *__restrict__ x, float *tend, R3 r) {
+  GET_IKJ(r);
+  const float *msftx = FL(msftx), *rdzw = FL(rdnw);
+  const float *fqx = SCR(S_FQX), *fqy = SCR(S_FQY), *fqz = SCR(S_FQZ), *fqxl = SCR(S_FQXL), *fqyl = SCR(S_FQYL), *fqzl = SCR(S_FQZL);
+  const float *sc = SCR(S_SCALE);
+  const float rdx = x->rdx, rdy = x->rdy;
+  // face fluxes after limiting
+  float zt = A3(fqz, i, k + 1, j), zb = A3(fqz, i, k, j);
+  if (zt < 0.f) zt *= A3(sc, i, k, j);          // outflow upward from this cell
+  else if (zt > 0.f && k + 1 <= x->kde - 1) zt *= A3(sc, i, k + 1, j);
+  if (zb > 0.f) zb *= A3(sc, i, k, j);
+  else if (zb < 0.f && k - 1 >= 1) zb *= A3(sc, i, k - 1, j);
+  float t = A3(tend, i, k, j);
+  t += -A1(rdzw, k) * (zt - zb + A3(fqzl, i, k + 1, j) - A3(fqzl, i, k, j));
+  if (i >= 2 && i <= x->ide - 2) {
+    float xe = A3(fqx, i + 1, k, j), xw = A3(fqx, i, k, j);
+    if (xe > 0.f) xe *= A3(sc, i, k, j);
+    else if (xe < 0.f) xe *= A3(sc, i + 1, k, j);
+    if (xw < 0.f) xw *= A3(sc, i, k, j);
+    else if (xw > 0.f) xw *= A3(sc, i - 1, k, j);
+    t += -A2(msftx, i, j) * (rdx * (xe - xw + A3(fqxl, i + 1, k, j) - A3(fqxl, i, k, j)));
+  }
+  if (j >= 2 && j <= x->jde - 2) {
+    float yn = A3(fqy, i, k, j + 1), ys = A3(fqy, i, k, j);
+    if (yn > 0.f) yn *= A3(sc, i, k, j);
+    else if (yn < 0.f) yn *= A3(sc, i, k, j + 1);
+    if (ys < 0.f) ys *= A3(sc, i, k, j);
+    else if (ys > 0.f) ys *= A3(sc, i, k, j - 1);
+    t += -A2(msftx, i, j) * (rdy * (yn - ys + A3(fqyl, i, k, j + 1) - A3(fqyl, i, k, j)));
+  }
+  A3(tend, i, k, j) = t;
+}

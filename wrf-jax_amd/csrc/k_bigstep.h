@@ -1,0 +1,342 @@
+// RK-stage (big step) kernels: rk_step_prep, rhs_ph, horizontal pressure
+// gradient, pg_buoy_w, Coriolis, curvature, rk_addtend_dry, calc_p_rho_phi.
+// Semantics: src/wrf_jax/dyn_em/module_big_step_utilities_em.py,
+// module_big_step_utilities_em_fast.py and module_em.py.
+#pragma once
+#include "wrfdc_common.h"
+
+__global__ void k_fill(float *p, float v, size_t n) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; t < n; t += stride) p[t] = v;
+}
+__global__ void k_copy(float *dst, const float *src, size_t n) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; t < n; t += stride) dst[t] = src[t];
+}
+
+// mut, muu, muv (calc_mu_uv, module_big_step_utilities_em.py:174-266) and
+// muus/muvs from muts (calc_mu_uv_1).  2-D over i [1, ide], j [1, jde].
+__global__ void k_mu_uv(const DevCtx *__restrict__ x, const float *mu, const float *mub, float *muu, float *muv, float *mut) {
+  const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = 1 + (int)blockIdx.y;
+  const int ide = x->ide, jde = x->jde;
+  if (i > ide || j > jde) return;
+  // mub == nullptr: calc_mu_uv_1 on the full column mass (adding the zero
+  // base state of the reference form leaves every sum unchanged)
+  if (mut) A2(mut, i, j) = A2(mub, i, j) + A2(mu, i, j);
+  if (j <= jde - 1) {
+    int ia = (i == ide) ? ide - 1 : i;
+    int ib = (i == 1) ? 1 : ((i == ide) ? ide - 1 : i - 1);
+    float s = A2(mu, ia, j) + A2(mu, ib, j);
+    if (mub) s = s + A2(mub, ia, j) + A2(mub, ib, j);
+    A2(muu, i, j) = 0.5f * s;
+  }
+  if (i <= ide - 1) {
+    int ja = (j == jde) ? jde - 1 : j;
+    int jb = (j == 1) ? 1 : ((j == jde) ? jde - 1 : j - 1);
+    float s = A2(mu, i, ja) + A2(mu, i, jb);
+    if (mub) s = s + A2(mub, i, ja) + A2(mub, i, jb);
+    A2(muv, i, j) = 0.5f * s;
+  }
+}
+
+// couple_momentum + calc_cq + calc_alt + calc_php over the full 3-D extent
+// (zero outside each field's range, as the reference's fresh outputs).
+__global__ void k_rk_prep3(const DevCtx *__restrict__ x, R3 r) {
+  GET_IKJ(r);
+  const int ide = x->ide, jde = x->jde, kde = x->kde;
+  const float *u = FL(u_2), *v = FL(v_2), *w = FL(w_2), *muu = FL(muu), *muv = FL(muv), *mut = FL(mut);
+  const float *c1h = FL(c1h), *c2h = FL(c2h), *c1f = FL(c1f), *c2f = FL(c2f);
+  const size_t o = XI3(i, k, j);
+  // ru / rv / rw
+  FL(ru)[o] = (j <= jde - 1 && k <= kde - 1) ? A3(u, i, k, j) * (A1(c1h, k) * A2(muu, i, j) + A1(c2h, k)) / A2(FL(msfuy), i, j) : 0.f;
+  FL(rv)[o] = (i <= ide - 1 && k <= kde - 1) ? A3(v, i, k, j) * (A1(c1h, k) * A2(muv, i, j) + A1(c2h, k)) * A2(FL(msfvx_inv), i, j) : 0.f;
+  FL(rw)[o] = (i <= ide - 1 && j <= jde - 1) ? A3(w, i, k, j) * (A1(c1f, k) * A2(mut, i, j) + A1(c2f, k)) / A2(FL(msfty), i, j) : 0.f;
+  // alt / php on mass points
+  const bool mass = (i <= ide - 1 && j <= jde - 1 && k <= kde - 1);
+  FL(alt)[o] = mass ? A3(FL(al), i, k, j) + A3(FL(alb), i, k, j) : 0.f;
+  if (i == ide || j == jde) FL(ww)[o] = 0.f;
+  FL(php)[o] = mass ? 0.5f * (A3(FL(phb), i, k, j) + A3(FL(phb), i, k + 1, j) + A3(FL(ph_2), i, k, j) + A3(FL(ph_2), i, k + 1, j)) : 0.f;
+  // cqu / cqv / cqw; neighbour i-1 / j-1 at the low edge wraps to the last
+  // stored index like the reference's negative indexing
+  float cu = 0.f, cv = 0.f, cw = 0.f;
+  if (k <= kde - 1) {
+    const int im = (i == 1) ? ide : i - 1;
+    const int jm = (j == 1) ? jde : j - 1;
+    float su = 0.f, sv = 0.f, sw = 0.f;
+    for (int s = 0; s < x->nmoist; s++) {
+      const float *q = SP(moist, s);
+      float q0 = A3(q, i, k, j);
+      su = su + q0 + A3(q, im, k, j);
+      sv = sv + q0 + A3(q, i, k, jm);
+      if (k > 1) sw = sw + q0 + A3(q, i, k - 1, j);
+    }
+    if (j < jde) cu = 1.f / (1.f + 0.5f * su);
+    if (i < ide) cv = 1.f / (1.f + 0.5f * sv);
+    if (i < ide && j < jde && k > 1) cw = 0.5f * sw;
+  }
+  FL(cqu)[o] = cu;
+  FL(cqv)[o] = cv;
+  FL(cqw)[o] = cw;
+}
+
+// calc_ww_cp (module_big_step_utilities_em.py:655-803): column kernel over
+// mass columns.  Omega is the bottom-up integral of the coupled horizontal
+// mass divergence minus its column share; the divergence of level k-1 is
+// recomputed in the second pass instead of being held in a per-thread array.
+__device__ __forceinline__ float ww_divv(const DevCtx *__restrict__ x, int i, int k, int j, float muu_i, float muu_ip, float muv_j,
+                                         float muv_jp) {
+  const float *u = FL(u_2), *v = FL(v_2);
+  const float c1 = A1(FL(c1h), k), c2 = A1(FL(c2h), k);
+  float xf = (c1 * muu_ip + c2) * A3(u, i + 1, k, j) / A2(FL(msfuy), i + 1, j) - (c1 * muu_i + c2) * A3(u, i, k, j) / A2(FL(msfuy), i, j);
+  float yf = (c1 * muv_jp + c2) * A3(v, i, k, j + 1) * A2(FL(msfvx_inv), i, j + 1) - (c1 * muv_j + c2) * A3(v, i, k, j) * A2(FL(msfvx_inv), i, j);
+  return A2(FL(msftx), i, j) * A1(FL(dnw), k) * (x->rdx * xf + x->rdy * yf);
+}
+__global__ void k_ww_cp(const DevCtx *__restrict__ x, R2 r) {
+  GET_IJ(r);
+  const float *mup = FL(mu_2), *mub = FL(mub);
+  float *ww = FL(ww);
+  const int ktf = x->kde - 1;
+  const int im = (i - 1 < 1) ? i : i - 1, jm = (j - 1 < 1) ? j : j - 1;
+  const float muu_i = 0.5f * (A2(mup, i, j) + A2(mub, i, j) + A2(mup, im, j) + A2(mub, im, j));
+  const float muu_ip = 0.5f * (A2(mup, i + 1, j) + A2(mub, i + 1, j) + A2(mup, i, j) + A2(mub, i, j));
+  const float muv_j = 0.5f * (A2(mup, i, j) + A2(mub, i, j) + A2(mup, i, jm) + A2(mub, i, jm));
+  const float muv_jp = 0.5f * (A2(mup, i, j + 1) + A2(mub, i, j + 1) + A2(mup, i, j) + A2(mub, i, j));
+  float dmdt = 0.f;
+  for (int k = 1; k <= ktf; k++) dmdt += ww_divv(x, i, k, j, muu_i, muu_ip, muv_j, muv_jp);
+  A3(ww, i, 1, j) = 0.f;
+  float prev = 0.f;
+  for (int k = 2; k <= ktf; k++) {
+    const float cur = prev - A1(FL(dnw), k - 1) * A1(FL(c1h), k - 1) * dmdt - ww_divv(x, i, k - 1, j, muu_i, muu_ip, muv_j, muv_jp);
+    A3(ww, i, k, j) = cur;
+    prev = cur;
+  }
+  A3(ww, i, x->kde, j) = 0.f;
+}
+
+// rhs_ph (module_big_step_utilities_em.py:2386-2794): vertical advection of
+// phi, the w g term and horizontal phi advection (6th/4th/2nd order with the
+// reference's specified-boundary column pattern).  k in [2, kde].
+__global__ void k_rhs_ph(const DevCtx *__restrict__ x, R3 r) {
+  GET_IKJ(r);
+  const int ide = x->ide, jde = x->jde, kde = x->kde, kte = kde;
+  const float *u = FL(u_2), *v = FL(v_2), *ww = FL(ww), *ph = FL(ph_2), *phb = FL(phb), *w = FL(w_2);
+  const float *mut = FL(mut), *muu = FL(muu), *muv = FL(muv), *c1f = FL(c1f), *c2f = FL(c2f);
+  const float *fnm = FL(fnm), *fnp = FL(fnp), *rdnw = FL(rdnw), *msfux = FL(msfux), *msfvy = FL(msfvy), *msfty = FL(msfty);
+  const float rdx = x->rdx, rdy = x->rdy, g = x->g, cfn = x->cfn, cfn1 = x->cfn1;
+  float out = A3(FL(ph_tend), i, k, j);
+  if (k <= kte - 1) {
+    float wdk = 0.5f * (A3(ww, i, k, j) + A3(ww, i, k - 1, j)) * A1(rdnw, k - 1) *
+                (A3(ph, i, k, j) - A3(ph, i, k - 1, j) + A3(phb, i, k, j) - A3(phb, i, k - 1, j));
+    float wdk1 = 0.5f * (A3(ww, i, k + 1, j) + A3(ww, i, k, j)) * A1(rdnw, k) *
+                 (A3(ph, i, k + 1, j) - A3(ph, i, k, j) + A3(phb, i, k + 1, j) - A3(phb, i, k, j));
+    out += -(A1(fnm, k) * wdk1 + A1(fnp, k) * wdk);
+  } else {
+    out = 0.f;
+  }
+  out += (A1(c1f, k) * A2(mut, i, j) + A1(c2f, k)) * g * A3(w, i, k, j) / A2(msfty, i, j);
+  // y direction
+  {
+    int mode = 0;  // 0 none, 2 second, 4 fourth, 6 sixth
+    if (j >= 4 && j <= jde - 4) mode = 6;
+    else if (j == 3 || j == jde - 3) mode = 4;
+    else if (j == 2 || j == jde - 2) mode = 2;
+    if (mode) {
+      float va1, va0, fac;
+      if (k < kte) {
+        va1 = (A1(c1f, k) * A2(muv, i, j + 1) + A1(c2f, k)) * (A3(v, i, k, j + 1) + A3(v, i, k - 1, j + 1)) * A2(msfvy, i, j + 1);
+        va0 = (A1(c1f, k) * A2(muv, i, j) + A1(c2f, k)) * (A3(v, i, k, j) + A3(v, i, k - 1, j)) * A2(msfvy, i, j);
+        fac = 0.25f;
+      } else {
+        va1 = (A1(c1f, k) * A2(muv, i, j + 1) + A1(c2f, k)) * (cfn * A3(v, i, k - 1, j + 1) + cfn1 * A3(v, i, k - 2, j + 1)) * A2(msfvy, i, j + 1);
+        va0 = (A1(c1f, k) * A2(muv, i, j) + A1(c2f, k)) * (cfn * A3(v, i, k - 1, j) + cfn1 * A3(v, i, k - 2, j)) * A2(msfvy, i, j);
+        fac = 0.5f;
+      }
+      if (mode == 2) {
+        float term = va1 * (A3(phb, i, k, j + 1) - A3(phb, i, k, j) + A3(ph, i, k, j + 1) - A3(ph, i, k, j)) +
+                     va0 * (A3(phb, i, k, j) - A3(phb, i, k, j - 1) + A3(ph, i, k, j) - A3(ph, i, k, j - 1));
+        out += -(fac * rdy / A2(msfty, i, j)) * term;
+      } else {
+        float ca = mode == 6 ? 45.f : 8.f, cb = mode == 6 ? -9.f : -1.f, cc = mode == 6 ? 1.f : 0.f;
+        float scale = mode == 6 ? (1.f / 60.f) : (1.f / 12.f);
+        int mo = mode == 6 ? 3 : 2;
+        float grad = ca * (A3(ph, i, k, j + 1) - A3(ph, i, k, j - 1)) + cb * (A3(ph, i, k, j + 2) - A3(ph, i, k, j - 2)) +
+                     cc * (A3(ph, i, k, j + mo) - A3(ph, i, k, j - mo)) + ca * (A3(phb, i, k, j + 1) - A3(phb, i, k, j - 1)) +
+                     cb * (A3(phb, i, k, j + 2) - A3(phb, i, k, j - 2)) + cc * (A3(phb, i, k, j + mo) - A3(phb, i, k, j - mo));
+        out += -(fac * rdy / A2(msfty, i, j)) * (va1 + va0) * scale * grad;
+      }
+    }
+  }
+  // x direction: columns ids+2 / ide-3 get no advection for specified
+  // boundaries (the reference applies their 4th-order term only for open ones)
+  {
+    int mode = 0;
+    if (i >= 4 && i <= ide - 4) mode = 6;
+    else if (i == 2 || i == ide - 2) mode = 2;
+    if (mode) {
+      float ua1, ua0, fac;
+      if (k < kte) {
+        ua1 = (A1(c1f, k) * A2(muu, i + 1, j) + A1(c2f, k)) * (A3(u, i + 1, k, j) + A3(u, i + 1, k - 1, j)) * A2(msfux, i + 1, j);
+        ua0 = (A1(c1f, k) * A2(muu, i, j) + A1(c2f, k)) * (A3(u, i, k, j) + A3(u, i, k - 1, j)) * A2(msfux, i, j);
+        fac = 0.25f;
+      } else {
+        ua1 = (A1(c1f, k) * A2(muu, i + 1, j) + A1(c2f, k)) * (cfn * A3(u, i + 1, k - 1, j) + cfn1 * A3(u, i + 1, k - 2, j)) * A2(msfux, i + 1, j);
+        ua0 = (A1(c1f, k) * A2(muu, i, j) + A1(c2f, k)) * (cfn * A3(u, i, k - 1, j) + cfn1 * A3(u, i, k - 2, j)) * A2(msfux, i, j);
+        fac = 0.5f;
+      }
+      if (mode == 2) {
+        float term = ua1 * (A3(phb, i + 1, k, j) - A3(phb, i, k, j) + A3(ph, i + 1, k, j) - A3(ph, i, k, j)) +
+                     ua0 * (A3(phb, i, k, j) - A3(phb, i - 1, k, j) + A3(ph, i, k, j) - A3(ph, i - 1, k, j));
+        out += -(fac * rdx / A2(msfty, i, j)) * term;
+      } else {
+        float grad = 45.f * (A3(ph, i + 1, k, j) - A3(ph, i - 1, k, j)) - 9.f * (A3(ph, i + 2, k, j) - A3(ph, i - 2, k, j)) +
+                     1.f * (A3(ph, i + 3, k, j) - A3(ph, i - 3, k, j)) + 45.f * (A3(phb, i + 1, k, j) - A3(phb, i - 1, k, j)) -
+                     9.f * (A3(phb, i + 2, k, j) - A3(phb, i - 2, k, j)) + 1.f * (A3(phb, i + 3, k, j) - A3(phb, i - 3, k, j));
+        out += -(fac * rdx / A2(msfty, i, j)) * (ua1 + ua0) * (1.f / 60.f) * grad;
+      }
+    }
+  }
+  A3(FL(ph_tend), i, k, j) = out;
+}
+
+// horizontal_pressure_gradient (module_big_step_utilities_em.py:1452-1671).
+// axis 0: u points i [2, ide-1], j [1, jde-1]; axis 1: v points i [1, ide-1], j [2, jde-1].
+__device__ __forceinline__ float hpg_dpn(const DevCtx *__restrict__ x, const float *p, int i, int j, int di, int dj, int k) {
+  if (k == 1)
+    return 0.5f * (x->cf1 * (A3(p, i - di, 1, j - dj) + A3(p, i, 1, j)) + x->cf2 * (A3(p, i - di, 2, j - dj) + A3(p, i, 2, j)) +
+                   x->cf3 * (A3(p, i - di, 3, j - dj) + A3(p, i, 3, j)));
+  if (k >= x->kde) return 0.f;
+  return 0.5f * (A1(FL(fnm), k) * (A3(p, i - di, k, j - dj) + A3(p, i, k, j)) + A1(FL(fnp), k) * (A3(p, i - di, k - 1, j - dj) + A3(p, i, k - 1, j)));
+}
+__global__ void k_hpg(const DevCtx *__restrict__ x, int axis, R3 r) {
+  GET_IKJ(r);
+  const float *ph = FL(ph_2), *alt = FL(alt), *p = FL(p), *pb = FL(pb), *al = FL(al), *php = FL(php), *mu = FL(mu_2);
+  const float *c1h = FL(c1h), *c2h = FL(c2h), *rdnw = FL(rdnw);
+  const int di = axis == 0 ? 1 : 0, dj = axis == 0 ? 0 : 1;
+  const float rd = axis == 0 ? x->rdx : x->rdy;
+  const float rr = axis == 0 ? A2(FL(msfux), i, j) / A2(FL(msfuy), i, j) : A2(FL(msfvy), i, j) / A2(FL(msfvx), i, j);
+  const float m = axis == 0 ? A2(FL(muu), i, j) : A2(FL(muv), i, j);
+  float dp = rr * 0.5f * rd * (A1(c1h, k) * m + A1(c2h, k)) *
+             ((A3(ph, i, k + 1, j) - A3(ph, i - di, k + 1, j - dj) + A3(ph, i, k, j) - A3(ph, i - di, k, j - dj)) +
+              (A3(alt, i, k, j) + A3(alt, i - di, k, j - dj)) * (A3(p, i, k, j) - A3(p, i - di, k, j - dj)) +
+              (A3(al, i, k, j) + A3(al, i - di, k, j - dj)) * (A3(pb, i, k, j) - A3(pb, i - di, k, j - dj)));
+  dp = dp + rr * rd * (A3(php, i, k, j) - A3(php, i - di, k, j - dj)) *
+                (A1(rdnw, k) * (hpg_dpn(x, p, i, j, di, dj, k + 1) - hpg_dpn(x, p, i, j, di, dj, k)) -
+                 0.5f * (A1(c1h, k) * A2(mu, i - di, j - dj) + A1(c1h, k) * A2(mu, i, j)));
+  if (axis == 0) A3(FL(ru_tend), i, k, j) = A3(FL(ru_tend), i, k, j) - A3(FL(cqu), i, k, j) * dp;
+  else A3(FL(rv_tend), i, k, j) = A3(FL(rv_tend), i, k, j) - A3(FL(cqv), i, k, j) * dp;
+}
+
+// pg_buoy_w (module_big_step_utilities_em.py:1674-1800): column kernel
+__global__ void k_pg_buoy_w(const DevCtx *__restrict__ x, R2 r) {
+  GET_IJ(r);
+  float *rwt = FL(rw_tend), *cqw = FL(cqw);
+  const float *p = FL(p), *mu = FL(mu_2), *mub = FL(mub), *c1f = FL(c1f), *c2f = FL(c2f), *rdnw = FL(rdnw), *rdn = FL(rdn);
+  const int kde = x->kde;
+  const float gm = (1.f / A2(FL(msfty), i, j)) * x->g;
+  float cq = A3(cqw, i, kde - 1, j);
+  float cq1 = 1.f / (1.f + cq), cq2 = cq * cq1;
+  A3(rwt, i, kde, j) += gm * (cq1 * 2.f * A1(rdnw, kde - 1) * (-A3(p, i, kde - 1, j)) - (A1(c1f, kde) * A2(mu, i, j)) -
+                              cq2 * (A1(c1f, kde) * A2(mub, i, j) + A1(c2f, kde)));
+  for (int k = 2; k <= kde - 1; k++) {
+    cq = A3(cqw, i, k, j);
+    cq1 = 1.f / (1.f + cq);
+    cq2 = cq * cq1;
+    A3(rwt, i, k, j) += gm * (cq1 * A1(rdn, k) * (A3(p, i, k, j) - A3(p, i, k - 1, j)) - (A1(c1f, k) * A2(mu, i, j)) -
+                              cq2 * (A1(c1f, k) * A2(mub, i, j) + A1(c2f, k)));
+    A3(cqw, i, k, j) = cq1;
+  }
+}
+
+// coriolis + curvature (module_big_step_utilities_em.py:978-1120, 1214-1449)
+// for specified boundaries.  comp 0: u points; 1: v points; 2: w points.
+__device__ __forceinline__ float vxgm(const DevCtx *__restrict__ x, int i, int k, int j) {
+  const float *u = FL(u_2), *v = FL(v_2);
+  return 0.5f * (A3(u, i, k, j) + A3(u, i + 1, k, j)) * (A2(FL(msfvx), i, j + 1) - A2(FL(msfvx), i, j)) * x->rdy -
+         0.5f * (A3(v, i, k, j) + A3(v, i, k, j + 1)) * (A2(FL(msfuy), i + 1, j) - A2(FL(msfuy), i, j)) * x->rdx;
+}
+__global__ void k_cor_curv(const DevCtx *__restrict__ x, int comp, R3 r) {
+  GET_IKJ(r);
+  const float *ru = FL(ru), *rv = FL(rv), *rw = FL(rw), *u = FL(u_2), *v = FL(v_2);
+  const float *f = FL(f), *e = FL(e), *sina = FL(sina), *cosa = FL(cosa), *fzm = FL(fnm), *fzp = FL(fnp);
+  const float rer = x->reradius;
+  if (comp == 0) {
+    float val = A3(FL(ru_tend), i, k, j);
+    val = val + (A2(FL(msfux), i, j) / A2(FL(msfuy), i, j)) * 0.5f * (A2(f, i, j) + A2(f, i - 1, j)) * 0.25f *
+                    (A3(rv, i - 1, k, j + 1) + A3(rv, i, k, j + 1) + A3(rv, i - 1, k, j) + A3(rv, i, k, j));
+    val = val - 0.5f * (A2(e, i, j) + A2(e, i - 1, j)) * 0.5f * (A2(cosa, i, j) + A2(cosa, i - 1, j)) * 0.25f *
+                    (A3(rw, i - 1, k + 1, j) + A3(rw, i - 1, k, j) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
+    val = val + 0.5f * (vxgm(x, i, k, j) + vxgm(x, i - 1, k, j)) * 0.25f *
+                    (A3(rv, i - 1, k, j + 1) + A3(rv, i, k, j + 1) + A3(rv, i - 1, k, j) + A3(rv, i, k, j)) -
+          A3(u, i, k, j) * rer * 0.25f * (A3(rw, i - 1, k + 1, j) + A3(rw, i - 1, k, j) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
+    A3(FL(ru_tend), i, k, j) = val;
+  } else if (comp == 1) {
+    const float r = A2(FL(msfvy), i, j) / A2(FL(msfvx), i, j);
+    float val = A3(FL(rv_tend), i, k, j);
+    val = val - r * 0.5f * (A2(f, i, j) + A2(f, i, j - 1)) * 0.25f *
+                    (A3(ru, i, k, j) + A3(ru, i + 1, k, j) + A3(ru, i, k, j - 1) + A3(ru, i + 1, k, j - 1));
+    val = val + r * 0.5f * (A2(e, i, j) + A2(e, i, j - 1)) * 0.5f * (A2(sina, i, j) + A2(sina, i, j - 1)) * 0.25f *
+                    (A3(rw, i, k + 1, j - 1) + A3(rw, i, k, j - 1) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
+    val = val - 0.5f * (vxgm(x, i, k, j) + vxgm(x, i, k, j - 1)) * 0.25f *
+                    (A3(ru, i, k, j) + A3(ru, i + 1, k, j) + A3(ru, i, k, j - 1) + A3(ru, i + 1, k, j - 1)) -
+          r * A3(v, i, k, j) * rer * 0.25f * (A3(rw, i, k + 1, j - 1) + A3(rw, i, k, j - 1) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
+    A3(FL(rv_tend), i, k, j) = val;
+  } else {
+    const float mr = A2(FL(msftx), i, j) / A2(FL(msfty), i, j);
+    float val = A3(FL(rw_tend), i, k, j);
+    val = val + A2(e, i, j) * (A2(cosa, i, j) * 0.5f * (A1(fzm, k) * (A3(ru, i, k, j) + A3(ru, i + 1, k, j)) + A1(fzp, k) * (A3(ru, i, k - 1, j) + A3(ru, i + 1, k - 1, j))) -
+                               mr * A2(sina, i, j) * 0.5f * (A1(fzm, k) * (A3(rv, i, k, j) + A3(rv, i, k, j + 1)) + A1(fzp, k) * (A3(rv, i, k - 1, j) + A3(rv, i, k - 1, j + 1))));
+    val = val + rer * (0.5f * (A1(fzm, k) * (A3(ru, i, k, j) + A3(ru, i + 1, k, j)) + A1(fzp, k) * (A3(ru, i, k - 1, j) + A3(ru, i + 1, k - 1, j))) *
+                           0.5f * (A1(fzm, k) * (A3(u, i, k, j) + A3(u, i + 1, k, j)) + A1(fzp, k) * (A3(u, i, k - 1, j) + A3(u, i + 1, k - 1, j))) +
+                       mr * 0.5f * (A1(fzm, k) * (A3(rv, i, k, j) + A3(rv, i, k, j + 1)) + A1(fzp, k) * (A3(rv, i, k - 1, j) + A3(rv, i, k - 1, j + 1))) *
+                           0.5f * (A1(fzm, k) * (A3(v, i, k, j) + A3(v, i, k, j + 1)) + A1(fzp, k) * (A3(v, i, k - 1, j) + A3(v, i, k - 1, j + 1))));
+    A3(FL(rw_tend), i, k, j) = val;
+  }
+}
+
+// rk_addtend_dry (module_em.py:959-1148); comp 0 u, 1 v, 2 w/ph, 3 theta
+__global__ void k_addtend(const DevCtx *__restrict__ x, int comp, int rk, R3 r) {
+  GET_IKJ(r);
+  const size_t o = XI3(i, k, j);
+  if (comp == 0) {
+    const float m = A2(FL(msfuy), i, j);
+    if (rk == 1) FL(ru_tendf)[o] = FL(ru_tendf)[o] + FL(u_save)[o] * m;
+    FL(ru_tend)[o] = FL(ru_tend)[o] + FL(ru_tendf)[o] / m;
+  } else if (comp == 1) {
+    if (rk == 1) FL(rv_tendf)[o] = FL(rv_tendf)[o] + FL(v_save)[o] * A2(FL(msfvx), i, j);
+    FL(rv_tend)[o] = FL(rv_tend)[o] + FL(rv_tendf)[o] * A2(FL(msfvx_inv), i, j);
+  } else if (comp == 2) {
+    const float m = A2(FL(msfty), i, j);
+    if (rk == 1) FL(rw_tendf)[o] = FL(rw_tendf)[o] + FL(w_save)[o] * m;
+    FL(rw_tend)[o] = FL(rw_tend)[o] + FL(rw_tendf)[o] / m;
+    if (rk == 1) FL(ph_tendf)[o] = FL(ph_tendf)[o] + FL(ph_save)[o];
+    FL(ph_tend)[o] = FL(ph_tend)[o] + FL(ph_tendf)[o] / m;
+  } else {
+    const float m = A2(FL(msfty), i, j);
+    const float mass = A1(FL(c1h), k) * A2(FL(mut), i, j) + A1(FL(c2h), k);
+    if (rk == 1) FL(t_tendf)[o] = FL(t_tendf)[o] + FL(t_save)[o];
+    const float heating = mass * FL(h_diabatic)[o] / m;
+    FL(t_tend)[o] = FL(t_tend)[o] + FL(t_tendf)[o] / m + heating;
+    if (k == 1) A2(FL(mu_tend), i, j) = A2(FL(mu_tend), i, j) + A2(FL(mu_tendf), i, j);
+  }
+}
+
+// _calc_p_rho_phi_nonhydro_opt2 (runtime.py:12369-12537): hypsometric al, EOS p
+__global__ void k_calc_p_rho_phi(const DevCtx *__restrict__ x, R3 r) {
+  GET_IKJ(r);
+  const float *muts = FL(muts), *ph = FL(ph_2), *phb = FL(phb), *alb = FL(alb), *pb = FL(pb), *t = FL(t_2);
+  const float m = A2(muts, i, j), ptop = x->ptop;
+  const float pfu = A1(FL(c3f), k + 1) * m + A1(FL(c4f), k + 1) + ptop;
+  const float pfd = A1(FL(c3f), k) * m + A1(FL(c4f), k) + ptop;
+  const float phm = A1(FL(c3h), k) * m + A1(FL(c4h), k) + ptop;
+  // log and pow in FP64, rounded once to FP32 (the correctly rounded FP32
+  // values in all but vanishingly rare cases)
+  const float lg = (float)log((double)(pfd / pfu));
+  const float alv = (A3(ph, i, k + 1, j) - A3(ph, i, k, j) + A3(phb, i, k + 1, j) - A3(phb, i, k, j)) / phm / lg - A3(alb, i, k, j);
+  A3(FL(al), i, k, j) = alv;
+  const float qvf = 1.f + x->rvovrd * A3(SP(moist, 0), i, k, j);
+  const float temp = (x->r_d * (x->t0 + A3(t, i, k, j)) * qvf) / (x->p0 * (alv + A3(alb, i, k, j)));
+  A3(FL(p), i, k, j) = (float)pow((double)temp, (double)x->cpovcv) * x->p0 - A3(pb, i, k, j);
+}

@@ -1,0 +1,367 @@
+// diff_opt = 2 / km_opt = 4 kernels of the first RK stage: metric terms,
+// deformation / divergence, 2-D Smagorinsky diffusivity and the horizontal
+// diffusion of momentum, theta and the transported species.
+// Semantics: src/wrf_jax/dyn_em/module_diffusion_em.py (specified
+// boundaries, no SFS/NBA terms, mix_full_fields = false).
+#pragma once
+#include "wrfdc_common.h"
+
+// rho = (1 + qv) / alt on mass points (phy_prep,
+// module_big_step_utilities_em.py:1823-2081, rho at :1965)
+__global__ void k_rho(const DevCtx *__restrict__ x, R3 r) {
+  GET_IKJ(r);
+  A3(FL(rho), i, k, j) = (1.f / A3(FL(alt), i, k, j)) * (1.f + A3(SP(moist, 0), i, k, j));
+}
+
+// compute_diff_metrics (module_diffusion_em.py:1825-2043), full extent.
+__global__ void k_diff_metrics(const DevCtx *__restrict__ x, R3 r) {
+  GET_IKJ(r);
+  const int ide = x->ide, jde = x->jde, kde = x->kde, ktf = kde - 1;
+  const float *ph = FL(ph_2), *phb = FL(phb);
+  const float g = x->g;
+#define ZW(ii, kk, jj) ((A3(ph, ii, kk, jj) + A3(phb, ii, kk, jj)) / g)
+  if (i <= ide - 1) {
+    if (k <= ktf) A3(FL(rdzw), i, k, j) = 1.f / (ZW(i, k + 1, j) - ZW(i, k, j));
+    if (k >= 2 && k <= ktf) A3(FL(rdz), i, k, j) = 2.f / (ZW(i, k + 1, j) - ZW(i, k - 1, j));
+    if (k == 1) A3(FL(rdz), i, 1, j) = 2.f / (ZW(i, 2, j) - ZW(i, 1, j));
+  }
+#undef ZW
+  if (j <= jde - 1) {
+    if (i >= 2 && i <= ide - 1) {
+      const float zb = (x->rdx * (A3(phb, i, k, j) - A3(phb, i - 1, k, j))) / g;
+      const float zp = (x->rdx * (A3(ph, i, k, j) - A3(ph, i - 1, k, j))) / g;
+      A3(FL(zx), i, k, j) = zb + zp;
+    } else if (k <= ktf) {
+      A3(FL(zx), i, k, j) = 0.f;
+    }
+  }
+  if (i <= ide - 1) {
+    if (j >= 2 && j <= jde - 1) {
+      const float zb = (x->rdy * (A3(phb, i, k, j) - A3(phb, i, k, j - 1))) / g;
+      const float zp = (x->rdy * (A3(ph, i, k, j) - A3(ph, i, k, j - 1))) / g;
+      A3(FL(zy), i, k, j) = zb + zp;
+    } else if (k <= ktf) {
+      A3(FL(zy), i, k, j) = 0.f;
+    }
+  }
+  if (i <= ide - 1 && j <= jde - 1 && k <= ktf) {
+    float s = A3(ph, i, k, j) + A3(phb, i, k, j);
+    s = s + A3(ph, i, k + 1, j);
+    s = s + A3(phb, i, k + 1, j);
+    A3(FL(z), i, k, j) = (0.5f * s) / g;
+  }
+}
+
+// value of a staggered field scaled by a map factor
+__device__ __forceinline__ float hatv(const DevCtx *__restrict__ x, const float *f, const float *m, int i, int k, int j) {
+  return A3(f, i, k, j) / A2(m, i, j);
+}
+// average of the two columns A and B of hat = f / m onto w level k
+// (the hatavg construction shared by the deformation components)
+__device__ __forceinline__ float hat_avg(const DevCtx *__restrict__ x, const float *f, const float *m, int ia, int ja, int ib, int jb, int k) {
+  const int kte = x->kde, ktes1 = kte - 1, ktes2 = kte - 2;
+  if (k == 1)
+    return 0.5f * (x->cf1 * hatv(x, f, m, ia, 1, ja) + x->cf2 * hatv(x, f, m, ia, 2, ja) + x->cf3 * hatv(x, f, m, ia, 3, ja) +
+                   x->cf1 * hatv(x, f, m, ib, 1, jb) + x->cf2 * hatv(x, f, m, ib, 2, jb) + x->cf3 * hatv(x, f, m, ib, 3, jb));
+  if (k == kte) {
+    const float cft2 = -0.5f * A1(FL(dnw), ktes1) / A1(FL(dn), ktes1), cft1 = 1.f - cft2;
+    return 0.5f * (cft1 * (hatv(x, f, m, ia, ktes1, ja) + hatv(x, f, m, ib, ktes1, jb)) +
+                   cft2 * (hatv(x, f, m, ia, ktes2, ja) + hatv(x, f, m, ib, ktes2, jb)));
+  }
+  return 0.5f * (A1(FL(fnm), k) * (hatv(x, f, m, ia, k, ja) + hatv(x, f, m, ib, k, jb)) +
+                 A1(FL(fnp), k) * (hatv(x, f, m, ia, k - 1, ja) + hatv(x, f, m, ib, k - 1, jb)));
+}
+
+// cal_deform_and_div (module_diffusion_em.py:366-891): D11, D22, D33 and the
+// divergence on mass points.
+__global__ void k_deform_diag(const DevCtx *__restrict__ x, R3 r) {
+  GET_IKJ(r);
+  const float *u = FL(u_2), *v = FL(v_2), *w = FL(w_2), *zx = FL(zx), *zy = FL(zy), *rdzw = FL(rdzw);
+  const float *msfuy = FL(msfuy), *msfvx = FL(msfvx);
+  const float mm = A2(FL(msftx), i, j) * A2(FL(msfty), i, j);
+  float t1;
+  {
+    const float tmpzx = 0.25f * (A3(zx, i, k, j) + A3(zx, i + 1, k, j) + A3(zx, i, k + 1, j) + A3(zx, i + 1, k + 1, j));
+    t1 = (hat_avg(x, u, msfuy, i, j, i + 1, j, k + 1) - hat_avg(x, u, msfuy, i, j, i + 1, j, k)) * tmpzx * A3(rdzw, i, k, j);
+    t1 = mm * (x->rdx * (hatv(x, u, msfuy, i + 1, k, j) - hatv(x, u, msfuy, i, k, j)) - t1);
+    A3(FL(defor11), i, k, j) = 2.f * t1;
+  }
+  float dv = t1;
+  {
+    const float tmpzy = 0.25f * (A3(zy, i, k, j) + A3(zy, i, k, j + 1) + A3(zy, i, k + 1, j) + A3(zy, i, k + 1, j + 1));
+    t1 = (hat_avg(x, v, msfvx, i, j, i, j + 1, k + 1) - hat_avg(x, v, msfvx, i, j, i, j + 1, k)) * tmpzy * A3(rdzw, i, k, j);
+    t1 = mm * (x->rdy * (hatv(x, v, msfvx, i, k, j + 1) - hatv(x, v, msfvx, i, k, j)) - t1);
+    A3(FL(defor22), i, k, j) = 2.f * t1;
+  }
+  dv = dv + t1;
+  t1 = (A3(w, i, k + 1, j) - A3(w, i, k, j)) * A3(rdzw, i, k, j);
+  A3(FL(defor33), i, k, j) = 2.f * t1;
+  dv = dv + t1;
+  A3(FL(div), i, k, j) = dv;
+}
+
+// D12 at (u, v) corner points i [2, ide-1], j [2, jde-1], k [1, kde-1]
+__global__ void k_deform12(const DevCtx *__restrict__ x, R3 r) {
+  GET_IKJ(r);
+  const float *u = FL(u_2), *v = FL(v_2), *zx = FL(zx), *zy = FL(zy), *rdzw = FL(rdzw), *msfux = FL(msfux), *msfvy = FL(msfvy);
+  const float mm = 0.25f * (A2(msfux, i, j - 1) + A2(msfux, i, j)) * (A2(msfvy, i - 1, j) + A2(msfvy, i, j));
+  float d;
+  {
+    const float tmpzy = 0.25f * (A3(zy, i - 1, k, j) + A3(zy, i, k, j) + A3(zy, i - 1, k + 1, j) + A3(zy, i, k + 1, j));
+    const float t1 = (hat_avg(x, u, msfux, i, j - 1, i, j, k + 1) - hat_avg(x, u, msfux, i, j - 1, i, j, k)) * 0.25f * tmpzy *
+                     (A3(rdzw, i, k, j) + A3(rdzw, i - 1, k, j) + A3(rdzw, i - 1, k, j - 1) + A3(rdzw, i, k, j - 1));
+    d = mm * (x->rdy * (hatv(x, u, msfux, i, k, j) - hatv(x, u, msfux, i, k, j - 1)) - t1);
+  }
+  {
+    const float tmpzx = 0.25f * (A3(zx, i, k, j - 1) + A3(zx, i, k, j) + A3(zx, i, k + 1, j - 1) + A3(zx, i, k + 1, j));
+    const float t1 = (hat_avg(x, v, msfvy, i - 1, j, i, j, k + 1) - hat_avg(x, v, msfvy, i - 1, j, i, j, k)) * 0.25f * tmpzx *
+                     (A3(rdzw, i, k, j) + A3(rdzw, i, k, j - 1) + A3(rdzw, i - 1, k, j - 1) + A3(rdzw, i - 1, k, j));
+    const float second = mm * (x->rdx * (hatv(x, v, msfvy, i, k, j) - hatv(x, v, msfvy, i - 1, k, j)) - t1);
+    d = d + second;
+  }
+  A3(FL(defor12), i, k, j) = d;
+}
+
+// D13 / D23 at (u, w) and (v, w) points i [2, ide-1], j [2, jde-1], k [1, kde]
+__global__ void k_deform13_23(const DevCtx *__restrict__ x, R3 r) {
+  GET_IKJ(r);
+  const int ktf = x->kde - 1;
+  float *d13 = FL(defor13), *d23 = FL(defor23);
+  if (k == 1 || k == ktf + 1) {
+    A3(d13, i, k, j) = 0.f;
+    A3(d23, i, k, j) = 0.f;
+    return;
+  }
+  const float *w = FL(w_2), *u = FL(u_2), *v = FL(v_2), *zx = FL(zx), *zy = FL(zy), *rdz = FL(rdz);
+  const float *msfty = FL(msfty), *msftx = FL(msftx), *ub = FL(u_base), *vb = FL(v_base);
+  {
+#define H13(ii, kk) hatv(x, w, msfty, ii, kk, j)
+#define AV13(kk) (0.25f * (H13(i, kk) + H13(i, (kk) + 1) + H13(i - 1, kk) + H13(i - 1, (kk) + 1)))
+    const float mm = A2(FL(msfux), i, j) * A2(FL(msfuy), i, j);
+    const float t1 = (AV13(k) - AV13(k - 1)) * A3(zx, i, k, j) * 0.5f * (A3(rdz, i, k, j) + A3(rdz, i - 1, k, j));
+    float d = mm * (x->rdx * (H13(i, k) - H13(i - 1, k)) - t1);
+    const float val = (A3(u, i, k, j) - A1(ub, k) - A3(u, i, k - 1, j) + A1(ub, k - 1)) * 0.5f * (A3(rdz, i, k, j) + A3(rdz, i - 1, k, j));
+    A3(d13, i, k, j) = d + val;
+#undef AV13
+#undef H13
+  }
+  {
+#define H23(jj, kk) hatv(x, w, msftx, i, kk, jj)
+#define AV23(kk) (0.25f * (H23(j, kk) + H23(j, (kk) + 1) + H23(j - 1, kk) + H23(j - 1, (kk) + 1)))
+    const float mm = A2(FL(msfvx), i, j) * A2(FL(msfvy), i, j);
+    const float t1 = (AV23(k) - AV23(k - 1)) * A3(zy, i, k, j) * 0.5f * (A3(rdz, i, k, j) + A3(rdz, i, k, j - 1));
+    float d = mm * (x->rdy * (H23(j, k) - H23(j - 1, k)) - t1);
+    const float val = (A3(v, i, k, j) - A1(vb, k) - A3(v, i, k - 1, j) + A1(vb, k - 1)) * 0.5f * (A3(rdz, i, k, j) + A3(rdz, i, k, j - 1));
+    A3(d23, i, k, j) = d + val;
+#undef AV23
+#undef H23
+  }
+}
+
+// edge copy of whole planes: i = idst <- isrc for every (k, j) (axis 0) or
+// j = jdst <- jsrc for every (i, k) (axis 1); up to two fields per launch.
+__global__ void k_edge_copy(const DevCtx *__restrict__ x, float *f0, float *f1, int axis, int dst, int src, int kend) {
+  const int a = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int k = 1 + (int)blockIdx.y;
+  if (k > kend) return;
+  if (axis == 0) {
+    if (a > x->jde) return;
+    A3(f0, dst, k, a) = A3(f0, src, k, a);
+    if (f1) A3(f1, dst, k, a) = A3(f1, src, k, a);
+  } else {
+    if (a > x->ide) return;
+    A3(f0, a, k, dst) = A3(f0, a, k, src);
+    if (f1) A3(f1, a, k, dst) = A3(f1, a, k, src);
+  }
+}
+
+// smag2d_km (module_diffusion_em.py:1502-1646) with the slope correction;
+// i [2, ide-2], j [2, jde-2], k [1, kde-1].
+__global__ void k_smag(const DevCtx *__restrict__ x, R3 r) {
+  GET_IKJ(r);
+  const float *d11 = FL(defor11), *d22 = FL(defor22), *d12 = FL(defor12), *zx = FL(zx), *zy = FL(zy), *rdzw = FL(rdzw);
+  const float dx = x->dx, dy = x->dy, c_s = x->c_s;
+  const float mx = A2(FL(msftx), i, j), my = A2(FL(msfty), i, j);
+  const float dd = A3(d11, i, k, j) - A3(d22, i, k, j);
+  float def2 = 0.25f * dd * dd;
+  const float tmp = 0.25f * (A3(d12, i, k, j) + A3(d12, i, k, j + 1) + A3(d12, i + 1, k, j) + A3(d12, i + 1, k, j + 1));
+  def2 = def2 + tmp * tmp;
+  const float mlen = sqrtf(dx / mx * dy / my);
+  const float sd = sqrtf(def2);
+  float km = c_s * c_s * mlen * mlen * sd;
+  km = fminf(km, 10.f * mlen);
+  const float dxm = dx / mx, dym = dy / my;
+  const float tzx = 0.25f * (fabsf(A3(zx, i, k, j)) + fabsf(A3(zx, i + 1, k, j)) + fabsf(A3(zx, i, k + 1, j)) + fabsf(A3(zx, i + 1, k + 1, j))) *
+                    A3(rdzw, i, k, j) * dxm;
+  const float tzy = 0.25f * (fabsf(A3(zy, i, k, j)) + fabsf(A3(zy, i, k, j + 1)) + fabsf(A3(zy, i, k + 1, j)) + fabsf(A3(zy, i, k + 1, j + 1))) *
+                    A3(rdzw, i, k, j) * dym;
+  const float alpha = fmaxf(sqrtf(tzx * tzx + tzy * tzy), 1.f);
+  const float lim = fmaxf(10.f / mlen, 1.0e-3f);
+  const float divisor = sd > lim ? alpha * alpha : alpha;
+  km = km / divisor;
+  A3(FL(xkmh), i, k, j) = km;
+  A3(FL(xkmv), i, k, j) = km;
+  A3(FL(xkhh), i, k, j) = km / x->prandtl;
+  A3(FL(xkhv), i, k, j) = 0.f;
+}
+
+// titau (module_diffusion_em.py:894-1316): stress component with rho and the
+// diffusivity averaged to the component's point; zero outside [i0,i1]x[j0,j1]
+// (and at k = 1, kde for the 13 / 23 components).  kind 11, 12, 13, 23.
+__global__ void k_titau(const DevCtx *__restrict__ x, float *tt, const float *xk, const float *defor, int kind, int i0, int i1, int j0,
+                        int j1, R3 r) {
+  GET_IKJ(r);
+  const float *rho = FL(rho);
+  const int ktf = x->kde - 1;
+  float val = 0.f;
+  if (i >= i0 && i <= i1 && j >= j0 && j <= j1 && k <= ktf) {
+    if (kind == 11) {
+      val = -A3(rho, i, k, j) * A3(xk, i, k, j) * A3(defor, i, k, j);
+    } else if (kind == 12) {
+      const float ra = 0.25f * (A3(rho, i - 1, k, j) + A3(rho, i, k, j) + A3(rho, i - 1, k, j - 1) + A3(rho, i, k, j - 1));
+      const float xa = ra * 0.25f * (A3(xk, i - 1, k, j) + A3(xk, i, k, j) + A3(xk, i - 1, k, j - 1) + A3(xk, i, k, j - 1));
+      val = -xa * A3(defor, i, k, j);
+    } else if (k >= 2) {
+      const float *fnm = FL(fnm), *fnp = FL(fnp);
+      const int di = kind == 13 ? 1 : 0, dj = kind == 13 ? 0 : 1;
+      const float ra = 0.5f * (A1(fnm, k) * (A3(rho, i - di, k, j - dj) + A3(rho, i, k, j)) + A1(fnp, k) * (A3(rho, i - di, k - 1, j - dj) + A3(rho, i, k - 1, j)));
+      const float xa = ra * 0.5f * (A1(fnm, k) * (A3(xk, i, k, j) + A3(xk, i - di, k, j - dj)) + A1(fnp, k) * (A3(xk, i, k - 1, j) + A3(xk, i - di, k - 1, j - dj)));
+      val = -xa * A3(defor, i, k, j);
+    }
+  }
+  A3(tt, i, k, j) = val;
+}
+
+// horizontal_diffusion_u_2 (module_diffusion_em.py:2302-2532):
+// u points i [2, ide-1], j [2, jde-2], k [1, kde-1].
+__device__ __forceinline__ float hd_a1u(const DevCtx *__restrict__ x, const float *t1, int i, int k, int j) {
+  if (k == 1 || k == x->kde) return 0.f;
+  return 0.5f * (A1(FL(fnm), k) * (A3(t1, i - 1, k, j) + A3(t1, i, k, j)) + A1(FL(fnp), k) * (A3(t1, i - 1, k - 1, j) + A3(t1, i, k - 1, j)));
+}
+__device__ __forceinline__ float hd_a2u(const DevCtx *__restrict__ x, const float *t2, int i, int k, int j) {
+  if (k == 1 || k == x->kde) return 0.f;
+  return 0.5f * (A1(FL(fnm), k) * (A3(t2, i, k, j + 1) + A3(t2, i, k, j)) + A1(FL(fnp), k) * (A3(t2, i, k - 1, j + 1) + A3(t2, i, k - 1, j)));
+}
+__global__ void k_hdiff_u(const DevCtx *__restrict__ x, const float *t1, const float *t2, R3 r) {
+  GET_IKJ(r);
+  const float *zx = FL(zx), *zy = FL(zy), *rdzw = FL(rdzw);
+  const float zxu = 0.5f * (A3(zx, i, k, j) + A3(zx, i, k + 1, j));
+  const float zyu = 0.125f * (A3(zy, i - 1, k, j) + A3(zy, i, k, j) + A3(zy, i - 1, k, j + 1) + A3(zy, i, k, j + 1) + A3(zy, i - 1, k + 1, j) +
+                              A3(zy, i, k + 1, j) + A3(zy, i - 1, k + 1, j + 1) + A3(zy, i, k + 1, j + 1));
+  const float msfux = A2(FL(msfux), i, j), msfuy = A2(FL(msfuy), i, j);
+  const float mrdx = msfux * x->rdx, mrdy = msfuy * x->rdy;
+  const float tmpdz = (1.f / A3(rdzw, i, k, j) + 1.f / A3(rdzw, i - 1, k, j)) / 2.f;
+  const float inc = x->g * tmpdz / A1(FL(dnw), k) *
+                    (mrdx * (A3(t1, i, k, j) - A3(t1, i - 1, k, j)) + mrdy * (A3(t2, i, k, j + 1) - A3(t2, i, k, j)) -
+                     msfux * zxu * (hd_a1u(x, t1, i, k + 1, j) - hd_a1u(x, t1, i, k, j)) / tmpdz -
+                     msfuy * zyu * (hd_a2u(x, t2, i, k + 1, j) - hd_a2u(x, t2, i, k, j)) / tmpdz);
+  A3(FL(ru_tendf), i, k, j) = A3(FL(ru_tendf), i, k, j) + inc;
+}
+
+// horizontal_diffusion_v_2 (module_diffusion_em.py:2535-2765):
+// v points i [2, ide-2], j [2, jde-1], k [1, kde-1].
+__device__ __forceinline__ float hd_a1v(const DevCtx *__restrict__ x, const float *t1, int i, int k, int j) {
+  if (k == 1 || k == x->kde) return 0.f;
+  return 0.5f * (A1(FL(fnm), k) * (A3(t1, i + 1, k, j) + A3(t1, i, k, j)) + A1(FL(fnp), k) * (A3(t1, i + 1, k - 1, j) + A3(t1, i, k - 1, j)));
+}
+__device__ __forceinline__ float hd_a2v(const DevCtx *__restrict__ x, const float *t2, int i, int k, int j) {
+  if (k == 1 || k == x->kde) return 0.f;
+  return 0.5f * (A1(FL(fnm), k) * (A3(t2, i, k, j - 1) + A3(t2, i, k, j)) + A1(FL(fnp), k) * (A3(t2, i, k - 1, j - 1) + A3(t2, i, k - 1, j)));
+}
+__global__ void k_hdiff_v(const DevCtx *__restrict__ x, const float *t1, const float *t2, R3 r) {
+  GET_IKJ(r);
+  const float *zx = FL(zx), *zy = FL(zy), *rdzw = FL(rdzw);
+  const float zxv = 0.125f * (A3(zx, i, k, j) + A3(zx, i + 1, k, j) + A3(zx, i, k, j - 1) + A3(zx, i + 1, k, j - 1) + A3(zx, i, k + 1, j) +
+                              A3(zx, i + 1, k + 1, j) + A3(zx, i, k + 1, j - 1) + A3(zx, i + 1, k + 1, j - 1));
+  const float zyv = 0.5f * (A3(zy, i, k, j) + A3(zy, i, k + 1, j));
+  const float msfvx = A2(FL(msfvx), i, j), msfvy = A2(FL(msfvy), i, j);
+  const float mrdx = msfvx * x->rdx, mrdy = msfvy * x->rdy;
+  const float tmpdz = (1.f / A3(rdzw, i, k, j) + 1.f / A3(rdzw, i, k, j - 1)) / 2.f;
+  const float inc = x->g * tmpdz / A1(FL(dnw), k) *
+                    (mrdy * (A3(t2, i, k, j) - A3(t2, i, k, j - 1)) + mrdx * (A3(t1, i + 1, k, j) - A3(t1, i, k, j)) -
+                     msfvx * zxv * (hd_a1v(x, t1, i, k + 1, j) - hd_a1v(x, t1, i, k, j)) / tmpdz -
+                     msfvy * zyv * (hd_a2v(x, t2, i, k + 1, j) - hd_a2v(x, t2, i, k, j)) / tmpdz);
+  A3(FL(rv_tendf), i, k, j) = A3(FL(rv_tendf), i, k, j) + inc;
+}
+
+// horizontal_diffusion_w_2 (module_diffusion_em.py:2768-2985):
+// w points i [2, ide-2], j [2, jde-2], k [2, kde-1].
+__global__ void k_hdiff_w(const DevCtx *__restrict__ x, const float *t1, const float *t2, R3 r) {
+  GET_IKJ(r);
+  const float *zx = FL(zx), *zy = FL(zy), *rdz = FL(rdz);
+#define HA1(kk) (0.25f * (A3(t1, i + 1, (kk) + 1, j) + A3(t1, i, (kk) + 1, j) + A3(t1, i + 1, kk, j) + A3(t1, i, kk, j)))
+#define HA2(kk) (0.25f * (A3(t2, i, (kk) + 1, j + 1) + A3(t2, i, (kk) + 1, j) + A3(t2, i, kk, j + 1) + A3(t2, i, kk, j)))
+  const float zxw = 0.5f * (A3(zx, i, k, j) + A3(zx, i + 1, k, j));
+  const float zyw = 0.5f * (A3(zy, i, k, j) + A3(zy, i, k, j + 1));
+  const float mrdx = A2(FL(msftx), i, j) * x->rdx, mrdy = A2(FL(msfty), i, j) * x->rdy;
+  const float rz = A3(rdz, i, k, j);
+  const float inc = x->g / (A1(FL(dn), k) * rz) *
+                    (mrdx * (A3(t1, i + 1, k, j) - A3(t1, i, k, j)) + mrdy * (A3(t2, i, k, j + 1) - A3(t2, i, k, j)) -
+                     A2(FL(msfty), i, j) * rz * (zxw * (HA1(k) - HA1(k - 1)) + zyw * (HA2(k) - HA2(k - 1))));
+#undef HA1
+#undef HA2
+  A3(FL(rw_tendf), i, k, j) = A3(FL(rw_tendf), i, k, j) + inc;
+}
+
+// horizontal_diffusion_s (module_diffusion_em.py:2046-2291), pass 1: the
+// face fluxes H1 (u faces i [2, ide-1], j [2, jde-2]) and H2 (v faces
+// i [2, ide-2], j [2, jde-1]) of one scalar.
+__device__ __forceinline__ float hs_hav(const DevCtx *__restrict__ x, const float *var, int ia, int ja, int ib, int jb, int k) {
+  // average of columns A (the face's own cell) and B (its upwind neighbour)
+  const int ktf = x->kde - 1, ktes1 = x->kde - 1, ktes2 = x->kde - 2;
+  if (k == 1)
+    return 0.5f * (x->cf1 * A3(var, ia, 1, ja) + x->cf2 * A3(var, ia, 2, ja) + x->cf3 * A3(var, ia, 3, ja) + x->cf1 * A3(var, ib, 1, jb) +
+                   x->cf2 * A3(var, ib, 2, jb) + x->cf3 * A3(var, ib, 3, jb));
+  if (k == ktf + 1) {
+    const float dnw1 = A1(FL(dnw), ktes1), dn1 = A1(FL(dn), ktes1);
+    return 0.5f * (A3(var, ia, ktes1, ja) + (A3(var, ia, ktes1, ja) - A3(var, ia, ktes2, ja)) * 0.5f * dnw1 / dn1 + A3(var, ib, ktes1, jb) +
+                   (A3(var, ib, ktes1, jb) - A3(var, ib, ktes2, jb)) * 0.5f * dnw1 / dn1);
+  }
+  return 0.5f * (A1(FL(fnm), k) * (A3(var, ib, k, jb) + A3(var, ia, k, ja)) + A1(FL(fnp), k) * (A3(var, ib, k - 1, jb) + A3(var, ia, k - 1, ja)));
+}
+__global__ void k_hdiff_s_faces(const DevCtx *__restrict__ x, const float *var, float *h1, float *h2, R3 r) {
+  GET_IKJ(r);
+  const int ide = x->ide, jde = x->jde;
+  const float *xkhh = FL(xkhh), *rho = FL(rho), *zx = FL(zx), *zy = FL(zy), *rdzw = FL(rdzw);
+  if (i >= 2 && i <= ide - 1 && j >= 2 && j <= jde - 2) {
+    const float xkx = 0.5f * (A3(xkhh, i - 1, k, j) + A3(xkhh, i, k, j)) * 0.5f * (A3(rho, i - 1, k, j) + A3(rho, i, k, j));
+    const float tmpzx = 0.5f * (A3(zx, i, k, j) + A3(zx, i, k + 1, j));
+    const float rdzu = 2.f / (1.f / A3(rdzw, i, k, j) + 1.f / A3(rdzw, i - 1, k, j));
+    A3(h1, i, k, j) = -A2(FL(msfux), i, j) * xkx *
+                      (x->rdx * (A3(var, i, k, j) - A3(var, i - 1, k, j)) -
+                       tmpzx * (hs_hav(x, var, i, j, i - 1, j, k + 1) - hs_hav(x, var, i, j, i - 1, j, k)) * rdzu);
+  }
+  if (i >= 2 && i <= ide - 2 && j >= 2 && j <= jde - 1) {
+    const float xkx = 0.5f * (A3(xkhh, i, k, j - 1) + A3(xkhh, i, k, j)) * 0.5f * (A3(rho, i, k, j - 1) + A3(rho, i, k, j));
+    const float tmpzy = 0.5f * (A3(zy, i, k, j) + A3(zy, i, k + 1, j));
+    const float rdzv = 2.f / (1.f / A3(rdzw, i, k, j) + 1.f / A3(rdzw, i, k, j - 1));
+    A3(h2, i, k, j) = -A2(FL(msfvy), i, j) * xkx *
+                      (x->rdy * (A3(var, i, k, j) - A3(var, i, k, j - 1)) -
+                       tmpzy * (hs_hav(x, var, i, j, i, j - 1, k + 1) - hs_hav(x, var, i, j, i, j - 1, k)) * rdzv);
+  }
+}
+// pass 2: tendency on mass points i [2, ide-2], j [2, jde-2], k [1, kde-1]
+__global__ void k_hdiff_s_tend(const DevCtx *__restrict__ x, float *tend, const float *h1, const float *h2, R3 r) {
+  GET_IKJ(r);
+  const float *zx = FL(zx), *zy = FL(zy), *rdzw = FL(rdzw), *fnm = FL(fnm), *fnp = FL(fnp);
+  const int ktf = x->kde - 1;
+#define H1A(kk)                                                                                                                  \
+  (((kk) == 1 || (kk) == ktf + 1) ? 0.f                                                                                          \
+                                  : 0.5f * (A1(fnm, kk) * (A3(h1, i + 1, kk, j) + A3(h1, i, kk, j)) +                             \
+                                            A1(fnp, kk) * (A3(h1, i + 1, (kk) - 1, j) + A3(h1, i, (kk) - 1, j))))
+#define H2A(kk)                                                                                                                  \
+  (((kk) == 1 || (kk) == ktf + 1) ? 0.f                                                                                          \
+                                  : 0.5f * (A1(fnm, kk) * (A3(h2, i, kk, j + 1) + A3(h2, i, kk, j)) +                             \
+                                            A1(fnp, kk) * (A3(h2, i, (kk) - 1, j + 1) + A3(h2, i, (kk) - 1, j))))
+  const float zxm = 0.25f * (A3(zx, i, k, j) + A3(zx, i + 1, k, j) + A3(zx, i, k + 1, j) + A3(zx, i + 1, k + 1, j));
+  const float zym = 0.25f * (A3(zy, i, k, j) + A3(zy, i, k, j + 1) + A3(zy, i, k + 1, j) + A3(zy, i, k + 1, j + 1));
+  const float msftx = A2(FL(msftx), i, j), msfty = A2(FL(msfty), i, j);
+  const float mrdx = msftx * x->rdx, mrdy = msfty * x->rdy;
+  const float rz = A3(rdzw, i, k, j);
+  const float inc = x->g / (A1(FL(dnw), k) * rz) *
+                    (mrdx * (A3(h1, i + 1, k, j) - A3(h1, i, k, j)) + mrdy * (A3(h2, i, k, j + 1) - A3(h2, i, k, j)) -
+                     msftx * zxm * (H1A(k + 1) - H1A(k)) * rz - msfty * zym * (H2A(k + 1) - H2A(k)) * rz);
+#undef H1A
+#undef H2A
+  A3(tend, i, k, j) = A3(tend, i, k, j) + inc;
+}

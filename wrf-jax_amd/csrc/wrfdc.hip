@@ -1,0 +1,883 @@
+// MI355X (gfx950) RK3 split-explicit dycore: host orchestration and the C ABI
+// declared in include/wrfdc.h.
+//
+// One wrfdc_ctx owns one domain's device-resident DevCtx (dimensions,
+// scalars and the caller's field pointers) plus library scratch.  A model
+// step is a fixed sequence of kernel launches on the context's stream; the
+// stage order follows src/wrf_jax/core/direct_step.py:3037-3531 (RK and
+// acoustic loops) and src/wrf_jax/dyn_em/solve_em.py:260-800 (final
+// boundary stages).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "wrfdc_common.h"
+#include "k_advect.h"
+#include "k_bc.h"
+#include "k_bigstep.h"
+#include "k_diff.h"
+#include "k_small.h"
+
+// ---------------------------------------------------------------------------
+// small helper kernels local to the orchestration
+// ---------------------------------------------------------------------------
+
+// field times its column mass (mass_weight, module_bc_em.py:561-624), zero
+// outside the coupled range; full extent.  stag_w selects w levels and c1f/c2f.
+__global__ void k_mass_weight(const DevCtx *__restrict__ x, const float *fl, float *out, int stag_w, R3 r) {
+  GET_IKJ(r);
+  const bool in = (i <= x->ide - 1 && j <= x->jde - 1 && (stag_w || k <= x->kde - 1));
+  float v = 0.f;
+  if (in) {
+    const float *c1 = stag_w ? FL(c1f) : FL(c1h), *c2 = stag_w ? FL(c2f) : FL(c2h);
+    v = A3(fl, i, k, j) * (A1(c1, k) * A2(FL(mut), i, j) + A1(c2, k));
+  }
+  A3(out, i, k, j) = v;
+}
+
+// dst = src on mass points (initialize_moist_old)
+__global__ void k_copy_mass(const DevCtx *__restrict__ x, float *dst, const float *src, R3 r) {
+  GET_IKJ(r);
+  A3(dst, i, k, j) = A3(src, i, k, j);
+}
+
+// FP64 checksums over mass points: [0] sum(mu_2 + mub), [1] sum(t_2),
+// [2] sum of all moist species, [3] count of non-finite prognostic values.
+// Pass 1 writes one partial per block (vector stores, no atomics).
+#define CK_BLOCK 256
+__global__ void __launch_bounds__(CK_BLOCK) k_checksum_partial(const DevCtx *__restrict__ x, double *part) {
+  const int ni = x->ide - 1, nk = x->kde - 1, nj = x->jde - 1;
+  const size_t ncol = (size_t)ni * nj, npt = ncol * nk;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < npt; t += (size_t)gridDim.x * blockDim.x) {
+    const int i = 1 + (int)(t % ni);
+    const int k = 1 + (int)((t / ni) % nk);
+    const int j = 1 + (int)(t / ((size_t)ni * nk));
+    const size_t o = XI3(i, k, j);
+    const float tv = FL(t_2)[o], uv = FL(u_2)[o], vv = FL(v_2)[o], wv = FL(w_2)[o], pv = FL(ph_2)[o];
+    s1 += (double)tv;
+    float nf = 0.f;
+    nf += isfinite(tv) ? 0.f : 1.f;
+    nf += isfinite(uv) ? 0.f : 1.f;
+    nf += isfinite(vv) ? 0.f : 1.f;
+    nf += isfinite(wv) ? 0.f : 1.f;
+    nf += isfinite(pv) ? 0.f : 1.f;
+    for (int s = 0; s < x->nmoist; s++) {
+      const float q = SP(moist, s)[o];
+      s2 += (double)q;
+      nf += isfinite(q) ? 0.f : 1.f;
+    }
+    if (k == 1) {
+      const float m = A2(FL(mu_2), i, j);
+      s0 += (double)m + (double)A2(FL(mub), i, j);
+      nf += isfinite(m) ? 0.f : 1.f;
+    }
+    s3 += (double)nf;
+  }
+  __shared__ double red[4][CK_BLOCK];
+  red[0][threadIdx.x] = s0;
+  red[1][threadIdx.x] = s1;
+  red[2][threadIdx.x] = s2;
+  red[3][threadIdx.x] = s3;
+  __syncthreads();
+  for (int w = CK_BLOCK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int c = 0; c < 4; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) part[(size_t)blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+__global__ void __launch_bounds__(CK_BLOCK) k_checksum_final(const double *part, int nblocks, double *out) {
+  __shared__ double red[4][CK_BLOCK];
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nblocks; b += CK_BLOCK)
+    for (int c = 0; c < 4; c++) s[c] += part[(size_t)b * 4 + c];
+  for (int c = 0; c < 4; c++) red[c][threadIdx.x] = s[c];
+  __syncthreads();
+  for (int w = CK_BLOCK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int c = 0; c < 4; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+static const char *kNames[] = {
+#define WRFDC_NAME(name, kind) #name,
+    WRFDC_FIELDS(WRFDC_NAME)
+#undef WRFDC_NAME
+};
+static const char kKinds[] = {
+#define WRFDC_KIND(name, kind) kind,
+    WRFDC_FIELDS(WRFDC_KIND)
+#undef WRFDC_KIND
+};
+
+static const int kCkBlocks = 512;
+
+struct wrfdc_ctx {
+  wrfdc_config c;
+  int ide, jde, kde, ip, ns;
+  size_t n3, n2;
+  DevCtx h;
+  DevCtx *d = nullptr;
+  hipStream_t s = nullptr;
+  float *scratch = nullptr;
+  double *ck = nullptr;  // kCkBlocks * 4 partials + 4 results
+  bool dirty = true;
+  int rk = 1, nsmall = 1, iteration = 1;
+  double dt_rk = 0.0, dts_rk = 0.0, dtbc = 0.0;
+  std::string err;
+};
+
+namespace {
+
+struct HipError {
+  hipError_t e;
+  const char *what;
+};
+
+inline void hchk(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw HipError{e, what};
+}
+
+inline unsigned cdiv(int a, int b) { return (unsigned)((a + b - 1) / b); }
+
+struct Run {
+  wrfdc_ctx *X;
+  const DevCtx *d;
+  hipStream_t s;
+  int ide, jde, kde;
+  explicit Run(wrfdc_ctx *x) : X(x), d(x->d), s(x->s), ide(x->ide), jde(x->jde), kde(x->kde) {}
+
+  float *F(int id) const { return X->h.F[id]; }
+  float *Sp(int id, int s_) const { return X->h.F[id] + (size_t)s_ * X->n3; }
+  float *Scr(int id) const { return X->h.S[id]; }
+
+  R3 full3() const { return R3{1, ide, 1, kde, 1, jde}; }
+  R3 mass3() const { return R3{1, ide - 1, 1, kde - 1, 1, jde - 1}; }
+  R2 mass2() const { return R2{1, ide - 1, 1, jde - 1}; }
+  static dim3 g3(const R3 &r) { return dim3(cdiv(r.i1 - r.i0 + 1, 64), cdiv(r.k1 - r.k0 + 1, 4), (unsigned)(r.j1 - r.j0 + 1)); }
+  static dim3 g2(const R2 &r) { return dim3(cdiv(r.i1 - r.i0 + 1, 64), (unsigned)(r.j1 - r.j0 + 1)); }
+  static dim3 b3() { return dim3(64, 4, 1); }
+
+  void fill(float *p, size_t n, float v = 0.f) const {
+    unsigned nb = (unsigned)((n + 255) / 256);
+    if (nb > 8192) nb = 8192;
+    k_fill<<<nb, 256, 0, s>>>(p, v, n);
+  }
+  void zero3(int id) const { fill(F(id), X->n3); }
+  void zero2(int id) const { fill(F(id), X->n2); }
+
+  // ---- physical boundary copies -----------------------------------------
+  void bc3(float *fl, char var) const {
+    const bool full_k = (var == 'e' || var == 'f' || var == 'w');
+    const int kend = full_k ? kde : kde - 1;
+    if (!(var == 'u' || var == 'x')) k_bc_x<<<dim3(cdiv(jde, 64), kend), 64, 0, s>>>(d, fl, kend, 0);
+    if (!(var == 'v' || var == 'y')) k_bc_y<<<dim3(cdiv(ide, 64), kend), 64, 0, s>>>(d, fl, kend, 0);
+  }
+  void bc2(float *fl, char var) const {
+    if (!(var == 'u' || var == 'x')) k_bc_x<<<dim3(cdiv(jde, 64), 1), 64, 0, s>>>(d, fl, 1, 1);
+    if (!(var == 'v' || var == 'y')) k_bc_y<<<dim3(cdiv(ide, 64), 1), 64, 0, s>>>(d, fl, 1, 1);
+  }
+
+  // ---- specified / relaxation zones --------------------------------------
+  struct Z {
+    int ibe, jbe, kt;
+  };
+  Z zd(char var, bool ktop, bool is2d) const {
+    Z z{ide - 1, jde - 1, ktop ? kde : kde - 1};
+    if (var == 'u') z.ibe = ide;
+    if (var == 'v') z.jbe = jde;
+    if (is2d) z.kt = 1;
+    return z;
+  }
+  static int zone_n(const Z &z) { return 2 * z.ibe + 2 * (z.jbe - 2); }
+  BPtrs bp(int ys, int ye, int xs, int xe) const { return BPtrs{{F(ys), F(ye), F(xs), F(xe)}}; }
+
+  void spec_overlay(float *tend, char var, bool ktop, bool is2d, BPtrs bt) const {
+    Z z = zd(var, ktop, is2d);
+    k_spec_overlay<<<dim3(cdiv(zone_n(z), 64), z.kt), 64, 0, s>>>(d, tend, z.ibe, z.jbe, z.kt, is2d ? 1 : 0, bt);
+  }
+  void spec_update(float *fl, const float *tend, double dt, char var, bool is2d) const {
+    Z z = zd(var, var == 'm' || var == 'h', is2d);
+    k_spec_update<<<dim3(cdiv(zone_n(z), 64), z.kt), 64, 0, s>>>(d, fl, tend, (float)dt, z.ibe, z.jbe, z.kt, is2d ? 1 : 0);
+  }
+  void relax(const float *fld, float *tend, char var, bool ktop, bool is2d, BPtrs b, BPtrs bt) const {
+    Z z = zd(var, ktop, is2d);
+    const int rz = X->c.relax_zone;
+    const int along = z.ibe > z.jbe ? z.ibe : z.jbe;
+    k_relax<<<dim3(cdiv(along, 64), z.kt, 4 * (rz - 1)), 64, 0, s>>>(d, fld, tend, z.ibe, z.jbe, z.kt, is2d ? 1 : 0, b, bt, (float)X->dtbc);
+  }
+  void spec_final(float *fl, char var, bool is2d, const float *mu, const float *msf, const float *c1, const float *c2, BPtrs b,
+                  BPtrs bt) const {
+    Z z = zd(var, var == 'm' || var == 'h' || var == 'w', is2d);
+    const int msfc = (var == 'u' || var == 'v' || var == 'w');
+    const int muc = (var != 'm');
+    k_spec_final<<<dim3(cdiv(zone_n(z), 64), z.kt), 64, 0, s>>>(d, fl, z.ibe, z.jbe, z.kt, is2d ? 1 : 0, msfc, muc, mu, msf, c1, c2, b,
+                                                               bt, (float)X->dtbc);
+  }
+  void edge_copy(float *f0, float *f1, int axis, int dst, int src_, int kend) const {
+    const int n = axis == 0 ? jde : ide;
+    k_edge_copy<<<dim3(cdiv(n, 64), kend), 64, 0, s>>>(d, f0, f1, axis, dst, src_, kend);
+  }
+
+  // ---- RK-stage pieces -----------------------------------------------------
+  void rk_step_prep() const {
+    k_mu_uv<<<dim3(cdiv(ide, 64), jde), 64, 0, s>>>(d, F(F_mu_2), F(F_mub), F(F_muu), F(F_muv), F(F_mut));
+    R3 r = full3();
+    k_rk_prep3<<<g3(r), b3(), 0, s>>>(d, r);
+    R2 c = mass2();
+    k_ww_cp<<<g2(c), 64, 0, s>>>(d, c);
+  }
+  void first_rk_step_part1() const {
+    const int dz[] = {F_ru_tendf, F_rv_tendf, F_rw_tendf, F_ph_tendf, F_t_tendf};
+    for (int id : dz) zero3(id);
+    zero2(F_mu_tendf);
+    fill(F(F_moist_tend), X->n3 * (size_t)X->c.nmoist);
+    if (X->c.nscalar > 0) fill(F(F_scalar_tend), X->n3 * (size_t)X->c.nscalar);
+    R3 m = mass3();
+    k_rho<<<g3(m), b3(), 0, s>>>(d, m);
+  }
+  void compute_diff_metrics() const {
+    R3 r = full3();
+    k_diff_metrics<<<g3(r), b3(), 0, s>>>(d, r);
+  }
+  void cal_deform_and_div() const {
+    R3 m = mass3();
+    k_deform_diag<<<g3(m), b3(), 0, s>>>(d, m);
+    R3 r12{2, ide - 1, 1, kde - 1, 2, jde - 1};
+    k_deform12<<<g3(r12), b3(), 0, s>>>(d, r12);
+    edge_copy(F(F_defor12), nullptr, 0, 1, 2, kde);
+    edge_copy(F(F_defor12), nullptr, 1, 1, 2, kde);
+    edge_copy(F(F_defor12), nullptr, 0, ide, ide - 1, kde);
+    edge_copy(F(F_defor12), nullptr, 1, jde, jde - 1, kde);
+    R3 r13{2, ide - 1, 1, kde, 2, jde - 1};
+    k_deform13_23<<<g3(r13), b3(), 0, s>>>(d, r13);
+    edge_copy(F(F_defor13), F(F_defor23), 0, 1, 2, kde);
+    edge_copy(F(F_defor13), F(F_defor23), 1, 1, 2, kde);
+    edge_copy(F(F_defor13), F(F_defor23), 0, ide, ide - 1, kde);
+    edge_copy(F(F_defor13), F(F_defor23), 1, jde, jde - 1, kde);
+  }
+  void smag2d_km() const {
+    R3 r{2, ide - 2, 1, kde - 1, 2, jde - 2};
+    k_smag<<<g3(r), b3(), 0, s>>>(d, r);
+  }
+  void titau(float *tt, int xk, int defor, int kind, int i0, int i1, int j0, int j1) const {
+    R3 r = full3();
+    k_titau<<<g3(r), b3(), 0, s>>>(d, tt, F(xk), F(defor), kind, i0, i1, j0, j1, r);
+  }
+  void hdiff_s(float *tend, const float *var) const {
+    float *t1 = Scr(S_T1), *t2 = Scr(S_T2);
+    R3 rf{2, ide - 1, 1, kde - 1, 2, jde - 1};
+    k_hdiff_s_faces<<<g3(rf), b3(), 0, s>>>(d, var, t1, t2, rf);
+    R3 rt{2, ide - 2, 1, kde - 1, 2, jde - 2};
+    k_hdiff_s_tend<<<g3(rt), b3(), 0, s>>>(d, tend, t1, t2, rt);
+  }
+  void horizontal_diffusion_2() const {
+    float *t1 = Scr(S_T1), *t2 = Scr(S_T2);
+    titau(t1, F_xkmh, F_defor11, 11, 1, ide - 1, 2, jde - 1);
+    titau(t2, F_xkmh, F_defor12, 12, 2, ide - 1, 2, jde);
+    R3 ru{2, ide - 1, 1, kde - 1, 2, jde - 2};
+    k_hdiff_u<<<g3(ru), b3(), 0, s>>>(d, t1, t2, ru);
+    titau(t1, F_xkmh, F_defor12, 12, 2, ide, 2, jde - 1);
+    titau(t2, F_xkmh, F_defor22, 11, 2, ide - 1, 1, jde - 1);
+    R3 rv{2, ide - 2, 1, kde - 1, 2, jde - 1};
+    k_hdiff_v<<<g3(rv), b3(), 0, s>>>(d, t1, t2, rv);
+    titau(t1, F_xkmv, F_defor13, 13, 2, ide, 2, jde - 2);
+    titau(t2, F_xkmv, F_defor23, 23, 2, ide - 2, 2, jde);
+    R3 rw{2, ide - 2, 2, kde - 1, 2, jde - 2};
+    k_hdiff_w<<<g3(rw), b3(), 0, s>>>(d, t1, t2, rw);
+    hdiff_s(F(F_t_tendf), F(F_t_2));
+    for (int q = 0; q < X->c.nmoist; q++) hdiff_s(Sp(F_moist_tend, q), Sp(F_moist, q));
+    for (int q = 0; q < X->c.nscalar; q++) hdiff_s(Sp(F_scalar_tend, q), Sp(F_scalar, q));
+  }
+  void first_rk_step_part2() const {
+    compute_diff_metrics();
+    bc3(F(F_rdzw), 'w');
+    bc3(F(F_rdz), 'w');
+    bc3(F(F_z), 'w');
+    bc3(F(F_zx), 'e');
+    bc3(F(F_zy), 'f');
+    cal_deform_and_div();
+    smag2d_km();
+    bc3(F(F_xkmh), 't');
+    bc3(F(F_xkhh), 't');
+    bc3(F(F_xkmv), 't');
+    bc3(F(F_xkhv), 't');
+    bc3(F(F_div), 't');
+    bc3(F(F_defor11), 't');
+    bc3(F(F_defor22), 't');
+    bc3(F(F_defor33), 't');
+    bc3(F(F_defor12), 'd');
+    bc3(F(F_defor13), 'e');
+    bc3(F(F_defor23), 'f');
+    bc3(F(F_rho), 't');
+    horizontal_diffusion_2();
+  }
+  void advect_u() const {
+    R3 r{2, ide - 1, 1, kde - 1, 1, jde - 1};
+    k_advect_u<<<g3(r), b3(), 0, s>>>(d, F(F_u_2), F(F_ru_tend), F(F_ru), F(F_rv), F(F_ww), r);
+  }
+  void advect_v() const {
+    R3 r{1, ide - 1, 1, kde - 1, 2, jde - 1};
+    k_advect_v<<<g3(r), b3(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), r);
+  }
+  void advect_w() const {
+    R3 r{1, ide - 1, 2, kde, 1, jde - 1};
+    k_advect_w<<<g3(r), b3(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), r);
+  }
+  void advect_scalar(const float *q, float *tend, const float *ru, const float *rv, const float *rom) const {
+    R3 m = mass3();
+    k_advect_scalar<<<g3(m), b3(), 0, s>>>(d, q, tend, ru, rv, rom, m);
+  }
+  void advect_scalar_pd(const float *q, const float *q_old, float *tend, double dt) const {
+    R3 r = full3();
+    k_pd_fluxes<<<g3(r), b3(), 0, s>>>(d, q, q_old, F(F_ru_m), F(F_rv_m), F(F_ww_m), F(F_muts), (float)dt, r);
+    k_pd_scale<<<g3(r), b3(), 0, s>>>(d, q_old, F(F_mub), F(F_mu_1), (float)dt, r);
+    R3 m = mass3();
+    k_pd_tend<<<g3(m), b3(), 0, s>>>(d, tend, m);
+  }
+  void rhs_ph() const {
+    R3 r{1, ide - 1, 2, kde, 1, jde - 1};
+    k_rhs_ph<<<g3(r), b3(), 0, s>>>(d, r);
+  }
+  void horizontal_pressure_gradient() const {
+    R3 rv{1, ide - 1, 1, kde - 1, 2, jde - 1};
+    k_hpg<<<g3(rv), b3(), 0, s>>>(d, 1, rv);
+    R3 ru{2, ide - 1, 1, kde - 1, 1, jde - 1};
+    k_hpg<<<g3(ru), b3(), 0, s>>>(d, 0, ru);
+  }
+  void pg_buoy_w() const {
+    R2 c = mass2();
+    k_pg_buoy_w<<<g2(c), 64, 0, s>>>(d, c);
+  }
+  void coriolis_curvature() const {
+    R3 ru{2, ide - 1, 1, kde - 1, 1, jde - 1};
+    k_cor_curv<<<g3(ru), b3(), 0, s>>>(d, 0, ru);
+    R3 rv{1, ide - 1, 1, kde - 1, 2, jde - 1};
+    k_cor_curv<<<g3(rv), b3(), 0, s>>>(d, 1, rv);
+    R3 rw{1, ide - 1, 2, kde - 1, 1, jde - 1};
+    k_cor_curv<<<g3(rw), b3(), 0, s>>>(d, 2, rw);
+  }
+  void rk_tendency() const {
+    const int z3[] = {F_ru_tend, F_rv_tend, F_rw_tend, F_t_tend, F_ph_tend, F_u_save, F_v_save, F_w_save, F_ph_save, F_t_save};
+    for (int id : z3) zero3(id);
+    zero2(F_mu_tend);
+    zero2(F_mu_save);
+    advect_u();
+    advect_v();
+    advect_w();
+    advect_scalar(F(F_t_2), F(F_t_tend), F(F_ru), F(F_rv), F(F_ww));
+    rhs_ph();
+    horizontal_pressure_gradient();
+    pg_buoy_w();
+    coriolis_curvature();
+  }
+  void dry_boundary_tendency() const {
+    if (X->rk == 1) {
+      relax(F(F_ru), F(F_ru_tendf), 'u', false, false, bp(F_u_bys, F_u_bye, F_u_bxs, F_u_bxe), bp(F_u_btys, F_u_btye, F_u_btxs, F_u_btxe));
+      relax(F(F_rv), F(F_rv_tendf), 'v', false, false, bp(F_v_bys, F_v_bye, F_v_bxs, F_v_bxe), bp(F_v_btys, F_v_btye, F_v_btxs, F_v_btxe));
+      float *tmp = Scr(S_HAT);
+      R3 r = full3();
+      k_mass_weight<<<g3(r), b3(), 0, s>>>(d, F(F_ph_2), tmp, 1, r);
+      relax(tmp, F(F_ph_tendf), 'h', true, false, bp(F_ph_bys, F_ph_bye, F_ph_bxs, F_ph_bxe), bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe));
+      k_mass_weight<<<g3(r), b3(), 0, s>>>(d, F(F_t_2), tmp, 0, r);
+      relax(tmp, F(F_t_tendf), 't', false, false, bp(F_t_bys, F_t_bye, F_t_bxs, F_t_bxe), bp(F_t_btys, F_t_btye, F_t_btxs, F_t_btxe));
+      relax(F(F_mu_2), F(F_mu_tend), 'm', true, true, bp(F_mu_bys, F_mu_bye, F_mu_bxs, F_mu_bxe),
+            bp(F_mu_btys, F_mu_btye, F_mu_btxs, F_mu_btxe));
+    }
+    const int rk = X->rk;
+    R3 r0{1, ide, 1, kde - 1, 1, jde - 1};
+    k_addtend<<<g3(r0), b3(), 0, s>>>(d, 0, rk, r0);
+    R3 r1{1, ide - 1, 1, kde - 1, 1, jde};
+    k_addtend<<<g3(r1), b3(), 0, s>>>(d, 1, rk, r1);
+    R3 r2{1, ide - 1, 1, kde, 1, jde - 1};
+    k_addtend<<<g3(r2), b3(), 0, s>>>(d, 2, rk, r2);
+    R3 r3 = mass3();
+    k_addtend<<<g3(r3), b3(), 0, s>>>(d, 3, rk, r3);
+    spec_overlay(F(F_ru_tend), 'u', false, false, bp(F_u_btys, F_u_btye, F_u_btxs, F_u_btxe));
+    spec_overlay(F(F_rv_tend), 'v', false, false, bp(F_v_btys, F_v_btye, F_v_btxs, F_v_btxe));
+    spec_overlay(F(F_ph_tend), 'h', true, false, bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe));
+    spec_overlay(F(F_t_tend), 't', false, false, bp(F_t_btys, F_t_btye, F_t_btxs, F_t_btxe));
+    spec_overlay(F(F_mu_tend), 'm', true, true, bp(F_mu_btys, F_mu_btye, F_mu_btxs, F_mu_btxe));
+  }
+  void small_step_prep() const {
+    k_ssp_mass<<<dim3(cdiv(ide, 64), jde), 64, 0, s>>>(d, X->rk);
+    R3 r = full3();
+    k_ssp_3d<<<g3(r), b3(), 0, s>>>(d, X->rk, r);
+  }
+  void calc_p_rho(int step) const {
+    R3 m = mass3();
+    k_calc_p_rho<<<g3(m), b3(), 0, s>>>(d, step, m);
+  }
+  void calc_coef_w() const {
+    R2 c = mass2();
+    k_calc_coef_w<<<g2(c), 64, 0, s>>>(d, (float)X->dts_rk, c);
+  }
+  void advance_uv() const {
+    const int sz = X->c.spec_zone;
+    R3 ru{1 + sz, ide - sz, 1, kde - 1, 1 + sz, jde - sz - 1};
+    k_advance_uv<<<g3(ru), b3(), 0, s>>>(d, 0, (float)X->dts_rk, ru);
+    R3 rv{1 + sz, ide - sz - 1, 1, kde - 1, 1 + sz, jde - sz};
+    k_advance_uv<<<g3(rv), b3(), 0, s>>>(d, 1, (float)X->dts_rk, rv);
+  }
+  void advance_mu_t() const {
+    R2 c{2, ide - 2, 2, jde - 2};
+    k_advance_mu_t<<<g2(c), 64, 0, s>>>(d, (float)X->dts_rk, c);
+  }
+  void advance_w() const {
+    R2 c{2, ide - 2, 2, jde - 2};
+    k_advance_w<<<g2(c), 64, 0, s>>>(d, (float)X->dts_rk, Scr(S_HAT2), c);
+  }
+  void sumflux() const {
+    R3 r = full3();
+    k_sumflux<<<g3(r), b3(), 0, s>>>(d, X->iteration == 1, X->iteration == X->nsmall, (float)X->nsmall, r);
+  }
+  void spec_bdyupdate_ph() const {
+    Z z = zd('h', true, false);
+    k_spec_update_ph<<<dim3(cdiv(zone_n(z), 64), kde), 64, 0, s>>>(d, (float)X->dts_rk);
+  }
+  void zero_grad_bdy_w() const {
+    k_zero_grad_rows<<<dim3(cdiv(ide - 1, 64), kde, 2), 64, 0, s>>>(d, F(F_w_2));
+    k_zero_grad_cols<<<dim3(cdiv(jde - 1, 64), kde, 2), 64, 0, s>>>(d, F(F_w_2));
+  }
+  void small_step_finish() const {
+    k_mu_uv<<<dim3(cdiv(ide, 64), jde), 64, 0, s>>>(d, F(F_muts), nullptr, F(F_muus), F(F_muvs), nullptr);
+    R3 r = full3();
+    k_ssf<<<g3(r), b3(), 0, s>>>(d, X->rk == 3, (float)X->dts_rk, (float)X->nsmall, r);
+  }
+  void calc_p_rho_phi() const {
+    R3 m = mass3();
+    k_calc_p_rho_phi<<<g3(m), b3(), 0, s>>>(d, m);
+  }
+  void update_scalar_pd(float *q_old, float *q_tend) const {
+    R3 m = mass3();
+    k_update_scalar_pd<<<g3(m), b3(), 0, s>>>(d, q_old, q_tend, (float)X->dt_rk, m);
+  }
+  void species_advance(float *q, float *q_old, float *q_tend, bool is_qv, bool is_moist, bool pd) const {
+    float *adv = F(F_advect_tend);
+    zero3(F_advect_tend);
+    if (X->rk == 3 && pd) advect_scalar_pd(q, q_old, adv, X->dt_rk);
+    else advect_scalar(q, adv, F(F_ru_m), F(F_rv_m), F(F_ww_m));
+    R3 m = mass3();
+    if (is_qv && X->rk == 1) {
+      float *tmp = Scr(S_HAT);
+      R3 r = full3();
+      k_mass_weight<<<g3(r), b3(), 0, s>>>(d, q, tmp, 0, r);
+      relax(tmp, q_tend, 'q', false, false, bp(F_qv_bys, F_qv_bye, F_qv_bxs, F_qv_bxe), bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe));
+      spec_overlay(q_tend, 't', false, false, bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe));
+    }
+    k_update_scalar<<<g3(m), b3(), 0, s>>>(d, q, q_old, q_tend, X->rk, (float)X->dt_rk, m);
+    if (is_moist && !is_qv) {
+      k_flow_dep_rows<<<dim3(cdiv(ide - 1, 64), kde - 1, 2), 64, 0, s>>>(d, q, F(F_rv_m));
+      k_flow_dep_cols<<<dim3(cdiv(jde - 1, 64), kde - 1, 2), 64, 0, s>>>(d, q, F(F_ru_m));
+    }
+  }
+
+  void rk_stage() const {
+    rk_step_prep();
+    bc3(F(F_ru), 'u');
+    bc3(F(F_rv), 'v');
+    bc3(F(F_rw), 'w');
+    bc3(F(F_ww), 'w');
+    bc3(F(F_php), 'w');
+    bc3(F(F_alt), 't');
+    bc3(F(F_p), 'p');
+    bc2(F(F_muu), 'u');
+    bc2(F(F_muv), 'v');
+    bc2(F(F_mut), 't');
+    bc3(F(F_al), 'p');
+    bc3(F(F_ph_2), 'w');
+    if (X->rk == 1) {
+      first_rk_step_part1();
+      first_rk_step_part2();
+    }
+    rk_tendency();
+    dry_boundary_tendency();
+    small_step_prep();
+    calc_p_rho(0);
+    calc_coef_w();
+    bc3(F(F_ru_tend), 'u');
+    bc3(F(F_rv_tend), 'v');
+    bc3(F(F_ph_2), 'w');
+    bc3(F(F_al), 'p');
+    bc3(F(F_p), 'p');
+    bc3(F(F_t_1), 'p');
+    bc3(F(F_t_save), 't');
+    bc2(F(F_mu_1), 't');
+    bc2(F(F_mu_2), 't');
+    bc2(F(F_mudf), 't');
+    for (int it = 1; it <= X->nsmall; it++) {
+      X->iteration = it;
+      advance_uv();
+      spec_update(F(F_u_2), F(F_ru_tend), X->dts_rk, 'u', false);
+      spec_update(F(F_v_2), F(F_rv_tend), X->dts_rk, 'v', false);
+      advance_mu_t();
+      spec_update(F(F_t_2), F(F_t_tend), X->dts_rk, 't', false);
+      spec_update(F(F_mu_2), F(F_mu_tend), X->dts_rk, 'm', true);
+      spec_update(F(F_muts), F(F_mu_tend), X->dts_rk, 'm', true);
+      advance_w();
+      sumflux();
+      spec_bdyupdate_ph();
+      zero_grad_bdy_w();
+      calc_p_rho(it);
+      bc3(F(F_ph_2), 'w');
+      bc3(F(F_al), 'p');
+      bc3(F(F_p), 'p');
+      bc2(F(F_muts), 't');
+      bc2(F(F_mu_2), 't');
+      bc2(F(F_mudf), 't');
+    }
+    small_step_finish();
+    if (X->rk == 3) {
+      bc3(F(F_ru_m), 'u');
+      bc3(F(F_rv_m), 'v');
+      bc3(F(F_ww_m), 'w');
+      bc2(F(F_mut), 't');
+      bc2(F(F_muts), 't');
+    }
+    const int nm = X->c.nmoist, nsc = X->c.nscalar;
+    if (X->rk == 3 && X->c.pd_moist)
+      for (int q = 0; q < nm; q++) {
+        update_scalar_pd(Sp(F_moist_old, q), Sp(F_moist_tend, q));
+        bc3(Sp(F_moist_old, q), 'p');
+      }
+    if (X->rk == 3 && X->c.pd_scalar)
+      for (int q = 0; q < nsc; q++) {
+        update_scalar_pd(Sp(F_scalar_old, q), Sp(F_scalar_tend, q));
+        bc3(Sp(F_scalar_old, q), 'p');
+      }
+    for (int q = 0; q < nm; q++) species_advance(Sp(F_moist, q), Sp(F_moist_old, q), Sp(F_moist_tend, q), q == 0, true, X->c.pd_moist != 0);
+    for (int q = 0; q < nsc; q++) species_advance(Sp(F_scalar, q), Sp(F_scalar_old, q), Sp(F_scalar_tend, q), false, false, X->c.pd_scalar != 0);
+    calc_p_rho_phi();
+    if (X->rk < 3) {
+      bc3(F(F_u_2), 'u');
+      bc3(F(F_v_2), 'v');
+      bc3(F(F_w_2), 'w');
+      bc3(F(F_t_2), 'p');
+      bc3(F(F_ph_2), 'w');
+      bc2(F(F_mu_2), 't');
+      for (int q = 0; q < nm; q++) bc3(Sp(F_moist, q), 'p');
+      for (int q = 0; q < nsc; q++) bc3(Sp(F_scalar, q), 'p');
+    }
+  }
+
+  void step() const {
+    R3 m = mass3();
+    for (int q = 0; q < X->c.nmoist; q++) {
+      k_copy_mass<<<g3(m), b3(), 0, s>>>(d, Sp(F_moist_old, q), Sp(F_moist, q), m);
+      bc3(Sp(F_moist_old, q), 'p');
+    }
+    for (int rk = 1; rk <= 3; rk++) {
+      set_rk(rk);
+      rk_stage();
+    }
+    bc3(F(F_u_1), 'u');
+    bc3(F(F_u_2), 'u');
+    bc3(F(F_v_1), 'v');
+    bc3(F(F_v_2), 'v');
+    bc3(F(F_w_1), 'w');
+    bc3(F(F_w_2), 'w');
+    bc3(F(F_t_1), 'p');
+    bc3(F(F_t_2), 'p');
+    bc3(F(F_ph_1), 'w');
+    bc3(F(F_ph_2), 'w');
+    bc2(F(F_mu_1), 't');
+    bc2(F(F_mu_2), 't');
+    bc3(Sp(F_moist, 0), 'p');
+    spec_final(F(F_u_2), 'u', false, F(F_muus), F(F_msfuy), F(F_c1h), F(F_c2h), bp(F_u_bys, F_u_bye, F_u_bxs, F_u_bxe),
+               bp(F_u_btys, F_u_btye, F_u_btxs, F_u_btxe));
+    spec_final(F(F_v_2), 'v', false, F(F_muvs), F(F_msfvx), F(F_c1h), F(F_c2h), bp(F_v_bys, F_v_bye, F_v_bxs, F_v_bxe),
+               bp(F_v_btys, F_v_btye, F_v_btxs, F_v_btxe));
+    spec_final(F(F_t_2), 't', false, F(F_muts), F(F_msfty), F(F_c1h), F(F_c2h), bp(F_t_bys, F_t_bye, F_t_bxs, F_t_bxe),
+               bp(F_t_btys, F_t_btye, F_t_btxs, F_t_btxe));
+    spec_final(F(F_ph_2), 'h', false, F(F_muts), F(F_msfty), F(F_c1f), F(F_c2f), bp(F_ph_bys, F_ph_bye, F_ph_bxs, F_ph_bxe),
+               bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe));
+    spec_final(F(F_mu_2), 'm', true, F(F_muts), F(F_msfty), F(F_c1h), F(F_c2h), bp(F_mu_bys, F_mu_bye, F_mu_bxs, F_mu_bxe),
+               bp(F_mu_btys, F_mu_btye, F_mu_btxs, F_mu_btxe));
+    spec_final(Sp(F_moist, 0), 't', false, F(F_muts), F(F_msfty), F(F_c1h), F(F_c2h), bp(F_qv_bys, F_qv_bye, F_qv_bxs, F_qv_bxe),
+               bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe));
+    k_set_w_surface<<<dim3(cdiv(ide - 1, 64), jde - 1), 64, 0, s>>>(d);
+  }
+
+  // _rk_step_sizes (solve_em.py:243-257), rk_order = 3
+  void set_rk(int rk) const {
+    const double dt = X->c.dt;
+    const int ns = X->ns;
+    X->rk = rk;
+    if (rk == 1) {
+      X->dt_rk = dt / 3.0;
+      X->dts_rk = X->dt_rk;
+      X->nsmall = 1;
+    } else if (rk == 2) {
+      X->dt_rk = 0.5 * dt;
+      X->dts_rk = dt / (double)ns;
+      X->nsmall = ns / 2 > 1 ? ns / 2 : 1;
+    } else {
+      X->dt_rk = dt;
+      X->dts_rk = dt / (double)ns;
+      X->nsmall = ns;
+    }
+  }
+};
+
+// WRF's acoustic step count (solve_em.py:220-240)
+int num_sound_steps(const wrfdc_config &c) {
+  if (c.time_step_sound != 0) return c.time_step_sound;
+  const double sp = c.dx < c.dy ? c.dx : c.dy;
+  const int n = 2 * ((int)(300.0 * c.dt / sp - 0.01) + 1);
+  return n > 4 ? n : 4;
+}
+
+int set_err(wrfdc_ctx *x, int code, const std::string &msg) {
+  if (x) x->err = msg;
+  return code;
+}
+
+int validate(const wrfdc_config &c, std::string &msg) {
+  if (c.nx < 12 || c.ny < 12 || c.nz < 6) { msg = "domain too small: need nx, ny >= 12 and nz >= 6"; return WRFDC_EINVAL; }
+  if (c.ip < c.nx + 1) { msg = "ip must be >= nx + 1"; return WRFDC_EINVAL; }
+  if (c.nmoist < 1) { msg = "nmoist must be >= 1 (species 0 is QV)"; return WRFDC_EINVAL; }
+  if (c.nscalar < 0) { msg = "nscalar must be >= 0"; return WRFDC_EINVAL; }
+  if (!(c.dx > 0.0 && c.dy > 0.0 && c.dt > 0.0)) { msg = "dx, dy and dt must be positive"; return WRFDC_EINVAL; }
+  if (c.spec_zone != 1) { msg = "only spec_zone = 1 is implemented"; return WRFDC_ENOTIMPL; }
+  if (c.relax_zone < 2) { msg = "relax_zone must be >= 2"; return WRFDC_EINVAL; }
+  if (c.spec_bdy_width != c.spec_zone + c.relax_zone || c.spec_bdy_width > 14) {
+    msg = "spec_bdy_width must equal spec_zone + relax_zone (at most 14)";
+    return WRFDC_EINVAL;
+  }
+  if (c.top_lid != 0) { msg = "top_lid is not implemented"; return WRFDC_ENOTIMPL; }
+  if (c.time_step_sound < 0) { msg = "time_step_sound must be >= 0"; return WRFDC_EINVAL; }
+  return WRFDC_OK;
+}
+
+int ensure_ready(wrfdc_ctx *x) {
+  for (int f = 0; f < NFIELDS; f++) {
+    if (x->h.F[f]) continue;
+    if (kKinds[f] == 'S' && x->c.nscalar == 0) continue;
+    return set_err(x, WRFDC_EUNBOUND, std::string("field not bound: ") + kNames[f]);
+  }
+  if (x->dirty) {
+    hchk(hipMemcpy(x->d, &x->h, sizeof(DevCtx), hipMemcpyHostToDevice), "upload context");
+    x->dirty = false;
+  }
+  return WRFDC_OK;
+}
+
+template <class Fn>
+int guarded(wrfdc_ctx *x, Fn fn) {
+  if (!x) return WRFDC_EINVAL;
+  try {
+    int rc = ensure_ready(x);
+    if (rc != WRFDC_OK) return rc;
+    rc = fn();
+    if (rc != WRFDC_OK) return rc;
+    hchk(hipGetLastError(), "kernel launch");
+    return WRFDC_OK;
+  } catch (const HipError &e) {
+    return set_err(x, WRFDC_EHIP, std::string(e.what) + ": " + hipGetErrorString(e.e));
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int wrfdc_field_count(void) { return NFIELDS; }
+const char *wrfdc_field_name(int index) { return (index >= 0 && index < NFIELDS) ? kNames[index] : nullptr; }
+int wrfdc_field_kind(int index) { return (index >= 0 && index < NFIELDS) ? (int)kKinds[index] : 0; }
+
+int wrfdc_create(const wrfdc_config *cfg, wrfdc_ctx **out) {
+  if (!cfg || !out) return WRFDC_EINVAL;
+  *out = nullptr;
+  std::string msg;
+  int rc = validate(*cfg, msg);
+  if (rc != WRFDC_OK) {
+    fprintf(stderr, "wrfdc_create: %s\n", msg.c_str());
+    return rc;
+  }
+  wrfdc_ctx *x = new wrfdc_ctx();
+  x->c = *cfg;
+  x->ide = cfg->nx + 1;
+  x->jde = cfg->ny + 1;
+  x->kde = cfg->nz + 1;
+  x->ip = cfg->ip;
+  x->n3 = (size_t)x->jde * x->kde * x->ip;
+  x->n2 = (size_t)x->jde * x->ip;
+  x->ns = num_sound_steps(*cfg);
+  x->dtbc = cfg->dtbc;
+  DevCtx &h = x->h;
+  memset(&h, 0, sizeof(h));
+  h.ide = x->ide;
+  h.jde = x->jde;
+  h.kde = x->kde;
+  h.ip = x->ip;
+  h.nmoist = cfg->nmoist;
+  h.nscalar = cfg->nscalar;
+  h.spec_zone = cfg->spec_zone;
+  h.relax_zone = cfg->relax_zone;
+  h.spec_bdy_width = cfg->spec_bdy_width;
+  h.top_lid = cfg->top_lid;
+  h.n3 = x->n3;
+  h.n2 = x->n2;
+  h.rdx = (float)(1.0 / cfg->dx);
+  h.rdy = (float)(1.0 / cfg->dy);
+  h.dx = (float)cfg->dx;
+  h.dy = (float)cfg->dy;
+  h.dt = (float)cfg->dt;
+  h.g = (float)cfg->g;
+  h.t0 = (float)cfg->t0;
+  h.p0 = (float)cfg->p0;
+  h.ptop = (float)cfg->ptop;
+  h.r_d = (float)cfg->r_d;
+  h.cpovcv = (float)cfg->cpovcv;
+  h.rvovrd = (float)cfg->rvovrd;
+  h.epssm = (float)cfg->epssm;
+  h.smdiv = (float)cfg->smdiv;
+  h.emdiv = (float)cfg->emdiv;
+  h.c_s = (float)cfg->c_s;
+  h.prandtl = (float)cfg->prandtl;
+  h.cf1 = (float)cfg->cf1;
+  h.cf2 = (float)cfg->cf2;
+  h.cf3 = (float)cfg->cf3;
+  h.cfn = (float)cfg->cfn;
+  h.cfn1 = (float)cfg->cfn1;
+  h.reradius = (float)cfg->reradius;
+  // lbc_fcx_gcx (boundary/lbc_fcx_gcx.py:10-56), spec_exp = 0, in FP32
+  for (int b = cfg->spec_zone + 1; b <= cfg->spec_zone + cfg->relax_zone && b <= cfg->spec_bdy_width; b++) {
+    const float dist = (float)(cfg->spec_zone + cfg->relax_zone - b);
+    const float den = (float)(cfg->relax_zone - 1);
+    h.fcx[b] = ((0.1f / (float)cfg->dt) * dist) / den;
+    h.gcx[b] = (((1.0f / (float)cfg->dt) / 50.0f) * dist) / den;
+  }
+  try {
+    hchk(hipMalloc((void **)&x->d, sizeof(DevCtx)), "hipMalloc context");
+    hchk(hipMalloc((void **)&x->scratch, sizeof(float) * x->n3 * NSCRATCH), "hipMalloc scratch");
+    hchk(hipMalloc((void **)&x->ck, sizeof(double) * (kCkBlocks * 4 + 4)), "hipMalloc checksum");
+  } catch (const HipError &e) {
+    fprintf(stderr, "wrfdc_create: %s: %s\n", e.what, hipGetErrorString(e.e));
+    if (x->d) (void)hipFree(x->d);
+    if (x->scratch) (void)hipFree(x->scratch);
+    delete x;
+    return WRFDC_EHIP;
+  }
+  for (int i = 0; i < NSCRATCH; i++) h.S[i] = x->scratch + (size_t)i * x->n3;
+  x->dirty = true;
+  *out = x;
+  return WRFDC_OK;
+}
+
+void wrfdc_destroy(wrfdc_ctx *x) {
+  if (!x) return;
+  if (x->s) (void)hipStreamSynchronize(x->s);
+  else (void)hipDeviceSynchronize();
+  if (x->d) (void)hipFree(x->d);
+  if (x->scratch) (void)hipFree(x->scratch);
+  if (x->ck) (void)hipFree(x->ck);
+  delete x;
+}
+
+int wrfdc_bind(wrfdc_ctx *x, const char *name, void *device_ptr) {
+  if (!x || !name) return WRFDC_EINVAL;
+  for (int f = 0; f < NFIELDS; f++)
+    if (strcmp(kNames[f], name) == 0) {
+      x->h.F[f] = (float *)device_ptr;
+      x->dirty = true;
+      return WRFDC_OK;
+    }
+  return set_err(x, WRFDC_EINVAL, std::string("unknown field: ") + name);
+}
+
+int wrfdc_set_stream(wrfdc_ctx *x, void *hip_stream) {
+  if (!x) return WRFDC_EINVAL;
+  x->s = (hipStream_t)hip_stream;
+  return WRFDC_OK;
+}
+
+int wrfdc_step(wrfdc_ctx *x, int nsteps, double dtbc0) {
+  if (nsteps < 0) return set_err(x, WRFDC_EINVAL, "nsteps must be >= 0");
+  return guarded(x, [&]() {
+    Run run(x);
+    for (int n = 0; n < nsteps; n++) {
+      x->dtbc = dtbc0 + (double)(n + 1) * x->c.dt;
+      run.step();
+    }
+    return WRFDC_OK;
+  });
+}
+
+int wrfdc_stage(wrfdc_ctx *x, const char *name, int rk_step, int iteration) {
+  if (!x || !name) return WRFDC_EINVAL;
+  if (rk_step < 1 || rk_step > 3) return set_err(x, WRFDC_EINVAL, "rk_step must be 1, 2 or 3");
+  return guarded(x, [&]() {
+    Run run(x);
+    run.set_rk(rk_step);
+    if (iteration < 0 || iteration > x->nsmall) return set_err(x, WRFDC_EINVAL, "iteration outside [0, nsmall]");
+    x->iteration = iteration;
+    std::string n(name);
+    if (n == "rk_step_prep") run.rk_step_prep();
+    else if (n == "first_rk_step_part1") run.first_rk_step_part1();
+    else if (n == "first_rk_step_part2") run.first_rk_step_part2();
+    else if (n == "compute_diff_metrics") run.compute_diff_metrics();
+    else if (n == "cal_deform_and_div") run.cal_deform_and_div();
+    else if (n == "smag2d_km") run.smag2d_km();
+    else if (n == "horizontal_diffusion_2") run.horizontal_diffusion_2();
+    else if (n == "rk_tendency") run.rk_tendency();
+    else if (n == "advect_u") run.advect_u();
+    else if (n == "advect_v") run.advect_v();
+    else if (n == "advect_w") run.advect_w();
+    else if (n == "advect_scalar") run.advect_scalar(run.F(F_t_2), run.F(F_t_tend), run.F(F_ru), run.F(F_rv), run.F(F_ww));
+    else if (n == "advect_scalar_pd") run.advect_scalar_pd(run.Sp(F_moist, 0), run.Sp(F_moist_old, 0), run.F(F_advect_tend), x->dt_rk);
+    else if (n == "rhs_ph") run.rhs_ph();
+    else if (n == "horizontal_pressure_gradient") run.horizontal_pressure_gradient();
+    else if (n == "pg_buoy_w") run.pg_buoy_w();
+    else if (n == "dry_boundary_tendency") run.dry_boundary_tendency();
+    else if (n == "small_step_prep") run.small_step_prep();
+    else if (n == "calc_p_rho") run.calc_p_rho(iteration);
+    else if (n == "calc_coef_w") run.calc_coef_w();
+    else if (n == "advance_uv") run.advance_uv();
+    else if (n == "advance_mu_t") run.advance_mu_t();
+    else if (n == "advance_w") run.advance_w();
+    else if (n == "sumflux") run.sumflux();
+    else if (n == "small_step_finish") run.small_step_finish();
+    else if (n == "calc_p_rho_phi") run.calc_p_rho_phi();
+    else if (n == "rk_stage") run.rk_stage();
+    else if (n == "step") run.step();
+    else return set_err(x, WRFDC_EINVAL, "unknown stage: " + n);
+    return WRFDC_OK;
+  });
+}
+
+int wrfdc_checksums(wrfdc_ctx *x, double out[4]) {
+  if (!out) return set_err(x, WRFDC_EINVAL, "out is null");
+  int rc = guarded(x, [&]() {
+    k_checksum_partial<<<kCkBlocks, CK_BLOCK, 0, x->s>>>(x->d, x->ck);
+    k_checksum_final<<<1, CK_BLOCK, 0, x->s>>>(x->ck, kCkBlocks, x->ck + kCkBlocks * 4);
+    hchk(hipMemcpyAsync(out, x->ck + kCkBlocks * 4, sizeof(double) * 4, hipMemcpyDeviceToHost, x->s), "copy checksums");
+    hchk(hipStreamSynchronize(x->s), "synchronize checksums");
+    return WRFDC_OK;
+  });
+  if (rc != WRFDC_OK) return rc;
+  if (out[3] > 0.0) return set_err(x, WRFDC_ENONFINITE, "non-finite values in the prognostic state");
+  return WRFDC_OK;
+}
+
+int wrfdc_sync(wrfdc_ctx *x) {
+  if (!x) return WRFDC_EINVAL;
+  hipError_t e = x->s ? hipStreamSynchronize(x->s) : hipDeviceSynchronize();
+  if (e != hipSuccess) return set_err(x, WRFDC_EHIP, std::string("synchronize: ") + hipGetErrorString(e));
+  return WRFDC_OK;
+}
+
+const char *wrfdc_last_error(wrfdc_ctx *x) { return x ? x->err.c_str() : "null context"; }
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+// Shared definitions of the MI355X dycore library (device + host side).
+//
+// Layout in HBM: every 3-D field is [j][k][i] float32 with i contiguous and a
+// row pitch `ip` (>= ide, padded to a multiple of 16 floats = 64 B); 2-D
+// fields are [j][i] with the same pitch; 1-D column arrays are [k].  Indices
+// in kernels are Fortran-style 1-based (i, k, j) exactly as in the reference
+// dycore (src/wrf_jax/dyn_em), with ide = nx+1, jde = ny+1, kde = nz+1.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/wrfdc.h"
+
+#define WRFDC_FIELDS(X)                                                       \
+  X(u_1, '3') X(u_2, '3') X(v_1, '3') X(v_2, '3') X(w_1, '3') X(w_2, '3')     \
+  X(t_1, '3') X(t_2, '3') X(ph_1, '3') X(ph_2, '3') X(p, '3') X(al, '3')      \
+  X(pb, '3') X(phb, '3') X(alb, '3') X(ru, '3') X(rv, '3') X(rw, '3')         \
+  X(ww, '3') X(php, '3') X(alt, '3') X(cqu, '3') X(cqv, '3') X(cqw, '3')      \
+  X(c2a, '3') X(pm1, '3') X(a, '3') X(alpha, '3') X(gamma, '3')               \
+  X(ru_tend, '3') X(rv_tend, '3') X(rw_tend, '3') X(t_tend, '3')              \
+  X(ph_tend, '3') X(ru_tendf, '3') X(rv_tendf, '3') X(rw_tendf, '3')          \
+  X(t_tendf, '3') X(ph_tendf, '3') X(u_save, '3') X(v_save, '3')              \
+  X(w_save, '3') X(t_save, '3') X(ph_save, '3') X(ww_1, '3') X(t_2ave, '3')   \
+  X(ru_m, '3') X(rv_m, '3') X(ww_m, '3') X(h_diabatic, '3')                  \
+  X(advect_tend, '3') X(rho, '3') X(z, '3') X(rdz, '3') X(rdzw, '3')          \
+  X(zx, '3') X(zy, '3') X(div, '3') X(defor11, '3') X(defor22, '3')           \
+  X(defor33, '3') X(defor12, '3') X(defor13, '3') X(defor23, '3')             \
+  X(xkmh, '3') X(xkmv, '3') X(xkhh, '3') X(xkhv, '3')                         \
+  X(mu_1, '2') X(mu_2, '2') X(mub, '2') X(mut, '2') X(muu, '2') X(muv, '2')   \
+  X(muts, '2') X(muus, '2') X(muvs, '2') X(muave, '2') X(mudf, '2')           \
+  X(mu_tend, '2') X(mu_tendf, '2') X(mu_save, '2') X(ht, '2') X(msfux, '2')   \
+  X(msfuy, '2') X(msfvx, '2') X(msfvx_inv, '2') X(msfvy, '2') X(msftx, '2')   \
+  X(msfty, '2') X(f, '2') X(e, '2') X(sina, '2') X(cosa, '2') X(clat, '2')    \
+  X(znw, '1') X(znu, '1') X(dnw, '1') X(rdnw, '1') X(dn, '1') X(rdn, '1')     \
+  X(fnm, '1') X(fnp, '1') X(c1h, '1') X(c2h, '1') X(c1f, '1') X(c2f, '1')     \
+  X(c3h, '1') X(c4h, '1') X(c3f, '1') X(c4f, '1') X(u_base, '1')              \
+  X(v_base, '1') X(moist, 'M') X(moist_old, 'M') X(moist_tend, 'M')           \
+  X(scalar, 'S') X(scalar_old, 'S') X(scalar_tend, 'S')                       \
+  X(u_bxs, 'X') X(u_bxe, 'X') X(u_bys, 'Y') X(u_bye, 'Y')                     \
+  X(u_btxs, 'X') X(u_btxe, 'X') X(u_btys, 'Y') X(u_btye, 'Y')                 \
+  X(v_bxs, 'X') X(v_bxe, 'X') X(v_bys, 'Y') X(v_bye, 'Y')                     \
+  X(v_btxs, 'X') X(v_btxe, 'X') X(v_btys, 'Y') X(v_btye, 'Y')                 \
+  X(t_bxs, 'X') X(t_bxe, 'X') X(t_bys, 'Y') X(t_bye, 'Y')                     \
+  X(t_btxs, 'X') X(t_btxe, 'X') X(t_btys, 'Y') X(t_btye, 'Y')                 \
+  X(ph_bxs, 'X') X(ph_bxe, 'X') X(ph_bys, 'Y') X(ph_bye, 'Y')                 \
+  X(ph_btxs, 'X') X(ph_btxe, 'X') X(ph_btys, 'Y') X(ph_btye, 'Y')             \
+  X(qv_bxs, 'X') X(qv_bxe, 'X') X(qv_bys, 'Y') X(qv_bye, 'Y')                 \
+  X(qv_btxs, 'X') X(qv_btxe, 'X') X(qv_btys, 'Y') X(qv_btye, 'Y')             \
+  X(mu_bxs, 'x') X(mu_bxe, 'x') X(mu_bys, 'y') X(mu_bye, 'y')                 \
+  X(mu_btxs, 'x') X(mu_btxe, 'x') X(mu_btys, 'y') X(mu_btye, 'y')
+
+enum {
+#define WRFDC_ENUM(name, kind) F_##name,
+  WRFDC_FIELDS(WRFDC_ENUM)
+#undef WRFDC_ENUM
+  NFIELDS
+};
+
+// Library-owned scratch arrays (3-D, same layout as fields).
+enum { S_T1, S_T2, S_FQX, S_FQY, S_FQZ, S_FQXL, S_FQYL, S_FQZL, S_SCALE, S_HAT, S_HAT2, S_VXGM, NSCRATCH };
+
+// Device-resident view of one domain: dimensions, scalars and pointers.
+struct DevCtx {
+  int ide, jde, kde, ip;
+  int nmoist, nscalar, spec_zone, relax_zone, spec_bdy_width, top_lid;
+  size_t n3, n2;
+  float rdx, rdy, dx, dy, dt, g, t0, p0, ptop, r_d, cpovcv, rvovrd, epssm, smdiv, emdiv, c_s, prandtl;
+  float cf1, cf2, cf3, cfn, cfn1, reradius;
+  float fcx[16], gcx[16];
+  float *F[NFIELDS];
+  float *S[NSCRATCH];
+};
+
+#define XI3(i, k, j) ((((size_t)(j) - 1) * (size_t)x->kde + ((size_t)(k) - 1)) * (size_t)x->ip + ((size_t)(i) - 1))
+#define XI2(i, j) (((size_t)(j) - 1) * (size_t)x->ip + ((size_t)(i) - 1))
+#define A3(fl, i, k, j) ((fl)[XI3(i, k, j)])
+#define A2(fl, i, j) ((fl)[XI2(i, j)])
+#define A1(fl, k) ((fl)[(k) - 1])
+#define BX(fl, j, k, b) ((fl)[(((size_t)(b) - 1) * (size_t)x->kde + ((size_t)(k) - 1)) * (size_t)x->jde + ((size_t)(j) - 1)])
+#define BY(fl, i, k, b) ((fl)[(((size_t)(b) - 1) * (size_t)x->kde + ((size_t)(k) - 1)) * (size_t)x->ip + ((size_t)(i) - 1)])
+#define BX2(fl, j, b) ((fl)[((size_t)(b) - 1) * (size_t)x->jde + ((size_t)(j) - 1)])
+#define BY2(fl, i, b) ((fl)[((size_t)(b) - 1) * (size_t)x->ip + ((size_t)(i) - 1)])
+#define FL(name) (x->F[F_##name])
+#define SP(name, s) (x->F[F_##name] + (size_t)(s) * x->n3)
+#define SCR(id) (x->S[id])
+
+// Inclusive Fortran index ranges of a launch.
+struct R3 {
+  int i0, i1, k0, k1, j0, j1;
+};
+struct R2 {
+  int i0, i1, j0, j1;
+};
+
+// thread -> (i, k, j) with blockDim (64, 4, 1), grid (ceil(ni/64), ceil(nk/4), nj)
+#define GET_IKJ(r)                                                  \
+  const int i = (r).i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x); \
+  const int k = (r).k0 + (int)(blockIdx.y * blockDim.y + threadIdx.y); \
+  const int j = (r).j0 + (int)blockIdx.z;                               \
+  if (i > (r).i1 || k > (r).k1 || j > (r).j1) return
+// thread -> (i, j) for column kernels: blockDim (64, 1, 1), grid (ceil(ni/64), nj)
+#define GET_IJ(r)                                                   \
+  const int i = (r).i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x); \
+  const int j = (r).j0 + (int)blockIdx.y;                               \
+  if (i > (r).i1 || j > (r).j1) return
+
+__device__ __forceinline__ float sgn1(float v) { return v >= 0.f ? 1.f : -1.f; }
+__device__ __forceinline__ int imin_(int a, int b) { return a < b ? a : b; }
+__device__ __forceinline__ int imax_(int a, int b) { return a > b ? a : b; }

@@ -1,0 +1,95 @@
+"""Algorithmic HBM byte models of the dycore (version 1).
+
+Step model: compulsory traffic of the minimal fused schedule, 4 B per mass
+point per 3-D field pass (2-D/1-D arrays and the +1 staggering ignored):
+
+    B_alg(step) = 4 * N_mass * (274 + 52 * (1 + 1.5 * ns) + 12 * N_s)
+
+with ns acoustic substeps per dt and N_s advected species; at ns = 4 this
+is 4 * N_mass * (638 + 12 * N_s).  Per RK stage: the tendency pass
+(rk_step_prep .. rk_tendency, diffusion at rk1) 29/27/27 field passes, the
+acoustic prologue (small_step_prep + calc_p_rho + calc_coef_w) 38/33/33, per
+acoustic substep 14 (advance_uv) + 38 (advance_mu_t + advance_w + sumflux +
+calc_p_rho), the epilogue (small_step_finish) 18, species 3 shared + 4 per
+species, the equation of state 8.
+
+Kernel model: the distinct 3-D fields a kernel must read and write once
+each (a read-modify-write field counts twice), 4 B * N_mass per field.
+"""
+
+from __future__ import annotations
+
+import os
+import re
+from dataclasses import dataclass
+
+MODEL_VERSION = 1
+
+
+def step_passes(n_species: int, ns: int = 4) -> float:
+    return 274 + 52 * (1 + 1.5 * ns) + 12 * n_species
+
+
+def step_bytes(n_mass: int, n_species: int, ns: int = 4) -> float:
+    return 4.0 * n_mass * step_passes(n_species, ns)
+
+
+@dataclass(frozen=True)
+class KernelModel:
+    name: str          # HIP kernel symbol
+    stage: str         # wrfdc_stage entry that launches exactly this kernel
+    rk_step: int
+    iteration: int
+    reads: tuple
+    writes: tuple
+
+    def passes(self) -> int:
+        return len(self.reads) + len(self.writes)
+
+    def bytes(self, cfg) -> float:
+        return 4.0 * cfg.nx * cfg.ny * cfg.nz * self.passes()
+
+
+KERNELS = {
+    "k_advance_w": KernelModel(
+        "k_advance_w", "advance_w", 3, 1,
+        reads=("w_2", "rw_tend", "ww", "t_2", "t_2ave", "t_save", "ph_save", "phb", "ph_tend", "ph_2", "c2a", "cqw",
+               "alt", "a", "alpha", "gamma"),
+        writes=("t_2ave", "w_2", "ph_2")),
+    "k_advance_mu_t": KernelModel(
+        "k_advance_mu_t", "advance_mu_t", 3, 1,
+        reads=("u_2", "u_save", "v_2", "v_save", "ww", "ww_1", "t_2", "t_save", "t_tend"),
+        writes=("ww", "t_2", "t_2ave")),
+    "k_advect_u": KernelModel("k_advect_u", "advect_u", 2, 0, reads=("u_2", "ru", "rv", "ww", "ru_tend"), writes=("ru_tend",)),
+    "k_advect_scalar": KernelModel("k_advect_scalar", "advect_scalar", 2, 0, reads=("t_2", "ru", "rv", "ww", "t_tend"),
+                                   writes=("t_tend",)),
+    "k_calc_p_rho": KernelModel("k_calc_p_rho", "calc_p_rho", 3, 1, reads=("alt", "t_save", "ph_2", "t_2", "c2a", "pm1"),
+                                writes=("al", "p", "pm1")),
+    "k_rhs_ph": KernelModel("k_rhs_ph", "rhs_ph", 2, 0, reads=("u_2", "v_2", "ww", "ph_2", "phb", "w_2", "ph_tend"),
+                            writes=("ph_tend",)),
+}
+
+DOMINANT = "k_advance_w"
+
+
+def dominant_kernel(cfg) -> KernelModel:
+    return KERNELS[DOMINANT]
+
+
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (already scaled by the copy-kernel calibration), or None."""
+    repo = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    path = os.path.join(repo, "profiles", f"traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    pat = re.compile(rf"^\|\s*{re.escape(kernel)}\s*\|\s*([0-9.eE+]+)\s*\|\s*([0-9.eE+]+)\s*\|\s*([0-9.eE+]+)\s*\|")
+    with open(path) as fh:
+        for line in fh:
+            m = pat.match(line.strip())
+            if m:
+                return float(m.group(3))
+    return None
+
+
+__all__ = ["KERNELS", "KernelModel", "MODEL_VERSION", "dominant_kernel", "measured_traffic", "step_bytes", "step_passes"]
