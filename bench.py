@@ -36,6 +36,9 @@ def parse():
     ap.add_argument("--workload", default="c1", choices=["c1", "c0"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only launch the dominant kernel --kernel-reps times after one step, then a 256 MiB "
+                         "calibration copy (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs)")
     return ap.parse_args()
 
 
@@ -79,6 +82,21 @@ def main():
     dom.upload(case.arrays)
     n_mass = cfg.nx * cfg.ny * cfg.nz
     n_species = cfg.nmoist + cfg.nscalar
+
+    if args.traffic_probe:
+        dom.step(1, 0.0)
+        kern = roofline.dominant_kernel(cfg)
+        for _ in range(args.kernel_reps):
+            dom.stage(kern.stage, kern.rk_step, kern.iteration)
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] traffic probe: {args.kernel_reps} x {kern.name}, then a copy of {src.numel() * 4} bytes",
+              file=sys.stderr, flush=True)
+        dom.close()
+        return
 
     for w in range(args.warmup):
         dom.step(1, w * cfg.dt)

@@ -286,6 +286,25 @@ static void o_rk_update_scalar_pd(OCtx *x, real *q_old, real *q_tend) {
       }
 }
 
+/* One acoustic substep (direct_step.py:3224-3390). */
+void o_acoustic_substep(OCtx *x, int it) {
+  x->iteration = it;
+  o_advance_uv(x);
+  o_spec_bdyupdate(x, FLD(u_2), FLD(ru_tend), x->dts_rk, 'u', 0);
+  o_spec_bdyupdate(x, FLD(v_2), FLD(rv_tend), x->dts_rk, 'v', 0);
+  o_advance_mu_t(x);
+  o_spec_bdyupdate(x, FLD(t_2), FLD(t_tend), x->dts_rk, 't', 0);
+  o_spec_bdyupdate(x, FLD(mu_2), FLD(mu_tend), x->dts_rk, 'm', 1);
+  o_spec_bdyupdate(x, FLD(muts), FLD(mu_tend), x->dts_rk, 'm', 1);
+  o_advance_w(x);
+  o_sumflux(x);
+  o_spec_bdyupdate_ph(x, x->dts_rk);
+  o_zero_grad_bdy_w(x, FLD(w_2));
+  o_calc_p_rho(x, it);
+  o_bc3(x, FLD(ph_2), 'w'); o_bc3(x, FLD(al), 'p'); o_bc3(x, FLD(p), 'p');
+  o_bc2(x, FLD(muts), 't'); o_bc2(x, FLD(mu_2), 't'); o_bc2(x, FLD(mudf), 't');
+}
+
 /* One RK stage (direct_step.py:3120-3422). */
 void o_rk_stage(OCtx *x, int rk_step) {
   o_set_rk_stage(x, rk_step);
@@ -308,23 +327,7 @@ void o_rk_stage(OCtx *x, int rk_step) {
   o_bc3(x, FLD(ru_tend), 'u'); o_bc3(x, FLD(rv_tend), 'v'); o_bc3(x, FLD(ph_2), 'w'); o_bc3(x, FLD(al), 'p');
   o_bc3(x, FLD(p), 'p'); o_bc3(x, FLD(t_1), 'p'); o_bc3(x, FLD(t_save), 't');
   o_bc2(x, FLD(mu_1), 't'); o_bc2(x, FLD(mu_2), 't'); o_bc2(x, FLD(mudf), 't');
-  for (int it = 1; it <= x->nsmall; it++) {
-    x->iteration = it;
-    o_advance_uv(x);
-    o_spec_bdyupdate(x, FLD(u_2), FLD(ru_tend), x->dts_rk, 'u', 0);
-    o_spec_bdyupdate(x, FLD(v_2), FLD(rv_tend), x->dts_rk, 'v', 0);
-    o_advance_mu_t(x);
-    o_spec_bdyupdate(x, FLD(t_2), FLD(t_tend), x->dts_rk, 't', 0);
-    o_spec_bdyupdate(x, FLD(mu_2), FLD(mu_tend), x->dts_rk, 'm', 1);
-    o_spec_bdyupdate(x, FLD(muts), FLD(mu_tend), x->dts_rk, 'm', 1);
-    o_advance_w(x);
-    o_sumflux(x);
-    o_spec_bdyupdate_ph(x, x->dts_rk);
-    o_zero_grad_bdy_w(x, FLD(w_2));
-    o_calc_p_rho(x, it);
-    o_bc3(x, FLD(ph_2), 'w'); o_bc3(x, FLD(al), 'p'); o_bc3(x, FLD(p), 'p');
-    o_bc2(x, FLD(muts), 't'); o_bc2(x, FLD(mu_2), 't'); o_bc2(x, FLD(mudf), 't');
-  }
+  for (int it = 1; it <= x->nsmall; it++) o_acoustic_substep(x, it);
   o_calc_mu_uv_1(x);
   o_small_step_finish(x, rk_step == 3);
   if (rk_step == 3) {
@@ -414,6 +417,7 @@ int o_run_stage(OCtx *x, const char *name, int rk_step, int iteration) {
   else if (IS("advance_mu_t")) o_advance_mu_t(x);
   else if (IS("advance_w")) o_advance_w(x);
   else if (IS("sumflux")) o_sumflux(x);
+  else if (IS("acoustic_substep")) o_acoustic_substep(x, iteration);
   else if (IS("small_step_finish")) { o_calc_mu_uv_1(x); o_small_step_finish(x, rk_step == 3); }
   else if (IS("calc_p_rho_phi")) o_calc_p_rho_phi(x);
   else if (IS("rk_stage")) o_rk_stage(x, rk_step);

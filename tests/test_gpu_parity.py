@@ -61,7 +61,8 @@ STAGES = [
     ("small_step_prep", 2, 0), ("calc_p_rho", 2, 0), ("calc_p_rho", 2, 1), ("calc_coef_w", 2, 0),
     ("advance_uv", 2, 1), ("advance_mu_t", 2, 1), ("advance_w", 2, 1), ("sumflux", 2, 1),
     ("sumflux", 2, 2), ("sumflux", 1, 1), ("small_step_finish", 2, 0), ("small_step_finish", 3, 0),
-    ("calc_p_rho_phi", 2, 0), ("rk_stage", 1, 0), ("rk_stage", 2, 0), ("rk_stage", 3, 0),
+    ("calc_p_rho_phi", 2, 0), ("acoustic_substep", 1, 1), ("acoustic_substep", 2, 1), ("acoustic_substep", 2, 2),
+    ("acoustic_substep", 3, 1), ("acoustic_substep", 3, 3), ("rk_stage", 1, 0), ("rk_stage", 2, 0), ("rk_stage", 3, 0),
 ]
 
 

@@ -23,6 +23,46 @@ __global__ void k_bc_y(const DevCtx *__restrict__ x, float *fl, int kend, int is
   else A3(fl, i, k, x->jde) = A3(fl, i, k, x->jde - 1);
 }
 
+// Batched physical BC copies: up to kBcMax fields per launch, blockIdx.z =
+// field.  Column ide and row jde are written in one pass from their interior
+// neighbours with the sequential result of "plane, then row": the corner
+// (ide, jde) takes (ide-1, jde-1) when both copies apply.
+constexpr int kBcMax = 24;
+struct BcList {
+  float *f[kBcMax];
+  int kend[kBcMax];
+  int flags[kBcMax];  // bit 0: copy column ide, bit 1: copy row jde, bit 2: 2-D
+};
+__global__ void k_bc_batch(const DevCtx *__restrict__ x, BcList L) {
+  const int a = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int k = 1 + (int)blockIdx.y;
+  const int q = (int)blockIdx.z;
+  const int fl = L.flags[q];
+  if (k > L.kend[q]) return;
+  float *f = L.f[q];
+  const int ide = x->ide, jde = x->jde;
+  const bool cx = fl & 1, cy = fl & 2, is2d = fl & 4;
+#define AT(ii, jj) (is2d ? A2(f, ii, jj) : A3(f, ii, k, jj))
+  if (cx && a <= jde) {
+    if (a == jde && cy) AT(ide, jde) = AT(ide - 1, jde - 1);
+    else AT(ide, a) = AT(ide - 1, a);
+  }
+  if (cy && a <= ide && !(a == ide && cx)) AT(a, jde) = AT(a, jde - 1);
+#undef AT
+}
+
+// Batched fills: blockIdx.y = field, grid-stride over its elements.
+constexpr int kFillMax = 16;
+struct FillList {
+  float *p[kFillMax];
+  size_t n[kFillMax];
+};
+__global__ void k_fill_batch(FillList L) {
+  float *p = L.p[blockIdx.y];
+  const size_t n = L.n[blockIdx.y];
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = 0.f;
+}
+
 // ---- specified zone (spec_zone = 1): ring of the variable's grid ----------
 // zone point t -> (i, j, edge) with edge 0=south 1=north 2=west 3=east.
 // south/north rows span i in [1, ibe]; west/east columns j in [2, jbe-1].

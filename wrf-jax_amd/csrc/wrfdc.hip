@@ -9,7 +9,10 @@
 // boundary stages).
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
+#include <initializer_list>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -19,6 +22,7 @@
 #include "k_bigstep.h"
 #include "k_diff.h"
 #include "k_small.h"
+#include "k_acoustic.h"
 
 // ---------------------------------------------------------------------------
 // small helper kernels local to the orchestration
@@ -174,6 +178,54 @@ struct Run {
   void zero3(int id) const { fill(F(id), X->n3); }
   void zero2(int id) const { fill(F(id), X->n2); }
 
+  // ---- batched physical boundary copies and fills ------------------------
+  struct BcBatch {
+    const Run *r;
+    BcList L;
+    int n = 0, kmax = 1;
+    explicit BcBatch(const Run *run) : r(run) { memset(&L, 0, sizeof(L)); }
+    BcBatch &add3(float *fl, char var) {
+      const bool full_k = (var == 'e' || var == 'f' || var == 'w');
+      return add(fl, full_k ? r->kde : r->kde - 1, var, false);
+    }
+    BcBatch &add2(float *fl, char var) { return add(fl, 1, var, true); }
+    BcBatch &add(float *fl, int kend, char var, bool is2d) {
+      if (n == kBcMax) flush();
+      L.f[n] = fl;
+      L.kend[n] = kend;
+      L.flags[n] = ((var == 'u' || var == 'x') ? 0 : 1) | ((var == 'v' || var == 'y') ? 0 : 2) | (is2d ? 4 : 0);
+      kmax = kend > kmax ? kend : kmax;
+      n++;
+      return *this;
+    }
+    void flush() {
+      if (n == 0) return;
+      const int along = r->ide > r->jde ? r->ide : r->jde;
+      k_bc_batch<<<dim3(cdiv(along, 64), kmax, n), 64, 0, r->s>>>(r->d, L);
+      n = 0;
+      kmax = 1;
+    }
+    ~BcBatch() { flush(); }
+  };
+  void fill_many(std::initializer_list<std::pair<float *, size_t>> items) const {
+    FillList L;
+    memset(&L, 0, sizeof(L));
+    int n = 0;
+    size_t nmax = 0;
+    for (auto &it : items) {
+      if (it.second == 0) continue;
+      L.p[n] = it.first;
+      L.n[n] = it.second;
+      nmax = it.second > nmax ? it.second : nmax;
+      if (++n == kFillMax) {
+        k_fill_batch<<<dim3((unsigned)std::min<size_t>((nmax + 255) / 256, 2048), n), 256, 0, s>>>(L);
+        n = 0;
+        nmax = 0;
+      }
+    }
+    if (n) k_fill_batch<<<dim3((unsigned)std::min<size_t>((nmax + 255) / 256, 2048), n), 256, 0, s>>>(L);
+  }
+
   // ---- physical boundary copies -----------------------------------------
   void bc3(float *fl, char var) const {
     const bool full_k = (var == 'e' || var == 'f' || var == 'w');
@@ -236,11 +288,10 @@ struct Run {
     k_ww_cp<<<g2(c), 64, 0, s>>>(d, c);
   }
   void first_rk_step_part1() const {
-    const int dz[] = {F_ru_tendf, F_rv_tendf, F_rw_tendf, F_ph_tendf, F_t_tendf};
-    for (int id : dz) zero3(id);
-    zero2(F_mu_tendf);
-    fill(F(F_moist_tend), X->n3 * (size_t)X->c.nmoist);
-    if (X->c.nscalar > 0) fill(F(F_scalar_tend), X->n3 * (size_t)X->c.nscalar);
+    const size_t n3 = X->n3;
+    fill_many({{F(F_ru_tendf), n3}, {F(F_rv_tendf), n3}, {F(F_rw_tendf), n3}, {F(F_ph_tendf), n3}, {F(F_t_tendf), n3},
+               {F(F_mu_tendf), X->n2}, {F(F_moist_tend), n3 * (size_t)X->c.nmoist},
+               {F(F_scalar_tend), n3 * (size_t)X->c.nscalar}});
     R3 m = mass3();
     k_rho<<<g3(m), b3(), 0, s>>>(d, m);
   }
@@ -299,25 +350,13 @@ struct Run {
   }
   void first_rk_step_part2() const {
     compute_diff_metrics();
-    bc3(F(F_rdzw), 'w');
-    bc3(F(F_rdz), 'w');
-    bc3(F(F_z), 'w');
-    bc3(F(F_zx), 'e');
-    bc3(F(F_zy), 'f');
+    BcBatch(this).add3(F(F_rdzw), 'w').add3(F(F_rdz), 'w').add3(F(F_z), 'w').add3(F(F_zx), 'e').add3(F(F_zy), 'f').flush();
     cal_deform_and_div();
     smag2d_km();
-    bc3(F(F_xkmh), 't');
-    bc3(F(F_xkhh), 't');
-    bc3(F(F_xkmv), 't');
-    bc3(F(F_xkhv), 't');
-    bc3(F(F_div), 't');
-    bc3(F(F_defor11), 't');
-    bc3(F(F_defor22), 't');
-    bc3(F(F_defor33), 't');
-    bc3(F(F_defor12), 'd');
-    bc3(F(F_defor13), 'e');
-    bc3(F(F_defor23), 'f');
-    bc3(F(F_rho), 't');
+    BcBatch(this)
+        .add3(F(F_xkmh), 't').add3(F(F_xkhh), 't').add3(F(F_xkmv), 't').add3(F(F_xkhv), 't').add3(F(F_div), 't')
+        .add3(F(F_defor11), 't').add3(F(F_defor22), 't').add3(F(F_defor33), 't').add3(F(F_defor12), 'd')
+        .add3(F(F_defor13), 'e').add3(F(F_defor23), 'f').add3(F(F_rho), 't').flush();
     horizontal_diffusion_2();
   }
   void advect_u() const {
@@ -366,10 +405,10 @@ struct Run {
     k_cor_curv<<<g3(rw), b3(), 0, s>>>(d, 2, rw);
   }
   void rk_tendency() const {
-    const int z3[] = {F_ru_tend, F_rv_tend, F_rw_tend, F_t_tend, F_ph_tend, F_u_save, F_v_save, F_w_save, F_ph_save, F_t_save};
-    for (int id : z3) zero3(id);
-    zero2(F_mu_tend);
-    zero2(F_mu_save);
+    const size_t n3 = X->n3, n2 = X->n2;
+    fill_many({{F(F_ru_tend), n3}, {F(F_rv_tend), n3}, {F(F_rw_tend), n3}, {F(F_t_tend), n3}, {F(F_ph_tend), n3},
+               {F(F_u_save), n3}, {F(F_v_save), n3}, {F(F_w_save), n3}, {F(F_ph_save), n3}, {F(F_t_save), n3},
+               {F(F_mu_tend), n2}, {F(F_mu_save), n2}});
     advect_u();
     advect_v();
     advect_w();
@@ -447,6 +486,17 @@ struct Run {
     k_zero_grad_rows<<<dim3(cdiv(ide - 1, 64), kde, 2), 64, 0, s>>>(d, F(F_w_2));
     k_zero_grad_cols<<<dim3(cdiv(jde - 1, 64), kde, 2), 64, 0, s>>>(d, F(F_w_2));
   }
+  // one fused acoustic substep: u/v, the column kernel, zero_grad_bdy of w
+  void acoustic_substep(int it) const {
+    X->iteration = it;
+    const float dts = (float)X->dts_rk;
+    R3 r{1, ide, 1, kde - 1, 1, jde};
+    k_uv_substep<<<g3(r), b3(), 0, s>>>(d, dts, r);
+    k_acoustic_col<<<dim3(cdiv(ide, 64), jde), dim3(64, 4), acoustic_col_lds(kde), s>>>(d, dts, it == 1, it == X->nsmall,
+                                                                                     (float)X->nsmall);
+    Z z = zd('w', true, false);
+    k_zero_grad_w<<<dim3(cdiv(zone_n(z), 64), kde), 64, 0, s>>>(d);
+  }
   void small_step_finish() const {
     k_mu_uv<<<dim3(cdiv(ide, 64), jde), 64, 0, s>>>(d, F(F_muts), nullptr, F(F_muus), F(F_muvs), nullptr);
     R3 r = full3();
@@ -482,18 +532,10 @@ struct Run {
 
   void rk_stage() const {
     rk_step_prep();
-    bc3(F(F_ru), 'u');
-    bc3(F(F_rv), 'v');
-    bc3(F(F_rw), 'w');
-    bc3(F(F_ww), 'w');
-    bc3(F(F_php), 'w');
-    bc3(F(F_alt), 't');
-    bc3(F(F_p), 'p');
-    bc2(F(F_muu), 'u');
-    bc2(F(F_muv), 'v');
-    bc2(F(F_mut), 't');
-    bc3(F(F_al), 'p');
-    bc3(F(F_ph_2), 'w');
+    BcBatch(this)
+        .add3(F(F_ru), 'u').add3(F(F_rv), 'v').add3(F(F_rw), 'w').add3(F(F_ww), 'w').add3(F(F_php), 'w')
+        .add3(F(F_alt), 't').add3(F(F_p), 'p').add2(F(F_muu), 'u').add2(F(F_muv), 'v').add2(F(F_mut), 't')
+        .add3(F(F_al), 'p').add3(F(F_ph_2), 'w').flush();
     if (X->rk == 1) {
       first_rk_step_part1();
       first_rk_step_part2();
@@ -503,94 +545,56 @@ struct Run {
     small_step_prep();
     calc_p_rho(0);
     calc_coef_w();
-    bc3(F(F_ru_tend), 'u');
-    bc3(F(F_rv_tend), 'v');
-    bc3(F(F_ph_2), 'w');
-    bc3(F(F_al), 'p');
-    bc3(F(F_p), 'p');
-    bc3(F(F_t_1), 'p');
-    bc3(F(F_t_save), 't');
-    bc2(F(F_mu_1), 't');
-    bc2(F(F_mu_2), 't');
-    bc2(F(F_mudf), 't');
-    for (int it = 1; it <= X->nsmall; it++) {
-      X->iteration = it;
-      advance_uv();
-      spec_update(F(F_u_2), F(F_ru_tend), X->dts_rk, 'u', false);
-      spec_update(F(F_v_2), F(F_rv_tend), X->dts_rk, 'v', false);
-      advance_mu_t();
-      spec_update(F(F_t_2), F(F_t_tend), X->dts_rk, 't', false);
-      spec_update(F(F_mu_2), F(F_mu_tend), X->dts_rk, 'm', true);
-      spec_update(F(F_muts), F(F_mu_tend), X->dts_rk, 'm', true);
-      advance_w();
-      sumflux();
-      spec_bdyupdate_ph();
-      zero_grad_bdy_w();
-      calc_p_rho(it);
-      bc3(F(F_ph_2), 'w');
-      bc3(F(F_al), 'p');
-      bc3(F(F_p), 'p');
-      bc2(F(F_muts), 't');
-      bc2(F(F_mu_2), 't');
-      bc2(F(F_mudf), 't');
-    }
+    BcBatch(this)
+        .add3(F(F_ru_tend), 'u').add3(F(F_rv_tend), 'v').add3(F(F_ph_2), 'w').add3(F(F_al), 'p').add3(F(F_p), 'p')
+        .add3(F(F_t_1), 'p').add3(F(F_t_save), 't').add2(F(F_mu_1), 't').add2(F(F_mu_2), 't').add2(F(F_mudf), 't')
+        .flush();
+    for (int it = 1; it <= X->nsmall; it++) acoustic_substep(it);
     small_step_finish();
-    if (X->rk == 3) {
-      bc3(F(F_ru_m), 'u');
-      bc3(F(F_rv_m), 'v');
-      bc3(F(F_ww_m), 'w');
-      bc2(F(F_mut), 't');
-      bc2(F(F_muts), 't');
-    }
+    if (X->rk == 3)
+      BcBatch(this).add3(F(F_ru_m), 'u').add3(F(F_rv_m), 'v').add3(F(F_ww_m), 'w').add2(F(F_mut), 't').add2(F(F_muts), 't').flush();
     const int nm = X->c.nmoist, nsc = X->c.nscalar;
-    if (X->rk == 3 && X->c.pd_moist)
-      for (int q = 0; q < nm; q++) {
-        update_scalar_pd(Sp(F_moist_old, q), Sp(F_moist_tend, q));
-        bc3(Sp(F_moist_old, q), 'p');
-      }
-    if (X->rk == 3 && X->c.pd_scalar)
-      for (int q = 0; q < nsc; q++) {
-        update_scalar_pd(Sp(F_scalar_old, q), Sp(F_scalar_tend, q));
-        bc3(Sp(F_scalar_old, q), 'p');
-      }
+    {
+      BcBatch bc(this);
+      if (X->rk == 3 && X->c.pd_moist)
+        for (int q = 0; q < nm; q++) {
+          update_scalar_pd(Sp(F_moist_old, q), Sp(F_moist_tend, q));
+          bc.add3(Sp(F_moist_old, q), 'p');
+        }
+      if (X->rk == 3 && X->c.pd_scalar)
+        for (int q = 0; q < nsc; q++) {
+          update_scalar_pd(Sp(F_scalar_old, q), Sp(F_scalar_tend, q));
+          bc.add3(Sp(F_scalar_old, q), 'p');
+        }
+    }
     for (int q = 0; q < nm; q++) species_advance(Sp(F_moist, q), Sp(F_moist_old, q), Sp(F_moist_tend, q), q == 0, true, X->c.pd_moist != 0);
     for (int q = 0; q < nsc; q++) species_advance(Sp(F_scalar, q), Sp(F_scalar_old, q), Sp(F_scalar_tend, q), false, false, X->c.pd_scalar != 0);
     calc_p_rho_phi();
     if (X->rk < 3) {
-      bc3(F(F_u_2), 'u');
-      bc3(F(F_v_2), 'v');
-      bc3(F(F_w_2), 'w');
-      bc3(F(F_t_2), 'p');
-      bc3(F(F_ph_2), 'w');
-      bc2(F(F_mu_2), 't');
-      for (int q = 0; q < nm; q++) bc3(Sp(F_moist, q), 'p');
-      for (int q = 0; q < nsc; q++) bc3(Sp(F_scalar, q), 'p');
+      BcBatch bc(this);
+      bc.add3(F(F_u_2), 'u').add3(F(F_v_2), 'v').add3(F(F_w_2), 'w').add3(F(F_t_2), 'p').add3(F(F_ph_2), 'w').add2(F(F_mu_2), 't');
+      for (int q = 0; q < nm; q++) bc.add3(Sp(F_moist, q), 'p');
+      for (int q = 0; q < nsc; q++) bc.add3(Sp(F_scalar, q), 'p');
     }
   }
 
   void step() const {
     R3 m = mass3();
-    for (int q = 0; q < X->c.nmoist; q++) {
-      k_copy_mass<<<g3(m), b3(), 0, s>>>(d, Sp(F_moist_old, q), Sp(F_moist, q), m);
-      bc3(Sp(F_moist_old, q), 'p');
+    {
+      BcBatch bc(this);
+      for (int q = 0; q < X->c.nmoist; q++) {
+        k_copy_mass<<<g3(m), b3(), 0, s>>>(d, Sp(F_moist_old, q), Sp(F_moist, q), m);
+        bc.add3(Sp(F_moist_old, q), 'p');
+      }
     }
     for (int rk = 1; rk <= 3; rk++) {
       set_rk(rk);
       rk_stage();
     }
-    bc3(F(F_u_1), 'u');
-    bc3(F(F_u_2), 'u');
-    bc3(F(F_v_1), 'v');
-    bc3(F(F_v_2), 'v');
-    bc3(F(F_w_1), 'w');
-    bc3(F(F_w_2), 'w');
-    bc3(F(F_t_1), 'p');
-    bc3(F(F_t_2), 'p');
-    bc3(F(F_ph_1), 'w');
-    bc3(F(F_ph_2), 'w');
-    bc2(F(F_mu_1), 't');
-    bc2(F(F_mu_2), 't');
-    bc3(Sp(F_moist, 0), 'p');
+    BcBatch(this)
+        .add3(F(F_u_1), 'u').add3(F(F_u_2), 'u').add3(F(F_v_1), 'v').add3(F(F_v_2), 'v').add3(F(F_w_1), 'w')
+        .add3(F(F_w_2), 'w').add3(F(F_t_1), 'p').add3(F(F_t_2), 'p').add3(F(F_ph_1), 'w').add3(F(F_ph_2), 'w')
+        .add2(F(F_mu_1), 't').add2(F(F_mu_2), 't').add3(Sp(F_moist, 0), 'p').flush();
     spec_final(F(F_u_2), 'u', false, F(F_muus), F(F_msfuy), F(F_c1h), F(F_c2h), bp(F_u_bys, F_u_bye, F_u_bxs, F_u_bxe),
                bp(F_u_btys, F_u_btye, F_u_btxs, F_u_btxe));
     spec_final(F(F_v_2), 'v', false, F(F_muvs), F(F_msfvx), F(F_c1h), F(F_c2h), bp(F_v_bys, F_v_bye, F_v_bxs, F_v_bxe),
@@ -653,6 +657,7 @@ int validate(const wrfdc_config &c, std::string &msg) {
     return WRFDC_EINVAL;
   }
   if (c.top_lid != 0) { msg = "top_lid is not implemented"; return WRFDC_ENOTIMPL; }
+  if (c.nz > 150) { msg = "at most 150 vertical levels (LDS-resident columns)"; return WRFDC_EINVAL; }
   if (c.time_step_sound < 0) { msg = "time_step_sound must be >= 0"; return WRFDC_EINVAL; }
   return WRFDC_OK;
 }
@@ -771,6 +776,15 @@ int wrfdc_create(const wrfdc_config *cfg, wrfdc_ctx **out) {
     return WRFDC_EHIP;
   }
   for (int i = 0; i < NSCRATCH; i++) h.S[i] = x->scratch + (size_t)i * x->n3;
+  if (acoustic_col_lds(x->kde) > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_acoustic_col, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)acoustic_col_lds(x->kde));
+    if (e != hipSuccess) {
+      fprintf(stderr, "wrfdc_create: LDS size for nz=%d: %s\n", cfg->nz, hipGetErrorString(e));
+      wrfdc_destroy(x);
+      return WRFDC_EHIP;
+    }
+  }
   x->dirty = true;
   *out = x;
   return WRFDC_OK;
@@ -848,6 +862,22 @@ int wrfdc_stage(wrfdc_ctx *x, const char *name, int rk_step, int iteration) {
     else if (n == "advance_mu_t") run.advance_mu_t();
     else if (n == "advance_w") run.advance_w();
     else if (n == "sumflux") run.sumflux();
+    else if (n == "acoustic_uv" || n == "acoustic_column") {
+      // single kernels of the fused substep (for timing and profiling)
+      const float dts = (float)x->dts_rk;
+      x->iteration = iteration < 1 ? 1 : iteration;
+      if (n == "acoustic_uv") {
+        R3 r{1, run.ide, 1, run.kde - 1, 1, run.jde};
+        k_uv_substep<<<Run::g3(r), Run::b3(), 0, x->s>>>(x->d, dts, r);
+      } else {
+        k_acoustic_col<<<dim3(cdiv(run.ide, 64), run.jde), dim3(64, 4), acoustic_col_lds(run.kde), x->s>>>(
+            x->d, dts, x->iteration == 1, x->iteration == x->nsmall, (float)x->nsmall);
+      }
+    }
+    else if (n == "acoustic_substep") {
+      if (iteration < 1) return set_err(x, WRFDC_EINVAL, "acoustic_substep needs iteration >= 1");
+      run.acoustic_substep(iteration);
+    }
     else if (n == "small_step_finish") run.small_step_finish();
     else if (n == "calc_p_rho_phi") run.calc_p_rho_phi();
     else if (n == "rk_stage") run.rk_stage();

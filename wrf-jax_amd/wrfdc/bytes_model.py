@@ -51,6 +51,15 @@ class KernelModel:
 
 
 KERNELS = {
+    "k_acoustic_col": KernelModel(
+        "k_acoustic_col", "acoustic_column", 3, 2,
+        reads=("u_2", "u_save", "v_2", "v_save", "ww", "ww_1", "t_2", "t_save", "t_tend", "ph_save", "phb", "ph_tend",
+               "w_2", "ph_2", "c2a", "cqw", "alt", "rw_tend", "a", "alpha", "gamma", "ru_m", "rv_m", "ww_m", "pm1"),
+        writes=("ww", "t_2", "t_2ave", "w_2", "ph_2", "ru_m", "rv_m", "ww_m", "al", "p", "pm1")),
+    "k_uv_substep": KernelModel(
+        "k_uv_substep", "acoustic_uv", 3, 2,
+        reads=("u_2", "v_2", "ru_tend", "rv_tend", "cqu", "cqv", "p", "pb", "ph_2", "alt", "al", "php"),
+        writes=("u_2", "v_2")),
     "k_advance_w": KernelModel(
         "k_advance_w", "advance_w", 3, 1,
         reads=("w_2", "rw_tend", "ww", "t_2", "t_2ave", "t_save", "ph_save", "phb", "ph_tend", "ph_2", "c2a", "cqw",
@@ -69,7 +78,7 @@ KERNELS = {
                             writes=("ph_tend",)),
 }
 
-DOMINANT = "k_advance_w"
+DOMINANT = "k_acoustic_col"
 
 
 def dominant_kernel(cfg) -> KernelModel:

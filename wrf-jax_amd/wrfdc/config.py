@@ -86,8 +86,8 @@ class DycoreConfig:
             raise ValueError("spec_bdy_width must equal spec_zone + relax_zone with relax_zone > 1")
         if self.ip < self.nx + 1:
             raise ValueError("i pitch smaller than the staggered extent")
-        if self.nz + 1 > 500:
-            raise ValueError("at most 499 vertical levels are supported")
+        if self.nz > 150:
+            raise ValueError("at most 150 vertical levels are supported (LDS-resident columns)")
         return self
 
     def with_dtbc(self, dtbc: float) -> "DycoreConfig":
