@@ -10,8 +10,8 @@ species, lateral boundaries).
 Prints one JSON line.  ``roofline`` covers the dominant kernel (algorithmic
 bytes per launch / mean launch time from HIP events on the dycore's stream);
 ``step_roofline`` the whole step against the minimal fused traffic model.
-``cpu_baseline`` times the CPU oracle (one core) on a bounded sample of the
-same workload.
+``cpu_baseline`` times one RK3 step of the CPU oracle (one core) on the same
+case (a bounded sample: one step instead of the timed run).
 """
 
 from __future__ import annotations
@@ -42,24 +42,20 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(workload: str) -> dict:
-    """One RK3 step of the oracle on a 400x100x60 strip of the same workload."""
+def cpu_baseline(case) -> dict:
+    """One RK3 step of the CPU oracle (FP32, one thread) on the same case."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     from oracle import Oracle
-    from wrfdc.cases import CONFIGS, make_case
 
-    params = dict(CONFIGS[workload])
-    nx, ny, nz = params.pop("nx"), params.pop("ny"), params.pop("nz")
-    ny_s = max(12, ny // 4)
-    case = make_case(nx, ny_s, nz, name=f"{workload}-strip", **params)
-    o = Oracle(case.config, case.arrays)
+    cfg = case.config
+    o = Oracle(cfg, case.arrays)
     t0 = time.perf_counter()
     o.step(1, 0.0)
     sec = time.perf_counter() - t0
     o.close()
-    npts = nx * ny_s * nz
+    npts = cfg.nx * cfg.ny * cfg.nz
     return {"value": npts / sec / 1e6, "unit": "Mpoint-steps/s", "cores": 1, "kind": "port",
-            "sample": f"1 RK3 step of the CPU oracle (FP32, 1 thread) on a {nx}x{ny_s}x{nz} strip of the same workload, {sec:.1f} s"}
+            "sample": f"1 RK3 step of the CPU oracle (FP32, 1 thread) on the same {cfg.nx}x{cfg.ny}x{cfg.nz} case, {sec:.1f} s"}
 
 
 def main():
@@ -161,7 +157,7 @@ def main():
     dom.close()
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(args.workload)
+        out["cpu_baseline"] = cpu_baseline(case)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

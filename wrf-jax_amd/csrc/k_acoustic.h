@@ -10,17 +10,21 @@
 //                   ide-1 / row jde-1 also writes its copy);
 //   k_zero_grad_w   zero_grad_bdy of w, rows and columns in one launch.
 //
-// The column kernel runs 64 columns (one j row, lanes = i) per block of four
-// waves.  Work that is pointwise in k is spread over the waves with loads
-// coalesced along i; the two k recurrences (column sum + omega, tridiagonal
-// sweeps) run in wave 0 over LDS-resident columns, in the same order as the
-// reference's loops, so every rounded operation matches it.
+// The column kernel runs 64 columns (one j row, lanes = i) per block of
+// eight waves.  Work that is pointwise in k is spread over the waves (wave w
+// owns levels w+1, w+1+8, ...) with loads coalesced along i; every phase first
+// loads the inputs of a group of its levels into registers, then computes and
+// stores, so a wave keeps a group's worth of loads in flight.  The two k
+// recurrences (column sum + omega, tridiagonal sweeps) run in wave 0 over
+// LDS-resident columns, in the same order as the reference's loops, so every
+// rounded operation matches it.
 #pragma once
 #include "wrfdc_common.h"
 #include "k_small.h"
 
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_uv_substep(const DevCtx *__restrict__ x, float dts, R3 r) {
+__global__ void __launch_bounds__(256) k_uv_substep(const DevCtx xv, float dts, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const int ide = x->ide, jde = x->jde;
   const size_t o = XI3(i, k, j);
@@ -74,7 +78,8 @@ __global__ void __launch_bounds__(256) k_uv_substep(const DevCtx *__restrict__ x
 
 // zero_grad_bdy of w: every destination (the zone ring of the mass grid)
 // copies an interior point, so rows and columns go in one race-free pass.
-__global__ void k_zero_grad_w(const DevCtx *__restrict__ x) {
+__global__ void k_zero_grad_w(const DevCtx xv) {
+  const DevCtx *__restrict__ x = &xv;
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = 1 + (int)blockIdx.y;
   const int ibe = x->ide - 1, jbe = x->jde - 1;
@@ -91,18 +96,30 @@ __global__ void k_zero_grad_w(const DevCtx *__restrict__ x) {
 }
 
 // ---------------------------------------------------------------------------
+constexpr int kColWaves = 8;  // waves per block of k_acoustic_col
+constexpr int kColGroup = 4;  // levels per register group
+
 // bytes of dynamic LDS of k_acoustic_col for kde levels
 __host__ __device__ inline size_t acoustic_col_lds(int kde) { return (size_t)4 * (kde + 1) * 64 * sizeof(float); }
+// levels per wave the kernel is instantiated for (kde <= kColWaves * NPW)
+inline int acoustic_col_npw(int kde) {
+  const int need = (kde + kColWaves - 1) / kColWaves;
+  return need <= 4 ? 4 : need <= 8 ? 8 : need <= 12 ? 12 : need <= 16 ? 16 : 20;
+}
 
-__global__ void __launch_bounds__(256) k_acoustic_col(const DevCtx *__restrict__ x, float dts, int first, int fin, float ns) {
+template <int NPW>
+__global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv, float dts, int first, int fin, float ns) {
+  const DevCtx *__restrict__ x = &xv;
+  constexpr int NW = kColWaves, G = kColGroup;
   extern __shared__ float sm[];
-  __shared__ float c_mts[64], c_mav[64], c_dmdt[64];
+  __shared__ float c_mts[64], c_mav[64];
   const int lane = (int)threadIdx.x, wv = (int)threadIdx.y;
   const int i = (int)blockIdx.x * 64 + lane + 1, j = (int)blockIdx.y + 1;
   const int ide = x->ide, jde = x->jde, kde = x->kde, kend = kde - 1;
   const int S = (kde + 1) * 64;
   float *B0 = sm, *B1 = sm + S, *B2 = sm + 2 * S, *B3 = sm + 3 * S;
 #define LS(buf, kk) (buf)[(kk) * 64 + lane]
+#define LEV(n) (1 + wv + NW * (n))
   const bool mass = (i <= ide - 1 && j <= jde - 1);
   const bool interior = (i >= 2 && i <= ide - 2 && j >= 2 && j <= jde - 2);
   const bool zone = mass && !interior;
@@ -110,14 +127,49 @@ __global__ void __launch_bounds__(256) k_acoustic_col(const DevCtx *__restrict__
   const float ope = 1.f + epssm, ome = 1.f - epssm;
   const float *c1h = FL(c1h), *c2h = FL(c2h), *c1f = FL(c1f), *c2f = FL(c2f), *dnw = FL(dnw), *fnm = FL(fnm), *fnp = FL(fnp);
   const float *rdnw = FL(rdnw), *rdn = FL(rdn);
-  const float *t_1 = FL(t_save);
+  const float *t_1 = FL(t_save), *u = FL(u_2), *v = FL(v_2);
   float *ww = FL(ww);
+  const int ic = mass ? i : 1, jc = mass ? j : 1;  // clamped column for address arithmetic
+  const float msfty = A2(FL(msfty), ic, jc), msftx = A2(FL(msftx), ic, jc);
 
-  // ---- advance_mu_t: horizontal mass divergence per level
-  if (interior)
-    for (int k = 1 + wv; k <= kend; k += 4) LS(B0, k) = mut_dvdxi(x, i, k, j);
+  // ---- 1. advance_mu_t: horizontal mass divergence per level -> B0
+  if (interior) {
+    const float *us = FL(u_save), *vs = FL(v_save), *muu = FL(muu), *muv = FL(muv);
+    const float muu0 = A2(muu, i, j), muu1 = A2(muu, i + 1, j), muv0 = A2(muv, i, j), muv1 = A2(muv, i, j + 1);
+    const float iuy0 = A2(FL(msfuy), i, j), iuy1 = A2(FL(msfuy), i + 1, j);
+    const float vx0 = A2(FL(msfvx_inv), i, j), vx1 = A2(FL(msfvx_inv), i, j + 1);
+    const float mm = msftx * msfty;
+#pragma unroll
+    for (int gq = 0; gq < NPW; gq += G) {
+      float r_ue[G], r_uw[G], r_use[G], r_usw[G], r_vn[G], r_vs[G], r_vsn[G], r_vss[G];
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const int k = imin_(LEV(gq + q), kend);
+        r_ue[q] = A3(u, i + 1, k, j);
+        r_uw[q] = A3(u, i, k, j);
+        r_use[q] = A3(us, i + 1, k, j);
+        r_usw[q] = A3(us, i, k, j);
+        r_vn[q] = A3(v, i, k, j + 1);
+        r_vs[q] = A3(v, i, k, j);
+        r_vsn[q] = A3(vs, i, k, j + 1);
+        r_vss[q] = A3(vs, i, k, j);
+      }
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const int k = LEV(gq + q);
+        if (k > kend) break;
+        const float c1 = A1(c1h, k), c2 = A1(c2h, k);
+        const float vn = r_vn[q] + ((c1 * muv1 + c2) * r_vsn[q]) * vx1;
+        const float vsv = r_vs[q] + ((c1 * muv0 + c2) * r_vss[q]) * vx0;
+        const float ue = r_ue[q] + ((c1 * muu1 + c2) * r_use[q]) / iuy1;
+        const float uw = r_uw[q] + ((c1 * muu0 + c2) * r_usw[q]) / iuy0;
+        const float hf = x->rdy * (vn - vsv) + x->rdx * (ue - uw);
+        LS(B0, k) = mm * hf;
+      }
+    }
+  }
   __syncthreads();
-  // ---- column sum, column-mass update and the omega recurrence (wave 0)
+  // ---- 2. column sum, column-mass update and the omega recurrence (wave 0)
   if (wv == 0 && interior) {
     float dmdt = 0.f;
     for (int k = 1; k <= kend; k++) dmdt = dmdt + A1(dnw, k) * LS(B0, k);
@@ -134,8 +186,6 @@ __global__ void __launch_bounds__(256) k_acoustic_col(const DevCtx *__restrict__
     A2(FL(muave), i, j) = mav;
     c_mts[lane] = mts;
     c_mav[lane] = mav;
-    c_dmdt[lane] = dmdt;
-    const float msfty = A2(FL(msfty), i, j);
     float prev = A3(ww, i, 1, j);
     LS(B1, 1) = prev;
     for (int k = 2; k <= kend; k++) {
@@ -154,133 +204,220 @@ __global__ void __launch_bounds__(256) k_acoustic_col(const DevCtx *__restrict__
     c_mts[lane] = mts;
   }
   __syncthreads();
-  // ---- omega minus the stage-start omega; ww(kde) kept for the phi advection
+  // ---- 3. omega minus the stage-start omega (B1 = final omega, k = 1..kde)
   if (interior) {
-    for (int k = 1 + wv; k <= kend; k += 4) {
-      const float v = LS(B1, k) - A3(FL(ww_1), i, k, j);
-      A3(ww, i, k, j) = v;
-      LS(B1, k) = v;
+    const float *ww_1 = FL(ww_1);
+#pragma unroll
+    for (int gq = 0; gq < NPW; gq += G) {
+      float r[G];
+#pragma unroll
+      for (int q = 0; q < G; q++) r[q] = A3(ww_1, i, imin_(LEV(gq + q), kend), j);
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const int k = LEV(gq + q);
+        if (k > kend) break;
+        const float vv = LS(B1, k) - r[q];
+        A3(ww, i, k, j) = vv;
+        LS(B1, k) = vv;
+      }
     }
-    if (wv == 0) LS(B1, kde) = A3(ww, i, kde, j);
+    if (wv == NW - 1) LS(B1, kde) = A3(ww, i, kde, j);
   }
   __syncthreads();
-  // ---- theta (advance_mu_t) and its acoustic average (advance_w); zone theta
+  // ---- 4. theta (advance_mu_t) and its acoustic average (advance_w) -> B2
   if (interior) {
-    const float msfty = A2(FL(msfty), i, j), msftx = A2(FL(msftx), i, j);
     const float mts = c_mts[lane], mav = c_mav[lane];
-    const float *u = FL(u_2), *v = FL(v_2), *ft = FL(t_tend);
-    float *t = FL(t_2);
-    for (int k = 1 + wv; k <= kend; k += 4) {
-      const float wlo = (k >= 2) ? LS(B1, k) * (A1(fnm, k) * A3(t_1, i, k, j) + A1(fnp, k) * A3(t_1, i, k - 1, j)) : 0.f;
-      const float whi = (k + 1 <= kend) ? LS(B1, k + 1) * (A1(fnm, k + 1) * A3(t_1, i, k + 1, j) + A1(fnp, k + 1) * A3(t_1, i, k, j)) : 0.f;
-      const float tk = A3(t, i, k, j);
-      const float ta = tk + (msfty * dts) * A3(ft, i, k, j);
-      const float vnorth = A3(v, i, k, j + 1) * (A3(t_1, i, k, j + 1) + A3(t_1, i, k, j));
-      const float vsouth = A3(v, i, k, j) * (A3(t_1, i, k, j) + A3(t_1, i, k, j - 1));
-      const float vft = (0.5f * x->rdy) * (vnorth - vsouth);
-      const float ueast = A3(u, i + 1, k, j) * (A3(t_1, i + 1, k, j) + A3(t_1, i, k, j));
-      const float uwest = A3(u, i, k, j) * (A3(t_1, i, k, j) + A3(t_1, i - 1, k, j));
-      const float uft = (0.5f * x->rdx) * (ueast - uwest);
-      const float horiz = msftx * (vft + uft);
-      const float vert = A1(rdnw, k) * (whi - wlo);
-      const float tnew = ta - (dts * msfty) * (horiz + vert);
-      A3(t, i, k, j) = tnew;
-      float tav = 0.5f * (ope * tnew + ome * tk);
-      tav = (tav + (A1(c1h, k) * mav) * t0) / ((A1(c1h, k) * mts + A1(c2h, k)) * (t0 + A3(t_1, i, k, j)));
-      A3(FL(t_2ave), i, k, j) = tav;
-      LS(B2, k) = tav;
+    const float *ft = FL(t_tend);
+    float *t = FL(t_2), *tav_out = FL(t_2ave);
+#pragma unroll
+    for (int gq = 0; gq < NPW; gq += G) {
+      float r_t1[G], r_t1m[G], r_t1p[G], r_t1e[G], r_t1w[G], r_t1n[G], r_t1s[G], r_ue[G], r_uw[G], r_vn[G], r_vs[G], r_t[G], r_ft[G];
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const int k = imin_(LEV(gq + q), kend);
+        r_t1[q] = A3(t_1, i, k, j);
+        r_t1m[q] = A3(t_1, i, imax_(k - 1, 1), j);
+        r_t1p[q] = A3(t_1, i, imin_(k + 1, kend), j);
+        r_t1e[q] = A3(t_1, i + 1, k, j);
+        r_t1w[q] = A3(t_1, i - 1, k, j);
+        r_t1n[q] = A3(t_1, i, k, j + 1);
+        r_t1s[q] = A3(t_1, i, k, j - 1);
+        r_ue[q] = A3(u, i + 1, k, j);
+        r_uw[q] = A3(u, i, k, j);
+        r_vn[q] = A3(v, i, k, j + 1);
+        r_vs[q] = A3(v, i, k, j);
+        r_t[q] = A3(t, i, k, j);
+        r_ft[q] = A3(ft, i, k, j);
+      }
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const int k = LEV(gq + q);
+        if (k > kend) break;
+        const float wlo = (k >= 2) ? LS(B1, k) * (A1(fnm, k) * r_t1[q] + A1(fnp, k) * r_t1m[q]) : 0.f;
+        const float whi = (k + 1 <= kend) ? LS(B1, k + 1) * (A1(fnm, k + 1) * r_t1p[q] + A1(fnp, k + 1) * r_t1[q]) : 0.f;
+        const float tk = r_t[q];
+        const float ta = tk + (msfty * dts) * r_ft[q];
+        const float vnorth = r_vn[q] * (r_t1n[q] + r_t1[q]);
+        const float vsouth = r_vs[q] * (r_t1[q] + r_t1s[q]);
+        const float vft = (0.5f * x->rdy) * (vnorth - vsouth);
+        const float ueast = r_ue[q] * (r_t1e[q] + r_t1[q]);
+        const float uwest = r_uw[q] * (r_t1[q] + r_t1w[q]);
+        const float uft = (0.5f * x->rdx) * (ueast - uwest);
+        const float horiz = msftx * (vft + uft);
+        const float vert = A1(rdnw, k) * (whi - wlo);
+        const float tnew = ta - (dts * msfty) * (horiz + vert);
+        A3(t, i, k, j) = tnew;
+        float tav = 0.5f * (ope * tnew + ome * tk);
+        tav = (tav + (A1(c1h, k) * mav) * t0) / ((A1(c1h, k) * mts + A1(c2h, k)) * (t0 + r_t1[q]));
+        A3(tav_out, i, k, j) = tav;
+        LS(B2, k) = tav;
+      }
     }
   } else if (zone) {
     float *t = FL(t_2);
-    for (int k = 1 + wv; k <= kend; k += 4) A3(t, i, k, j) = A3(t, i, k, j) + dts * A3(FL(t_tend), i, k, j);
+    const float *ft = FL(t_tend);
+    for (int k = 1 + wv; k <= kend; k += NW) A3(t, i, k, j) = A3(t, i, k, j) + dts * A3(ft, i, k, j);
   }
   __syncthreads();
-  // ---- advance_w: right-hand side of the phi equation, levels 2..kde
+  // ---- 5. advance_w: right-hand side of the phi equation (levels 2..kde) -> B0
   const float *ph = FL(ph_2);
-  const float m = mass ? A2(FL(mut), i, j) : 0.f;
-  const float msfty = mass ? A2(FL(msfty), i, j) : 1.f;
+  const float m = A2(FL(mut), ic, jc);
   if (interior) {
     const float *ph_1 = FL(ph_save), *phb = FL(phb), *ph_tend = FL(ph_tend), *w = FL(w_2);
-#define WDWN(kk) (((0.5f * (LS(B1, kk) + LS(B1, (kk) - 1))) * A1(rdnw, (kk) - 1)) * \
-                  (A3(ph_1, i, kk, j) - A3(ph_1, i, (kk) - 1, j) + A3(phb, i, kk, j) - A3(phb, i, (kk) - 1, j)))
-    for (int k = 1 + wv; k <= kde; k += 4) {
-      if (k == 1) {
-        LS(B0, 1) = 0.f;
-        continue;
+#pragma unroll
+    for (int gq = 0; gq < NPW; gq += G) {
+      float r_pt[G], r_w[G], r_p1[G], r_p1m[G], r_p1p[G], r_pb[G], r_pbm[G], r_pbp[G], r_ph[G];
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const int k = imax_(imin_(LEV(gq + q), kde), 2);
+        r_pt[q] = A3(ph_tend, i, k, j);
+        r_w[q] = A3(w, i, k, j);
+        r_p1[q] = A3(ph_1, i, k, j);
+        r_p1m[q] = A3(ph_1, i, k - 1, j);
+        r_p1p[q] = A3(ph_1, i, imin_(k + 1, kde), j);
+        r_pb[q] = A3(phb, i, k, j);
+        r_pbm[q] = A3(phb, i, k - 1, j);
+        r_pbp[q] = A3(phb, i, imin_(k + 1, kde), j);
+        r_ph[q] = A3(ph, i, k, j);
       }
-      float rk = dts * (A3(ph_tend, i, k, j) + ((0.5f * g) * ome) * A3(w, i, k, j));
-      if (k <= kend) rk = rk - dts * (A1(fnm, k) * WDWN(k + 1) + A1(fnp, k) * WDWN(k));
-      rk = A3(ph, i, k, j) + (msfty * rk) / (A1(c1f, k) * m + A1(c2f, k));
-      LS(B0, k) = rk;
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const int k = LEV(gq + q);
+        if (k > kde) break;
+        if (k == 1) {
+          LS(B0, 1) = 0.f;
+          continue;
+        }
+        float rk = dts * (r_pt[q] + ((0.5f * g) * ome) * r_w[q]);
+        if (k <= kend) {
+          const float wdn_k = ((0.5f * (LS(B1, k) + LS(B1, k - 1))) * A1(rdnw, k - 1)) * (r_p1[q] - r_p1m[q] + r_pb[q] - r_pbm[q]);
+          const float wdn_kp = ((0.5f * (LS(B1, k + 1) + LS(B1, k))) * A1(rdnw, k)) * (r_p1p[q] - r_p1[q] + r_pbp[q] - r_pb[q]);
+          rk = rk - dts * (A1(fnm, k) * wdn_kp + A1(fnp, k) * wdn_k);
+        }
+        rk = r_ph[q] + (msfty * rk) / (A1(c1f, k) * m + A1(c2f, k));
+        LS(B0, k) = rk;
+      }
     }
-#undef WDWN
   }
   __syncthreads();
-  // ---- explicit w (raw right-hand side of the tridiagonal system)
+  // ---- 6. explicit w (raw right-hand side of the tridiagonal system) -> B3
   if (interior) {
     const float *w = FL(w_2), *rw_tend = FL(rw_tend), *c2a = FL(c2a), *cqw = FL(cqw), *alt = FL(alt);
     const float mav = c_mav[lane], msft_inv = 1.f / msfty;
-    for (int k = 1 + wv; k <= kde; k += 4) {
-      float wl;
-      if (k == 1) {
-        const float *ht = FL(ht), *u = FL(u_2), *v = FL(v_2);
-        const float cf1 = x->cf1, cf2 = x->cf2, cf3 = x->cf3;
-        const float lv = (A2(ht, i, j + 1) - A2(ht, i, j)) * (cf1 * A3(v, i, 1, j + 1) + cf2 * A3(v, i, 2, j + 1) + cf3 * A3(v, i, 3, j + 1)) +
-                         (A2(ht, i, j) - A2(ht, i, j - 1)) * (cf1 * A3(v, i, 1, j) + cf2 * A3(v, i, 2, j) + cf3 * A3(v, i, 3, j));
-        const float lu = (A2(ht, i + 1, j) - A2(ht, i, j)) * (cf1 * A3(u, i + 1, 1, j) + cf2 * A3(u, i + 1, 2, j) + cf3 * A3(u, i + 1, 3, j)) +
-                         (A2(ht, i, j) - A2(ht, i - 1, j)) * (cf1 * A3(u, i, 1, j) + cf2 * A3(u, i, 2, j) + cf3 * A3(u, i, 3, j));
-        wl = ((msfty * 0.5f) * x->rdy) * lv + ((A2(FL(msftx), i, j) * 0.5f) * x->rdx) * lu;
-      } else if (k <= kend) {
-        const float rkp = LS(B0, k + 1), rk0 = LS(B0, k), rkm = LS(B0, k - 1);
-        const float pu = ((A3(c2a, i, k, j) * A1(rdnw, k)) / (A1(c1h, k) * m + A1(c2h, k))) *
-                         (ope * (rkp - rk0) + ome * (A3(ph, i, k + 1, j) - A3(ph, i, k, j)));
-        const float pl = ((A3(c2a, i, k - 1, j) * A1(rdnw, k - 1)) / (A1(c1h, k - 1) * m + A1(c2h, k - 1))) *
-                         (ope * (rk0 - rkm) + ome * (A3(ph, i, k, j) - A3(ph, i, k - 1, j)));
-        const float ppart = (((0.5f * dts) * g) * A1(rdn, k)) * (pu - pl);
-        const float tu = (A3(c2a, i, k, j) * A3(alt, i, k, j)) * LS(B2, k);
-        const float tl = (A3(c2a, i, k - 1, j) * A3(alt, i, k - 1, j)) * LS(B2, k - 1);
-        const float bpart = A1(rdn, k) * (tu - tl) - A1(c1f, k) * mav;
-        wl = ((A3(w, i, k, j) + dts * A3(rw_tend, i, k, j)) + (msft_inv * A3(cqw, i, k, j)) * ppart) + ((dts * g) * msft_inv) * bpart;
-      } else {
-        float fac = ((-0.5f * dts) * g) / (A1(c1h, k - 1) * m + A1(c2h, k - 1));
-        fac = fac * (A1(rdnw, k - 1) * A1(rdnw, k - 1));
-        fac = fac * 2.f;
-        fac = fac * A3(c2a, i, k - 1, j);
-        const float ptop = fac * (ope * (LS(B0, k) - LS(B0, k - 1)) + ome * (A3(ph, i, k, j) - A3(ph, i, k - 1, j)));
-        const float btop =
-            (-dts * g) * (((2.f * A1(rdnw, k - 1)) * A3(c2a, i, k - 1, j)) * A3(alt, i, k - 1, j) * LS(B2, k - 1) + A1(c1f, k) * mav);
-        wl = A3(w, i, k, j) + dts * A3(rw_tend, i, k, j) + (1.f / msfty) * (ptop + btop);
-        if (x->top_lid) wl = 0.f;
+#pragma unroll
+    for (int gq = 0; gq < NPW; gq += G) {
+      float r_c2a[G], r_c2am[G], r_alt[G], r_altm[G], r_ph[G], r_phm[G], r_php[G], r_w[G], r_rwt[G], r_cqw[G];
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const int k = imax_(imin_(LEV(gq + q), kde), 2);
+        r_c2a[q] = A3(c2a, i, imin_(k, kend), j);
+        r_c2am[q] = A3(c2a, i, k - 1, j);
+        r_alt[q] = A3(alt, i, imin_(k, kend), j);
+        r_altm[q] = A3(alt, i, k - 1, j);
+        r_ph[q] = A3(ph, i, k, j);
+        r_phm[q] = A3(ph, i, k - 1, j);
+        r_php[q] = A3(ph, i, imin_(k + 1, kde), j);
+        r_w[q] = A3(w, i, k, j);
+        r_rwt[q] = A3(rw_tend, i, k, j);
+        r_cqw[q] = A3(cqw, i, k, j);
       }
-      LS(B3, k) = wl;
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const int k = LEV(gq + q);
+        if (k > kde) break;
+        float wl;
+        if (k == 1) {
+          const float *ht = FL(ht);
+          const float cf1 = x->cf1, cf2 = x->cf2, cf3 = x->cf3;
+          const float lv = (A2(ht, i, j + 1) - A2(ht, i, j)) * (cf1 * A3(v, i, 1, j + 1) + cf2 * A3(v, i, 2, j + 1) + cf3 * A3(v, i, 3, j + 1)) +
+                           (A2(ht, i, j) - A2(ht, i, j - 1)) * (cf1 * A3(v, i, 1, j) + cf2 * A3(v, i, 2, j) + cf3 * A3(v, i, 3, j));
+          const float lu = (A2(ht, i + 1, j) - A2(ht, i, j)) * (cf1 * A3(u, i + 1, 1, j) + cf2 * A3(u, i + 1, 2, j) + cf3 * A3(u, i + 1, 3, j)) +
+                           (A2(ht, i, j) - A2(ht, i - 1, j)) * (cf1 * A3(u, i, 1, j) + cf2 * A3(u, i, 2, j) + cf3 * A3(u, i, 3, j));
+          wl = ((msfty * 0.5f) * x->rdy) * lv + ((msftx * 0.5f) * x->rdx) * lu;
+        } else if (k <= kend) {
+          const float rkp = LS(B0, k + 1), rk0 = LS(B0, k), rkm = LS(B0, k - 1);
+          const float pu = ((r_c2a[q] * A1(rdnw, k)) / (A1(c1h, k) * m + A1(c2h, k))) * (ope * (rkp - rk0) + ome * (r_php[q] - r_ph[q]));
+          const float pl = ((r_c2am[q] * A1(rdnw, k - 1)) / (A1(c1h, k - 1) * m + A1(c2h, k - 1))) * (ope * (rk0 - rkm) + ome * (r_ph[q] - r_phm[q]));
+          const float ppart = (((0.5f * dts) * g) * A1(rdn, k)) * (pu - pl);
+          const float tu = (r_c2a[q] * r_alt[q]) * LS(B2, k);
+          const float tl = (r_c2am[q] * r_altm[q]) * LS(B2, k - 1);
+          const float bpart = A1(rdn, k) * (tu - tl) - A1(c1f, k) * mav;
+          wl = ((r_w[q] + dts * r_rwt[q]) + (msft_inv * r_cqw[q]) * ppart) + ((dts * g) * msft_inv) * bpart;
+        } else {
+          float fac = ((-0.5f * dts) * g) / (A1(c1h, k - 1) * m + A1(c2h, k - 1));
+          fac = fac * (A1(rdnw, k - 1) * A1(rdnw, k - 1));
+          fac = fac * 2.f;
+          fac = fac * r_c2am[q];
+          const float ptop = fac * (ope * (LS(B0, k) - LS(B0, k - 1)) + ome * (r_ph[q] - r_phm[q]));
+          const float btop = (-dts * g) * (((2.f * A1(rdnw, k - 1)) * r_c2am[q]) * r_altm[q] * LS(B2, k - 1) + A1(c1f, k) * mav);
+          wl = r_w[q] + dts * r_rwt[q] + (1.f / msfty) * (ptop + btop);
+          if (x->top_lid) wl = 0.f;
+        }
+        LS(B3, k) = wl;
+      }
     }
   }
   __syncthreads();
-  // ---- tridiagonal sweeps (wave 0, one column per lane)
+  // ---- 7. LU factors of the forward sweep -> B1 (a), B2 (alpha)
+  if (interior) {
+    const float *a = FL(a), *alpha = FL(alpha);
+    for (int k = 2 + wv; k <= kde; k += NW) {
+      LS(B1, k) = A3(a, i, k, j);
+      LS(B2, k) = A3(alpha, i, k, j);
+    }
+  }
+  __syncthreads();
   if (wv == 0 && interior) {
-    const float *a = FL(a), *alpha = FL(alpha), *gamma = FL(gamma);
     float wprev = LS(B3, 1);
-#pragma unroll 4
     for (int k = 2; k <= kde; k++) {
-      const float wl = (LS(B3, k) - A3(a, i, k, j) * wprev) * A3(alpha, i, k, j);
-      LS(B3, k) = wl;
-      wprev = wl;
-    }
-#pragma unroll 4
-    for (int k = kend; k >= 2; k--) {
-      const float wl = LS(B3, k) - A3(gamma, i, k, j) * wprev;
+      const float wl = (LS(B3, k) - LS(B1, k) * wprev) * LS(B2, k);
       LS(B3, k) = wl;
       wprev = wl;
     }
   }
   __syncthreads();
-  // ---- new w and phi (interior); spec_bdyupdate_ph (zone); sumflux (all)
+  // ---- 8. back substitution factors -> B1 (gamma), then the backward sweep
+  if (interior) {
+    const float *gamma = FL(gamma);
+    for (int k = 2 + wv; k <= kend; k += NW) LS(B1, k) = A3(gamma, i, k, j);
+  }
+  __syncthreads();
+  if (wv == 0 && interior) {
+    float wprev = LS(B3, kde);
+    for (int k = kend; k >= 2; k--) {
+      const float wl = LS(B3, k) - LS(B1, k) * wprev;
+      LS(B3, k) = wl;
+      wprev = wl;
+    }
+  }
+  __syncthreads();
+  // ---- 9. new w and phi (interior); spec_bdyupdate_ph (zone) -> B2 = final phi
   float *php2 = FL(ph_2);
   if (interior) {
     const float mts = c_mts[lane];
     const float pcoef = (((msfty * 0.5f) * dts) * g) * ope;
     float *w = FL(w_2);
-    for (int k = 1 + wv; k <= kde; k += 4) {
+    for (int k = 1 + wv; k <= kde; k += NW) {
       const float wl = LS(B3, k);
       A3(w, i, k, j) = wl;
       float phn;
@@ -295,84 +432,137 @@ __global__ void __launch_bounds__(256) k_acoustic_col(const DevCtx *__restrict__
     const float mts = c_mts[lane];
     const float mtend = A2(FL(mu_tend), i, j);
     const float mu_old = mts - dts * mtend;
-    for (int k = 1 + wv; k <= kde; k += 4) {
-      const float den = A1(c1f, k) * mts + A1(c2f, k);
-      const float ratio = (A1(c1f, k) * mu_old + A1(c2f, k)) / den;
-      const float phn = A3(php2, i, k, j) * ratio + dts * A3(FL(ph_tend), i, k, j) / den + A3(FL(ph_save), i, k, j) * (ratio - 1.f);
-      A3(php2, i, k, j) = phn;
-      LS(B2, k) = phn;
+    const float *ph_tend = FL(ph_tend), *ph_save = FL(ph_save);
+#pragma unroll
+    for (int gq = 0; gq < NPW; gq += G) {
+      float r_ph[G], r_pt[G], r_ps[G];
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const int k = imin_(LEV(gq + q), kde);
+        r_ph[q] = A3(php2, i, k, j);
+        r_pt[q] = A3(ph_tend, i, k, j);
+        r_ps[q] = A3(ph_save, i, k, j);
+      }
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const int k = LEV(gq + q);
+        if (k > kde) break;
+        const float den = A1(c1f, k) * mts + A1(c2f, k);
+        const float ratio = (A1(c1f, k) * mu_old + A1(c2f, k)) / den;
+        const float phn = r_ph[q] * ratio + dts * r_pt[q] / den + r_ps[q] * (ratio - 1.f);
+        A3(php2, i, k, j) = phn;
+        LS(B2, k) = phn;
+      }
     }
   }
-  {
-    // sumflux over the column's u, v and w points (first iteration clears
-    // the whole arrays, as the reference's fresh accumulators)
+  // ---- sumflux over the column's u, v and w points (the first iteration
+  // clears the whole arrays, as the reference's fresh accumulators)
+  if (i <= ide && j <= jde) {
     float *ru_m = FL(ru_m), *rv_m = FL(rv_m), *ww_m = FL(ww_m);
-    if (i <= ide && j <= jde)
-      for (int k = 1 + wv; k <= kde; k += 4) {
+    const bool upt = (j <= jde - 1), vpt = (i <= ide - 1);
+    const float cmu_u = upt ? A2(FL(muu), i, j) : 0.f, cmsf_u = upt ? A2(FL(msfuy), i, j) : 1.f;
+    const float cmu_v = vpt ? A2(FL(muv), i, j) : 0.f, cmsf_v = vpt ? A2(FL(msfvx_inv), i, j) : 1.f;
+#pragma unroll
+    for (int gq = 0; gq < NPW; gq += G) {
+      float r_rum[G], r_rvm[G], r_wwm[G], r_u[G], r_v[G], r_ww[G], r_us[G], r_vs[G], r_ww1[G];
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const int k = imin_(LEV(gq + q), kde);
         const size_t o = XI3(i, k, j);
-        if (j <= jde - 1 && k <= kend) {
-          float acc = (first ? 0.f : ru_m[o]) + FL(u_2)[o];
-          if (fin) acc = acc / ns + ((A1(c1h, k) * A2(FL(muu), i, j) + A1(c2h, k)) * FL(u_save)[o]) / A2(FL(msfuy), i, j);
+        r_rum[q] = first ? 0.f : ru_m[o];
+        r_rvm[q] = first ? 0.f : rv_m[o];
+        r_wwm[q] = first ? 0.f : ww_m[o];
+        r_u[q] = upt ? u[o] : 0.f;
+        r_v[q] = vpt ? v[o] : 0.f;
+        r_ww[q] = mass ? ww[o] : 0.f;
+        r_us[q] = (fin && upt) ? FL(u_save)[o] : 0.f;
+        r_vs[q] = (fin && vpt) ? FL(v_save)[o] : 0.f;
+        r_ww1[q] = (fin && mass) ? FL(ww_1)[o] : 0.f;
+      }
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const int k = LEV(gq + q);
+        if (k > kde) break;
+        const size_t o = XI3(i, k, j);
+        if (upt && k <= kend) {
+          float acc = r_rum[q] + r_u[q];
+          if (fin) acc = acc / ns + ((A1(c1h, k) * cmu_u + A1(c2h, k)) * r_us[q]) / cmsf_u;
           ru_m[o] = acc;
         } else if (first) {
           ru_m[o] = 0.f;
         }
-        if (i <= ide - 1 && k <= kend) {
-          float acc = (first ? 0.f : rv_m[o]) + FL(v_2)[o];
-          if (fin) acc = acc / ns + ((A1(c1h, k) * A2(FL(muv), i, j) + A1(c2h, k)) * FL(v_save)[o]) * A2(FL(msfvx_inv), i, j);
+        if (vpt && k <= kend) {
+          float acc = r_rvm[q] + r_v[q];
+          if (fin) acc = acc / ns + ((A1(c1h, k) * cmu_v + A1(c2h, k)) * r_vs[q]) * cmsf_v;
           rv_m[o] = acc;
         } else if (first) {
           rv_m[o] = 0.f;
         }
         if (mass) {
-          float acc = (first ? 0.f : ww_m[o]) + ww[o];
-          if (fin) acc = acc / ns + FL(ww_1)[o];
+          float acc = r_wwm[q] + r_ww[q];
+          if (fin) acc = acc / ns + r_ww1[q];
           ww_m[o] = acc;
         } else if (first) {
           ww_m[o] = 0.f;
         }
       }
+    }
   }
   __syncthreads();
-  // ---- calc_p_rho (iteration >= 1) and the physical-boundary copies
+  // ---- 10. calc_p_rho (iteration >= 1) and the physical-boundary copies
   if (mass) {
     const float mts = c_mts[lane], mu2 = A2(FL(mu_2), i, j);
     const bool gx = (i == ide - 1), gy = (j == jde - 1);
     const float *alt = FL(alt), *t_2 = FL(t_2), *c2a = FL(c2a);
     float *al = FL(al), *p = FL(p), *pm1 = FL(pm1);
-    for (int k = 1 + wv; k <= kend; k += 4) {
-      const size_t o = XI3(i, k, j);
-      const float mass_ = A1(c1h, k) * mts + A1(c2h, k);
-      const float dry = A1(c1h, k) * mu2;
-      const float altk = alt[o], t1 = t_1[o];
-      const float bracket = altk * dry + A1(rdnw, k) * (LS(B2, k + 1) - LS(B2, k));
-      const float alv = (-1.f / mass_) * bracket;
-      const float tdel = t_2[o] - dry * t1;
-      const float frac = (altk * tdel) / (mass_ * (t0 + t1));
-      const float pr = c2a[o] * (frac - alv);
-      const float pn = pr + x->smdiv * (pr - pm1[o]);
-      al[o] = alv;
-      p[o] = pn;
-      pm1[o] = pr;
-      if (gx) {
-        A3(al, ide, k, j) = alv;
-        A3(p, ide, k, j) = pn;
+#pragma unroll
+    for (int gq = 0; gq < NPW; gq += G) {
+      float r_alt[G], r_t1[G], r_t2[G], r_c2a[G], r_pm1[G];
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const size_t o = XI3(i, imin_(LEV(gq + q), kend), j);
+        r_alt[q] = alt[o];
+        r_t1[q] = t_1[o];
+        r_t2[q] = t_2[o];
+        r_c2a[q] = c2a[o];
+        r_pm1[q] = pm1[o];
       }
-      if (gy) {
-        A3(al, i, k, jde) = alv;
-        A3(p, i, k, jde) = pn;
-      }
-      if (gx && gy) {
-        A3(al, ide, k, jde) = alv;
-        A3(p, ide, k, jde) = pn;
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const int k = LEV(gq + q);
+        if (k > kend) break;
+        const size_t o = XI3(i, k, j);
+        const float mass_ = A1(c1h, k) * mts + A1(c2h, k);
+        const float dry = A1(c1h, k) * mu2;
+        const float bracket = r_alt[q] * dry + A1(rdnw, k) * (LS(B2, k + 1) - LS(B2, k));
+        const float alv = (-1.f / mass_) * bracket;
+        const float tdel = r_t2[q] - dry * r_t1[q];
+        const float frac = (r_alt[q] * tdel) / (mass_ * (t0 + r_t1[q]));
+        const float pr = r_c2a[q] * (frac - alv);
+        const float pn = pr + x->smdiv * (pr - r_pm1[q]);
+        al[o] = alv;
+        p[o] = pn;
+        pm1[o] = pr;
+        if (gx) {
+          A3(al, ide, k, j) = alv;
+          A3(p, ide, k, j) = pn;
+        }
+        if (gy) {
+          A3(al, i, k, jde) = alv;
+          A3(p, i, k, jde) = pn;
+        }
+        if (gx && gy) {
+          A3(al, ide, k, jde) = alv;
+          A3(p, ide, k, jde) = pn;
+        }
       }
     }
     if (gx || gy) {
-      for (int k = 1 + wv; k <= kde; k += 4) {
-        const float v = LS(B2, k);
-        if (gx) A3(php2, ide, k, j) = v;
-        if (gy) A3(php2, i, k, jde) = v;
-        if (gx && gy) A3(php2, ide, k, jde) = v;
+      for (int k = 1 + wv; k <= kde; k += NW) {
+        const float vv = LS(B2, k);
+        if (gx) A3(php2, ide, k, j) = vv;
+        if (gy) A3(php2, i, k, jde) = vv;
+        if (gx && gy) A3(php2, ide, k, jde) = vv;
       }
       if (wv == 0) {
         const float a2 = A2(FL(muts), i, j), b2 = A2(FL(mu_2), i, j), c2 = A2(FL(mudf), i, j);
@@ -388,4 +578,30 @@ __global__ void __launch_bounds__(256) k_acoustic_col(const DevCtx *__restrict__
     }
   }
 #undef LS
+#undef LEV
+}
+
+// host launcher: instantiation by levels per wave
+inline void launch_acoustic_col(const DevCtx &d, hipStream_t s, float dts, int first, int fin, float ns) {
+  const dim3 grid((unsigned)((d.ide + 63) / 64), (unsigned)d.jde), block(64, kColWaves);
+  const size_t lds = acoustic_col_lds(d.kde);
+  switch (acoustic_col_npw(d.kde)) {
+    case 4: k_acoustic_col<4><<<grid, block, lds, s>>>(d, dts, first, fin, ns); break;
+    case 8: k_acoustic_col<8><<<grid, block, lds, s>>>(d, dts, first, fin, ns); break;
+    case 12: k_acoustic_col<12><<<grid, block, lds, s>>>(d, dts, first, fin, ns); break;
+    case 16: k_acoustic_col<16><<<grid, block, lds, s>>>(d, dts, first, fin, ns); break;
+    default: k_acoustic_col<20><<<grid, block, lds, s>>>(d, dts, first, fin, ns); break;
+  }
+}
+// raise the dynamic-LDS limit of the instantiation kde needs (beyond 64 KB)
+inline hipError_t acoustic_col_allow_lds(int kde) {
+  const int lds = (int)acoustic_col_lds(kde);
+  if (lds <= 65536) return hipSuccess;
+  switch (acoustic_col_npw(kde)) {
+    case 4: return hipFuncSetAttribute((const void *)k_acoustic_col<4>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    case 8: return hipFuncSetAttribute((const void *)k_acoustic_col<8>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    case 12: return hipFuncSetAttribute((const void *)k_acoustic_col<12>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    case 16: return hipFuncSetAttribute((const void *)k_acoustic_col<16>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    default: return hipFuncSetAttribute((const void *)k_acoustic_col<20>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  }
 }

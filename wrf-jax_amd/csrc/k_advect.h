@@ -55,8 +55,9 @@ __device__ __forceinline__ float fz_s(const DevCtx *__restrict__ x, const float 
 }
 
 // advect_scalar (module_advect_em.py:90-369): tend -= div(flux), mass points
-__global__ void k_advect_scalar(const DevCtx *__restrict__ x, const float *q, float *tend, const float *ru, const float *rv,
+__global__ void k_advect_scalar(const DevCtx xv, const float *q, float *tend, const float *ru, const float *rv,
                                 const float *rom, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *msftx = FL(msftx), *rdzw = FL(rdnw);
   float t = A3(tend, i, k, j);
@@ -106,8 +107,9 @@ __device__ __forceinline__ float fz_u(const DevCtx *__restrict__ x, const float 
   return vel * flux3(A3(u, i, k - 2, j), A3(u, i, k - 1, j), A3(u, i, k, j), A3(u, i, k + 1, j), -vel);
 }
 // advect_u (module_advect_em.py:762-1034); r covers i [2, ide-1], j [1, jde-1]
-__global__ void k_advect_u(const DevCtx *__restrict__ x, const float *u, float *tend, const float *ru, const float *rv,
+__global__ void k_advect_u(const DevCtx xv, const float *u, float *tend, const float *ru, const float *rv,
                            const float *rom, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *msfux = FL(msfux), *rdzw = FL(rdnw);
   float t = A3(tend, i, k, j);
@@ -157,8 +159,9 @@ __device__ __forceinline__ float fz_v(const DevCtx *__restrict__ x, const float 
   return vel * flux3(A3(v, i, k - 2, j), A3(v, i, k - 1, j), A3(v, i, k, j), A3(v, i, k + 1, j), -vel);
 }
 // advect_v (module_advect_em.py:1037-1354); r covers i [1, ide-1], j [2, jde-1]
-__global__ void k_advect_v(const DevCtx *__restrict__ x, const float *v, float *tend, const float *ru, const float *rv,
+__global__ void k_advect_v(const DevCtx xv, const float *v, float *tend, const float *ru, const float *rv,
                            const float *rom, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *msfvy = FL(msfvy), *msfvx = FL(msfvx), *rdzw = FL(rdnw);
   float t = A3(tend, i, k, j);
@@ -197,8 +200,9 @@ __device__ __forceinline__ float fz_w(const DevCtx *__restrict__ x, const float 
   return vel * flux3(A3(w, i, k - 2, j), A3(w, i, k - 1, j), A3(w, i, k, j), A3(w, i, k + 1, j), -vel);
 }
 // advect_w (module_advect_em.py:1357-1640); r covers i [1, ide-1], k [2, kde], j [1, jde-1]
-__global__ void k_advect_w(const DevCtx *__restrict__ x, const float *w, float *tend, const float *ru, const float *rv,
+__global__ void k_advect_w(const DevCtx xv, const float *w, float *tend, const float *ru, const float *rv,
                            const float *rom, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *msftx = FL(msftx), *rdzu = FL(rdn);
   float t = A3(tend, i, k, j);
@@ -212,8 +216,9 @@ __global__ void k_advect_w(const DevCtx *__restrict__ x, const float *w, float *
 // ---------------- positive-definite scalar advection ------------------------
 // pass 1: high-order minus upwind (antidiffusive) and upwind fluxes
 // (module_advect_em.py:537-660) on all faces the limiter and tendency read.
-__global__ void k_pd_fluxes(const DevCtx *__restrict__ x, const float *q, const float *q_old, const float *ru, const float *rv,
+__global__ void k_pd_fluxes(const DevCtx xv, const float *q, const float *q_old, const float *ru, const float *rv,
                             const float *rom, const float *mut, float dt, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *c1 = FL(c1h), *c2 = FL(c2h), *msftx = FL(msftx), *msfty = FL(msfty), *rdzw = FL(rdnw);
   const float *fzm = FL(fnm), *fzp = FL(fnp);
@@ -279,7 +284,8 @@ __global__ void k_pd_fluxes(const DevCtx *__restrict__ x, const float *q, const 
 }
 // pass 2: renormalisation factor of each limited cell (1 = not limited)
 // (module_advect_em.py:662-720); cells i in [2, ide-2], j in [2, jde-2].
-__global__ void k_pd_scale(const DevCtx *__restrict__ x, const float *q_old, const float *mub, const float *mu_old, float dt, R3 r) {
+__global__ void k_pd_scale(const DevCtx xv, const float *q_old, const float *mub, const float *mu_old, float dt, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *c1 = FL(c1h), *c2 = FL(c2h), *msftx = FL(msftx), *msfty = FL(msfty), *rdzw = FL(rdnw);
   const float *fqx = SCR(S_FQX), *fqy = SCR(S_FQY), *fqz = SCR(S_FQZ), *fqxl = SCR(S_FQXL), *fqyl = SCR(S_FQYL), *fqzl = SCR(S_FQZL);
@@ -300,7 +306,8 @@ __global__ void k_pd_scale(const DevCtx *__restrict__ x, const float *q_old, con
 }
 // pass 3: tendency from the renormalised faces; a face is scaled by the
 // cell its antidiffusive flux leaves (the only cell that may limit it).
-__global__ void k_pd_tend(const DevCtx *__restrict__ x, float *tend, R3 r) {
+__global__ void k_pd_tend(const DevCtx xv, float *tend, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *msftx = FL(msftx), *rdzw = FL(rdnw);
   const float *fqx = SCR(S_FQX), *fqy = SCR(S_FQY), *fqz = SCR(S_FQZ), *fqxl = SCR(S_FQXL), *fqyl = SCR(S_FQYL), *fqzl = SCR(S_FQZL);

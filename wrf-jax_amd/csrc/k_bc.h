@@ -7,7 +7,8 @@
 
 // ---- physical BC copy (set_physical_bc3d, no-halo specified) -------------
 // plane i = ide-1 -> ide for every j (skipped for x-staggered variables)
-__global__ void k_bc_x(const DevCtx *__restrict__ x, float *fl, int kend, int is2d) {
+__global__ void k_bc_x(const DevCtx xv, float *fl, int kend, int is2d) {
+  const DevCtx *__restrict__ x = &xv;
   const int j = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = 1 + (int)blockIdx.y;
   if (j > x->jde || k > kend) return;
@@ -15,7 +16,8 @@ __global__ void k_bc_x(const DevCtx *__restrict__ x, float *fl, int kend, int is
   else A3(fl, x->ide, k, j) = A3(fl, x->ide - 1, k, j);
 }
 // row j = jde-1 -> jde for every i (skipped for y-staggered variables)
-__global__ void k_bc_y(const DevCtx *__restrict__ x, float *fl, int kend, int is2d) {
+__global__ void k_bc_y(const DevCtx xv, float *fl, int kend, int is2d) {
+  const DevCtx *__restrict__ x = &xv;
   const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = 1 + (int)blockIdx.y;
   if (i > x->ide || k > kend) return;
@@ -33,7 +35,8 @@ struct BcList {
   int kend[kBcMax];
   int flags[kBcMax];  // bit 0: copy column ide, bit 1: copy row jde, bit 2: 2-D
 };
-__global__ void k_bc_batch(const DevCtx *__restrict__ x, BcList L) {
+__global__ void k_bc_batch(const DevCtx xv, BcList L) {
+  const DevCtx *__restrict__ x = &xv;
   const int a = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = 1 + (int)blockIdx.y;
   const int q = (int)blockIdx.z;
@@ -86,7 +89,8 @@ struct BPtrs {
 };
 
 // spec_bdytend overlay: tend on the zone = boundary tendency (b = 1)
-__global__ void k_spec_overlay(const DevCtx *__restrict__ x, float *tend, int ibe, int jbe, int kt, int is2d, BPtrs bt) {
+__global__ void k_spec_overlay(const DevCtx xv, float *tend, int ibe, int jbe, int kt, int is2d, BPtrs bt) {
+  const DevCtx *__restrict__ x = &xv;
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = 1 + (int)blockIdx.y;
   int i, j, e;
@@ -98,7 +102,8 @@ __global__ void k_spec_overlay(const DevCtx *__restrict__ x, float *tend, int ib
 }
 
 // spec_bdyupdate: field += dt * tend on the zone
-__global__ void k_spec_update(const DevCtx *__restrict__ x, float *fl, const float *tend, float dt, int ibe, int jbe, int kt, int is2d) {
+__global__ void k_spec_update(const DevCtx xv, float *fl, const float *tend, float dt, int ibe, int jbe, int kt, int is2d) {
+  const DevCtx *__restrict__ x = &xv;
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = 1 + (int)blockIdx.y;
   int i, j, e;
@@ -108,7 +113,8 @@ __global__ void k_spec_update(const DevCtx *__restrict__ x, float *fl, const flo
 }
 
 // spec_bdyupdate_ph (module_bc_em.py:1429-1560)
-__global__ void k_spec_update_ph(const DevCtx *__restrict__ x, float dts) {
+__global__ void k_spec_update_ph(const DevCtx xv, float dts) {
+  const DevCtx *__restrict__ x = &xv;
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = 1 + (int)blockIdx.y;
   int i, j, e;
@@ -123,7 +129,8 @@ __global__ void k_spec_update_ph(const DevCtx *__restrict__ x, float dts) {
 }
 
 // zero_grad_bdy for w (zero_grad_bdy.py:21-128): rows first, then columns
-__global__ void k_zero_grad_rows(const DevCtx *__restrict__ x, float *w) {
+__global__ void k_zero_grad_rows(const DevCtx xv, float *w) {
+  const DevCtx *__restrict__ x = &xv;
   const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = 1 + (int)blockIdx.y;
   const int side = (int)blockIdx.z;
@@ -133,7 +140,8 @@ __global__ void k_zero_grad_rows(const DevCtx *__restrict__ x, float *w) {
   if (side == 0) A3(w, i, k, 1) = A3(w, ii, k, 2);
   else A3(w, i, k, jbe) = A3(w, ii, k, jbe - 1);
 }
-__global__ void k_zero_grad_cols(const DevCtx *__restrict__ x, float *w) {
+__global__ void k_zero_grad_cols(const DevCtx xv, float *w) {
+  const DevCtx *__restrict__ x = &xv;
   const int j = 2 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = 1 + (int)blockIdx.y;
   const int side = (int)blockIdx.z;
@@ -147,8 +155,9 @@ __global__ void k_zero_grad_cols(const DevCtx *__restrict__ x, float *w) {
 // ---- relaxation zone (relax_bdytend_core.py:27-294) -----------------------
 // blockIdx.z = side*(rz-1) + (bd-1), side 0..3 = south, north, west, east.
 // fld is the mass-coupled field, tend receives the increment.
-__global__ void k_relax(const DevCtx *__restrict__ x, const float *fld, float *tend, int ibe, int jbe, int kt, int is2d,
+__global__ void k_relax(const DevCtx xv, const float *fld, float *tend, int ibe, int jbe, int kt, int is2d,
                         BPtrs b, BPtrs bt, float dt_bc) {
+  const DevCtx *__restrict__ x = &xv;
   const int a = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = 1 + (int)blockIdx.y;
   const int rz = x->relax_zone;
@@ -199,7 +208,8 @@ __global__ void k_relax(const DevCtx *__restrict__ x, const float *fld, float *t
 }
 
 // flow_dep_bdy (flow_dep_bdy.py:27-176): rows then columns
-__global__ void k_flow_dep_rows(const DevCtx *__restrict__ x, float *q, const float *v) {
+__global__ void k_flow_dep_rows(const DevCtx xv, float *q, const float *v) {
+  const DevCtx *__restrict__ x = &xv;
   const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = 1 + (int)blockIdx.y;
   const int side = (int)blockIdx.z;
@@ -209,7 +219,8 @@ __global__ void k_flow_dep_rows(const DevCtx *__restrict__ x, float *q, const fl
   if (side == 0) A3(q, i, k, 1) = A3(v, i, k, 1) < 0.f ? A3(q, ii, k, 2) : 0.f;
   else A3(q, i, k, jbe) = A3(v, i, k, jbe + 1) > 0.f ? A3(q, ii, k, jbe - 1) : 0.f;
 }
-__global__ void k_flow_dep_cols(const DevCtx *__restrict__ x, float *q, const float *u) {
+__global__ void k_flow_dep_cols(const DevCtx xv, float *q, const float *u) {
+  const DevCtx *__restrict__ x = &xv;
   const int j = 2 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = 1 + (int)blockIdx.y;
   const int side = (int)blockIdx.z;
@@ -221,8 +232,9 @@ __global__ void k_flow_dep_cols(const DevCtx *__restrict__ x, float *q, const fl
 }
 
 // spec_bdy_final (spec_bdy_final.py:27-191)
-__global__ void k_spec_final(const DevCtx *__restrict__ x, float *fl, int ibe, int jbe, int kt, int is2d, int msfc, int muc,
+__global__ void k_spec_final(const DevCtx xv, float *fl, int ibe, int jbe, int kt, int is2d, int msfc, int muc,
                              const float *mu, const float *msf, const float *c1, const float *c2, BPtrs b, BPtrs bt, float dt_bc) {
+  const DevCtx *__restrict__ x = &xv;
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = 1 + (int)blockIdx.y;
   int i, j, e;
@@ -236,7 +248,8 @@ __global__ void k_spec_final(const DevCtx *__restrict__ x, float *fl, int ibe, i
 }
 
 // set_w_surface (module_bc_em.py:448-560) without the fill option
-__global__ void k_set_w_surface(const DevCtx *__restrict__ x) {
+__global__ void k_set_w_surface(const DevCtx xv) {
+  const DevCtx *__restrict__ x = &xv;
   const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int j = 1 + (int)blockIdx.y;
   if (i > x->ide - 1 || j > x->jde - 1) return;

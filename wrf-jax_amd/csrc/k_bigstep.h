@@ -18,7 +18,8 @@ __global__ void k_copy(float *dst, const float *src, size_t n) {
 
 // mut, muu, muv (calc_mu_uv, module_big_step_utilities_em.py:174-266) and
 // muus/muvs from muts (calc_mu_uv_1).  2-D over i [1, ide], j [1, jde].
-__global__ void k_mu_uv(const DevCtx *__restrict__ x, const float *mu, const float *mub, float *muu, float *muv, float *mut) {
+__global__ void k_mu_uv(const DevCtx xv, const float *mu, const float *mub, float *muu, float *muv, float *mut) {
+  const DevCtx *__restrict__ x = &xv;
   const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int j = 1 + (int)blockIdx.y;
   const int ide = x->ide, jde = x->jde;
@@ -44,7 +45,8 @@ __global__ void k_mu_uv(const DevCtx *__restrict__ x, const float *mu, const flo
 
 // couple_momentum + calc_cq + calc_alt + calc_php over the full 3-D extent
 // (zero outside each field's range, as the reference's fresh outputs).
-__global__ void k_rk_prep3(const DevCtx *__restrict__ x, R3 r) {
+__global__ void k_rk_prep3(const DevCtx xv, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const int ide = x->ide, jde = x->jde, kde = x->kde;
   const float *u = FL(u_2), *v = FL(v_2), *w = FL(w_2), *muu = FL(muu), *muv = FL(muv), *mut = FL(mut);
@@ -94,7 +96,8 @@ __device__ __forceinline__ float ww_divv(const DevCtx *__restrict__ x, int i, in
   float yf = (c1 * muv_jp + c2) * A3(v, i, k, j + 1) * A2(FL(msfvx_inv), i, j + 1) - (c1 * muv_j + c2) * A3(v, i, k, j) * A2(FL(msfvx_inv), i, j);
   return A2(FL(msftx), i, j) * A1(FL(dnw), k) * (x->rdx * xf + x->rdy * yf);
 }
-__global__ void k_ww_cp(const DevCtx *__restrict__ x, R2 r) {
+__global__ void k_ww_cp(const DevCtx xv, R2 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IJ(r);
   const float *mup = FL(mu_2), *mub = FL(mub);
   float *ww = FL(ww);
@@ -119,7 +122,8 @@ __global__ void k_ww_cp(const DevCtx *__restrict__ x, R2 r) {
 // rhs_ph (module_big_step_utilities_em.py:2386-2794): vertical advection of
 // phi, the w g term and horizontal phi advection (6th/4th/2nd order with the
 // reference's specified-boundary column pattern).  k in [2, kde].
-__global__ void k_rhs_ph(const DevCtx *__restrict__ x, R3 r) {
+__global__ void k_rhs_ph(const DevCtx xv, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const int ide = x->ide, jde = x->jde, kde = x->kde, kte = kde;
   const float *u = FL(u_2), *v = FL(v_2), *ww = FL(ww), *ph = FL(ph_2), *phb = FL(phb), *w = FL(w_2);
@@ -210,7 +214,8 @@ __device__ __forceinline__ float hpg_dpn(const DevCtx *__restrict__ x, const flo
   if (k >= x->kde) return 0.f;
   return 0.5f * (A1(FL(fnm), k) * (A3(p, i - di, k, j - dj) + A3(p, i, k, j)) + A1(FL(fnp), k) * (A3(p, i - di, k - 1, j - dj) + A3(p, i, k - 1, j)));
 }
-__global__ void k_hpg(const DevCtx *__restrict__ x, int axis, R3 r) {
+__global__ void k_hpg(const DevCtx xv, int axis, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *ph = FL(ph_2), *alt = FL(alt), *p = FL(p), *pb = FL(pb), *al = FL(al), *php = FL(php), *mu = FL(mu_2);
   const float *c1h = FL(c1h), *c2h = FL(c2h), *rdnw = FL(rdnw);
@@ -230,7 +235,8 @@ __global__ void k_hpg(const DevCtx *__restrict__ x, int axis, R3 r) {
 }
 
 // pg_buoy_w (module_big_step_utilities_em.py:1674-1800): column kernel
-__global__ void k_pg_buoy_w(const DevCtx *__restrict__ x, R2 r) {
+__global__ void k_pg_buoy_w(const DevCtx xv, R2 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IJ(r);
   float *rwt = FL(rw_tend), *cqw = FL(cqw);
   const float *p = FL(p), *mu = FL(mu_2), *mub = FL(mub), *c1f = FL(c1f), *c2f = FL(c2f), *rdnw = FL(rdnw), *rdn = FL(rdn);
@@ -257,7 +263,8 @@ __device__ __forceinline__ float vxgm(const DevCtx *__restrict__ x, int i, int k
   return 0.5f * (A3(u, i, k, j) + A3(u, i + 1, k, j)) * (A2(FL(msfvx), i, j + 1) - A2(FL(msfvx), i, j)) * x->rdy -
          0.5f * (A3(v, i, k, j) + A3(v, i, k, j + 1)) * (A2(FL(msfuy), i + 1, j) - A2(FL(msfuy), i, j)) * x->rdx;
 }
-__global__ void k_cor_curv(const DevCtx *__restrict__ x, int comp, R3 r) {
+__global__ void k_cor_curv(const DevCtx xv, int comp, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *ru = FL(ru), *rv = FL(rv), *rw = FL(rw), *u = FL(u_2), *v = FL(v_2);
   const float *f = FL(f), *e = FL(e), *sina = FL(sina), *cosa = FL(cosa), *fzm = FL(fnm), *fzp = FL(fnp);
@@ -297,7 +304,8 @@ __global__ void k_cor_curv(const DevCtx *__restrict__ x, int comp, R3 r) {
 }
 
 // rk_addtend_dry (module_em.py:959-1148); comp 0 u, 1 v, 2 w/ph, 3 theta
-__global__ void k_addtend(const DevCtx *__restrict__ x, int comp, int rk, R3 r) {
+__global__ void k_addtend(const DevCtx xv, int comp, int rk, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const size_t o = XI3(i, k, j);
   if (comp == 0) {
@@ -324,7 +332,8 @@ __global__ void k_addtend(const DevCtx *__restrict__ x, int comp, int rk, R3 r) 
 }
 
 // _calc_p_rho_phi_nonhydro_opt2 (runtime.py:12369-12537): hypsometric al, EOS p
-__global__ void k_calc_p_rho_phi(const DevCtx *__restrict__ x, R3 r) {
+__global__ void k_calc_p_rho_phi(const DevCtx xv, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *muts = FL(muts), *ph = FL(ph_2), *phb = FL(phb), *alb = FL(alb), *pb = FL(pb), *t = FL(t_2);
   const float m = A2(muts, i, j), ptop = x->ptop;

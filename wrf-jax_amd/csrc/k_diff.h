@@ -8,13 +8,15 @@
 
 // rho = (1 + qv) / alt on mass points (phy_prep,
 // module_big_step_utilities_em.py:1823-2081, rho at :1965)
-__global__ void k_rho(const DevCtx *__restrict__ x, R3 r) {
+__global__ void k_rho(const DevCtx xv, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   A3(FL(rho), i, k, j) = (1.f / A3(FL(alt), i, k, j)) * (1.f + A3(SP(moist, 0), i, k, j));
 }
 
 // compute_diff_metrics (module_diffusion_em.py:1825-2043), full extent.
-__global__ void k_diff_metrics(const DevCtx *__restrict__ x, R3 r) {
+__global__ void k_diff_metrics(const DevCtx xv, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const int ide = x->ide, jde = x->jde, kde = x->kde, ktf = kde - 1;
   const float *ph = FL(ph_2), *phb = FL(phb);
@@ -74,7 +76,8 @@ __device__ __forceinline__ float hat_avg(const DevCtx *__restrict__ x, const flo
 
 // cal_deform_and_div (module_diffusion_em.py:366-891): D11, D22, D33 and the
 // divergence on mass points.
-__global__ void k_deform_diag(const DevCtx *__restrict__ x, R3 r) {
+__global__ void k_deform_diag(const DevCtx xv, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *u = FL(u_2), *v = FL(v_2), *w = FL(w_2), *zx = FL(zx), *zy = FL(zy), *rdzw = FL(rdzw);
   const float *msfuy = FL(msfuy), *msfvx = FL(msfvx);
@@ -101,7 +104,8 @@ __global__ void k_deform_diag(const DevCtx *__restrict__ x, R3 r) {
 }
 
 // D12 at (u, v) corner points i [2, ide-1], j [2, jde-1], k [1, kde-1]
-__global__ void k_deform12(const DevCtx *__restrict__ x, R3 r) {
+__global__ void k_deform12(const DevCtx xv, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *u = FL(u_2), *v = FL(v_2), *zx = FL(zx), *zy = FL(zy), *rdzw = FL(rdzw), *msfux = FL(msfux), *msfvy = FL(msfvy);
   const float mm = 0.25f * (A2(msfux, i, j - 1) + A2(msfux, i, j)) * (A2(msfvy, i - 1, j) + A2(msfvy, i, j));
@@ -123,7 +127,8 @@ __global__ void k_deform12(const DevCtx *__restrict__ x, R3 r) {
 }
 
 // D13 / D23 at (u, w) and (v, w) points i [2, ide-1], j [2, jde-1], k [1, kde]
-__global__ void k_deform13_23(const DevCtx *__restrict__ x, R3 r) {
+__global__ void k_deform13_23(const DevCtx xv, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const int ktf = x->kde - 1;
   float *d13 = FL(defor13), *d23 = FL(defor23);
@@ -160,7 +165,8 @@ __global__ void k_deform13_23(const DevCtx *__restrict__ x, R3 r) {
 
 // edge copy of whole planes: i = idst <- isrc for every (k, j) (axis 0) or
 // j = jdst <- jsrc for every (i, k) (axis 1); up to two fields per launch.
-__global__ void k_edge_copy(const DevCtx *__restrict__ x, float *f0, float *f1, int axis, int dst, int src, int kend) {
+__global__ void k_edge_copy(const DevCtx xv, float *f0, float *f1, int axis, int dst, int src, int kend) {
+  const DevCtx *__restrict__ x = &xv;
   const int a = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = 1 + (int)blockIdx.y;
   if (k > kend) return;
@@ -177,7 +183,8 @@ __global__ void k_edge_copy(const DevCtx *__restrict__ x, float *f0, float *f1, 
 
 // smag2d_km (module_diffusion_em.py:1502-1646) with the slope correction;
 // i [2, ide-2], j [2, jde-2], k [1, kde-1].
-__global__ void k_smag(const DevCtx *__restrict__ x, R3 r) {
+__global__ void k_smag(const DevCtx xv, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *d11 = FL(defor11), *d22 = FL(defor22), *d12 = FL(defor12), *zx = FL(zx), *zy = FL(zy), *rdzw = FL(rdzw);
   const float dx = x->dx, dy = x->dy, c_s = x->c_s;
@@ -208,8 +215,9 @@ __global__ void k_smag(const DevCtx *__restrict__ x, R3 r) {
 // titau (module_diffusion_em.py:894-1316): stress component with rho and the
 // diffusivity averaged to the component's point; zero outside [i0,i1]x[j0,j1]
 // (and at k = 1, kde for the 13 / 23 components).  kind 11, 12, 13, 23.
-__global__ void k_titau(const DevCtx *__restrict__ x, float *tt, const float *xk, const float *defor, int kind, int i0, int i1, int j0,
+__global__ void k_titau(const DevCtx xv, float *tt, const float *xk, const float *defor, int kind, int i0, int i1, int j0,
                         int j1, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *rho = FL(rho);
   const int ktf = x->kde - 1;
@@ -242,7 +250,8 @@ __device__ __forceinline__ float hd_a2u(const DevCtx *__restrict__ x, const floa
   if (k == 1 || k == x->kde) return 0.f;
   return 0.5f * (A1(FL(fnm), k) * (A3(t2, i, k, j + 1) + A3(t2, i, k, j)) + A1(FL(fnp), k) * (A3(t2, i, k - 1, j + 1) + A3(t2, i, k - 1, j)));
 }
-__global__ void k_hdiff_u(const DevCtx *__restrict__ x, const float *t1, const float *t2, R3 r) {
+__global__ void k_hdiff_u(const DevCtx xv, const float *t1, const float *t2, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *zx = FL(zx), *zy = FL(zy), *rdzw = FL(rdzw);
   const float zxu = 0.5f * (A3(zx, i, k, j) + A3(zx, i, k + 1, j));
@@ -268,7 +277,8 @@ __device__ __forceinline__ float hd_a2v(const DevCtx *__restrict__ x, const floa
   if (k == 1 || k == x->kde) return 0.f;
   return 0.5f * (A1(FL(fnm), k) * (A3(t2, i, k, j - 1) + A3(t2, i, k, j)) + A1(FL(fnp), k) * (A3(t2, i, k - 1, j - 1) + A3(t2, i, k - 1, j)));
 }
-__global__ void k_hdiff_v(const DevCtx *__restrict__ x, const float *t1, const float *t2, R3 r) {
+__global__ void k_hdiff_v(const DevCtx xv, const float *t1, const float *t2, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *zx = FL(zx), *zy = FL(zy), *rdzw = FL(rdzw);
   const float zxv = 0.125f * (A3(zx, i, k, j) + A3(zx, i + 1, k, j) + A3(zx, i, k, j - 1) + A3(zx, i + 1, k, j - 1) + A3(zx, i, k + 1, j) +
@@ -286,7 +296,8 @@ __global__ void k_hdiff_v(const DevCtx *__restrict__ x, const float *t1, const f
 
 // horizontal_diffusion_w_2 (module_diffusion_em.py:2768-2985):
 // w points i [2, ide-2], j [2, jde-2], k [2, kde-1].
-__global__ void k_hdiff_w(const DevCtx *__restrict__ x, const float *t1, const float *t2, R3 r) {
+__global__ void k_hdiff_w(const DevCtx xv, const float *t1, const float *t2, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *zx = FL(zx), *zy = FL(zy), *rdz = FL(rdz);
 #define HA1(kk) (0.25f * (A3(t1, i + 1, (kk) + 1, j) + A3(t1, i, (kk) + 1, j) + A3(t1, i + 1, kk, j) + A3(t1, i, kk, j)))
@@ -319,7 +330,8 @@ __device__ __forceinline__ float hs_hav(const DevCtx *__restrict__ x, const floa
   }
   return 0.5f * (A1(FL(fnm), k) * (A3(var, ib, k, jb) + A3(var, ia, k, ja)) + A1(FL(fnp), k) * (A3(var, ib, k - 1, jb) + A3(var, ia, k - 1, ja)));
 }
-__global__ void k_hdiff_s_faces(const DevCtx *__restrict__ x, const float *var, float *h1, float *h2, R3 r) {
+__global__ void k_hdiff_s_faces(const DevCtx xv, const float *var, float *h1, float *h2, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const int ide = x->ide, jde = x->jde;
   const float *xkhh = FL(xkhh), *rho = FL(rho), *zx = FL(zx), *zy = FL(zy), *rdzw = FL(rdzw);
@@ -341,7 +353,8 @@ __global__ void k_hdiff_s_faces(const DevCtx *__restrict__ x, const float *var, 
   }
 }
 // pass 2: tendency on mass points i [2, ide-2], j [2, jde-2], k [1, kde-1]
-__global__ void k_hdiff_s_tend(const DevCtx *__restrict__ x, float *tend, const float *h1, const float *h2, R3 r) {
+__global__ void k_hdiff_s_tend(const DevCtx xv, float *tend, const float *h1, const float *h2, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *zx = FL(zx), *zy = FL(zy), *rdzw = FL(rdzw), *fnm = FL(fnm), *fnp = FL(fnp);
   const int ktf = x->kde - 1;

@@ -8,7 +8,8 @@
 
 // small_step_prep, 2-D part (module_small_step_em.py:272-420): stage-start
 // column masses.  i [1, ide], j [1, jde].
-__global__ void k_ssp_mass(const DevCtx *__restrict__ x, int rk) {
+__global__ void k_ssp_mass(const DevCtx xv, int rk) {
+  const DevCtx *__restrict__ x = &xv;
   const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int j = 1 + (int)blockIdx.y;
   const int ide = x->ide, jde = x->jde;
@@ -54,7 +55,8 @@ __global__ void k_ssp_mass(const DevCtx *__restrict__ x, int rk) {
 // small_step_prep, 3-D part (module_small_step_em.py:420-717): save the
 // stage-start state, form the acoustic perturbation variables and c2a.
 // Full extent i [1, ide], k [1, kde], j [1, jde].
-__global__ void k_ssp_3d(const DevCtx *__restrict__ x, int rk, R3 r) {
+__global__ void k_ssp_3d(const DevCtx xv, int rk, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const int ide = x->ide, jde = x->jde, kde = x->kde;
   const size_t o = XI3(i, k, j);
@@ -101,7 +103,8 @@ __global__ void k_ssp_3d(const DevCtx *__restrict__ x, int rk, R3 r) {
 
 // calc_p_rho, non-hydrostatic (module_small_step_em.py:720-1034) with
 // ph = ph_2, t_1 = t_save, mu = mu_2, mut = muts; mass points.
-__global__ void k_calc_p_rho(const DevCtx *__restrict__ x, int step, R3 r) {
+__global__ void k_calc_p_rho(const DevCtx xv, int step, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const size_t o = XI3(i, k, j);
   const float mass = A1(FL(c1h), k) * A2(FL(muts), i, j) + A1(FL(c2h), k);
@@ -120,7 +123,8 @@ __global__ void k_calc_p_rho(const DevCtx *__restrict__ x, int step, R3 r) {
 
 // calc_coef_w (module_small_step_em.py:1359-1586): LU factors of the
 // implicit w-phi system, one thread per mass column.
-__global__ void k_calc_coef_w(const DevCtx *__restrict__ x, float dts, R2 r) {
+__global__ void k_calc_coef_w(const DevCtx xv, float dts, R2 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IJ(r);
   float *a = FL(a), *alpha = FL(alpha), *gamma = FL(gamma);
   const float *cqw = FL(cqw), *c2a = FL(c2a), *c1h = FL(c1h), *c2h = FL(c2h), *c1f = FL(c1f), *c2f = FL(c2f);
@@ -171,7 +175,8 @@ __device__ __forceinline__ float uv_iface(const DevCtx *__restrict__ x, const fl
 
 // advance_uv (module_small_step_em.py:1589-1987), specified boundaries.
 // axis 0: u on i [2, ide-1], j [2, jde-2]; axis 1: v on i [2, ide-2], j [2, jde-1].
-__global__ void k_advance_uv(const DevCtx *__restrict__ x, int axis, float dts, R3 r) {
+__global__ void k_advance_uv(const DevCtx xv, int axis, float dts, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const int di = axis == 0 ? 1 : 0, dj = axis == 0 ? 0 : 1;
   float *vel = axis == 0 ? FL(u_2) : FL(v_2);
@@ -215,7 +220,8 @@ __device__ __forceinline__ float mut_dvdxi(const DevCtx *__restrict__ x, int i, 
 // advance_mu_t (module_small_step_em.py:2024-2403) with u_1 = u_save,
 // v_1 = v_save, t_1 = t_save, t_ave = t_2ave, ft = t_tend; one thread per
 // column i [2, ide-2], j [2, jde-2].
-__global__ void k_advance_mu_t(const DevCtx *__restrict__ x, float dts, R2 r) {
+__global__ void k_advance_mu_t(const DevCtx xv, float dts, R2 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IJ(r);
   float *ww = FL(ww), *t = FL(t_2), *t_ave = FL(t_2ave);
   const float *ww_1 = FL(ww_1), *t_1 = FL(t_save), *ft = FL(t_tend), *u = FL(u_2), *v = FL(v_2);
@@ -269,7 +275,8 @@ __global__ void k_advance_mu_t(const DevCtx *__restrict__ x, float dts, R2 r) {
 // ph_1 = ph_save; one thread per column i [2, ide-2], j [2, jde-2].  The
 // right-hand side of the phi equation is staged in the scratch array `rhs`
 // (w levels); the forward sweep of the tridiagonal solve runs in w itself.
-__global__ void k_advance_w(const DevCtx *__restrict__ x, float dts, float *__restrict__ rhs, R2 r) {
+__global__ void k_advance_w(const DevCtx xv, float dts, float *__restrict__ rhs, R2 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IJ(r);
   float *w = FL(w_2), *ph = FL(ph_2), *t_2ave = FL(t_2ave);
   const float *rw_tend = FL(rw_tend), *ww = FL(ww), *u = FL(u_2), *v = FL(v_2), *t_2 = FL(t_2), *t_1 = FL(t_save);
@@ -351,7 +358,8 @@ __global__ void k_advance_w(const DevCtx *__restrict__ x, float dts, float *__re
 // sumflux (module_small_step_em.py:3134-3310) with ru = u_2, rv = v_2,
 // u_lin = u_save, v_lin = v_save, ww_lin = ww_1; full extent so the first
 // iteration also clears points outside the flux ranges.
-__global__ void k_sumflux(const DevCtx *__restrict__ x, int first, int fin, float ns, R3 r) {
+__global__ void k_sumflux(const DevCtx xv, int first, int fin, float ns, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const int ide = x->ide, jde = x->jde, kde = x->kde;
   const size_t o = XI3(i, k, j);
@@ -382,7 +390,8 @@ __global__ void k_sumflux(const DevCtx *__restrict__ x, int first, int fin, floa
 
 // small_step_finish (module_small_step_em.py:1119-1332): recouple the
 // acoustic perturbations onto the stage-start state.  Full extent.
-__global__ void k_ssf(const DevCtx *__restrict__ x, int final_step, float dts, float nsm, R3 r) {
+__global__ void k_ssf(const DevCtx xv, int final_step, float dts, float nsm, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const int ide = x->ide, jde = x->jde, kde = x->kde;
   const size_t o = XI3(i, k, j);
@@ -415,7 +424,8 @@ __global__ void k_ssf(const DevCtx *__restrict__ x, int final_step, float dts, f
 
 // rk_update_scalar_pd (module_em.py:1601-1669): last-stage physics update of
 // the old scalar with mu_old = mu_new = mu_1, then the tendency is cleared.
-__global__ void k_update_scalar_pd(const DevCtx *__restrict__ x, float *q_old, float *q_tend, float dt, R3 r) {
+__global__ void k_update_scalar_pd(const DevCtx xv, float *q_old, float *q_tend, float dt, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const size_t o = XI3(i, k, j);
   const float m = A1(FL(c1h), k) * (A2(FL(mu_1), i, j) + A2(FL(mub), i, j)) + A1(FL(c2h), k);
@@ -425,7 +435,8 @@ __global__ void k_update_scalar_pd(const DevCtx *__restrict__ x, float *q_old, f
 
 // rk_update_scalar (module_em.py:1437-1598): advective tendency (interior
 // only, times msfty) plus the physics tendency, mass-weighted update.
-__global__ void k_update_scalar(const DevCtx *__restrict__ x, float *q, float *q_old, const float *q_tend, int rk, float dt, R3 r) {
+__global__ void k_update_scalar(const DevCtx xv, float *q, float *q_old, const float *q_tend, int rk, float dt, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const size_t o = XI3(i, k, j);
   const int sz = x->spec_zone;

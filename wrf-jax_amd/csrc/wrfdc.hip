@@ -1,8 +1,9 @@
 // MI355X (gfx950) RK3 split-explicit dycore: host orchestration and the C ABI
 // declared in include/wrfdc.h.
 //
-// One wrfdc_ctx owns one domain's device-resident DevCtx (dimensions,
-// scalars and the caller's field pointers) plus library scratch.  A model
+// One wrfdc_ctx owns one domain's DevCtx (dimensions, scalars and the
+// caller's field pointers), which every kernel receives by value so that the
+// pointers it loads are known to be global, plus library scratch.  A model
 // step is a fixed sequence of kernel launches on the context's stream; the
 // stage order follows src/wrf_jax/core/direct_step.py:3037-3531 (RK and
 // acoustic loops) and src/wrf_jax/dyn_em/solve_em.py:260-800 (final
@@ -30,7 +31,8 @@
 
 // field times its column mass (mass_weight, module_bc_em.py:561-624), zero
 // outside the coupled range; full extent.  stag_w selects w levels and c1f/c2f.
-__global__ void k_mass_weight(const DevCtx *__restrict__ x, const float *fl, float *out, int stag_w, R3 r) {
+__global__ void k_mass_weight(const DevCtx xv, const float *fl, float *out, int stag_w, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const bool in = (i <= x->ide - 1 && j <= x->jde - 1 && (stag_w || k <= x->kde - 1));
   float v = 0.f;
@@ -42,7 +44,8 @@ __global__ void k_mass_weight(const DevCtx *__restrict__ x, const float *fl, flo
 }
 
 // dst = src on mass points (initialize_moist_old)
-__global__ void k_copy_mass(const DevCtx *__restrict__ x, float *dst, const float *src, R3 r) {
+__global__ void k_copy_mass(const DevCtx xv, float *dst, const float *src, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   A3(dst, i, k, j) = A3(src, i, k, j);
 }
@@ -51,7 +54,8 @@ __global__ void k_copy_mass(const DevCtx *__restrict__ x, float *dst, const floa
 // [2] sum of all moist species, [3] count of non-finite prognostic values.
 // Pass 1 writes one partial per block (vector stores, no atomics).
 #define CK_BLOCK 256
-__global__ void __launch_bounds__(CK_BLOCK) k_checksum_partial(const DevCtx *__restrict__ x, double *part) {
+__global__ void __launch_bounds__(CK_BLOCK) k_checksum_partial(const DevCtx xv, double *part) {
+  const DevCtx *__restrict__ x = &xv;
   const int ni = x->ide - 1, nk = x->kde - 1, nj = x->jde - 1;
   const size_t ncol = (size_t)ni * nj, npt = ncol * nk;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
@@ -129,11 +133,9 @@ struct wrfdc_ctx {
   int ide, jde, kde, ip, ns;
   size_t n3, n2;
   DevCtx h;
-  DevCtx *d = nullptr;
   hipStream_t s = nullptr;
   float *scratch = nullptr;
   double *ck = nullptr;  // kCkBlocks * 4 partials + 4 results
-  bool dirty = true;
   int rk = 1, nsmall = 1, iteration = 1;
   double dt_rk = 0.0, dts_rk = 0.0, dtbc = 0.0;
   std::string err;
@@ -154,10 +156,10 @@ inline unsigned cdiv(int a, int b) { return (unsigned)((a + b - 1) / b); }
 
 struct Run {
   wrfdc_ctx *X;
-  const DevCtx *d;
+  DevCtx d;  // passed by value: kernels read the field pointers from kernel arguments
   hipStream_t s;
   int ide, jde, kde;
-  explicit Run(wrfdc_ctx *x) : X(x), d(x->d), s(x->s), ide(x->ide), jde(x->jde), kde(x->kde) {}
+  explicit Run(wrfdc_ctx *x) : X(x), d(x->h), s(x->s), ide(x->ide), jde(x->jde), kde(x->kde) {}
 
   float *F(int id) const { return X->h.F[id]; }
   float *Sp(int id, int s_) const { return X->h.F[id] + (size_t)s_ * X->n3; }
@@ -492,8 +494,7 @@ struct Run {
     const float dts = (float)X->dts_rk;
     R3 r{1, ide, 1, kde - 1, 1, jde};
     k_uv_substep<<<g3(r), b3(), 0, s>>>(d, dts, r);
-    k_acoustic_col<<<dim3(cdiv(ide, 64), jde), dim3(64, 4), acoustic_col_lds(kde), s>>>(d, dts, it == 1, it == X->nsmall,
-                                                                                     (float)X->nsmall);
+    launch_acoustic_col(d, s, dts, it == 1, it == X->nsmall, (float)X->nsmall);
     Z z = zd('w', true, false);
     k_zero_grad_w<<<dim3(cdiv(zone_n(z), 64), kde), 64, 0, s>>>(d);
   }
@@ -668,10 +669,6 @@ int ensure_ready(wrfdc_ctx *x) {
     if (kKinds[f] == 'S' && x->c.nscalar == 0) continue;
     return set_err(x, WRFDC_EUNBOUND, std::string("field not bound: ") + kNames[f]);
   }
-  if (x->dirty) {
-    hchk(hipMemcpy(x->d, &x->h, sizeof(DevCtx), hipMemcpyHostToDevice), "upload context");
-    x->dirty = false;
-  }
   return WRFDC_OK;
 }
 
@@ -765,27 +762,23 @@ int wrfdc_create(const wrfdc_config *cfg, wrfdc_ctx **out) {
     h.gcx[b] = (((1.0f / (float)cfg->dt) / 50.0f) * dist) / den;
   }
   try {
-    hchk(hipMalloc((void **)&x->d, sizeof(DevCtx)), "hipMalloc context");
     hchk(hipMalloc((void **)&x->scratch, sizeof(float) * x->n3 * NSCRATCH), "hipMalloc scratch");
     hchk(hipMalloc((void **)&x->ck, sizeof(double) * (kCkBlocks * 4 + 4)), "hipMalloc checksum");
   } catch (const HipError &e) {
     fprintf(stderr, "wrfdc_create: %s: %s\n", e.what, hipGetErrorString(e.e));
-    if (x->d) (void)hipFree(x->d);
     if (x->scratch) (void)hipFree(x->scratch);
     delete x;
     return WRFDC_EHIP;
   }
   for (int i = 0; i < NSCRATCH; i++) h.S[i] = x->scratch + (size_t)i * x->n3;
-  if (acoustic_col_lds(x->kde) > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void *)k_acoustic_col, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)acoustic_col_lds(x->kde));
+  {
+    hipError_t e = acoustic_col_allow_lds(x->kde);
     if (e != hipSuccess) {
       fprintf(stderr, "wrfdc_create: LDS size for nz=%d: %s\n", cfg->nz, hipGetErrorString(e));
       wrfdc_destroy(x);
       return WRFDC_EHIP;
     }
   }
-  x->dirty = true;
   *out = x;
   return WRFDC_OK;
 }
@@ -794,7 +787,6 @@ void wrfdc_destroy(wrfdc_ctx *x) {
   if (!x) return;
   if (x->s) (void)hipStreamSynchronize(x->s);
   else (void)hipDeviceSynchronize();
-  if (x->d) (void)hipFree(x->d);
   if (x->scratch) (void)hipFree(x->scratch);
   if (x->ck) (void)hipFree(x->ck);
   delete x;
@@ -805,7 +797,6 @@ int wrfdc_bind(wrfdc_ctx *x, const char *name, void *device_ptr) {
   for (int f = 0; f < NFIELDS; f++)
     if (strcmp(kNames[f], name) == 0) {
       x->h.F[f] = (float *)device_ptr;
-      x->dirty = true;
       return WRFDC_OK;
     }
   return set_err(x, WRFDC_EINVAL, std::string("unknown field: ") + name);
@@ -868,10 +859,9 @@ int wrfdc_stage(wrfdc_ctx *x, const char *name, int rk_step, int iteration) {
       x->iteration = iteration < 1 ? 1 : iteration;
       if (n == "acoustic_uv") {
         R3 r{1, run.ide, 1, run.kde - 1, 1, run.jde};
-        k_uv_substep<<<Run::g3(r), Run::b3(), 0, x->s>>>(x->d, dts, r);
+        k_uv_substep<<<Run::g3(r), Run::b3(), 0, x->s>>>(x->h, dts, r);
       } else {
-        k_acoustic_col<<<dim3(cdiv(run.ide, 64), run.jde), dim3(64, 4), acoustic_col_lds(run.kde), x->s>>>(
-            x->d, dts, x->iteration == 1, x->iteration == x->nsmall, (float)x->nsmall);
+        launch_acoustic_col(x->h, x->s, dts, x->iteration == 1, x->iteration == x->nsmall, (float)x->nsmall);
       }
     }
     else if (n == "acoustic_substep") {
@@ -890,7 +880,7 @@ int wrfdc_stage(wrfdc_ctx *x, const char *name, int rk_step, int iteration) {
 int wrfdc_checksums(wrfdc_ctx *x, double out[4]) {
   if (!out) return set_err(x, WRFDC_EINVAL, "out is null");
   int rc = guarded(x, [&]() {
-    k_checksum_partial<<<kCkBlocks, CK_BLOCK, 0, x->s>>>(x->d, x->ck);
+    k_checksum_partial<<<kCkBlocks, CK_BLOCK, 0, x->s>>>(x->h, x->ck);
     k_checksum_final<<<1, CK_BLOCK, 0, x->s>>>(x->ck, kCkBlocks, x->ck + kCkBlocks * 4);
     hchk(hipMemcpyAsync(out, x->ck + kCkBlocks * 4, sizeof(double) * 4, hipMemcpyDeviceToHost, x->s), "copy checksums");
     hchk(hipStreamSynchronize(x->s), "synchronize checksums");
