@@ -24,6 +24,7 @@
 #include "k_diff.h"
 #include "k_small.h"
 #include "k_acoustic.h"
+#include "k_species.h"
 
 // ---------------------------------------------------------------------------
 // small helper kernels local to the orchestration
@@ -135,6 +136,8 @@ struct wrfdc_ctx {
   DevCtx h;
   hipStream_t s = nullptr;
   float *scratch = nullptr;
+  float *spec_alt = nullptr;    // second buffer of every species (nmoist + nscalar 3-D arrays)
+  float *spec_scale = nullptr;  // positive-definite renormalisation factors, one 3-D array per species
   double *ck = nullptr;  // kCkBlocks * 4 partials + 4 results
   int rk = 1, nsmall = 1, iteration = 1;
   double dt_rk = 0.0, dts_rk = 0.0, dtbc = 0.0;
@@ -156,13 +159,21 @@ inline unsigned cdiv(int a, int b) { return (unsigned)((a + b - 1) / b); }
 
 struct Run {
   wrfdc_ctx *X;
-  DevCtx d;  // passed by value: kernels read the field pointers from kernel arguments
+  // passed by value to every kernel: kernels read the field pointers from
+  // kernel arguments.  The moist / scalar entries point at whichever species
+  // buffer holds the current stage's values (the caller's or the library's
+  // second buffer, see species_stage).
+  mutable DevCtx d;
   hipStream_t s;
   int ide, jde, kde;
-  explicit Run(wrfdc_ctx *x) : X(x), d(x->h), s(x->s), ide(x->ide), jde(x->jde), kde(x->kde) {}
+  float *user_moist, *user_scalar, *alt_moist, *alt_scalar;
+  mutable bool old_synced = false;  // q_old == q on mass points for every species
+  explicit Run(wrfdc_ctx *x)
+      : X(x), d(x->h), s(x->s), ide(x->ide), jde(x->jde), kde(x->kde), user_moist(x->h.F[F_moist]), user_scalar(x->h.F[F_scalar]),
+        alt_moist(x->spec_alt), alt_scalar(x->spec_alt + x->n3 * (size_t)x->c.nmoist) {}
 
-  float *F(int id) const { return X->h.F[id]; }
-  float *Sp(int id, int s_) const { return X->h.F[id] + (size_t)s_ * X->n3; }
+  float *F(int id) const { return d.F[id]; }
+  float *Sp(int id, int s_) const { return d.F[id] + (size_t)s_ * X->n3; }
   float *Scr(int id) const { return X->h.S[id]; }
 
   R3 full3() const { return R3{1, ide, 1, kde, 1, jde}; }
@@ -531,6 +542,92 @@ struct Run {
     }
   }
 
+  // ---- species (all moist and scalar species of one stage) ----------------
+  bool species_in_alt() const { return d.F[F_moist] != user_moist; }
+  void use_species(bool alt) const {
+    d.F[F_moist] = alt ? alt_moist : user_moist;
+    d.F[F_scalar] = (X->c.nscalar > 0) ? (alt ? alt_scalar : user_scalar) : nullptr;
+  }
+  // copy the current species into the other buffer and switch to it
+  void species_to(bool alt) const {
+    if (species_in_alt() == alt) return;
+    const size_t n = X->n3;
+    float *dm = alt ? alt_moist : user_moist, *ds = alt ? alt_scalar : user_scalar;
+    fill_copy(dm, d.F[F_moist], n * (size_t)X->c.nmoist);
+    if (X->c.nscalar > 0) fill_copy(ds, d.F[F_scalar], n * (size_t)X->c.nscalar);
+    use_species(alt);
+  }
+  void fill_copy(float *dst, const float *src, size_t n) const {
+    unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    k_copy<<<nb, 256, 0, s>>>(dst, src, n);
+  }
+  // q_old <- q on mass points (the reference's rk1 update sets it; for the
+  // moist species the step start has already done so)
+  void sync_old(bool moist, bool scalar) const {
+    R3 m = mass3();
+    if (moist)
+      for (int q = 0; q < X->c.nmoist; q++) k_copy_mass<<<g3(m), b3(), 0, s>>>(d, Sp(F_moist_old, q), Sp(F_moist, q), m);
+    if (scalar)
+      for (int q = 0; q < X->c.nscalar; q++) k_copy_mass<<<g3(m), b3(), 0, s>>>(d, Sp(F_scalar_old, q), Sp(F_scalar, q), m);
+  }
+  // Buffers: rk1 reads the stencils from q_old (== q) and writes q in place;
+  // rk2 reads q and writes the second buffer, which later kernels of the step
+  // then see as the species; rk3 reads that buffer and writes the caller's.
+  void species_stage() const {
+    const int rk = X->rk, nm = X->c.nmoist, nsc = X->c.nscalar;
+    const float dt = (float)X->dt_rk;
+    R3 m = mass3();
+    if (rk == 1) {
+      if (species_in_alt()) species_to(false);
+      if (!old_synced) sync_old(true, true);
+      old_synced = true;
+      // relax_bdy_scalar + spec_bdy_scalar of QV into its tendency
+      float *tmp = Scr(S_HAT);
+      R3 r = full3();
+      k_mass_weight<<<g3(r), b3(), 0, s>>>(d, Sp(F_moist, 0), tmp, 0, r);
+      relax(tmp, Sp(F_moist_tend, 0), 'q', false, false, bp(F_qv_bys, F_qv_bye, F_qv_bxs, F_qv_bxe), bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe));
+      spec_overlay(Sp(F_moist_tend, 0), 't', false, false, bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe));
+    }
+    if (rk == 3 && !species_in_alt()) species_to(true);
+    const bool out_alt = (rk == 2);
+    SpecList plain, pd;
+    memset(&plain, 0, sizeof(plain));
+    memset(&pd, 0, sizeof(pd));
+    const size_t n3 = X->n3;
+    auto add = [&](SpecList &L, int id, int id_old, int id_tend, float *outbase, int q) {
+      L.in[L.n] = (rk == 1) ? Sp(id_old, q) : Sp(id, q);
+      L.out[L.n] = outbase + (size_t)q * n3;
+      L.old[L.n] = Sp(id_old, q);
+      L.tend[L.n] = Sp(id_tend, q);
+      L.scale[L.n] = nullptr;
+      L.n++;
+    };
+    const bool pdm = (rk == 3 && X->c.pd_moist), pds = (rk == 3 && X->c.pd_scalar);
+    for (int q = 0; q < nm; q++) add(pdm ? pd : plain, F_moist, F_moist_old, F_moist_tend, out_alt ? alt_moist : user_moist, q);
+    for (int q = 0; q < nsc; q++) add(pds ? pd : plain, F_scalar, F_scalar_old, F_scalar_tend, out_alt ? alt_scalar : user_scalar, q);
+    if (pd.n > 0) {
+      for (int q = 0; q < pd.n; q++) pd.scale[q] = X->spec_scale + (size_t)q * n3;
+      k_species_pd_prep<<<g3(m), b3(), 0, s>>>(d, pd, dt, m);
+      {
+        BcBatch bc(this);
+        for (int q = 0; q < pd.n; q++) bc.add3(pd.old[q], 'p');
+      }
+      R3 r = full3();
+      k_species_pd_scale<<<g3(r), b3(), 0, s>>>(d, pd, dt, r);
+      k_species_pd_update<<<g3(m), b3(), 0, s>>>(d, pd, dt, m);
+    }
+    if (plain.n > 0) k_species_update<<<g3(m), b3(), 0, s>>>(d, plain, rk, dt, m);
+    // outputs become the current species
+    use_species(out_alt);
+    if (nm > 1) {
+      FlowList fl;
+      memset(&fl, 0, sizeof(fl));
+      for (int q = 1; q < nm; q++) fl.q[fl.n++] = Sp(F_moist, q);
+      Z z = zd('t', false, false);
+      k_flow_dep_batch<<<dim3(cdiv(zone_n(z), 64), kde - 1, fl.n), 64, 0, s>>>(d, fl);
+    }
+  }
+
   void rk_stage() const {
     rk_step_prep();
     BcBatch(this)
@@ -555,21 +652,7 @@ struct Run {
     if (X->rk == 3)
       BcBatch(this).add3(F(F_ru_m), 'u').add3(F(F_rv_m), 'v').add3(F(F_ww_m), 'w').add2(F(F_mut), 't').add2(F(F_muts), 't').flush();
     const int nm = X->c.nmoist, nsc = X->c.nscalar;
-    {
-      BcBatch bc(this);
-      if (X->rk == 3 && X->c.pd_moist)
-        for (int q = 0; q < nm; q++) {
-          update_scalar_pd(Sp(F_moist_old, q), Sp(F_moist_tend, q));
-          bc.add3(Sp(F_moist_old, q), 'p');
-        }
-      if (X->rk == 3 && X->c.pd_scalar)
-        for (int q = 0; q < nsc; q++) {
-          update_scalar_pd(Sp(F_scalar_old, q), Sp(F_scalar_tend, q));
-          bc.add3(Sp(F_scalar_old, q), 'p');
-        }
-    }
-    for (int q = 0; q < nm; q++) species_advance(Sp(F_moist, q), Sp(F_moist_old, q), Sp(F_moist_tend, q), q == 0, true, X->c.pd_moist != 0);
-    for (int q = 0; q < nsc; q++) species_advance(Sp(F_scalar, q), Sp(F_scalar_old, q), Sp(F_scalar_tend, q), false, false, X->c.pd_scalar != 0);
+    species_stage();
     calc_p_rho_phi();
     if (X->rk < 3) {
       BcBatch bc(this);
@@ -581,13 +664,18 @@ struct Run {
 
   void step() const {
     R3 m = mass3();
+    if (species_in_alt()) species_to(false);
     {
+      // initialize_moist_old (module_big_step_utilities_em.py:935-975); the
+      // scalars' q_old <- q of the first-stage update is done here as well
       BcBatch bc(this);
       for (int q = 0; q < X->c.nmoist; q++) {
         k_copy_mass<<<g3(m), b3(), 0, s>>>(d, Sp(F_moist_old, q), Sp(F_moist, q), m);
         bc.add3(Sp(F_moist_old, q), 'p');
       }
     }
+    sync_old(false, true);
+    old_synced = true;
     for (int rk = 1; rk <= 3; rk++) {
       set_rk(rk);
       rk_stage();
@@ -650,6 +738,7 @@ int validate(const wrfdc_config &c, std::string &msg) {
   if (c.ip < c.nx + 1) { msg = "ip must be >= nx + 1"; return WRFDC_EINVAL; }
   if (c.nmoist < 1) { msg = "nmoist must be >= 1 (species 0 is QV)"; return WRFDC_EINVAL; }
   if (c.nscalar < 0) { msg = "nscalar must be >= 0"; return WRFDC_EINVAL; }
+  if (c.nmoist + c.nscalar > kMaxSpec) { msg = "at most 16 advected species (nmoist + nscalar)"; return WRFDC_EINVAL; }
   if (!(c.dx > 0.0 && c.dy > 0.0 && c.dt > 0.0)) { msg = "dx, dy and dt must be positive"; return WRFDC_EINVAL; }
   if (c.spec_zone != 1) { msg = "only spec_zone = 1 is implemented"; return WRFDC_ENOTIMPL; }
   if (c.relax_zone < 2) { msg = "relax_zone must be >= 2"; return WRFDC_EINVAL; }
@@ -764,9 +853,14 @@ int wrfdc_create(const wrfdc_config *cfg, wrfdc_ctx **out) {
   try {
     hchk(hipMalloc((void **)&x->scratch, sizeof(float) * x->n3 * NSCRATCH), "hipMalloc scratch");
     hchk(hipMalloc((void **)&x->ck, sizeof(double) * (kCkBlocks * 4 + 4)), "hipMalloc checksum");
+    hchk(hipMalloc((void **)&x->spec_alt, sizeof(float) * x->n3 * (size_t)(cfg->nmoist + cfg->nscalar)), "hipMalloc species");
+    hchk(hipMalloc((void **)&x->spec_scale, sizeof(float) * x->n3 * (size_t)(cfg->nmoist + cfg->nscalar)), "hipMalloc species scale");
   } catch (const HipError &e) {
     fprintf(stderr, "wrfdc_create: %s: %s\n", e.what, hipGetErrorString(e.e));
     if (x->scratch) (void)hipFree(x->scratch);
+    if (x->ck) (void)hipFree(x->ck);
+    if (x->spec_alt) (void)hipFree(x->spec_alt);
+    if (x->spec_scale) (void)hipFree(x->spec_scale);
     delete x;
     return WRFDC_EHIP;
   }
@@ -789,6 +883,8 @@ void wrfdc_destroy(wrfdc_ctx *x) {
   else (void)hipDeviceSynchronize();
   if (x->scratch) (void)hipFree(x->scratch);
   if (x->ck) (void)hipFree(x->ck);
+  if (x->spec_alt) (void)hipFree(x->spec_alt);
+  if (x->spec_scale) (void)hipFree(x->spec_scale);
   delete x;
 }
 
@@ -870,7 +966,10 @@ int wrfdc_stage(wrfdc_ctx *x, const char *name, int rk_step, int iteration) {
     }
     else if (n == "small_step_finish") run.small_step_finish();
     else if (n == "calc_p_rho_phi") run.calc_p_rho_phi();
-    else if (n == "rk_stage") run.rk_stage();
+    else if (n == "rk_stage") {
+      run.rk_stage();
+      if (run.species_in_alt()) run.species_to(false);
+    }
     else if (n == "step") run.step();
     else return set_err(x, WRFDC_EINVAL, "unknown stage: " + n);
     return WRFDC_OK;
