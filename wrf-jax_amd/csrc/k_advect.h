@@ -6,11 +6,12 @@
 #include "wrfdc_common.h"
 
 __device__ __forceinline__ float flux3(float qm2, float qm1, float q0, float qp1, float ua) {
-  return (7.f * (q0 + qm1) - (qp1 + qm2)) / 12.f + sgn1(ua) * ((qp1 - qm2) - 3.f * (q0 - qm1)) / 12.f;
+  return div_const(7.f * (q0 + qm1) - (qp1 + qm2), 12.f, 1.f / 12.f) +
+         div_const(sgn1(ua) * ((qp1 - qm2) - 3.f * (q0 - qm1)), 12.f, 1.f / 12.f);
 }
 __device__ __forceinline__ float flux5(float qm3, float qm2, float qm1, float q0, float qp1, float qp2, float ua) {
-  return (37.f * (q0 + qm1) - 8.f * (qp1 + qm2) + (qp2 + qm3)) / 60.f -
-         sgn1(ua) * ((qp2 - qm3) - 5.f * (qp1 - qm2) + 10.f * (q0 - qm1)) / 60.f;
+  return div_const(37.f * (q0 + qm1) - 8.f * (qp1 + qm2) + (qp2 + qm3), 60.f, 1.f / 60.f) -
+         div_const(sgn1(ua) * ((qp2 - qm3) - 5.f * (qp1 - qm2) + 10.f * (q0 - qm1)), 60.f, 1.f / 60.f);
 }
 __device__ __forceinline__ float flux3pd(float qm2, float qm1, float q0, float qp1, float ua) {
   float f4 = (7.f / 12.f) * (q0 + qm1) - (1.f / 12.f) * (qp1 + qm2);

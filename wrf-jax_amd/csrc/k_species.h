@@ -112,64 +112,69 @@ __global__ void k_species_pd_prep(const DevCtx xv, SpecList L, float dt, R3 r) {
 }
 
 // ---- positive-definite faces (same arithmetic as k_pd_fluxes) ------------
-// antidiffusive (high - low) and upwind (low) flux of the y face j
-__device__ __forceinline__ void pd_face_y(const DevCtx *__restrict__ x, const float *q, const float *qo, int i, int k, int j, float dt,
-                                          float &anti, float &low) {
-  anti = 0.f;
-  low = 0.f;
-  const int jde = x->jde;
-  if (!(j >= 2 && j <= jde - 1 && i <= x->ide - 1 && k <= x->kde - 1)) return;
-  const float *c1 = FL(c1h), *c2 = FL(c2h), *msftx = FL(msftx), *mut = FL(muts);
-  const float dy = 2.f / (A2(msftx, i, j) + A2(msftx, i, j - 1)) / x->rdy;
-  const float mu = 0.5f * ((A1(c1, k) * A2(mut, i, j) + A1(c2, k)) + (A1(c1, k) * A2(mut, i, j - 1) + A1(c2, k)));
-  const float vel = A3(FL(rv_m), i, k, j);
-  const float cr = vel * dt / dy / mu;
-  low = mu * (dy / dt) * flux_up(A3(qo, i, k, j - 1), A3(qo, i, k, j), cr);
-  float high;
-  if (j >= 4 && j <= jde - 3)
-    high = vel * flux5pd(A3(q, i, k, j - 3), A3(q, i, k, j - 2), A3(q, i, k, j - 1), A3(q, i, k, j), A3(q, i, k, j + 1), A3(q, i, k, j + 2), vel);
-  else if (j == 2 || j == jde - 1)
-    high = 0.5f * vel * (A3(q, i, k, j) + A3(q, i, k, j - 1));
-  else
-    high = vel * flux3pd(A3(q, i, k, j - 2), A3(q, i, k, j - 1), A3(q, i, k, j), A3(q, i, k, j + 1), vel);
-  anti = high - low;
+// Species-independent part of one face: velocity, Courant number, the
+// upwind-flux weight mu*(d/dt) and the order of the high-order flux
+// (5: 5th order, 3: 3rd order, 2: centred, 0: no face).
+struct PdFace {
+  float vel, cr, muf;
+  int mode;
+};
+__device__ __forceinline__ PdFace pd_face_geom(const DevCtx *__restrict__ x, int axis, int i, int k, int j, float dt) {
+  PdFace f{0.f, 0.f, 0.f, 0};
+  const float *c1 = FL(c1h), *c2 = FL(c2h), *mut = FL(muts);
+  const int ide = x->ide, jde = x->jde, ktf = x->kde - 1;
+  if (axis == 0) {
+    if (!(i >= 2 && i <= ide - 1 && j <= jde - 1 && k <= ktf)) return f;
+    const float dx = 2.f / (A2(FL(msfty), i, j) + A2(FL(msfty), i - 1, j)) / x->rdx;
+    const float mu = 0.5f * ((A1(c1, k) * A2(mut, i, j) + A1(c2, k)) + (A1(c1, k) * A2(mut, i - 1, j) + A1(c2, k)));
+    f.vel = A3(FL(ru_m), i, k, j);
+    f.cr = f.vel * dt / dx / mu;
+    f.muf = mu * (dx / dt);
+    f.mode = (i >= 4 && i <= ide - 3) ? 5 : ((i == 2 || i == ide - 1) ? 2 : 3);
+  } else if (axis == 1) {
+    if (!(j >= 2 && j <= jde - 1 && i <= ide - 1 && k <= ktf)) return f;
+    const float dy = 2.f / (A2(FL(msftx), i, j) + A2(FL(msftx), i, j - 1)) / x->rdy;
+    const float mu = 0.5f * ((A1(c1, k) * A2(mut, i, j) + A1(c2, k)) + (A1(c1, k) * A2(mut, i, j - 1) + A1(c2, k)));
+    f.vel = A3(FL(rv_m), i, k, j);
+    f.cr = f.vel * dt / dy / mu;
+    f.muf = mu * (dy / dt);
+    f.mode = (j >= 4 && j <= jde - 3) ? 5 : ((j == 2 || j == jde - 1) ? 2 : 3);
+  } else {
+    if (!(k >= 2 && k <= ktf && i <= ide - 1 && j <= jde - 1)) return f;
+    const float *rdzw = FL(rdnw);
+    const float dz = 2.f / (A1(rdzw, k) + A1(rdzw, k - 1));
+    const float mu = A1(c1, k) * A2(mut, i, j) + A1(c2, k);
+    f.vel = A3(FL(ww_m), i, k, j);
+    f.cr = f.vel * dt / dz / mu;
+    f.muf = mu * (dz / dt);
+    f.mode = (k == 2 || k == ktf) ? 2 : 3;
+  }
+  return f;
 }
-__device__ __forceinline__ void pd_face_x(const DevCtx *__restrict__ x, const float *q, const float *qo, int i, int k, int j, float dt,
-                                          float &anti, float &low) {
-  anti = 0.f;
-  low = 0.f;
-  const int ide = x->ide;
-  if (!(i >= 2 && i <= ide - 1 && j <= x->jde - 1 && k <= x->kde - 1)) return;
-  const float *c1 = FL(c1h), *c2 = FL(c2h), *msfty = FL(msfty), *mut = FL(muts);
-  const float dx = 2.f / (A2(msfty, i, j) + A2(msfty, i - 1, j)) / x->rdx;
-  const float mu = 0.5f * ((A1(c1, k) * A2(mut, i, j) + A1(c2, k)) + (A1(c1, k) * A2(mut, i - 1, j) + A1(c2, k)));
-  const float vel = A3(FL(ru_m), i, k, j);
-  const float cr = vel * dt / dx / mu;
-  low = mu * (dx / dt) * flux_up(A3(qo, i - 1, k, j), A3(qo, i, k, j), cr);
+// antidiffusive and upwind flux of one species through a face
+__device__ __forceinline__ void pd_face_flux(const DevCtx *__restrict__ x, const PdFace &f, int axis, const float *q, const float *qo,
+                                             int i, int k, int j, float &anti, float &low) {
+  if (f.mode == 0) {
+    anti = 0.f;
+    low = 0.f;
+    return;
+  }
+  // unit stride of the face normal in the [j][k][i] layout
+  const int st = axis == 0 ? 1 : (axis == 1 ? x->kde * x->ip : x->ip);
+  const unsigned o = XI3(i, k, j);
+  const float *p = q + o, *po = qo + o;
+  low = f.muf * flux_up(po[-st], po[0], f.cr);
   float high;
-  if (i >= 4 && i <= ide - 3)
-    high = vel * flux5pd(A3(q, i - 3, k, j), A3(q, i - 2, k, j), A3(q, i - 1, k, j), A3(q, i, k, j), A3(q, i + 1, k, j), A3(q, i + 2, k, j), vel);
-  else if (i == 2 || i == ide - 1)
-    high = 0.5f * vel * (A3(q, i, k, j) + A3(q, i - 1, k, j));
-  else
-    high = vel * flux3pd(A3(q, i - 2, k, j), A3(q, i - 1, k, j), A3(q, i, k, j), A3(q, i + 1, k, j), vel);
-  anti = high - low;
-}
-__device__ __forceinline__ void pd_face_z(const DevCtx *__restrict__ x, const float *q, const float *qo, int i, int k, int j, float dt,
-                                          float &anti, float &low) {
-  anti = 0.f;
-  low = 0.f;
-  const int ktf = x->kde - 1;
-  if (!(k >= 2 && k <= ktf && i <= x->ide - 1 && j <= x->jde - 1)) return;
-  const float *c1 = FL(c1h), *c2 = FL(c2h), *rdzw = FL(rdnw), *mut = FL(muts);
-  const float dz = 2.f / (A1(rdzw, k) + A1(rdzw, k - 1));
-  const float mu = A1(c1, k) * A2(mut, i, j) + A1(c2, k);
-  const float vel = A3(FL(ww_m), i, k, j);
-  const float cr = vel * dt / dz / mu;
-  low = mu * (dz / dt) * flux_up(A3(qo, i, k - 1, j), A3(qo, i, k, j), cr);
-  float high;
-  if (k == 2 || k == ktf) high = vel * (A1(FL(fnm), k) * A3(q, i, k, j) + A1(FL(fnp), k) * A3(q, i, k - 1, j));
-  else high = vel * flux3pd(A3(q, i, k - 2, j), A3(q, i, k - 1, j), A3(q, i, k, j), A3(q, i, k + 1, j), -vel);
+  if (axis == 2) {
+    if (f.mode == 2) high = f.vel * (A1(FL(fnm), k) * p[0] + A1(FL(fnp), k) * p[-st]);
+    else high = f.vel * flux3pd(p[-2 * st], p[-st], p[0], p[st], -f.vel);
+  } else if (f.mode == 5) {
+    high = f.vel * flux5pd(p[-3 * st], p[-2 * st], p[-st], p[0], p[st], p[2 * st], f.vel);
+  } else if (f.mode == 2) {
+    high = 0.5f * f.vel * (p[0] + p[-st]);
+  } else {
+    high = f.vel * flux3pd(p[-2 * st], p[-st], p[0], p[st], f.vel);
+  }
   anti = high - low;
 }
 
@@ -178,7 +183,7 @@ __device__ __forceinline__ void pd_face_z(const DevCtx *__restrict__ x, const fl
 __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecList L, float dt, R3 r) {
   const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
-  const size_t o = XI3(i, k, j);
+  const unsigned o = XI3(i, k, j);
   const bool lim = (i >= 2 && i <= x->ide - 2 && j >= 2 && j <= x->jde - 2 && k <= x->kde - 1);
   if (!lim) {
     for (int s = 0; s < L.n; s++) L.scale[s][o] = 1.f;
@@ -188,16 +193,19 @@ __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecL
   const float rdx = x->rdx, rdy = x->rdy;
   const float mm = A2(FL(msftx), i, j) * A2(FL(msfty), i, j), msfty = A2(FL(msfty), i, j);
   const float mold = (A1(c1, k) * A2(FL(mub), i, j) + A1(c2, k)) + (A1(c1, k) * A2(FL(mu_1), i, j));
+  const PdFace fx0 = pd_face_geom(x, 0, i, k, j, dt), fx1 = pd_face_geom(x, 0, i + 1, k, j, dt);
+  const PdFace fy0 = pd_face_geom(x, 1, i, k, j, dt), fy1 = pd_face_geom(x, 1, i, k, j + 1, dt);
+  const PdFace fz0 = pd_face_geom(x, 2, i, k, j, dt), fz1 = pd_face_geom(x, 2, i, k + 1, j, dt);
   for (int s = 0; s < L.n; s++) {
     const float *q = L.in[s], *qo = L.old[s];
     float ax0, lx0, ax1, lx1, ay0, ly0, ay1, ly1, az0, lz0, az1, lz1;
-    pd_face_x(x, q, qo, i, k, j, dt, ax0, lx0);
-    pd_face_x(x, q, qo, i + 1, k, j, dt, ax1, lx1);
-    pd_face_y(x, q, qo, i, k, j, dt, ay0, ly0);
-    pd_face_y(x, q, qo, i, k, j + 1, dt, ay1, ly1);
-    pd_face_z(x, q, qo, i, k, j, dt, az0, lz0);
-    pd_face_z(x, q, qo, i, k + 1, j, dt, az1, lz1);
-    const float ph_low = mold * A3(qo, i, k, j) - dt * (mm * (rdx * (lx1 - lx0) + rdy * (ly1 - ly0)) + msfty * A1(rdzw, k) * (lz1 - lz0));
+    pd_face_flux(x, fx0, 0, q, qo, i, k, j, ax0, lx0);
+    pd_face_flux(x, fx1, 0, q, qo, i + 1, k, j, ax1, lx1);
+    pd_face_flux(x, fy0, 1, q, qo, i, k, j, ay0, ly0);
+    pd_face_flux(x, fy1, 1, q, qo, i, k, j + 1, ay1, ly1);
+    pd_face_flux(x, fz0, 2, q, qo, i, k, j, az0, lz0);
+    pd_face_flux(x, fz1, 2, q, qo, i, k + 1, j, az1, lz1);
+    const float ph_low = mold * qo[o] - dt * (mm * (rdx * (lx1 - lx0) + rdy * (ly1 - ly0)) + msfty * A1(rdzw, k) * (lz1 - lz0));
     const float flux_out = dt * (mm * (rdx * (fmaxf(0.f, ax1) - fminf(0.f, ax0)) + rdy * (fmaxf(0.f, ay1) - fminf(0.f, ay0))) +
                                  msfty * A1(rdzw, k) * (fminf(0.f, az1) - fmaxf(0.f, az0)));
     float sc = 1.f;
@@ -207,46 +215,58 @@ __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecL
 }
 
 // pass 2: tendency from the renormalised faces, then the mass-weighted update
-// (old = q_old after k_species_pd_prep; the cleared physics tendency adds 0).
+// (old = q_old after k_species_pd_prep).
 __global__ void __launch_bounds__(256) k_species_pd_update(const DevCtx xv, SpecList L, float dt, R3 r) {
   const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
-  const size_t o = XI3(i, k, j);
+  const unsigned o = XI3(i, k, j);
   const float rdx = x->rdx, rdy = x->rdy, rdzw = A1(FL(rdnw), k), msftx = A2(FL(msftx), i, j);
   const bool dox = (i >= 2 && i <= x->ide - 2), doy = (j >= 2 && j <= x->jde - 2);
+  const bool up_ok = (k + 1 <= x->kde - 1), dn_ok = (k - 1 >= 1);
   const UpdCoef c = upd_coef(x, i, k, j);
+  const PdFace fz0 = pd_face_geom(x, 2, i, k, j, dt), fz1 = pd_face_geom(x, 2, i, k + 1, j, dt);
+  PdFace fx0{0.f, 0.f, 0.f, 0}, fx1 = fx0, fy0 = fx0, fy1 = fx0;
+  if (dox) {
+    fx0 = pd_face_geom(x, 0, i, k, j, dt);
+    fx1 = pd_face_geom(x, 0, i + 1, k, j, dt);
+  }
+  if (doy) {
+    fy0 = pd_face_geom(x, 1, i, k, j, dt);
+    fy1 = pd_face_geom(x, 1, i, k, j + 1, dt);
+  }
+  const int sj = x->kde * x->ip, sk = x->ip;
   for (int s = 0; s < L.n; s++) {
-    const float *q = L.in[s], *qo = L.old[s], *sc = L.scale[s];
+    const float *q = L.in[s], *qo = L.old[s], *sc = L.scale[s] + o;
     float az0, lz0, az1, lz1;
-    pd_face_z(x, q, qo, i, k, j, dt, az0, lz0);
-    pd_face_z(x, q, qo, i, k + 1, j, dt, az1, lz1);
+    pd_face_flux(x, fz0, 2, q, qo, i, k, j, az0, lz0);
+    pd_face_flux(x, fz1, 2, q, qo, i, k + 1, j, az1, lz1);
     float zt = az1, zb = az0;
-    if (zt < 0.f) zt *= A3(sc, i, k, j);
-    else if (zt > 0.f && k + 1 <= x->kde - 1) zt *= A3(sc, i, k + 1, j);
-    if (zb > 0.f) zb *= A3(sc, i, k, j);
-    else if (zb < 0.f && k - 1 >= 1) zb *= A3(sc, i, k - 1, j);
+    if (zt < 0.f) zt *= sc[0];
+    else if (zt > 0.f && up_ok) zt *= sc[sk];
+    if (zb > 0.f) zb *= sc[0];
+    else if (zb < 0.f && dn_ok) zb *= sc[-sk];
     float t = 0.f;
     t += -rdzw * (zt - zb + lz1 - lz0);
     if (dox) {
       float ax0, lx0, ax1, lx1;
-      pd_face_x(x, q, qo, i, k, j, dt, ax0, lx0);
-      pd_face_x(x, q, qo, i + 1, k, j, dt, ax1, lx1);
+      pd_face_flux(x, fx0, 0, q, qo, i, k, j, ax0, lx0);
+      pd_face_flux(x, fx1, 0, q, qo, i + 1, k, j, ax1, lx1);
       float xe = ax1, xw = ax0;
-      if (xe > 0.f) xe *= A3(sc, i, k, j);
-      else if (xe < 0.f) xe *= A3(sc, i + 1, k, j);
-      if (xw < 0.f) xw *= A3(sc, i, k, j);
-      else if (xw > 0.f) xw *= A3(sc, i - 1, k, j);
+      if (xe > 0.f) xe *= sc[0];
+      else if (xe < 0.f) xe *= sc[1];
+      if (xw < 0.f) xw *= sc[0];
+      else if (xw > 0.f) xw *= sc[-1];
       t += -msftx * (rdx * (xe - xw + lx1 - lx0));
     }
     if (doy) {
       float ay0, ly0, ay1, ly1;
-      pd_face_y(x, q, qo, i, k, j, dt, ay0, ly0);
-      pd_face_y(x, q, qo, i, k, j + 1, dt, ay1, ly1);
+      pd_face_flux(x, fy0, 1, q, qo, i, k, j, ay0, ly0);
+      pd_face_flux(x, fy1, 1, q, qo, i, k, j + 1, ay1, ly1);
       float yn = ay1, ys = ay0;
-      if (yn > 0.f) yn *= A3(sc, i, k, j);
-      else if (yn < 0.f) yn *= A3(sc, i, k, j + 1);
-      if (ys < 0.f) ys *= A3(sc, i, k, j);
-      else if (ys > 0.f) ys *= A3(sc, i, k, j - 1);
+      if (yn > 0.f) yn *= sc[0];
+      else if (yn < 0.f) yn *= sc[sj];
+      if (ys < 0.f) ys *= sc[0];
+      else if (ys > 0.f) ys *= sc[-sj];
       t += -msftx * (rdy * (yn - ys + ly1 - ly0));
     }
     float tend = c.inner ? t * c.msfty : 0.f;

@@ -548,14 +548,19 @@ struct Run {
     d.F[F_moist] = alt ? alt_moist : user_moist;
     d.F[F_scalar] = (X->c.nscalar > 0) ? (alt ? alt_scalar : user_scalar) : nullptr;
   }
-  // copy the current species into the other buffer and switch to it
+  // copy the current species (every point a stage writes: i <= ide,
+  // k <= kde-1, j <= jde) into the other buffer and switch to it
   void species_to(bool alt) const {
     if (species_in_alt() == alt) return;
-    const size_t n = X->n3;
-    float *dm = alt ? alt_moist : user_moist, *ds = alt ? alt_scalar : user_scalar;
-    fill_copy(dm, d.F[F_moist], n * (size_t)X->c.nmoist);
-    if (X->c.nscalar > 0) fill_copy(ds, d.F[F_scalar], n * (size_t)X->c.nscalar);
+    copy_species(R3{1, ide, 1, kde - 1, 1, jde}, alt);
     use_species(alt);
+  }
+  void copy_species(R3 r, bool to_alt) const {
+    const size_t n = X->n3;
+    const float *sm = to_alt ? user_moist : alt_moist, *ss = to_alt ? user_scalar : alt_scalar;
+    float *dm = to_alt ? alt_moist : user_moist, *ds = to_alt ? alt_scalar : user_scalar;
+    for (int q = 0; q < X->c.nmoist; q++) k_copy_mass<<<g3(r), b3(), 0, s>>>(d, dm + q * n, sm + q * n, r);
+    for (int q = 0; q < X->c.nscalar; q++) k_copy_mass<<<g3(r), b3(), 0, s>>>(d, ds + q * n, ss + q * n, r);
   }
   void fill_copy(float *dst, const float *src, size_t n) const {
     unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
@@ -619,6 +624,12 @@ struct Run {
     if (plain.n > 0) k_species_update<<<g3(m), b3(), 0, s>>>(d, plain, rk, dt, m);
     // outputs become the current species
     use_species(out_alt);
+    if (rk == 3) {
+      // the physical-boundary ring (column ide, row jde) is last written by
+      // the second stage's copies, which went to the other buffer
+      copy_species(R3{ide, ide, 1, kde - 1, 1, jde}, false);
+      copy_species(R3{1, ide, 1, kde - 1, jde, jde}, false);
+    }
     if (nm > 1) {
       FlowList fl;
       memset(&fl, 0, sizeof(fl));
@@ -736,6 +747,7 @@ int set_err(wrfdc_ctx *x, int code, const std::string &msg) {
 int validate(const wrfdc_config &c, std::string &msg) {
   if (c.nx < 12 || c.ny < 12 || c.nz < 6) { msg = "domain too small: need nx, ny >= 12 and nz >= 6"; return WRFDC_EINVAL; }
   if (c.ip < c.nx + 1) { msg = "ip must be >= nx + 1"; return WRFDC_EINVAL; }
+  if ((double)(c.ny + 1) * (c.nz + 1) * c.ip >= 2147483648.0) { msg = "a 3-D field must hold fewer than 2^31 elements"; return WRFDC_EINVAL; }
   if (c.nmoist < 1) { msg = "nmoist must be >= 1 (species 0 is QV)"; return WRFDC_EINVAL; }
   if (c.nscalar < 0) { msg = "nscalar must be >= 0"; return WRFDC_EINVAL; }
   if (c.nmoist + c.nscalar > kMaxSpec) { msg = "at most 16 advected species (nmoist + nscalar)"; return WRFDC_EINVAL; }

@@ -73,15 +73,18 @@ struct DevCtx {
   float *S[NSCRATCH];
 };
 
-#define XI3(i, k, j) ((((size_t)(j) - 1) * (size_t)x->kde + ((size_t)(k) - 1)) * (size_t)x->ip + ((size_t)(i) - 1))
-#define XI2(i, j) (((size_t)(j) - 1) * (size_t)x->ip + ((size_t)(i) - 1))
+// Element offsets in 32-bit arithmetic (a 3-D field holds < 2^31 elements;
+// wrfdc_create checks), added to 64-bit base pointers held in scalar
+// registers: the loads use the saddr + voffset form.
+#define XI3(i, k, j) ((unsigned)((((j) - 1) * x->kde + ((k) - 1)) * x->ip + ((i) - 1)))
+#define XI2(i, j) ((unsigned)(((j) - 1) * x->ip + ((i) - 1)))
 #define A3(fl, i, k, j) ((fl)[XI3(i, k, j)])
 #define A2(fl, i, j) ((fl)[XI2(i, j)])
 #define A1(fl, k) ((fl)[(k) - 1])
-#define BX(fl, j, k, b) ((fl)[(((size_t)(b) - 1) * (size_t)x->kde + ((size_t)(k) - 1)) * (size_t)x->jde + ((size_t)(j) - 1)])
-#define BY(fl, i, k, b) ((fl)[(((size_t)(b) - 1) * (size_t)x->kde + ((size_t)(k) - 1)) * (size_t)x->ip + ((size_t)(i) - 1)])
-#define BX2(fl, j, b) ((fl)[((size_t)(b) - 1) * (size_t)x->jde + ((size_t)(j) - 1)])
-#define BY2(fl, i, b) ((fl)[((size_t)(b) - 1) * (size_t)x->ip + ((size_t)(i) - 1)])
+#define BX(fl, j, k, b) ((fl)[(unsigned)((((b) - 1) * x->kde + ((k) - 1)) * x->jde + ((j) - 1))])
+#define BY(fl, i, k, b) ((fl)[(unsigned)((((b) - 1) * x->kde + ((k) - 1)) * x->ip + ((i) - 1))])
+#define BX2(fl, j, b) ((fl)[(unsigned)(((b) - 1) * x->jde + ((j) - 1))])
+#define BY2(fl, i, b) ((fl)[(unsigned)(((b) - 1) * x->ip + ((i) - 1))])
 #define FL(name) (x->F[F_##name])
 #define SP(name, s) (x->F[F_##name] + (size_t)(s) * x->n3)
 #define SCR(id) (x->S[id])
@@ -107,5 +110,15 @@ struct R2 {
   if (i > (r).i1 || j > (r).j1) return
 
 __device__ __forceinline__ float sgn1(float v) { return v >= 0.f ? 1.f : -1.f; }
+// x / c correctly rounded, for a constant c with rc = RN(1/c): one FMA
+// correction of x * rc is the IEEE quotient whenever the quotient is a
+// normal number (checked exhaustively over all FP32 x for c = 12 and 60);
+// subnormal quotients take the division itself.
+__device__ __forceinline__ float div_const(float xv, float c, float rc) {
+  const float q = xv * rc;
+  const float e = __builtin_fmaf(-q, c, xv);
+  const float q2 = __builtin_fmaf(e, rc, q);
+  return (fabsf(q2) < 2.4e-38f) ? xv / c : q2;
+}
 __device__ __forceinline__ int imin_(int a, int b) { return a < b ? a : b; }
 __device__ __forceinline__ int imax_(int a, int b) { return a > b ? a : b; }
