@@ -97,10 +97,10 @@ __global__ void k_zero_grad_w(const DevCtx xv) {
 
 // ---------------------------------------------------------------------------
 constexpr int kColWaves = 8;  // waves per block of k_acoustic_col
-constexpr int kColGroup = 4;  // levels per register group
+constexpr int kColGroup = 2;  // levels per register group
 
 // bytes of dynamic LDS of k_acoustic_col for kde levels
-__host__ __device__ inline size_t acoustic_col_lds(int kde) { return (size_t)4 * (kde + 1) * 64 * sizeof(float); }
+__host__ __device__ inline size_t acoustic_col_lds(int kde) { return (size_t)3 * (kde + 1) * 64 * sizeof(float); }
 // levels per wave the kernel is instantiated for (kde <= kColWaves * NPW)
 inline int acoustic_col_npw(int kde) {
   const int need = (kde + kColWaves - 1) / kColWaves;
@@ -117,7 +117,9 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
   const int i = (int)blockIdx.x * 64 + lane + 1, j = (int)blockIdx.y + 1;
   const int ide = x->ide, jde = x->jde, kde = x->kde, kend = kde - 1;
   const int S = (kde + 1) * 64;
-  float *B0 = sm, *B1 = sm + S, *B2 = sm + 2 * S, *B3 = sm + 3 * S;
+  // B0: divergence, then the phi right-hand side; B1: omega, then w (the
+  // tridiagonal unknowns); B2: the acoustic theta average, then final phi
+  float *B0 = sm, *B1 = sm + S, *B2 = sm + 2 * S, *B3 = B1;
 #define LS(buf, kk) (buf)[(kk) * 64 + lane]
 #define LEV(n) (1 + wv + NW * (n))
   const bool mass = (i <= ide - 1 && j <= jde - 1);
@@ -378,34 +380,20 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
     }
   }
   __syncthreads();
-  // ---- 7. LU factors of the forward sweep -> B1 (a), B2 (alpha)
-  if (interior) {
-    const float *a = FL(a), *alpha = FL(alpha);
-    for (int k = 2 + wv; k <= kde; k += NW) {
-      LS(B1, k) = A3(a, i, k, j);
-      LS(B2, k) = A3(alpha, i, k, j);
-    }
-  }
-  __syncthreads();
+  // ---- 7. tridiagonal sweeps (wave 0, one column per lane); the LU factors
+  // come from global memory, their loads are independent of the recurrence
   if (wv == 0 && interior) {
+    const float *a = FL(a), *alpha = FL(alpha), *gamma = FL(gamma);
     float wprev = LS(B3, 1);
+#pragma unroll 8
     for (int k = 2; k <= kde; k++) {
-      const float wl = (LS(B3, k) - LS(B1, k) * wprev) * LS(B2, k);
+      const float wl = (LS(B3, k) - A3(a, i, k, j) * wprev) * A3(alpha, i, k, j);
       LS(B3, k) = wl;
       wprev = wl;
     }
-  }
-  __syncthreads();
-  // ---- 8. back substitution factors -> B1 (gamma), then the backward sweep
-  if (interior) {
-    const float *gamma = FL(gamma);
-    for (int k = 2 + wv; k <= kend; k += NW) LS(B1, k) = A3(gamma, i, k, j);
-  }
-  __syncthreads();
-  if (wv == 0 && interior) {
-    float wprev = LS(B3, kde);
+#pragma unroll 8
     for (int k = kend; k >= 2; k--) {
-      const float wl = LS(B3, k) - LS(B1, k) * wprev;
+      const float wl = LS(B3, k) - A3(gamma, i, k, j) * wprev;
       LS(B3, k) = wl;
       wprev = wl;
     }
