@@ -36,6 +36,9 @@ def parse():
     ap.add_argument("--workload", default="c1", choices=["c1", "c0"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--traffic-calibration", action="store_true",
+                    help="after the run, one 256 MiB torch copy (known HBM traffic) for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only launch the dominant kernel --kernel-reps times after one step, then a 256 MiB "
                          "calibration copy (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs)")
@@ -155,6 +158,13 @@ def main():
         "checksums": ck.tolist(),
     }
     dom.close()
+    if args.traffic_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(case)

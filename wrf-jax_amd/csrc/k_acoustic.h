@@ -382,20 +382,74 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
   __syncthreads();
   // ---- 7. tridiagonal sweeps (wave 0, one column per lane); the LU factors
   // come from global memory, their loads are independent of the recurrence
+  // (chunks of C levels: the factors of chunk c+1 are requested before the
+  // recurrence of chunk c runs, so their latency overlaps it)
   if (wv == 0 && interior) {
+    constexpr int C = 8;
     const float *a = FL(a), *alpha = FL(alpha), *gamma = FL(gamma);
+    const int sk = x->ip;  // element stride between levels
+    const unsigned o1 = XI3(i, 1, j);
     float wprev = LS(B3, 1);
-#pragma unroll 8
-    for (int k = 2; k <= kde; k++) {
-      const float wl = (LS(B3, k) - A3(a, i, k, j) * wprev) * A3(alpha, i, k, j);
-      LS(B3, k) = wl;
-      wprev = wl;
+    // forward: levels 2..kde
+    {
+      const int n = kde - 1;
+      float ca[C], cl[C], na[C], nl[C];
+#pragma unroll
+      for (int q = 0; q < C; q++) {
+        const int k = imin_(2 + q, kde);
+        ca[q] = a[o1 + (unsigned)((k - 1) * sk)];
+        cl[q] = alpha[o1 + (unsigned)((k - 1) * sk)];
+      }
+      for (int c0 = 0; c0 < n; c0 += C) {
+#pragma unroll
+        for (int q = 0; q < C; q++) {
+          const int k = imin_(2 + c0 + C + q, kde);
+          na[q] = a[o1 + (unsigned)((k - 1) * sk)];
+          nl[q] = alpha[o1 + (unsigned)((k - 1) * sk)];
+        }
+#pragma unroll
+        for (int q = 0; q < C; q++) {
+          const int k = 2 + c0 + q;
+          if (k <= kde) {
+            const float wl = (LS(B3, k) - ca[q] * wprev) * cl[q];
+            LS(B3, k) = wl;
+            wprev = wl;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < C; q++) {
+          ca[q] = na[q];
+          cl[q] = nl[q];
+        }
+      }
     }
-#pragma unroll 8
-    for (int k = kend; k >= 2; k--) {
-      const float wl = LS(B3, k) - A3(gamma, i, k, j) * wprev;
-      LS(B3, k) = wl;
-      wprev = wl;
+    // backward: levels kend..2
+    {
+      const int n = kend - 1;
+      float cg[C], ng[C];
+#pragma unroll
+      for (int q = 0; q < C; q++) {
+        const int k = imax_(kend - q, 2);
+        cg[q] = gamma[o1 + (unsigned)((k - 1) * sk)];
+      }
+      for (int c0 = 0; c0 < n; c0 += C) {
+#pragma unroll
+        for (int q = 0; q < C; q++) {
+          const int k = imax_(kend - c0 - C - q, 2);
+          ng[q] = gamma[o1 + (unsigned)((k - 1) * sk)];
+        }
+#pragma unroll
+        for (int q = 0; q < C; q++) {
+          const int k = kend - c0 - q;
+          if (k >= 2) {
+            const float wl = LS(B3, k) - cg[q] * wprev;
+            LS(B3, k) = wl;
+            wprev = wl;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < C; q++) cg[q] = ng[q];
+      }
     }
   }
   __syncthreads();

@@ -114,11 +114,15 @@ __device__ __forceinline__ float sgn1(float v) { return v >= 0.f ? 1.f : -1.f; }
 // correction of x * rc is the IEEE quotient whenever the quotient is a
 // normal number (checked exhaustively over all FP32 x for c = 12 and 60);
 // subnormal quotients take the division itself.
+// The division of the rare case sits in a function that is not inlined, so
+// the compiler cannot hoist it into the common path.
+__device__ __attribute__((noinline)) float div_rare(float xv, float c) { return xv / c; }
 __device__ __forceinline__ float div_const(float xv, float c, float rc) {
   const float q = xv * rc;
   const float e = __builtin_fmaf(-q, c, xv);
-  const float q2 = __builtin_fmaf(e, rc, q);
-  return (fabsf(q2) < 2.4e-38f) ? xv / c : q2;
+  float q2 = __builtin_fmaf(e, rc, q);
+  if (__builtin_expect(fabsf(q2) < 2.4e-38f, 0)) q2 = div_rare(xv, c);
+  return q2;
 }
 __device__ __forceinline__ int imin_(int a, int b) { return a < b ? a : b; }
 __device__ __forceinline__ int imax_(int a, int b) { return a > b ? a : b; }
