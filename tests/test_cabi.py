@@ -72,6 +72,7 @@ def _create(lib, **over):
         (dict(spec_zone=2, spec_bdy_width=6), _lib.WRFDC_ENOTIMPL),
         (dict(top_lid=1), _lib.WRFDC_ENOTIMPL),
         (dict(spec_bdy_width=6), _lib.WRFDC_EINVAL),
+        (dict(nx=4095, ny=2047, nz=127, ip=4096), _lib.WRFDC_EINVAL),  # 2^30 elements per 3-D field
     ],
 )
 def test_create_rejects_bad_configs(lib, over, code):
@@ -94,6 +95,9 @@ def test_error_mapping_uses_reference_exception_types(lib):
 def test_python_validation_matches_library():
     with pytest.raises(ValueError):
         DycoreConfig(nx=8, ny=20, nz=10, ip=16, nmoist=1, nscalar=0).validate()
+    with pytest.raises(ValueError):
+        DycoreConfig(nx=4095, ny=2047, nz=127, ip=4096, nmoist=1, nscalar=0).validate()
+    DycoreConfig(nx=4095, ny=2047, nz=126, ip=4096, nmoist=1, nscalar=0).validate()
     with pytest.raises(ValueError):
         DycoreConfig(nx=24, ny=20, nz=10, ip=32, nmoist=1, nscalar=0, options={"rk_ord": 2}).validate()
     with pytest.raises(NotImplementedError):

@@ -114,7 +114,8 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
   extern __shared__ float sm[];
   __shared__ float c_mts[64], c_mav[64];
   const int lane = (int)threadIdx.x, wv = (int)threadIdx.y;
-  const int i = (int)blockIdx.x * 64 + lane + 1, j = (int)blockIdx.y + 1;
+  const unsigned gx = (unsigned)(x->ide + 63) / 64u, t = xcd_tile(blockIdx.x, gridDim.x);
+  const int i = (int)(t % gx) * 64 + lane + 1, j = (int)(t / gx) + 1;
   const int ide = x->ide, jde = x->jde, kde = x->kde, kend = kde - 1;
   const int S = (kde + 1) * 64;
   // B0: divergence, then the phi right-hand side; B1: omega, then w (the
@@ -134,7 +135,9 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
   const int ic = mass ? i : 1, jc = mass ? j : 1;  // clamped column for address arithmetic
   const float msfty = A2(FL(msfty), ic, jc), msftx = A2(FL(msftx), ic, jc);
 
-  // ---- 1. advance_mu_t: horizontal mass divergence per level -> B0
+  float *ru_m = FL(ru_m), *rv_m = FL(rv_m), *ww_m = FL(ww_m);
+  // ---- 1. advance_mu_t: horizontal mass divergence per level -> B0; with
+  // the same loads, sumflux of the column's u and v points (levels 1..kde-1)
   if (interior) {
     const float *us = FL(u_save), *vs = FL(v_save), *muu = FL(muu), *muv = FL(muv);
     const float muu0 = A2(muu, i, j), muu1 = A2(muu, i + 1, j), muv0 = A2(muv, i, j), muv1 = A2(muv, i, j + 1);
@@ -143,10 +146,12 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
     const float mm = msftx * msfty;
 #pragma unroll
     for (int gq = 0; gq < NPW; gq += G) {
-      float r_ue[G], r_uw[G], r_use[G], r_usw[G], r_vn[G], r_vs[G], r_vsn[G], r_vss[G];
+      float r_ue[G], r_uw[G], r_use[G], r_usw[G], r_vn[G], r_vs[G], r_vsn[G], r_vss[G], r_rum[G], r_rvm[G];
 #pragma unroll
       for (int q = 0; q < G; q++) {
         const int k = imin_(LEV(gq + q), kend);
+        r_rum[q] = first ? 0.f : A3(ru_m, i, k, j);
+        r_rvm[q] = first ? 0.f : A3(rv_m, i, k, j);
         r_ue[q] = A3(u, i + 1, k, j);
         r_uw[q] = A3(u, i, k, j);
         r_use[q] = A3(us, i + 1, k, j);
@@ -167,6 +172,12 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
         const float uw = r_uw[q] + ((c1 * muu0 + c2) * r_usw[q]) / iuy0;
         const float hf = x->rdy * (vn - vsv) + x->rdx * (ue - uw);
         LS(B0, k) = mm * hf;
+        float au = r_rum[q] + r_uw[q];
+        if (fin) au = au / ns + ((c1 * muu0 + c2) * r_usw[q]) / iuy0;
+        A3(ru_m, i, k, j) = au;
+        float av = r_rvm[q] + r_vs[q];
+        if (fin) av = av / ns + ((c1 * muv0 + c2) * r_vss[q]) * vx0;
+        A3(rv_m, i, k, j) = av;
       }
     }
   }
@@ -206,14 +217,19 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
     c_mts[lane] = mts;
   }
   __syncthreads();
-  // ---- 3. omega minus the stage-start omega (B1 = final omega, k = 1..kde)
+  // ---- 3. omega minus the stage-start omega (B1 = final omega, k = 1..kde),
+  // and sumflux of the column's w points
   if (interior) {
     const float *ww_1 = FL(ww_1);
 #pragma unroll
     for (int gq = 0; gq < NPW; gq += G) {
-      float r[G];
+      float r[G], r_wwm[G];
 #pragma unroll
-      for (int q = 0; q < G; q++) r[q] = A3(ww_1, i, imin_(LEV(gq + q), kend), j);
+      for (int q = 0; q < G; q++) {
+        const int k = imin_(LEV(gq + q), kend);
+        r[q] = A3(ww_1, i, k, j);
+        r_wwm[q] = first ? 0.f : A3(ww_m, i, k, j);
+      }
 #pragma unroll
       for (int q = 0; q < G; q++) {
         const int k = LEV(gq + q);
@@ -221,12 +237,24 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
         const float vv = LS(B1, k) - r[q];
         A3(ww, i, k, j) = vv;
         LS(B1, k) = vv;
+        float aw = r_wwm[q] + vv;
+        if (fin) aw = aw / ns + r[q];
+        A3(ww_m, i, k, j) = aw;
       }
     }
-    if (wv == NW - 1) LS(B1, kde) = A3(ww, i, kde, j);
+    if (wv == NW - 1) {
+      const float wtop = A3(ww, i, kde, j);
+      LS(B1, kde) = wtop;
+      float aw = (first ? 0.f : A3(ww_m, i, kde, j)) + wtop;
+      if (fin) aw = aw / ns + A3(ww_1, i, kde, j);
+      A3(ww_m, i, kde, j) = aw;
+    }
   }
   __syncthreads();
-  // ---- 4. theta (advance_mu_t) and its acoustic average (advance_w) -> B2
+  // ---- 4. theta (advance_mu_t) and its acoustic average (advance_w) -> B2;
+  // the stage-start and new theta of each level stay in registers for
+  // calc_p_rho (phase 10 handles the same levels in the same wave)
+  float c_t1[NPW], c_t2[NPW];
   if (interior) {
     const float mts = c_mts[lane], mav = c_mav[lane];
     const float *ft = FL(t_tend);
@@ -269,6 +297,8 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
         const float vert = A1(rdnw, k) * (whi - wlo);
         const float tnew = ta - (dts * msfty) * (horiz + vert);
         A3(t, i, k, j) = tnew;
+        c_t1[gq + q] = r_t1[q];
+        c_t2[gq + q] = tnew;
         float tav = 0.5f * (ope * tnew + ome * tk);
         tav = (tav + (A1(c1h, k) * mav) * t0) / ((A1(c1h, k) * mts + A1(c2h, k)) * (t0 + r_t1[q]));
         A3(tav_out, i, k, j) = tav;
@@ -497,10 +527,15 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
       }
     }
   }
-  // ---- sumflux over the column's u, v and w points (the first iteration
-  // clears the whole arrays, as the reference's fresh accumulators)
-  if (i <= ide && j <= jde) {
-    float *ru_m = FL(ru_m), *rv_m = FL(rv_m), *ww_m = FL(ww_m);
+  // ---- sumflux over the u, v and w points of the columns phases 1 and 3
+  // did not cover (the first iteration clears the whole arrays, as the
+  // reference's fresh accumulators)
+  if (interior) {
+    if (first && wv == 0) {
+      A3(ru_m, i, kde, j) = 0.f;
+      A3(rv_m, i, kde, j) = 0.f;
+    }
+  } else if (i <= ide && j <= jde) {
     const bool upt = (j <= jde - 1), vpt = (i <= ide - 1);
     const float cmu_u = upt ? A2(FL(muu), i, j) : 0.f, cmsf_u = upt ? A2(FL(msfuy), i, j) : 1.f;
     const float cmu_v = vpt ? A2(FL(muv), i, j) : 0.f, cmsf_v = vpt ? A2(FL(msfvx_inv), i, j) : 1.f;
@@ -564,8 +599,8 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
       for (int q = 0; q < G; q++) {
         const size_t o = XI3(i, imin_(LEV(gq + q), kend), j);
         r_alt[q] = alt[o];
-        r_t1[q] = t_1[o];
-        r_t2[q] = t_2[o];
+        r_t1[q] = interior ? c_t1[gq + q] : t_1[o];
+        r_t2[q] = interior ? c_t2[gq + q] : t_2[o];
         r_c2a[q] = c2a[o];
         r_pm1[q] = pm1[o];
       }
@@ -625,7 +660,7 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
 
 // host launcher: instantiation by levels per wave
 inline void launch_acoustic_col(const DevCtx &d, hipStream_t s, float dts, int first, int fin, float ns) {
-  const dim3 grid((unsigned)((d.ide + 63) / 64), (unsigned)d.jde), block(64, kColWaves);
+  const dim3 grid((unsigned)((d.ide + 63) / 64) * (unsigned)d.jde), block(64, kColWaves);
   const size_t lds = acoustic_col_lds(d.kde);
   switch (acoustic_col_npw(d.kde)) {
     case 4: k_acoustic_col<4><<<grid, block, lds, s>>>(d, dts, first, fin, ns); break;

@@ -25,6 +25,48 @@ __device__ __forceinline__ float flux_up(float qm1, float q0, float cr) {
   return 0.5f * fminf(1.f, cr + fabsf(cr)) * qm1 + 0.5f * fmaxf(-1.f, cr - fabsf(cr)) * q0;
 }
 
+// ---------------- face-sharing blocks ---------------------------------------
+// value of v in the next lane of the wave (lane 63 receives lane 0's)
+__device__ __forceinline__ float lane_next(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(((lane + 1) & 63) << 2, __float_as_int(v)));
+}
+// Blocks of the flux-form kernels: 64 lanes along i, of which the first 63
+// own a point (lane 63 only forms the west face its left neighbour needs as
+// east face), 4 waves on 4 consecutive rows j0..j0+3 (each wave forms the
+// south face of its row, the row below reads it from LDS as north face; the
+// top wave also forms its own north face), one level k.  Each x face flux is
+// evaluated once and each y face 5/4 times instead of twice; every thread
+// still owns one point, so the grid keeps one wave per 63 points.
+// Tile order: i fastest, then 4 consecutive levels, then rows, then groups
+// of 4 levels, XCD-contiguous: the neighbour rows of a y stencil and the
+// neighbour levels of a z stencil are read by the same XCD shortly before.
+struct FaceTiles {
+  unsigned ni, nj, nkc;  // tiles along i, row groups, level groups of 4
+  __host__ __device__ unsigned count() const { return ni * nj * nkc * 4u; }
+};
+__host__ __device__ inline FaceTiles face_tiles(const R3 &r) {
+  return FaceTiles{(unsigned)(r.i1 - r.i0 + 63) / 63u, (unsigned)(r.j1 - r.j0 + 4) / 4u, (unsigned)(r.k1 - r.k0 + 4) / 4u};
+}
+// thread -> lane, wave (row in the block), column i, level k, row j and the
+// block's first row j0; own: the lane holds a point of r; ic: i clamped into
+// r (in-range loads for every lane).  Blocks past the last level return as a
+// whole; waves past the last row (jrow false) still take part in the block's
+// barrier and form their south face.
+#define GET_FACE_BLOCK(r)                                                                \
+  const FaceTiles T_ = face_tiles(r);                                                    \
+  const unsigned t_ = xcd_tile(blockIdx.x, gridDim.x);                                   \
+  const int lane = (int)threadIdx.x, wv = (int)threadIdx.y;                              \
+  const unsigned u_ = t_ / T_.ni;                                                        \
+  const int i = (r).i0 + (int)(t_ % T_.ni) * 63 + lane;                                  \
+  const int k = (r).k0 + (int)((u_ / 4u) / T_.nj) * 4 + (int)(u_ % 4u);                  \
+  const int j0 = (r).j0 + (int)((u_ / 4u) % T_.nj) * 4;                                  \
+  const int j = j0 + wv;                                                                 \
+  if (k > (r).k1) return; /* whole block */                                              \
+  const bool jrow = (j <= (r).j1);                                                       \
+  const bool own = (lane < 63 && i <= (r).i1 && jrow);                                   \
+  const int ic = i <= (r).i1 ? i : (r).i1;                                               \
+  const int jc = jrow ? j : (r).j1
+
 // along-j / along-i stencils of a field at fixed (i,k) / (k,j)
 #define QJ(q, i, k, j) A3(q, i, k, j)
 #define QI(q, i, k, j) A3(q, i, k, j)
@@ -56,16 +98,29 @@ __device__ __forceinline__ float fz_s(const DevCtx *__restrict__ x, const float 
 }
 
 // advect_scalar (module_advect_em.py:90-369): tend -= div(flux), mass points
-__global__ void k_advect_scalar(const DevCtx xv, const float *q, float *tend, const float *ru, const float *rv,
-                                const float *rom, R3 r) {
+// (face-sharing blocks; r = mass points; y faces j in [2, jde-1])
+__global__ void __launch_bounds__(256) k_advect_scalar(const DevCtx xv, const float *q, float *tend, const float *ru, const float *rv,
+                                                       const float *rom, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
+  __shared__ float ys[4][64];
+  GET_FACE_BLOCK(r);
   const float *msftx = FL(msftx), *rdzw = FL(rdnw);
-  float t = A3(tend, i, k, j);
-  if (j >= 2 && j <= x->jde - 2) t -= A2(msftx, i, j) * x->rdy * (fy_s(x, q, rv, i, k, j + 1) - fy_s(x, q, rv, i, k, j));
-  if (i >= 2 && i <= x->ide - 2) t -= A2(msftx, i, j) * x->rdx * (fx_s(x, q, ru, i + 1, k, j) - fx_s(x, q, ru, i, k, j));
-  t -= A1(rdzw, k) * (fz_s(x, q, rom, i, k + 1, j) - fz_s(x, q, rom, i, k, j));
-  A3(tend, i, k, j) = t;
+  const int ide = x->ide, jde = x->jde;
+  const float fs = (j >= 2 && j <= jde - 1) ? fy_s(x, q, rv, ic, k, j) : 0.f;
+  ys[wv][lane] = fs;
+  __syncthreads();
+  if (!jrow) return;
+  float fn;
+  if (wv < 3) fn = ys[wv + 1][lane];
+  else fn = (j + 1 <= jde - 1) ? fy_s(x, q, rv, ic, k, j + 1) : 0.f;
+  const float fw = (i >= 2 && i <= ide - 1) ? fx_s(x, q, ru, i, k, j) : 0.f;
+  const float fe = lane_next(fw, lane);
+  if (!own) return;
+  float t = A3(tend, ic, k, j);
+  if (j >= 2 && j <= jde - 2) t -= A2(msftx, ic, j) * x->rdy * (fn - fs);
+  if (ic >= 2 && ic <= ide - 2) t -= A2(msftx, ic, j) * x->rdx * (fe - fw);
+  t -= A1(rdzw, k) * (fz_s(x, q, rom, ic, k + 1, j) - fz_s(x, q, rom, ic, k, j));
+  A3(tend, ic, k, j) = t;
 }
 
 // ---------------- u momentum (u points i in [2, ide-1]) ---------------------
@@ -108,18 +163,30 @@ __device__ __forceinline__ float fz_u(const DevCtx *__restrict__ x, const float 
   return vel * flux3(A3(u, i, k - 2, j), A3(u, i, k - 1, j), A3(u, i, k, j), A3(u, i, k + 1, j), -vel);
 }
 // advect_u (module_advect_em.py:762-1034); r covers i [2, ide-1], j [1, jde-1]
-__global__ void k_advect_u(const DevCtx xv, const float *u, float *tend, const float *ru, const float *rv,
-                           const float *rom, R3 r) {
+// (face-sharing blocks: x faces i in [2, ide], y faces j in [2, jde-1])
+__global__ void __launch_bounds__(256) k_advect_u(const DevCtx xv, const float *u, float *tend, const float *ru, const float *rv,
+                                                  const float *rom, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
+  __shared__ float ys[4][64];
+  GET_FACE_BLOCK(r);
   const float *msfux = FL(msfux), *rdzw = FL(rdnw);
-  float t = A3(tend, i, k, j);
-  if (j >= 2 && j <= x->jde - 2) t -= A2(msfux, i, j) * x->rdy * (fy_u(x, u, rv, i, k, j + 1) - fy_u(x, u, rv, i, k, j));
-  t -= A2(msfux, i, j) * x->rdx * (fx_u(x, u, ru, i + 1, k, j) - fx_u(x, u, ru, i, k, j));
-  t -= A1(rdzw, k) * (fz_u(x, u, rom, i, k + 1, j) - fz_u(x, u, rom, i, k, j));
-  A3(tend, i, k, j) = t;
+  const int ide = x->ide, jde = x->jde;
+  const float fs = (j >= 2 && j <= jde - 1) ? fy_u(x, u, rv, ic, k, j) : 0.f;
+  ys[wv][lane] = fs;
+  __syncthreads();
+  if (!jrow) return;
+  float fn;
+  if (wv < 3) fn = ys[wv + 1][lane];
+  else fn = (j + 1 <= jde - 1) ? fy_u(x, u, rv, ic, k, j + 1) : 0.f;
+  const float fw = (i >= 2 && i <= ide) ? fx_u(x, u, ru, i, k, j) : 0.f;
+  const float fe = lane_next(fw, lane);
+  if (!own) return;
+  float t = A3(tend, ic, k, j);
+  if (j >= 2 && j <= jde - 2) t -= A2(msfux, ic, j) * x->rdy * (fn - fs);
+  t -= A2(msfux, ic, j) * x->rdx * (fe - fw);
+  t -= A1(rdzw, k) * (fz_u(x, u, rom, ic, k + 1, j) - fz_u(x, u, rom, ic, k, j));
+  A3(tend, ic, k, j) = t;
 }
-
 // ---------------- v momentum (v rows j in [2, jde-1]) -----------------------
 __device__ __forceinline__ float fy_v(const DevCtx *__restrict__ x, const float *v, const float *rv, int i, int k, int j) {
   const int jde = x->jde;
@@ -160,18 +227,30 @@ __device__ __forceinline__ float fz_v(const DevCtx *__restrict__ x, const float 
   return vel * flux3(A3(v, i, k - 2, j), A3(v, i, k - 1, j), A3(v, i, k, j), A3(v, i, k + 1, j), -vel);
 }
 // advect_v (module_advect_em.py:1037-1354); r covers i [1, ide-1], j [2, jde-1]
-__global__ void k_advect_v(const DevCtx xv, const float *v, float *tend, const float *ru, const float *rv,
-                           const float *rom, R3 r) {
+// (face-sharing blocks: x faces i in [2, ide-1], y faces j in [2, jde])
+__global__ void __launch_bounds__(256) k_advect_v(const DevCtx xv, const float *v, float *tend, const float *ru, const float *rv,
+                                                  const float *rom, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
+  __shared__ float ys[4][64];
+  GET_FACE_BLOCK(r);
   const float *msfvy = FL(msfvy), *msfvx = FL(msfvx), *rdzw = FL(rdnw);
-  float t = A3(tend, i, k, j);
-  t -= A2(msfvy, i, j) * x->rdy * (fy_v(x, v, rv, i, k, j + 1) - fy_v(x, v, rv, i, k, j));
-  if (i >= 2 && i <= x->ide - 2) t -= A2(msfvy, i, j) * x->rdx * (fx_v(x, v, ru, i + 1, k, j) - fx_v(x, v, ru, i, k, j));
-  t -= (A2(msfvy, i, j) / A2(msfvx, i, j)) * A1(rdzw, k) * (fz_v(x, v, rom, i, k + 1, j) - fz_v(x, v, rom, i, k, j));
-  A3(tend, i, k, j) = t;
+  const int ide = x->ide, jde = x->jde;
+  const float fs = (j <= jde) ? fy_v(x, v, rv, ic, k, j) : 0.f;  // j >= 2
+  ys[wv][lane] = fs;
+  __syncthreads();
+  if (!jrow) return;
+  float fn;
+  if (wv < 3) fn = ys[wv + 1][lane];
+  else fn = fy_v(x, v, rv, ic, k, j + 1);  // j + 1 <= jde
+  const float fw = (i >= 2 && i <= ide - 1) ? fx_v(x, v, ru, i, k, j) : 0.f;
+  const float fe = lane_next(fw, lane);
+  if (!own) return;
+  float t = A3(tend, ic, k, j);
+  t -= A2(msfvy, ic, j) * x->rdy * (fn - fs);
+  if (ic >= 2 && ic <= ide - 2) t -= A2(msfvy, ic, j) * x->rdx * (fe - fw);
+  t -= (A2(msfvy, ic, j) / A2(msfvx, ic, j)) * A1(rdzw, k) * (fz_v(x, v, rom, ic, k + 1, j) - fz_v(x, v, rom, ic, k, j));
+  A3(tend, ic, k, j) = t;
 }
-
 // ---------------- w momentum (levels k in [2, kde]) -------------------------
 __device__ __forceinline__ float rvw(const DevCtx *__restrict__ x, const float *rv, int i, int k, int j) {
   const float *fzm = FL(fnm), *fzp = FL(fnp);
@@ -201,19 +280,31 @@ __device__ __forceinline__ float fz_w(const DevCtx *__restrict__ x, const float 
   return vel * flux3(A3(w, i, k - 2, j), A3(w, i, k - 1, j), A3(w, i, k, j), A3(w, i, k + 1, j), -vel);
 }
 // advect_w (module_advect_em.py:1357-1640); r covers i [1, ide-1], k [2, kde], j [1, jde-1]
-__global__ void k_advect_w(const DevCtx xv, const float *w, float *tend, const float *ru, const float *rv,
-                           const float *rom, R3 r) {
+// (face-sharing blocks: x faces i in [2, ide-1], y faces j in [2, jde-1])
+__global__ void __launch_bounds__(256) k_advect_w(const DevCtx xv, const float *w, float *tend, const float *ru, const float *rv,
+                                                  const float *rom, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
+  __shared__ float ys[4][64];
+  GET_FACE_BLOCK(r);
   const float *msftx = FL(msftx), *rdzu = FL(rdn);
-  float t = A3(tend, i, k, j);
-  if (j >= 2 && j <= x->jde - 2) t -= A2(msftx, i, j) * x->rdy * (fy_w(x, w, rv, i, k, j + 1) - fy_w(x, w, rv, i, k, j));
-  if (i >= 2 && i <= x->ide - 2) t -= A2(msftx, i, j) * x->rdx * (fx_w(x, w, ru, i + 1, k, j) - fx_w(x, w, ru, i, k, j));
-  if (k <= x->kde - 1) t -= A1(rdzu, k) * (fz_w(x, w, rom, i, k + 1, j) - fz_w(x, w, rom, i, k, j));
-  else t += 2.f * A1(rdzu, k - 1) * fz_w(x, w, rom, i, k, j);
-  A3(tend, i, k, j) = t;
+  const int ide = x->ide, jde = x->jde;
+  const float fs = (j >= 2 && j <= jde - 1) ? fy_w(x, w, rv, ic, k, j) : 0.f;
+  ys[wv][lane] = fs;
+  __syncthreads();
+  if (!jrow) return;
+  float fn;
+  if (wv < 3) fn = ys[wv + 1][lane];
+  else fn = (j + 1 <= jde - 1) ? fy_w(x, w, rv, ic, k, j + 1) : 0.f;
+  const float fw = (i >= 2 && i <= ide - 1) ? fx_w(x, w, ru, i, k, j) : 0.f;
+  const float fe = lane_next(fw, lane);
+  if (!own) return;
+  float t = A3(tend, ic, k, j);
+  if (j >= 2 && j <= jde - 2) t -= A2(msftx, ic, j) * x->rdy * (fn - fs);
+  if (ic >= 2 && ic <= ide - 2) t -= A2(msftx, ic, j) * x->rdx * (fe - fw);
+  if (k <= x->kde - 1) t -= A1(rdzu, k) * (fz_w(x, w, rom, ic, k + 1, j) - fz_w(x, w, rom, ic, k, j));
+  else t += 2.f * A1(rdzu, k - 1) * fz_w(x, w, rom, ic, k, j);
+  A3(tend, ic, k, j) = t;
 }
-
 // ---------------- positive-definite scalar advection ------------------------
 // pass 1: high-order minus upwind (antidiffusive) and upwind fluxes
 // (module_advect_em.py:537-660) on all faces the limiter and tendency read.

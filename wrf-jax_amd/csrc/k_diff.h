@@ -314,11 +314,19 @@ __global__ void k_hdiff_w(const DevCtx xv, const float *t1, const float *t2, R3 
   A3(FL(rw_tendf), i, k, j) = A3(FL(rw_tendf), i, k, j) + inc;
 }
 
-// horizontal_diffusion_s (module_diffusion_em.py:2046-2291), pass 1: the
-// face fluxes H1 (u faces i [2, ide-1], j [2, jde-2]) and H2 (v faces
-// i [2, ide-2], j [2, jde-1]) of one scalar.
+// horizontal_diffusion_s (module_diffusion_em.py:2046-2291) of every
+// diffused scalar (theta, moist, scalars), in two passes over all of them:
+// the face fluxes H1 (u faces) and H2 (v faces), then the tendency of the
+// mass points i [2, ide-2], j [2, jde-2], k [1, kde-1] from H1/H2 at levels
+// k-1, k, k+1.  The face geometry (diffusivity x density, slope, inverse
+// layer depth) and the tendency geometry do not depend on the scalar and are
+// formed once per point for all of them.  Every value is the reference's
+// expression in the reference's operation order.
+
+// average of columns A (the face's own cell) and B (its upwind neighbour)
+// at level k of the face (k = 1: extrapolated to the surface; k = kde:
+// extrapolated to the model top)
 __device__ __forceinline__ float hs_hav(const DevCtx *__restrict__ x, const float *var, int ia, int ja, int ib, int jb, int k) {
-  // average of columns A (the face's own cell) and B (its upwind neighbour)
   const int ktf = x->kde - 1, ktes1 = x->kde - 1, ktes2 = x->kde - 2;
   if (k == 1)
     return 0.5f * (x->cf1 * A3(var, ia, 1, ja) + x->cf2 * A3(var, ia, 2, ja) + x->cf3 * A3(var, ia, 3, ja) + x->cf1 * A3(var, ib, 1, jb) +
@@ -330,51 +338,83 @@ __device__ __forceinline__ float hs_hav(const DevCtx *__restrict__ x, const floa
   }
   return 0.5f * (A1(FL(fnm), k) * (A3(var, ib, k, jb) + A3(var, ia, k, ja)) + A1(FL(fnp), k) * (A3(var, ib, k - 1, jb) + A3(var, ia, k - 1, ja)));
 }
-__global__ void k_hdiff_s_faces(const DevCtx xv, const float *var, float *h1, float *h2, R3 r) {
+
+// scalar-independent part of one face at level k: -msf * xk * rho (c),
+// the face slope averaged to the level (tz) and the inverse layer depth at
+// the face (rdz), for the face between columns A (a) and B (b)
+struct HsFace {
+  float c, tz, rdz;
+};
+__device__ __forceinline__ HsFace hs_face(const DevCtx *__restrict__ x, const float *zs, float msf, int ia, int ja, int ib, int jb, int k) {
+  const float *xkhh = FL(xkhh), *rho = FL(rho), *rdzw = FL(rdzw);
+  HsFace f;
+  const float xk = 0.5f * (A3(xkhh, ib, k, jb) + A3(xkhh, ia, k, ja)) * 0.5f * (A3(rho, ib, k, jb) + A3(rho, ia, k, ja));
+  f.c = -msf * xk;
+  f.tz = 0.5f * (A3(zs, ia, k, ja) + A3(zs, ia, k + 1, ja));
+  f.rdz = 2.f / (1.f / A3(rdzw, ia, k, ja) + 1.f / A3(rdzw, ib, k, jb));
+  return f;
+}
+// H1 / H2 of one face: c * (rd * (var_A - var_B) - tz * (hav(k+1) - hav(k)) * rdz)
+__device__ __forceinline__ float hs_flux(const HsFace &f, float rd, float va, float vb, float hav_up, float hav_dn) {
+  return f.c * (rd * (va - vb) - f.tz * (hav_up - hav_dn) * f.rdz);
+}
+
+constexpr int kMaxDiff = 17;  // theta and up to 16 advected species
+struct HdsList {
+  const float *var[kMaxDiff];
+  float *tend[kMaxDiff];
+  float *h1[kMaxDiff], *h2[kMaxDiff];  // face fluxes (scratch)
+  int n;
+};
+
+// pass 1: H1 at u faces i [2, ide-1], j [2, jde-2] and H2 at v faces
+// i [2, ide-2], j [2, jde-1] of every scalar; r = i [2, ide-1], k [1, kde-1],
+// j [2, jde-1]
+__global__ void __launch_bounds__(256) k_hdiff_s_faces(const DevCtx xv, HdsList L, R3 r) {
   const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const int ide = x->ide, jde = x->jde;
-  const float *xkhh = FL(xkhh), *rho = FL(rho), *zx = FL(zx), *zy = FL(zy), *rdzw = FL(rdzw);
-  if (i >= 2 && i <= ide - 1 && j >= 2 && j <= jde - 2) {
-    const float xkx = 0.5f * (A3(xkhh, i - 1, k, j) + A3(xkhh, i, k, j)) * 0.5f * (A3(rho, i - 1, k, j) + A3(rho, i, k, j));
-    const float tmpzx = 0.5f * (A3(zx, i, k, j) + A3(zx, i, k + 1, j));
-    const float rdzu = 2.f / (1.f / A3(rdzw, i, k, j) + 1.f / A3(rdzw, i - 1, k, j));
-    A3(h1, i, k, j) = -A2(FL(msfux), i, j) * xkx *
-                      (x->rdx * (A3(var, i, k, j) - A3(var, i - 1, k, j)) -
-                       tmpzx * (hs_hav(x, var, i, j, i - 1, j, k + 1) - hs_hav(x, var, i, j, i - 1, j, k)) * rdzu);
-  }
-  if (i >= 2 && i <= ide - 2 && j >= 2 && j <= jde - 1) {
-    const float xkx = 0.5f * (A3(xkhh, i, k, j - 1) + A3(xkhh, i, k, j)) * 0.5f * (A3(rho, i, k, j - 1) + A3(rho, i, k, j));
-    const float tmpzy = 0.5f * (A3(zy, i, k, j) + A3(zy, i, k + 1, j));
-    const float rdzv = 2.f / (1.f / A3(rdzw, i, k, j) + 1.f / A3(rdzw, i, k, j - 1));
-    A3(h2, i, k, j) = -A2(FL(msfvy), i, j) * xkx *
-                      (x->rdy * (A3(var, i, k, j) - A3(var, i, k, j - 1)) -
-                       tmpzy * (hs_hav(x, var, i, j, i, j - 1, k + 1) - hs_hav(x, var, i, j, i, j - 1, k)) * rdzv);
+  const bool f1 = (i <= ide - 1 && j <= jde - 2), f2 = (i <= ide - 2 && j <= jde - 1);
+  const unsigned o = XI3(i, k, j);
+  HsFace a{0.f, 0.f, 0.f}, b{0.f, 0.f, 0.f};
+  if (f1) a = hs_face(x, FL(zx), A2(FL(msfux), i, j), i, j, i - 1, j, k);
+  if (f2) b = hs_face(x, FL(zy), A2(FL(msfvy), i, j), i, j, i, j - 1, k);
+  for (int s = 0; s < L.n; s++) {
+    const float *q = L.var[s];
+    const float qc = A3(q, i, k, j);
+    if (f1)
+      elem(L.h1[s], o) = hs_flux(a, x->rdx, qc, A3(q, i - 1, k, j), hs_hav(x, q, i, j, i - 1, j, k + 1), hs_hav(x, q, i, j, i - 1, j, k));
+    if (f2)
+      elem(L.h2[s], o) = hs_flux(b, x->rdy, qc, A3(q, i, k, j - 1), hs_hav(x, q, i, j, i, j - 1, k + 1), hs_hav(x, q, i, j, i, j - 1, k));
   }
 }
-// pass 2: tendency on mass points i [2, ide-2], j [2, jde-2], k [1, kde-1]
-__global__ void k_hdiff_s_tend(const DevCtx xv, float *tend, const float *h1, const float *h2, R3 r) {
+
+// pass 2: tendency of every scalar on mass points i [2, ide-2], j [2, jde-2],
+// k [1, kde-1] (r)
+__global__ void __launch_bounds__(256) k_hdiff_s_tend(const DevCtx xv, HdsList L, R3 r) {
   const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *zx = FL(zx), *zy = FL(zy), *rdzw = FL(rdzw), *fnm = FL(fnm), *fnp = FL(fnp);
   const int ktf = x->kde - 1;
-#define H1A(kk)                                                                                                                  \
-  (((kk) == 1 || (kk) == ktf + 1) ? 0.f                                                                                          \
-                                  : 0.5f * (A1(fnm, kk) * (A3(h1, i + 1, kk, j) + A3(h1, i, kk, j)) +                             \
-                                            A1(fnp, kk) * (A3(h1, i + 1, (kk) - 1, j) + A3(h1, i, (kk) - 1, j))))
-#define H2A(kk)                                                                                                                  \
-  (((kk) == 1 || (kk) == ktf + 1) ? 0.f                                                                                          \
-                                  : 0.5f * (A1(fnm, kk) * (A3(h2, i, kk, j + 1) + A3(h2, i, kk, j)) +                             \
-                                            A1(fnp, kk) * (A3(h2, i, (kk) - 1, j + 1) + A3(h2, i, (kk) - 1, j))))
   const float zxm = 0.25f * (A3(zx, i, k, j) + A3(zx, i + 1, k, j) + A3(zx, i, k + 1, j) + A3(zx, i + 1, k + 1, j));
   const float zym = 0.25f * (A3(zy, i, k, j) + A3(zy, i, k, j + 1) + A3(zy, i, k + 1, j) + A3(zy, i, k + 1, j + 1));
   const float msftx = A2(FL(msftx), i, j), msfty = A2(FL(msfty), i, j);
   const float mrdx = msftx * x->rdx, mrdy = msfty * x->rdy;
   const float rz = A3(rdzw, i, k, j);
-  const float inc = x->g / (A1(FL(dnw), k) * rz) *
-                    (mrdx * (A3(h1, i + 1, k, j) - A3(h1, i, k, j)) + mrdy * (A3(h2, i, k, j + 1) - A3(h2, i, k, j)) -
-                     msftx * zxm * (H1A(k + 1) - H1A(k)) * rz - msfty * zym * (H2A(k + 1) - H2A(k)) * rz);
-#undef H1A
-#undef H2A
-  A3(tend, i, k, j) = A3(tend, i, k, j) + inc;
+  const float fac = x->g / (A1(FL(dnw), k) * rz);
+  const float cx = msftx * zxm, cy = msfty * zym;
+  const bool lo = (k != 1), hi = (k + 1 != ktf + 1);  // H*A(k), H*A(k+1) nonzero
+  const float fm0 = A1(fnm, k), fp0 = A1(fnp, k), fm1 = A1(fnm, k + 1), fp1 = A1(fnp, k + 1);
+  const unsigned o = XI3(i, k, j), sk = (unsigned)x->ip, sj = (unsigned)(x->kde * x->ip);
+  for (int s = 0; s < L.n; s++) {
+    const float *h1 = L.h1[s], *h2 = L.h2[s];
+    const float a0 = elem(h1, o), a1 = elem(h1, o + 1), b0 = elem(h2, o), b1 = elem(h2, o + sj);
+    const float h1a0 = lo ? 0.5f * (fm0 * (a1 + a0) + fp0 * (elem(h1, o + 1 - sk) + elem(h1, o - sk))) : 0.f;
+    const float h1a1 = hi ? 0.5f * (fm1 * (elem(h1, o + 1 + sk) + elem(h1, o + sk)) + fp1 * (a1 + a0)) : 0.f;
+    const float h2a0 = lo ? 0.5f * (fm0 * (b1 + b0) + fp0 * (elem(h2, o + sj - sk) + elem(h2, o - sk))) : 0.f;
+    const float h2a1 = hi ? 0.5f * (fm1 * (elem(h2, o + sj + sk) + elem(h2, o + sk)) + fp1 * (b1 + b0)) : 0.f;
+    const float inc = fac * (mrdx * (a1 - a0) + mrdy * (b1 - b0) - cx * (h1a1 - h1a0) * rz - cy * (h2a1 - h2a0) * rz);
+    float *t = L.tend[s];
+    elem(t, o) = elem(t, o) + inc;
+  }
 }

@@ -12,8 +12,10 @@
 // Positive-definite advection on the last stage runs as two passes over all
 // PD species: the renormalisation factor of every limited cell, then the
 // tendency from the limited faces (each face scaled by the cell its
-// antidiffusive flux leaves) followed by the update.  Face fluxes are
-// recomputed in each pass instead of being stored.
+// antidiffusive flux leaves) followed by the update.  The first pass stores
+// the antidiffusive flux of the west, south and bottom face of every limited
+// cell (and of the east / north face of the last limited column / row); the
+// second reads them back and recomputes only the upwind fluxes.
 #pragma once
 #include "wrfdc_common.h"
 #include "k_advect.h"
@@ -25,6 +27,7 @@ struct SpecList {
   float *old[kMaxSpec];       // stage-start species (q_old)
   float *tend[kMaxSpec];      // accumulated physics / boundary tendency
   float *scale[kMaxSpec];     // PD renormalisation factors (scratch)
+  float *anti[kMaxSpec];      // PD antidiffusive face fluxes: x, y, z faces, n3 apart (scratch)
   int n;
 };
 
@@ -71,30 +74,51 @@ __device__ __forceinline__ UpdCoef upd_coef(const DevCtx *__restrict__ x, int i,
 
 // plain (non-PD) stages: advection with the time-averaged mass fluxes and
 // the update, all species of the list.  At rk = 1 `in` is the stage-start
-// copy q_old (equal to q there), so old = in.
+// copy q_old (equal to q there), so old = in.  Face-sharing blocks (block
+// (64, 4), 1-D grid of face_tiles(r).count() blocks): the south faces of all
+// species go to LDS first, one barrier, then each species' update.
 __global__ void __launch_bounds__(256) k_species_update(const DevCtx xv, SpecList L, int rk, float dt, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
-  const float *ru = FL(ru_m), *rv = FL(rv_m), *rom = FL(ww_m);
+  extern __shared__ float ys_[];  // L.n x 4 x 64: south face flux of every species
+  GET_FACE_BLOCK(r);
+#define YS(s, w) ys_[((s) * 4 + (w)) * 64 + lane]
   const int ide = x->ide, jde = x->jde, kde = x->kde;
-  const bool doy = (j >= 2 && j <= jde - 2), dox = (i >= 2 && i <= ide - 2);
-  const float vn = doy ? A3(rv, i, k, j + 1) : 0.f, vs = doy ? A3(rv, i, k, j) : 0.f;
-  const float ue = dox ? A3(ru, i + 1, k, j) : 0.f, uw = dox ? A3(ru, i, k, j) : 0.f;
-  const float wt = (k + 1 <= kde - 1) ? A3(rom, i, k + 1, j) : 0.f, wb = (k >= 2) ? A3(rom, i, k, j) : 0.f;
-  const float msftx = A2(FL(msftx), i, j), rdzw = A1(FL(rdnw), k);
-  const UpdCoef c = upd_coef(x, i, k, j);
-  const size_t o = XI3(i, k, j);
+  const float *ru = FL(ru_m), *rv = FL(rv_m), *rom = FL(ww_m);
+  {
+    const bool yf = (j >= 2 && j <= jde - 1);
+    const float vs = yf ? A3(rv, ic, k, j) : 0.f;
+    for (int s = 0; s < L.n; s++) YS(s, wv) = yf ? sfy(x, L.in[s], vs, ic, k, j) : 0.f;
+  }
+  __syncthreads();
+  if (!jrow) return;
+  const bool xface = (i >= 2 && i <= ide - 1);
+  const bool yface_n = (wv == 3 && j + 1 <= jde - 1);
+  const bool doy = (j >= 2 && j <= jde - 2), dox = (ic >= 2 && ic <= ide - 2);
+  const float uw = xface ? A3(ru, i, k, j) : 0.f;
+  const float vn = yface_n ? A3(rv, ic, k, j + 1) : 0.f;
+  const float wt = (k + 1 <= kde - 1) ? A3(rom, ic, k + 1, j) : 0.f, wb = (k >= 2) ? A3(rom, ic, k, j) : 0.f;
+  const float msftx = A2(FL(msftx), ic, j), rdzw = A1(FL(rdnw), k);
+  const UpdCoef c = upd_coef(x, ic, k, j);
+  const unsigned o = XI3(ic, k, j);
   for (int s = 0; s < L.n; s++) {
     const float *q = L.in[s];
+    const float fw = xface ? sfx(x, q, uw, i, k, j) : 0.f;
+    const float fe = lane_next(fw, lane);
+    const float fs = YS(s, wv);
+    float fn;
+    if (wv < 3) fn = YS(s, wv + 1);
+    else fn = yface_n ? sfy(x, q, vn, ic, k, j + 1) : 0.f;
     float adv = 0.f;
-    if (doy) adv -= msftx * x->rdy * (sfy(x, q, vn, i, k, j + 1) - sfy(x, q, vs, i, k, j));
-    if (dox) adv -= msftx * x->rdx * (sfx(x, q, ue, i + 1, k, j) - sfx(x, q, uw, i, k, j));
-    adv -= rdzw * (sfz(x, q, wt, i, k + 1, j) - sfz(x, q, wb, i, k, j));
+    if (doy) adv -= msftx * x->rdy * (fn - fs);
+    if (dox) adv -= msftx * x->rdx * (fe - fw);
+    adv -= rdzw * (sfz(x, q, wt, ic, k + 1, j) - sfz(x, q, wb, ic, k, j));
     float tend = c.inner ? adv * c.msfty : 0.f;
-    tend = tend + L.tend[s][o];
-    const float old = (rk == 1) ? q[o] : L.old[s][o];
-    L.out[s][o] = (c.oldm * old + dt * tend) / c.newm;
+    tend = tend + elem(L.tend[s], o);
+    const float old = (rk == 1) ? elem(q, o) : elem(L.old[s], o);
+    const float out = (c.oldm * old + dt * tend) / c.newm;
+    if (own) elem(L.out[s], o) = out;
   }
+#undef YS
 }
 
 // rk_update_scalar_pd (module_em.py:1601-1669) for every species of the list:
@@ -178,79 +202,127 @@ __device__ __forceinline__ void pd_face_flux(const DevCtx *__restrict__ x, const
   anti = high - low;
 }
 
-// pass 1: renormalisation factor of every cell (1 where not limited);
-// full extent so the factor array is defined wherever pass 2 reads it.
+// pass 1: renormalisation factor of every cell (1 where not limited), over
+// the full extent so the factor array is defined wherever pass 2 reads it.
+// Face-sharing blocks as k_species_update: the west face comes from this
+// lane, the east face from the next lane, the south face from this wave and
+// the north face from the wave above (LDS).
 __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecList L, float dt, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
-  const unsigned o = XI3(i, k, j);
-  const bool lim = (i >= 2 && i <= x->ide - 2 && j >= 2 && j <= x->jde - 2 && k <= x->kde - 1);
-  if (!lim) {
-    for (int s = 0; s < L.n; s++) L.scale[s][o] = 1.f;
-    return;
+  extern __shared__ float yal_[];  // 2 x L.n x 4 x 64: south antidiffusive / upwind flux of every species
+  GET_FACE_BLOCK(r);
+#define YA(s, w) yal_[((s) * 4 + (w)) * 64 + lane]
+#define YL(s, w) yal_[((L.n + (s)) * 4 + (w)) * 64 + lane]
+  const int ide = x->ide, jde = x->jde;
+  {
+    const PdFace fs = pd_face_geom(x, 1, ic, k, j, dt);
+    for (int s = 0; s < L.n; s++) {
+      float a, l;
+      pd_face_flux(x, fs, 1, L.in[s], L.old[s], ic, k, j, a, l);
+      YA(s, wv) = a;
+      YL(s, wv) = l;
+    }
   }
-  const float *c1 = FL(c1h), *c2 = FL(c2h), *rdzw = FL(rdnw);
-  const float rdx = x->rdx, rdy = x->rdy;
-  const float mm = A2(FL(msftx), i, j) * A2(FL(msfty), i, j), msfty = A2(FL(msfty), i, j);
-  const float mold = (A1(c1, k) * A2(FL(mub), i, j) + A1(c2, k)) + (A1(c1, k) * A2(FL(mu_1), i, j));
-  const PdFace fx0 = pd_face_geom(x, 0, i, k, j, dt), fx1 = pd_face_geom(x, 0, i + 1, k, j, dt);
-  const PdFace fy0 = pd_face_geom(x, 1, i, k, j, dt), fy1 = pd_face_geom(x, 1, i, k, j + 1, dt);
-  const PdFace fz0 = pd_face_geom(x, 2, i, k, j, dt), fz1 = pd_face_geom(x, 2, i, k + 1, j, dt);
+  __syncthreads();
+  if (!jrow) return;
+  const bool lim = own && (ic >= 2 && ic <= ide - 2 && j >= 2 && j <= jde - 2 && k <= x->kde - 1);
+  const unsigned o = XI3(ic, k, j);
+  const float *c1 = FL(c1h), *c2 = FL(c2h);
+  const float rdx = x->rdx, rdy = x->rdy, rz = A1(FL(rdnw), imin_(k, x->kde - 1));
+  const float mm = A2(FL(msftx), ic, j) * A2(FL(msfty), ic, j), msfty = A2(FL(msfty), ic, j);
+  const float mold = (A1(c1, k) * A2(FL(mub), ic, j) + A1(c2, k)) + (A1(c1, k) * A2(FL(mu_1), ic, j));
+  const PdFace fx = pd_face_geom(x, 0, i, k, j, dt);
+  const PdFace fyn = (wv == 3) ? pd_face_geom(x, 1, ic, k, j + 1, dt) : PdFace{0.f, 0.f, 0.f, 0};
+  const PdFace fz0 = pd_face_geom(x, 2, ic, k, j, dt), fz1 = pd_face_geom(x, 2, ic, k + 1, j, dt);
+  const unsigned sj = (unsigned)(x->kde * x->ip);
+  const size_t n3 = x->n3;
   for (int s = 0; s < L.n; s++) {
     const float *q = L.in[s], *qo = L.old[s];
-    float ax0, lx0, ax1, lx1, ay0, ly0, ay1, ly1, az0, lz0, az1, lz1;
-    pd_face_flux(x, fx0, 0, q, qo, i, k, j, ax0, lx0);
-    pd_face_flux(x, fx1, 0, q, qo, i + 1, k, j, ax1, lx1);
-    pd_face_flux(x, fy0, 1, q, qo, i, k, j, ay0, ly0);
-    pd_face_flux(x, fy1, 1, q, qo, i, k, j + 1, ay1, ly1);
-    pd_face_flux(x, fz0, 2, q, qo, i, k, j, az0, lz0);
-    pd_face_flux(x, fz1, 2, q, qo, i, k + 1, j, az1, lz1);
-    const float ph_low = mold * qo[o] - dt * (mm * (rdx * (lx1 - lx0) + rdy * (ly1 - ly0)) + msfty * A1(rdzw, k) * (lz1 - lz0));
+    float ax0, lx0, ay1, ly1, az0, lz0, az1, lz1;
+    pd_face_flux(x, fx, 0, q, qo, i, k, j, ax0, lx0);
+    const float ax1 = lane_next(ax0, lane), lx1 = lane_next(lx0, lane);
+    if (!own) continue;
+    if (!lim) {
+      L.scale[s][o] = 1.f;
+      continue;
+    }
+    const float ay0 = YA(s, wv), ly0 = YL(s, wv);
+    if (wv < 3) {
+      ay1 = YA(s, wv + 1);
+      ly1 = YL(s, wv + 1);
+    } else {
+      pd_face_flux(x, fyn, 1, q, qo, ic, k, j + 1, ay1, ly1);
+    }
+    pd_face_flux(x, fz0, 2, q, qo, ic, k, j, az0, lz0);
+    pd_face_flux(x, fz1, 2, q, qo, ic, k + 1, j, az1, lz1);
+    float *an = L.anti[s];
+    an[o] = ax0;
+    an[n3 + o] = ay0;
+    an[2 * n3 + o] = az0;
+    if (ic == ide - 2) an[o + 1] = ax1;
+    if (j == jde - 2) an[n3 + o + sj] = ay1;
+    const float ph_low = mold * qo[o] - dt * (mm * (rdx * (lx1 - lx0) + rdy * (ly1 - ly0)) + msfty * rz * (lz1 - lz0));
     const float flux_out = dt * (mm * (rdx * (fmaxf(0.f, ax1) - fminf(0.f, ax0)) + rdy * (fmaxf(0.f, ay1) - fminf(0.f, ay0))) +
-                                 msfty * A1(rdzw, k) * (fminf(0.f, az1) - fmaxf(0.f, az0)));
+                                 msfty * rz * (fminf(0.f, az1) - fmaxf(0.f, az0)));
     float sc = 1.f;
     if (flux_out > ph_low) sc = fmaxf(0.f, ph_low / (flux_out + 1.0e-20f));
     L.scale[s][o] = sc;
   }
+#undef YA
+#undef YL
+}
+
+// upwind flux of one species through a face whose geometry is given
+__device__ __forceinline__ float pd_face_low(const PdFace &f, const float *qo, unsigned o, int st) {
+  return f.muf * flux_up(qo[o - st], qo[o], f.cr);
 }
 
 // pass 2: tendency from the renormalised faces, then the mass-weighted update
-// (old = q_old after k_species_pd_prep).
+// (old = q_old after k_species_pd_prep).  Only the interior cells
+// i [2, ide-2], j [2, jde-2] (the cells pass 1 limits: the specified zone is
+// one cell wide) take an advective tendency; their antidiffusive fluxes come
+// from pass 1.
 __global__ void __launch_bounds__(256) k_species_pd_update(const DevCtx xv, SpecList L, float dt, R3 r) {
   const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const unsigned o = XI3(i, k, j);
-  const float rdx = x->rdx, rdy = x->rdy, rdzw = A1(FL(rdnw), k), msftx = A2(FL(msftx), i, j);
-  const bool dox = (i >= 2 && i <= x->ide - 2), doy = (j >= 2 && j <= x->jde - 2);
-  const bool up_ok = (k + 1 <= x->kde - 1), dn_ok = (k - 1 >= 1);
   const UpdCoef c = upd_coef(x, i, k, j);
+  if (!c.inner) {
+    for (int s = 0; s < L.n; s++) {
+      const float tend = 0.f + L.tend[s][o];
+      L.out[s][o] = (c.oldm * L.old[s][o] + dt * tend) / c.newm;
+    }
+    return;
+  }
+  const float rdx = x->rdx, rdy = x->rdy, rdzw = A1(FL(rdnw), k), msftx = A2(FL(msftx), i, j);
+  const bool up_ok = (k + 1 <= x->kde - 1), dn_ok = (k - 1 >= 1);
   const PdFace fz0 = pd_face_geom(x, 2, i, k, j, dt), fz1 = pd_face_geom(x, 2, i, k + 1, j, dt);
-  PdFace fx0{0.f, 0.f, 0.f, 0}, fx1 = fx0, fy0 = fx0, fy1 = fx0;
-  if (dox) {
-    fx0 = pd_face_geom(x, 0, i, k, j, dt);
-    fx1 = pd_face_geom(x, 0, i + 1, k, j, dt);
-  }
-  if (doy) {
-    fy0 = pd_face_geom(x, 1, i, k, j, dt);
-    fy1 = pd_face_geom(x, 1, i, k, j + 1, dt);
-  }
-  const int sj = x->kde * x->ip, sk = x->ip;
+  const PdFace fx0 = pd_face_geom(x, 0, i, k, j, dt), fx1 = pd_face_geom(x, 0, i + 1, k, j, dt);
+  const PdFace fy0 = pd_face_geom(x, 1, i, k, j, dt), fy1 = pd_face_geom(x, 1, i, k, j + 1, dt);
+  const unsigned sj = (unsigned)(x->kde * x->ip), sk = (unsigned)x->ip;
+  const size_t n3 = x->n3;
   for (int s = 0; s < L.n; s++) {
-    const float *q = L.in[s], *qo = L.old[s], *sc = L.scale[s] + o;
-    float az0, lz0, az1, lz1;
-    pd_face_flux(x, fz0, 2, q, qo, i, k, j, az0, lz0);
-    pd_face_flux(x, fz1, 2, q, qo, i, k + 1, j, az1, lz1);
+    const float *qo = L.old[s], *sc = L.scale[s] + o, *an = L.anti[s];
+    // z faces (no face below level 2 or above level kde-1)
+    float az0 = 0.f, lz0 = 0.f, az1 = 0.f, lz1 = 0.f;
+    if (fz0.mode != 0) {
+      az0 = an[2 * n3 + o];
+      lz0 = pd_face_low(fz0, qo, o, sk);
+    }
+    if (fz1.mode != 0) {
+      az1 = an[2 * n3 + o + sk];
+      lz1 = pd_face_low(fz1, qo, o + sk, sk);
+    }
     float zt = az1, zb = az0;
     if (zt < 0.f) zt *= sc[0];
     else if (zt > 0.f && up_ok) zt *= sc[sk];
     if (zb > 0.f) zb *= sc[0];
-    else if (zb < 0.f && dn_ok) zb *= sc[-sk];
+    else if (zb < 0.f && dn_ok) zb *= sc[-(int)sk];
     float t = 0.f;
     t += -rdzw * (zt - zb + lz1 - lz0);
-    if (dox) {
-      float ax0, lx0, ax1, lx1;
-      pd_face_flux(x, fx0, 0, q, qo, i, k, j, ax0, lx0);
-      pd_face_flux(x, fx1, 0, q, qo, i + 1, k, j, ax1, lx1);
+    {
+      const float ax0 = an[o], ax1 = an[o + 1];
+      const float lx0 = pd_face_low(fx0, qo, o, 1), lx1 = pd_face_low(fx1, qo, o + 1, 1);
       float xe = ax1, xw = ax0;
       if (xe > 0.f) xe *= sc[0];
       else if (xe < 0.f) xe *= sc[1];
@@ -258,18 +330,17 @@ __global__ void __launch_bounds__(256) k_species_pd_update(const DevCtx xv, Spec
       else if (xw > 0.f) xw *= sc[-1];
       t += -msftx * (rdx * (xe - xw + lx1 - lx0));
     }
-    if (doy) {
-      float ay0, ly0, ay1, ly1;
-      pd_face_flux(x, fy0, 1, q, qo, i, k, j, ay0, ly0);
-      pd_face_flux(x, fy1, 1, q, qo, i, k, j + 1, ay1, ly1);
+    {
+      const float ay0 = an[n3 + o], ay1 = an[n3 + o + sj];
+      const float ly0 = pd_face_low(fy0, qo, o, sj), ly1 = pd_face_low(fy1, qo, o + sj, sj);
       float yn = ay1, ys = ay0;
       if (yn > 0.f) yn *= sc[0];
       else if (yn < 0.f) yn *= sc[sj];
       if (ys < 0.f) ys *= sc[0];
-      else if (ys > 0.f) ys *= sc[-sj];
+      else if (ys > 0.f) ys *= sc[-(int)sj];
       t += -msftx * (rdy * (yn - ys + ly1 - ly0));
     }
-    float tend = c.inner ? t * c.msfty : 0.f;
+    float tend = t * c.msfty;
     tend = tend + L.tend[s][o];
     L.out[s][o] = (c.oldm * qo[o] + dt * tend) / c.newm;
   }

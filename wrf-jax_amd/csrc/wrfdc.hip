@@ -138,6 +138,8 @@ struct wrfdc_ctx {
   float *scratch = nullptr;
   float *spec_alt = nullptr;    // second buffer of every species (nmoist + nscalar 3-D arrays)
   float *spec_scale = nullptr;  // positive-definite renormalisation factors, one 3-D array per species
+  float *spec_anti = nullptr;   // positive-definite antidiffusive face fluxes, three 3-D arrays per species
+  float *hds = nullptr;         // horizontal scalar diffusion face fluxes, two 3-D arrays per diffused scalar
   double *ck = nullptr;  // kCkBlocks * 4 partials + 4 results
   int rk = 1, nsmall = 1, iteration = 1;
   double dt_rk = 0.0, dts_rk = 0.0, dtbc = 0.0;
@@ -179,8 +181,9 @@ struct Run {
   R3 full3() const { return R3{1, ide, 1, kde, 1, jde}; }
   R3 mass3() const { return R3{1, ide - 1, 1, kde - 1, 1, jde - 1}; }
   R2 mass2() const { return R2{1, ide - 1, 1, jde - 1}; }
-  static dim3 g3(const R3 &r) { return dim3(cdiv(r.i1 - r.i0 + 1, 64), cdiv(r.k1 - r.k0 + 1, 4), (unsigned)(r.j1 - r.j0 + 1)); }
-  static dim3 g2(const R2 &r) { return dim3(cdiv(r.i1 - r.i0 + 1, 64), (unsigned)(r.j1 - r.j0 + 1)); }
+  // 1-D grids decoded by GET_IKJ / GET_IJ (XCD-contiguous tiles)
+  static dim3 g3(const R3 &r) { return dim3(cdiv(r.i1 - r.i0 + 1, 64) * cdiv(r.k1 - r.k0 + 1, 4) * (unsigned)(r.j1 - r.j0 + 1)); }
+  static dim3 g2(const R2 &r) { return dim3(cdiv(r.i1 - r.i0 + 1, 64) * (unsigned)(r.j1 - r.j0 + 1)); }
   static dim3 b3() { return dim3(64, 4, 1); }
 
   void fill(float *p, size_t n, float v = 0.f) const {
@@ -336,12 +339,24 @@ struct Run {
     R3 r = full3();
     k_titau<<<g3(r), b3(), 0, s>>>(d, tt, F(xk), F(defor), kind, i0, i1, j0, j1, r);
   }
-  void hdiff_s(float *tend, const float *var) const {
-    float *t1 = Scr(S_T1), *t2 = Scr(S_T2);
-    R3 rf{2, ide - 1, 1, kde - 1, 2, jde - 1};
-    k_hdiff_s_faces<<<g3(rf), b3(), 0, s>>>(d, var, t1, t2, rf);
-    R3 rt{2, ide - 2, 1, kde - 1, 2, jde - 2};
-    k_hdiff_s_tend<<<g3(rt), b3(), 0, s>>>(d, tend, t1, t2, rt);
+  // horizontal_diffusion_s of theta and every species: faces, then tendencies
+  void hdiff_scalars() const {
+    HdsList L;
+    memset(&L, 0, sizeof(L));
+    auto add = [&](const float *v, float *t) {
+      L.var[L.n] = v;
+      L.tend[L.n] = t;
+      L.h1[L.n] = X->hds + (size_t)(2 * L.n) * X->n3;
+      L.h2[L.n] = X->hds + (size_t)(2 * L.n + 1) * X->n3;
+      L.n++;
+    };
+    add(F(F_t_2), F(F_t_tendf));
+    for (int q = 0; q < X->c.nmoist; q++) add(Sp(F_moist, q), Sp(F_moist_tend, q));
+    for (int q = 0; q < X->c.nscalar; q++) add(Sp(F_scalar, q), Sp(F_scalar_tend, q));
+    const R3 rf{2, ide - 1, 1, kde - 1, 2, jde - 1};
+    k_hdiff_s_faces<<<g3(rf), b3(), 0, s>>>(d, L, rf);
+    const R3 rt{2, ide - 2, 1, kde - 1, 2, jde - 2};
+    k_hdiff_s_tend<<<g3(rt), b3(), 0, s>>>(d, L, rt);
   }
   void horizontal_diffusion_2() const {
     float *t1 = Scr(S_T1), *t2 = Scr(S_T2);
@@ -357,9 +372,7 @@ struct Run {
     titau(t2, F_xkmv, F_defor23, 23, 2, ide - 2, 2, jde);
     R3 rw{2, ide - 2, 2, kde - 1, 2, jde - 2};
     k_hdiff_w<<<g3(rw), b3(), 0, s>>>(d, t1, t2, rw);
-    hdiff_s(F(F_t_tendf), F(F_t_2));
-    for (int q = 0; q < X->c.nmoist; q++) hdiff_s(Sp(F_moist_tend, q), Sp(F_moist, q));
-    for (int q = 0; q < X->c.nscalar; q++) hdiff_s(Sp(F_scalar_tend, q), Sp(F_scalar, q));
+    hdiff_scalars();
   }
   void first_rk_step_part2() const {
     compute_diff_metrics();
@@ -374,19 +387,19 @@ struct Run {
   }
   void advect_u() const {
     R3 r{2, ide - 1, 1, kde - 1, 1, jde - 1};
-    k_advect_u<<<g3(r), b3(), 0, s>>>(d, F(F_u_2), F(F_ru_tend), F(F_ru), F(F_rv), F(F_ww), r);
+    k_advect_u<<<dim3(face_tiles(r).count()), b3(), 0, s>>>(d, F(F_u_2), F(F_ru_tend), F(F_ru), F(F_rv), F(F_ww), r);
   }
   void advect_v() const {
     R3 r{1, ide - 1, 1, kde - 1, 2, jde - 1};
-    k_advect_v<<<g3(r), b3(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), r);
+    k_advect_v<<<dim3(face_tiles(r).count()), b3(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), r);
   }
   void advect_w() const {
     R3 r{1, ide - 1, 2, kde, 1, jde - 1};
-    k_advect_w<<<g3(r), b3(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), r);
+    k_advect_w<<<dim3(face_tiles(r).count()), b3(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), r);
   }
   void advect_scalar(const float *q, float *tend, const float *ru, const float *rv, const float *rom) const {
     R3 m = mass3();
-    k_advect_scalar<<<g3(m), b3(), 0, s>>>(d, q, tend, ru, rv, rom, m);
+    k_advect_scalar<<<dim3(face_tiles(m).count()), b3(), 0, s>>>(d, q, tend, ru, rv, rom, m);
   }
   void advect_scalar_pd(const float *q, const float *q_old, float *tend, double dt) const {
     R3 r = full3();
@@ -605,23 +618,27 @@ struct Run {
       L.old[L.n] = Sp(id_old, q);
       L.tend[L.n] = Sp(id_tend, q);
       L.scale[L.n] = nullptr;
+      L.anti[L.n] = nullptr;
       L.n++;
     };
     const bool pdm = (rk == 3 && X->c.pd_moist), pds = (rk == 3 && X->c.pd_scalar);
     for (int q = 0; q < nm; q++) add(pdm ? pd : plain, F_moist, F_moist_old, F_moist_tend, out_alt ? alt_moist : user_moist, q);
     for (int q = 0; q < nsc; q++) add(pds ? pd : plain, F_scalar, F_scalar_old, F_scalar_tend, out_alt ? alt_scalar : user_scalar, q);
     if (pd.n > 0) {
-      for (int q = 0; q < pd.n; q++) pd.scale[q] = X->spec_scale + (size_t)q * n3;
+      for (int q = 0; q < pd.n; q++) {
+        pd.scale[q] = X->spec_scale + (size_t)q * n3;
+        pd.anti[q] = X->spec_anti + (size_t)q * 3 * n3;
+      }
       k_species_pd_prep<<<g3(m), b3(), 0, s>>>(d, pd, dt, m);
       {
         BcBatch bc(this);
         for (int q = 0; q < pd.n; q++) bc.add3(pd.old[q], 'p');
       }
       R3 r = full3();
-      k_species_pd_scale<<<g3(r), b3(), 0, s>>>(d, pd, dt, r);
+      k_species_pd_scale<<<dim3(face_tiles(r).count()), b3(), sizeof(float) * 2 * 256 * pd.n, s>>>(d, pd, dt, r);
       k_species_pd_update<<<g3(m), b3(), 0, s>>>(d, pd, dt, m);
     }
-    if (plain.n > 0) k_species_update<<<g3(m), b3(), 0, s>>>(d, plain, rk, dt, m);
+    if (plain.n > 0) k_species_update<<<dim3(face_tiles(m).count()), b3(), sizeof(float) * 256 * plain.n, s>>>(d, plain, rk, dt, m);
     // outputs become the current species
     use_species(out_alt);
     if (rk == 3) {
@@ -747,7 +764,7 @@ int set_err(wrfdc_ctx *x, int code, const std::string &msg) {
 int validate(const wrfdc_config &c, std::string &msg) {
   if (c.nx < 12 || c.ny < 12 || c.nz < 6) { msg = "domain too small: need nx, ny >= 12 and nz >= 6"; return WRFDC_EINVAL; }
   if (c.ip < c.nx + 1) { msg = "ip must be >= nx + 1"; return WRFDC_EINVAL; }
-  if ((double)(c.ny + 1) * (c.nz + 1) * c.ip >= 2147483648.0) { msg = "a 3-D field must hold fewer than 2^31 elements"; return WRFDC_EINVAL; }
+  if ((double)(c.ny + 1) * (c.nz + 1) * c.ip >= 1073741824.0) { msg = "a 3-D field must hold fewer than 2^30 elements"; return WRFDC_EINVAL; }
   if (c.nmoist < 1) { msg = "nmoist must be >= 1 (species 0 is QV)"; return WRFDC_EINVAL; }
   if (c.nscalar < 0) { msg = "nscalar must be >= 0"; return WRFDC_EINVAL; }
   if (c.nmoist + c.nscalar > kMaxSpec) { msg = "at most 16 advected species (nmoist + nscalar)"; return WRFDC_EINVAL; }
@@ -867,12 +884,17 @@ int wrfdc_create(const wrfdc_config *cfg, wrfdc_ctx **out) {
     hchk(hipMalloc((void **)&x->ck, sizeof(double) * (kCkBlocks * 4 + 4)), "hipMalloc checksum");
     hchk(hipMalloc((void **)&x->spec_alt, sizeof(float) * x->n3 * (size_t)(cfg->nmoist + cfg->nscalar)), "hipMalloc species");
     hchk(hipMalloc((void **)&x->spec_scale, sizeof(float) * x->n3 * (size_t)(cfg->nmoist + cfg->nscalar)), "hipMalloc species scale");
+    if (cfg->pd_moist || cfg->pd_scalar)
+      hchk(hipMalloc((void **)&x->spec_anti, sizeof(float) * x->n3 * 3 * (size_t)(cfg->nmoist + cfg->nscalar)), "hipMalloc species fluxes");
+    hchk(hipMalloc((void **)&x->hds, sizeof(float) * x->n3 * 2 * (size_t)(1 + cfg->nmoist + cfg->nscalar)), "hipMalloc diffusion fluxes");
   } catch (const HipError &e) {
     fprintf(stderr, "wrfdc_create: %s: %s\n", e.what, hipGetErrorString(e.e));
     if (x->scratch) (void)hipFree(x->scratch);
     if (x->ck) (void)hipFree(x->ck);
     if (x->spec_alt) (void)hipFree(x->spec_alt);
     if (x->spec_scale) (void)hipFree(x->spec_scale);
+    if (x->spec_anti) (void)hipFree(x->spec_anti);
+    if (x->hds) (void)hipFree(x->hds);
     delete x;
     return WRFDC_EHIP;
   }
@@ -897,6 +919,8 @@ void wrfdc_destroy(wrfdc_ctx *x) {
   if (x->ck) (void)hipFree(x->ck);
   if (x->spec_alt) (void)hipFree(x->spec_alt);
   if (x->spec_scale) (void)hipFree(x->spec_scale);
+  if (x->spec_anti) (void)hipFree(x->spec_anti);
+  if (x->hds) (void)hipFree(x->hds);
   delete x;
 }
 

@@ -73,13 +73,19 @@ struct DevCtx {
   float *S[NSCRATCH];
 };
 
-// Element offsets in 32-bit arithmetic (a 3-D field holds < 2^31 elements;
+// Element offsets in 32-bit arithmetic, scaled to byte offsets in 32-bit
+// arithmetic too (a 3-D field holds < 2^30 elements, so its byte offsets fit;
 // wrfdc_create checks), added to 64-bit base pointers held in scalar
-// registers: the loads use the saddr + voffset form.
+// registers: the loads and stores use the saddr + 32-bit voffset form and
+// need no 64-bit address arithmetic per access.
 #define XI3(i, k, j) ((unsigned)((((j) - 1) * x->kde + ((k) - 1)) * x->ip + ((i) - 1)))
 #define XI2(i, j) ((unsigned)(((j) - 1) * x->ip + ((i) - 1)))
-#define A3(fl, i, k, j) ((fl)[XI3(i, k, j)])
-#define A2(fl, i, j) ((fl)[XI2(i, j)])
+__device__ __forceinline__ float &elem(float *p, unsigned e) { return *reinterpret_cast<float *>(reinterpret_cast<char *>(p) + (e << 2)); }
+__device__ __forceinline__ const float &elem(const float *p, unsigned e) {
+  return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(p) + (e << 2));
+}
+#define A3(fl, i, k, j) elem(fl, XI3(i, k, j))
+#define A2(fl, i, j) elem(fl, XI2(i, j))
 #define A1(fl, k) ((fl)[(k) - 1])
 #define BX(fl, j, k, b) ((fl)[(unsigned)((((b) - 1) * x->kde + ((k) - 1)) * x->jde + ((j) - 1))])
 #define BY(fl, i, k, b) ((fl)[(unsigned)((((b) - 1) * x->kde + ((k) - 1)) * x->ip + ((i) - 1))])
@@ -97,31 +103,65 @@ struct R2 {
   int i0, i1, j0, j1;
 };
 
-// thread -> (i, k, j) with blockDim (64, 4, 1), grid (ceil(ni/64), ceil(nk/4), nj)
-#define GET_IKJ(r)                                                  \
-  const int i = (r).i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x); \
-  const int k = (r).k0 + (int)(blockIdx.y * blockDim.y + threadIdx.y); \
-  const int j = (r).j0 + (int)blockIdx.z;                               \
+// Workgroups are dispatched to the 8 XCDs round-robin by linear id, and each
+// XCD has its own L2.  xcd_tile maps the linear id b of a grid of nb blocks
+// to a tile number so that XCD x walks the contiguous tiles
+// [x*q + min(x, r), ...) (q = nb / 8, r = nb % 8) in order: neighbouring rows
+// of a stencil are then read by the same XCD shortly after one another and
+// hit its L2 instead of being fetched again by another XCD.  Bijective for
+// every nb.
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
+  const unsigned xc = b & 7u, idx = b >> 3, q = nb >> 3, rm = nb & 7u;
+  return xc * q + (xc < rm ? xc : rm) + idx;
+}
+// thread -> (i, k, j) with blockDim (64, 4, 1) and a 1-D grid of
+// ceil(ni/64) * ceil(nk/4) * nj blocks; tiles run i fastest, then k, then j
+#define GET_IKJ(r)                                                                           \
+  const unsigned gx_ = (unsigned)((r).i1 - (r).i0 + 64) / 64u;                               \
+  const unsigned gy_ = (unsigned)((r).k1 - (r).k0 + 4) / 4u;                                 \
+  const unsigned t_ = xcd_tile(blockIdx.x, gridDim.x);                                       \
+  const int i = (r).i0 + (int)((t_ % gx_) * 64u + threadIdx.x);                              \
+  const int k = (r).k0 + (int)(((t_ / gx_) % gy_) * 4u + threadIdx.y);                       \
+  const int j = (r).j0 + (int)(t_ / (gx_ * gy_));                                            \
   if (i > (r).i1 || k > (r).k1 || j > (r).j1) return
-// thread -> (i, j) for column kernels: blockDim (64, 1, 1), grid (ceil(ni/64), nj)
-#define GET_IJ(r)                                                   \
-  const int i = (r).i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x); \
-  const int j = (r).j0 + (int)blockIdx.y;                               \
+// thread -> (i, j) for column kernels: blockDim (64, 1, 1), 1-D grid of
+// ceil(ni/64) * nj blocks
+#define GET_IJ(r)                                                                            \
+  const unsigned gx_ = (unsigned)((r).i1 - (r).i0 + 64) / 64u;                               \
+  const unsigned t_ = xcd_tile(blockIdx.x, gridDim.x);                                       \
+  const int i = (r).i0 + (int)((t_ % gx_) * 64u + threadIdx.x);                              \
+  const int j = (r).j0 + (int)(t_ / gx_);                                                    \
   if (i > (r).i1 || j > (r).j1) return
 
 __device__ __forceinline__ float sgn1(float v) { return v >= 0.f ? 1.f : -1.f; }
-// x / c correctly rounded, for a constant c with rc = RN(1/c): one FMA
-// correction of x * rc is the IEEE quotient whenever the quotient is a
-// normal number (checked exhaustively over all FP32 x for c = 12 and 60);
-// subnormal quotients take the division itself.
-// The division of the rare case sits in a function that is not inlined, so
-// the compiler cannot hoist it into the common path.
-__device__ __attribute__((noinline)) float div_rare(float xv, float c) { return xv / c; }
+// x / c correctly rounded (IEEE RN), for an integer constant c (12 or 60)
+// with rc = RN(1/c).  Checked bitwise against x / c for every finite FP32 x,
+// for c = 12 and c = 60.
+//  - A quotient of at least 2^-125 is one FMA correction of x * rc.  The
+//    sign is taken from x, which only changes the result for x = -0.
+//  - A nonzero x below c * 2^-125 has X = |x| * 2^149, an integer below
+//    2^30, and a quotient n * 2^-149 with n = X / c rounded to nearest, ties
+//    to even (n <= 2^24, so exact).  n comes from two rounded estimates,
+//    each corrected by the exact remainder X - n * c (one FMA), then the
+//    half-way cases; every constant is an inline operand or c / rc, so the
+//    rare path takes no scalar registers of its own.
+__device__ __forceinline__ float div_tiny(float xv, float c, float rc) {
+  const float X = __builtin_ldexpf(__builtin_ldexpf(__builtin_ldexpf(fabsf(xv), 64), 64), 21);
+  float n = __builtin_rintf(X * rc);
+  float r = __builtin_fmaf(-n, c, X);
+  n = n + __builtin_rintf(r * rc);
+  r = __builtin_fmaf(-n, c, X);
+  const float h = 0.5f * c;
+  const bool odd = (n - 2.f * __builtin_rintf(0.5f * n)) != 0.f;
+  if (r > h || (r == h && odd)) n = n + 1.f;
+  else if (r < -h || (r == -h && odd)) n = n - 1.f;
+  return copysignf(n * 0x1p-149f, xv);
+}
 __device__ __forceinline__ float div_const(float xv, float c, float rc) {
   const float q = xv * rc;
   const float e = __builtin_fmaf(-q, c, xv);
-  float q2 = __builtin_fmaf(e, rc, q);
-  if (__builtin_expect(fabsf(q2) < 2.4e-38f, 0)) q2 = div_rare(xv, c);
+  float q2 = copysignf(__builtin_fmaf(e, rc, q), xv);
+  if (__builtin_expect(fabsf(xv) < c * 0x1p-125f && xv != 0.f, 0)) q2 = div_tiny(xv, c, rc);
   return q2;
 }
 __device__ __forceinline__ int imin_(int a, int b) { return a < b ? a : b; }

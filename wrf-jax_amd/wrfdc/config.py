@@ -88,6 +88,8 @@ class DycoreConfig:
             raise ValueError("i pitch smaller than the staggered extent")
         if self.nz > 150:
             raise ValueError("at most 150 vertical levels are supported (LDS-resident columns)")
+        if (self.ny + 1) * (self.nz + 1) * self.ip >= 1 << 30:
+            raise ValueError("a 3-D field must hold fewer than 2^30 elements (32-bit byte offsets)")
         return self
 
     def with_dtbc(self, dtbc: float) -> "DycoreConfig":
