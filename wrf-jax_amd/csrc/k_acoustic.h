@@ -96,8 +96,16 @@ __global__ void k_zero_grad_w(const DevCtx xv) {
 }
 
 // ---------------------------------------------------------------------------
-constexpr int kColWaves = 8;  // waves per block of k_acoustic_col
-constexpr int kColGroup = 2;  // levels per register group
+// waves per block of k_acoustic_col and levels per register group (loads of
+// one group are issued together)
+#ifndef WRFDC_COL_WAVES
+#define WRFDC_COL_WAVES 8
+#endif
+#ifndef WRFDC_COL_GROUP
+#define WRFDC_COL_GROUP 2
+#endif
+constexpr int kColWaves = WRFDC_COL_WAVES;
+constexpr int kColGroup = WRFDC_COL_GROUP;
 
 // bytes of dynamic LDS of k_acoustic_col for kde levels
 __host__ __device__ inline size_t acoustic_col_lds(int kde) { return (size_t)3 * (kde + 1) * 64 * sizeof(float); }

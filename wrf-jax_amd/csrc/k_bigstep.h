@@ -306,7 +306,7 @@ __global__ void k_cor_curv(const DevCtx xv, int comp, R3 r) {
 // rk_addtend_dry (module_em.py:959-1148); comp 0 u, 1 v, 2 w/ph, 3 theta
 __global__ void k_addtend(const DevCtx xv, int comp, int rk, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
+  GET_IKJ_P(r);
   const size_t o = XI3(i, k, j);
   if (comp == 0) {
     const float m = A2(FL(msfuy), i, j);
@@ -334,7 +334,7 @@ __global__ void k_addtend(const DevCtx xv, int comp, int rk, R3 r) {
 // _calc_p_rho_phi_nonhydro_opt2 (runtime.py:12369-12537): hypsometric al, EOS p
 __global__ void k_calc_p_rho_phi(const DevCtx xv, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
+  GET_IKJ_P(r);
   const float *muts = FL(muts), *ph = FL(ph_2), *phb = FL(phb), *alb = FL(alb), *pb = FL(pb), *t = FL(t_2);
   const float m = A2(muts, i, j), ptop = x->ptop;
   const float pfu = A1(FL(c3f), k + 1) * m + A1(FL(c4f), k + 1) + ptop;

@@ -10,7 +10,7 @@
 // module_big_step_utilities_em.py:1823-2081, rho at :1965)
 __global__ void k_rho(const DevCtx xv, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
+  GET_IKJ_P(r);
   A3(FL(rho), i, k, j) = (1.f / A3(FL(alt), i, k, j)) * (1.f + A3(SP(moist, 0), i, k, j));
 }
 

@@ -57,7 +57,7 @@ __global__ void k_ssp_mass(const DevCtx xv, int rk) {
 // Full extent i [1, ide], k [1, kde], j [1, jde].
 __global__ void k_ssp_3d(const DevCtx xv, int rk, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
+  GET_IKJ_P(r);
   const int ide = x->ide, jde = x->jde, kde = x->kde;
   const size_t o = XI3(i, k, j);
   const float *c1h = FL(c1h), *c2h = FL(c2h), *c1f = FL(c1f), *c2f = FL(c2f);
@@ -105,7 +105,7 @@ __global__ void k_ssp_3d(const DevCtx xv, int rk, R3 r) {
 // ph = ph_2, t_1 = t_save, mu = mu_2, mut = muts; mass points.
 __global__ void k_calc_p_rho(const DevCtx xv, int step, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
+  GET_IKJ_P(r);
   const size_t o = XI3(i, k, j);
   const float mass = A1(FL(c1h), k) * A2(FL(muts), i, j) + A1(FL(c2h), k);
   const float dry = A1(FL(c1h), k) * A2(FL(mu_2), i, j);
@@ -360,7 +360,7 @@ __global__ void k_advance_w(const DevCtx xv, float dts, float *__restrict__ rhs,
 // iteration also clears points outside the flux ranges.
 __global__ void k_sumflux(const DevCtx xv, int first, int fin, float ns, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
+  GET_IKJ_P(r);
   const int ide = x->ide, jde = x->jde, kde = x->kde;
   const size_t o = XI3(i, k, j);
   const float *c1h = FL(c1h), *c2h = FL(c2h);
@@ -426,7 +426,7 @@ __global__ void k_ssf(const DevCtx xv, int final_step, float dts, float nsm, R3 
 // the old scalar with mu_old = mu_new = mu_1, then the tendency is cleared.
 __global__ void k_update_scalar_pd(const DevCtx xv, float *q_old, float *q_tend, float dt, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
+  GET_IKJ_P(r);
   const size_t o = XI3(i, k, j);
   const float m = A1(FL(c1h), k) * (A2(FL(mu_1), i, j) + A2(FL(mub), i, j)) + A1(FL(c2h), k);
   q_old[o] = (m * q_old[o] + dt * q_tend[o]) / m;
@@ -437,7 +437,7 @@ __global__ void k_update_scalar_pd(const DevCtx xv, float *q_old, float *q_tend,
 // only, times msfty) plus the physics tendency, mass-weighted update.
 __global__ void k_update_scalar(const DevCtx xv, float *q, float *q_old, const float *q_tend, int rk, float dt, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
+  GET_IKJ_P(r);
   const size_t o = XI3(i, k, j);
   const int sz = x->spec_zone;
   const bool inner = (i >= 1 + sz && i <= x->ide - sz - 1 && j >= 1 + sz && j <= x->jde - sz - 1);

@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(256) k_species_update(const DevCtx xv, SpecLis
 // cleared.
 __global__ void k_species_pd_prep(const DevCtx xv, SpecList L, float dt, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
+  GET_IKJ_P(r);
   const size_t o = XI3(i, k, j);
   const float m = A1(FL(c1h), k) * (A2(FL(mu_1), i, j) + A2(FL(mub), i, j)) + A1(FL(c2h), k);
   for (int s = 0; s < L.n; s++) {

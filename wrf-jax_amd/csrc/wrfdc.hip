@@ -34,7 +34,7 @@
 // outside the coupled range; full extent.  stag_w selects w levels and c1f/c2f.
 __global__ void k_mass_weight(const DevCtx xv, const float *fl, float *out, int stag_w, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
+  GET_IKJ_P(r);
   const bool in = (i <= x->ide - 1 && j <= x->jde - 1 && (stag_w || k <= x->kde - 1));
   float v = 0.f;
   if (in) {
@@ -47,7 +47,7 @@ __global__ void k_mass_weight(const DevCtx xv, const float *fl, float *out, int 
 // dst = src on mass points (initialize_moist_old)
 __global__ void k_copy_mass(const DevCtx xv, float *dst, const float *src, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
+  GET_IKJ_P(r);
   A3(dst, i, k, j) = A3(src, i, k, j);
 }
 
@@ -181,8 +181,10 @@ struct Run {
   R3 full3() const { return R3{1, ide, 1, kde, 1, jde}; }
   R3 mass3() const { return R3{1, ide - 1, 1, kde - 1, 1, jde - 1}; }
   R2 mass2() const { return R2{1, ide - 1, 1, jde - 1}; }
-  // 1-D grids decoded by GET_IKJ / GET_IJ (XCD-contiguous tiles)
-  static dim3 g3(const R3 &r) { return dim3(cdiv(r.i1 - r.i0 + 1, 64) * cdiv(r.k1 - r.k0 + 1, 4) * (unsigned)(r.j1 - r.j0 + 1)); }
+  // 1-D grids decoded by GET_IKJ (stencil kernels), GET_IKJ_P (pointwise
+  // kernels) and GET_IJ
+  static dim3 g3(const R3 &r) { return dim3(cdiv(r.i1 - r.i0 + 1, 64) * cdiv(r.k1 - r.k0 + 1, 4) * 4u * cdiv(r.j1 - r.j0 + 1, 4)); }
+  static dim3 g3p(const R3 &r) { return dim3(cdiv(r.i1 - r.i0 + 1, 64) * cdiv(r.k1 - r.k0 + 1, 4) * (unsigned)(r.j1 - r.j0 + 1)); }
   static dim3 g2(const R2 &r) { return dim3(cdiv(r.i1 - r.i0 + 1, 64) * (unsigned)(r.j1 - r.j0 + 1)); }
   static dim3 b3() { return dim3(64, 4, 1); }
 
@@ -309,7 +311,7 @@ struct Run {
                {F(F_mu_tendf), X->n2}, {F(F_moist_tend), n3 * (size_t)X->c.nmoist},
                {F(F_scalar_tend), n3 * (size_t)X->c.nscalar}});
     R3 m = mass3();
-    k_rho<<<g3(m), b3(), 0, s>>>(d, m);
+    k_rho<<<g3p(m), b3(), 0, s>>>(d, m);
   }
   void compute_diff_metrics() const {
     R3 r = full3();
@@ -450,22 +452,22 @@ struct Run {
       relax(F(F_rv), F(F_rv_tendf), 'v', false, false, bp(F_v_bys, F_v_bye, F_v_bxs, F_v_bxe), bp(F_v_btys, F_v_btye, F_v_btxs, F_v_btxe));
       float *tmp = Scr(S_HAT);
       R3 r = full3();
-      k_mass_weight<<<g3(r), b3(), 0, s>>>(d, F(F_ph_2), tmp, 1, r);
+      k_mass_weight<<<g3p(r), b3(), 0, s>>>(d, F(F_ph_2), tmp, 1, r);
       relax(tmp, F(F_ph_tendf), 'h', true, false, bp(F_ph_bys, F_ph_bye, F_ph_bxs, F_ph_bxe), bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe));
-      k_mass_weight<<<g3(r), b3(), 0, s>>>(d, F(F_t_2), tmp, 0, r);
+      k_mass_weight<<<g3p(r), b3(), 0, s>>>(d, F(F_t_2), tmp, 0, r);
       relax(tmp, F(F_t_tendf), 't', false, false, bp(F_t_bys, F_t_bye, F_t_bxs, F_t_bxe), bp(F_t_btys, F_t_btye, F_t_btxs, F_t_btxe));
       relax(F(F_mu_2), F(F_mu_tend), 'm', true, true, bp(F_mu_bys, F_mu_bye, F_mu_bxs, F_mu_bxe),
             bp(F_mu_btys, F_mu_btye, F_mu_btxs, F_mu_btxe));
     }
     const int rk = X->rk;
     R3 r0{1, ide, 1, kde - 1, 1, jde - 1};
-    k_addtend<<<g3(r0), b3(), 0, s>>>(d, 0, rk, r0);
+    k_addtend<<<g3p(r0), b3(), 0, s>>>(d, 0, rk, r0);
     R3 r1{1, ide - 1, 1, kde - 1, 1, jde};
-    k_addtend<<<g3(r1), b3(), 0, s>>>(d, 1, rk, r1);
+    k_addtend<<<g3p(r1), b3(), 0, s>>>(d, 1, rk, r1);
     R3 r2{1, ide - 1, 1, kde, 1, jde - 1};
-    k_addtend<<<g3(r2), b3(), 0, s>>>(d, 2, rk, r2);
+    k_addtend<<<g3p(r2), b3(), 0, s>>>(d, 2, rk, r2);
     R3 r3 = mass3();
-    k_addtend<<<g3(r3), b3(), 0, s>>>(d, 3, rk, r3);
+    k_addtend<<<g3p(r3), b3(), 0, s>>>(d, 3, rk, r3);
     spec_overlay(F(F_ru_tend), 'u', false, false, bp(F_u_btys, F_u_btye, F_u_btxs, F_u_btxe));
     spec_overlay(F(F_rv_tend), 'v', false, false, bp(F_v_btys, F_v_btye, F_v_btxs, F_v_btxe));
     spec_overlay(F(F_ph_tend), 'h', true, false, bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe));
@@ -475,11 +477,11 @@ struct Run {
   void small_step_prep() const {
     k_ssp_mass<<<dim3(cdiv(ide, 64), jde), 64, 0, s>>>(d, X->rk);
     R3 r = full3();
-    k_ssp_3d<<<g3(r), b3(), 0, s>>>(d, X->rk, r);
+    k_ssp_3d<<<g3p(r), b3(), 0, s>>>(d, X->rk, r);
   }
   void calc_p_rho(int step) const {
     R3 m = mass3();
-    k_calc_p_rho<<<g3(m), b3(), 0, s>>>(d, step, m);
+    k_calc_p_rho<<<g3p(m), b3(), 0, s>>>(d, step, m);
   }
   void calc_coef_w() const {
     R2 c = mass2();
@@ -502,7 +504,7 @@ struct Run {
   }
   void sumflux() const {
     R3 r = full3();
-    k_sumflux<<<g3(r), b3(), 0, s>>>(d, X->iteration == 1, X->iteration == X->nsmall, (float)X->nsmall, r);
+    k_sumflux<<<g3p(r), b3(), 0, s>>>(d, X->iteration == 1, X->iteration == X->nsmall, (float)X->nsmall, r);
   }
   void spec_bdyupdate_ph() const {
     Z z = zd('h', true, false);
@@ -529,11 +531,11 @@ struct Run {
   }
   void calc_p_rho_phi() const {
     R3 m = mass3();
-    k_calc_p_rho_phi<<<g3(m), b3(), 0, s>>>(d, m);
+    k_calc_p_rho_phi<<<g3p(m), b3(), 0, s>>>(d, m);
   }
   void update_scalar_pd(float *q_old, float *q_tend) const {
     R3 m = mass3();
-    k_update_scalar_pd<<<g3(m), b3(), 0, s>>>(d, q_old, q_tend, (float)X->dt_rk, m);
+    k_update_scalar_pd<<<g3p(m), b3(), 0, s>>>(d, q_old, q_tend, (float)X->dt_rk, m);
   }
   void species_advance(float *q, float *q_old, float *q_tend, bool is_qv, bool is_moist, bool pd) const {
     float *adv = F(F_advect_tend);
@@ -544,11 +546,11 @@ struct Run {
     if (is_qv && X->rk == 1) {
       float *tmp = Scr(S_HAT);
       R3 r = full3();
-      k_mass_weight<<<g3(r), b3(), 0, s>>>(d, q, tmp, 0, r);
+      k_mass_weight<<<g3p(r), b3(), 0, s>>>(d, q, tmp, 0, r);
       relax(tmp, q_tend, 'q', false, false, bp(F_qv_bys, F_qv_bye, F_qv_bxs, F_qv_bxe), bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe));
       spec_overlay(q_tend, 't', false, false, bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe));
     }
-    k_update_scalar<<<g3(m), b3(), 0, s>>>(d, q, q_old, q_tend, X->rk, (float)X->dt_rk, m);
+    k_update_scalar<<<g3p(m), b3(), 0, s>>>(d, q, q_old, q_tend, X->rk, (float)X->dt_rk, m);
     if (is_moist && !is_qv) {
       k_flow_dep_rows<<<dim3(cdiv(ide - 1, 64), kde - 1, 2), 64, 0, s>>>(d, q, F(F_rv_m));
       k_flow_dep_cols<<<dim3(cdiv(jde - 1, 64), kde - 1, 2), 64, 0, s>>>(d, q, F(F_ru_m));
@@ -572,8 +574,8 @@ struct Run {
     const size_t n = X->n3;
     const float *sm = to_alt ? user_moist : alt_moist, *ss = to_alt ? user_scalar : alt_scalar;
     float *dm = to_alt ? alt_moist : user_moist, *ds = to_alt ? alt_scalar : user_scalar;
-    for (int q = 0; q < X->c.nmoist; q++) k_copy_mass<<<g3(r), b3(), 0, s>>>(d, dm + q * n, sm + q * n, r);
-    for (int q = 0; q < X->c.nscalar; q++) k_copy_mass<<<g3(r), b3(), 0, s>>>(d, ds + q * n, ss + q * n, r);
+    for (int q = 0; q < X->c.nmoist; q++) k_copy_mass<<<g3p(r), b3(), 0, s>>>(d, dm + q * n, sm + q * n, r);
+    for (int q = 0; q < X->c.nscalar; q++) k_copy_mass<<<g3p(r), b3(), 0, s>>>(d, ds + q * n, ss + q * n, r);
   }
   void fill_copy(float *dst, const float *src, size_t n) const {
     unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
@@ -584,9 +586,9 @@ struct Run {
   void sync_old(bool moist, bool scalar) const {
     R3 m = mass3();
     if (moist)
-      for (int q = 0; q < X->c.nmoist; q++) k_copy_mass<<<g3(m), b3(), 0, s>>>(d, Sp(F_moist_old, q), Sp(F_moist, q), m);
+      for (int q = 0; q < X->c.nmoist; q++) k_copy_mass<<<g3p(m), b3(), 0, s>>>(d, Sp(F_moist_old, q), Sp(F_moist, q), m);
     if (scalar)
-      for (int q = 0; q < X->c.nscalar; q++) k_copy_mass<<<g3(m), b3(), 0, s>>>(d, Sp(F_scalar_old, q), Sp(F_scalar, q), m);
+      for (int q = 0; q < X->c.nscalar; q++) k_copy_mass<<<g3p(m), b3(), 0, s>>>(d, Sp(F_scalar_old, q), Sp(F_scalar, q), m);
   }
   // Buffers: rk1 reads the stencils from q_old (== q) and writes q in place;
   // rk2 reads q and writes the second buffer, which later kernels of the step
@@ -602,7 +604,7 @@ struct Run {
       // relax_bdy_scalar + spec_bdy_scalar of QV into its tendency
       float *tmp = Scr(S_HAT);
       R3 r = full3();
-      k_mass_weight<<<g3(r), b3(), 0, s>>>(d, Sp(F_moist, 0), tmp, 0, r);
+      k_mass_weight<<<g3p(r), b3(), 0, s>>>(d, Sp(F_moist, 0), tmp, 0, r);
       relax(tmp, Sp(F_moist_tend, 0), 'q', false, false, bp(F_qv_bys, F_qv_bye, F_qv_bxs, F_qv_bxe), bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe));
       spec_overlay(Sp(F_moist_tend, 0), 't', false, false, bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe));
     }
@@ -629,7 +631,7 @@ struct Run {
         pd.scale[q] = X->spec_scale + (size_t)q * n3;
         pd.anti[q] = X->spec_anti + (size_t)q * 3 * n3;
       }
-      k_species_pd_prep<<<g3(m), b3(), 0, s>>>(d, pd, dt, m);
+      k_species_pd_prep<<<g3p(m), b3(), 0, s>>>(d, pd, dt, m);
       {
         BcBatch bc(this);
         for (int q = 0; q < pd.n; q++) bc.add3(pd.old[q], 'p');
@@ -698,7 +700,7 @@ struct Run {
       // scalars' q_old <- q of the first-stage update is done here as well
       BcBatch bc(this);
       for (int q = 0; q < X->c.nmoist; q++) {
-        k_copy_mass<<<g3(m), b3(), 0, s>>>(d, Sp(F_moist_old, q), Sp(F_moist, q), m);
+        k_copy_mass<<<g3p(m), b3(), 0, s>>>(d, Sp(F_moist_old, q), Sp(F_moist, q), m);
         bc.add3(Sp(F_moist_old, q), 'p');
       }
     }

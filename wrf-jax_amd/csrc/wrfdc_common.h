@@ -114,15 +114,29 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
   const unsigned xc = b & 7u, idx = b >> 3, q = nb >> 3, rm = nb & 7u;
   return xc * q + (xc < rm ? xc : rm) + idx;
 }
-// thread -> (i, k, j) with blockDim (64, 4, 1) and a 1-D grid of
-// ceil(ni/64) * ceil(nk/4) * nj blocks; tiles run i fastest, then k, then j
+// Stencil kernels: thread -> (i, k, j) with blockDim (64, 4, 1) (a block is
+// 64 i x 4 levels of one row) and a 1-D grid of
+// ceil(ni/64) * ceil(nk/4) * 4 * ceil(nj/4) blocks.  Tiles run i fastest,
+// then 4 consecutive rows, then k, then groups of 4 rows, in XCD-contiguous
+// order: the j+-1 rows of a stencil were read a few blocks earlier by the
+// same XCD and are still in its L2.
 #define GET_IKJ(r)                                                                           \
   const unsigned gx_ = (unsigned)((r).i1 - (r).i0 + 64) / 64u;                               \
   const unsigned gy_ = (unsigned)((r).k1 - (r).k0 + 4) / 4u;                                 \
   const unsigned t_ = xcd_tile(blockIdx.x, gridDim.x);                                       \
+  const unsigned u_ = t_ / gx_;                                                              \
   const int i = (r).i0 + (int)((t_ % gx_) * 64u + threadIdx.x);                              \
-  const int k = (r).k0 + (int)(((t_ / gx_) % gy_) * 4u + threadIdx.y);                       \
-  const int j = (r).j0 + (int)(t_ / (gx_ * gy_));                                            \
+  const int k = (r).k0 + (int)(((u_ / 4u) % gy_) * 4u + threadIdx.y);                        \
+  const int j = (r).j0 + (int)((u_ / 4u / gy_) * 4u + u_ % 4u);                              \
+  if (i > (r).i1 || k > (r).k1 || j > (r).j1) return
+// Pointwise kernels (no neighbour in j): the same block shape, a 1-D grid of
+// ceil(ni/64) * ceil(nk/4) * nj blocks in plain order (i, then k, then j).
+#define GET_IKJ_P(r)                                                                         \
+  const unsigned gx_ = (unsigned)((r).i1 - (r).i0 + 64) / 64u;                               \
+  const unsigned gy_ = (unsigned)((r).k1 - (r).k0 + 4) / 4u;                                 \
+  const int i = (r).i0 + (int)((blockIdx.x % gx_) * 64u + threadIdx.x);                      \
+  const int k = (r).k0 + (int)(((blockIdx.x / gx_) % gy_) * 4u + threadIdx.y);               \
+  const int j = (r).j0 + (int)(blockIdx.x / (gx_ * gy_));                                    \
   if (i > (r).i1 || k > (r).k1 || j > (r).j1) return
 // thread -> (i, j) for column kernels: blockDim (64, 1, 1), 1-D grid of
 // ceil(ni/64) * nj blocks

@@ -85,19 +85,21 @@ def dominant_kernel(cfg) -> KernelModel:
     return KERNELS[DOMINANT]
 
 
-def measured_traffic(kernel: str, workload: str):
+def measured_traffic(kernel: str, workload: str, path: str | None = None):
     """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary (already scaled by the copy-kernel calibration), or None."""
-    repo = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
-    path = os.path.join(repo, "profiles", f"traffic_{workload}.md")
+    if path is None:
+        repo = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+        path = os.path.join(repo, "profiles", f"traffic_{workload}.md")
     if not os.path.exists(path):
         return None
-    pat = re.compile(rf"^\|\s*{re.escape(kernel)}\s*\|\s*([0-9.eE+]+)\s*\|\s*([0-9.eE+]+)\s*\|\s*([0-9.eE+]+)\s*\|")
+    # the kernel symbol may carry template arguments (k_acoustic_col<8>)
+    pat = re.compile(rf"^\|\s*{re.escape(kernel)}(<[^>|]*>)?\s*\|\s*([0-9.eE+]+)\s*\|\s*([0-9.eE+]+)\s*\|\s*([0-9.eE+]+)\s*\|")
     with open(path) as fh:
         for line in fh:
             m = pat.match(line.strip())
             if m:
-                return float(m.group(3))
+                return float(m.group(4))
     return None
 
 

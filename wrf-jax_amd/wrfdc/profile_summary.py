@@ -36,9 +36,10 @@ def kernels(db: str) -> str:
     cur = sqlite3.connect(db).cursor()
     rows = list(cur.execute("select name, total_calls, total_duration, average from top_kernels"))
     total = sum(r[2] for r in rows)
-    out = [f"Total kernel time {total / 1e6:.2f} ms", "", "| kernel | calls | total us | avg us | % |", "|---|---:|---:|---:|---:|"]
+    # the top_kernels view reports durations in microseconds
+    out = [f"Total kernel time {total / 1e3:.2f} ms", "", "| kernel | calls | total us | avg us | % |", "|---|---:|---:|---:|---:|"]
     for name, calls, dur, avg in rows:
-        out.append(f"| {_short(name)} | {calls} | {dur / 1e3:.0f} | {avg / 1e3:.1f} | {100.0 * dur / total:.2f} |")
+        out.append(f"| {_short(name)} | {calls} | {dur:.0f} | {avg:.1f} | {100.0 * dur / total:.2f} |")
     return "\n".join(out)
 
 
@@ -87,7 +88,7 @@ def traffic(fetch_db: str, write_db: str) -> str:
         fm, wm = fb / nf, wb / nw
         rows.append((fm + wm, k, fm, wm, n))
     for tot, k, fm, wm, n in sorted(rows, reverse=True):
-        out.append(f"| {k} | {fm / 2**20:.1f} | {wm / 2**20:.1f} | {tot:.4g} | {n} |")
+        out.append(f"| {k} | {fm / 2**20:.1f} | {wm / 2**20:.1f} | {tot:.0f} | {n} |")
     return "\n".join(out)
 
 
