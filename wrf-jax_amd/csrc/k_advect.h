@@ -4,6 +4,7 @@
 // Each thread owns one output point and evaluates the fluxes of its faces.
 #pragma once
 #include "wrfdc_common.h"
+#include "k_bigstep.h"
 
 __device__ __forceinline__ float flux3(float qm2, float qm1, float q0, float qp1, float ua) {
   return div_const(7.f * (q0 + qm1) - (qp1 + qm2), 12.f, 1.f / 12.f) +
@@ -98,9 +99,12 @@ __device__ __forceinline__ float fz_s(const DevCtx *__restrict__ x, const float 
 }
 
 // advect_scalar (module_advect_em.py:90-369): tend -= div(flux), mass points
-// (face-sharing blocks; r = mass points; y faces j in [2, jde-1])
+// (face-sharing blocks; r = mass points; y faces j in [2, jde-1]).
+// TEND: the theta tendency of the RK step, from zero (the cleared array) plus
+// rk_addtend_dry, written once.
+template <bool TEND>
 __global__ void __launch_bounds__(256) k_advect_scalar(const DevCtx xv, const float *q, float *tend, const float *ru, const float *rv,
-                                                       const float *rom, R3 r) {
+                                                       const float *rom, R3 r, int rk) {
   const DevCtx *__restrict__ x = &xv;
   __shared__ float ys[4][64];
   GET_FACE_BLOCK(r);
@@ -116,10 +120,11 @@ __global__ void __launch_bounds__(256) k_advect_scalar(const DevCtx xv, const fl
   const float fw = (i >= 2 && i <= ide - 1) ? fx_s(x, q, ru, i, k, j) : 0.f;
   const float fe = lane_next(fw, lane);
   if (!own) return;
-  float t = A3(tend, ic, k, j);
+  float t = TEND ? 0.f : A3(tend, ic, k, j);
   if (j >= 2 && j <= jde - 2) t -= A2(msftx, ic, j) * x->rdy * (fn - fs);
   if (ic >= 2 && ic <= ide - 2) t -= A2(msftx, ic, j) * x->rdx * (fe - fw);
   t -= A1(rdzw, k) * (fz_s(x, q, rom, ic, k + 1, j) - fz_s(x, q, rom, ic, k, j));
+  if (TEND) t = addtend_t(x, ic, k, j, rk, t);
   A3(tend, ic, k, j) = t;
 }
 
@@ -163,28 +168,40 @@ __device__ __forceinline__ float fz_u(const DevCtx *__restrict__ x, const float 
   return vel * flux3(A3(u, i, k - 2, j), A3(u, i, k - 1, j), A3(u, i, k, j), A3(u, i, k + 1, j), -vel);
 }
 // advect_u (module_advect_em.py:762-1034); r covers i [2, ide-1], j [1, jde-1]
-// (face-sharing blocks: x faces i in [2, ide], y faces j in [2, jde-1])
+// (face-sharing blocks: x faces i in [2, ide], y faces j in [2, jde-1]).
+// TEND: the u tendency of the RK step over i [1, ide]: from zero (the cleared
+// array), advection, horizontal pressure gradient and Coriolis/curvature at
+// i [2, ide-1] in the reference's order, then rk_addtend_dry; written once.
+template <bool TEND>
 __global__ void __launch_bounds__(256) k_advect_u(const DevCtx xv, const float *u, float *tend, const float *ru, const float *rv,
-                                                  const float *rom, R3 r) {
+                                                  const float *rom, R3 r, int rk) {
   const DevCtx *__restrict__ x = &xv;
   __shared__ float ys[4][64];
   GET_FACE_BLOCK(r);
   const float *msfux = FL(msfux), *rdzw = FL(rdnw);
   const int ide = x->ide, jde = x->jde;
-  const float fs = (j >= 2 && j <= jde - 1) ? fy_u(x, u, rv, ic, k, j) : 0.f;
+  const bool adv = !TEND || (ic >= 2 && ic <= ide - 1);
+  const float fs = (adv && j >= 2 && j <= jde - 1) ? fy_u(x, u, rv, ic, k, j) : 0.f;
   ys[wv][lane] = fs;
   __syncthreads();
   if (!jrow) return;
   float fn;
   if (wv < 3) fn = ys[wv + 1][lane];
-  else fn = (j + 1 <= jde - 1) ? fy_u(x, u, rv, ic, k, j + 1) : 0.f;
+  else fn = (adv && j + 1 <= jde - 1) ? fy_u(x, u, rv, ic, k, j + 1) : 0.f;
   const float fw = (i >= 2 && i <= ide) ? fx_u(x, u, ru, i, k, j) : 0.f;
   const float fe = lane_next(fw, lane);
   if (!own) return;
-  float t = A3(tend, ic, k, j);
-  if (j >= 2 && j <= jde - 2) t -= A2(msfux, ic, j) * x->rdy * (fn - fs);
-  t -= A2(msfux, ic, j) * x->rdx * (fe - fw);
-  t -= A1(rdzw, k) * (fz_u(x, u, rom, ic, k + 1, j) - fz_u(x, u, rom, ic, k, j));
+  float t = TEND ? 0.f : A3(tend, ic, k, j);
+  if (adv) {
+    if (j >= 2 && j <= jde - 2) t -= A2(msfux, ic, j) * x->rdy * (fn - fs);
+    t -= A2(msfux, ic, j) * x->rdx * (fe - fw);
+    t -= A1(rdzw, k) * (fz_u(x, u, rom, ic, k + 1, j) - fz_u(x, u, rom, ic, k, j));
+    if (TEND) {
+      t = t - hpg_term(x, 0, ic, k, j);
+      t = cor_curv_u(x, ic, k, j, t);
+    }
+  }
+  if (TEND) t = addtend_u(x, ic, k, j, rk, t);
   A3(tend, ic, k, j) = t;
 }
 // ---------------- v momentum (v rows j in [2, jde-1]) -----------------------
@@ -227,28 +244,39 @@ __device__ __forceinline__ float fz_v(const DevCtx *__restrict__ x, const float 
   return vel * flux3(A3(v, i, k - 2, j), A3(v, i, k - 1, j), A3(v, i, k, j), A3(v, i, k + 1, j), -vel);
 }
 // advect_v (module_advect_em.py:1037-1354); r covers i [1, ide-1], j [2, jde-1]
-// (face-sharing blocks: x faces i in [2, ide-1], y faces j in [2, jde])
+// (face-sharing blocks: x faces i in [2, ide-1], y faces j in [2, jde]).
+// TEND: the v tendency of the RK step over j [1, jde] (advection, pressure
+// gradient, Coriolis/curvature at j [2, jde-1], then rk_addtend_dry).
+template <bool TEND>
 __global__ void __launch_bounds__(256) k_advect_v(const DevCtx xv, const float *v, float *tend, const float *ru, const float *rv,
-                                                  const float *rom, R3 r) {
+                                                  const float *rom, R3 r, int rk) {
   const DevCtx *__restrict__ x = &xv;
   __shared__ float ys[4][64];
   GET_FACE_BLOCK(r);
   const float *msfvy = FL(msfvy), *msfvx = FL(msfvx), *rdzw = FL(rdnw);
   const int ide = x->ide, jde = x->jde;
-  const float fs = (j <= jde) ? fy_v(x, v, rv, ic, k, j) : 0.f;  // j >= 2
+  const bool adv = !TEND || (j >= 2 && j <= jde - 1);  // whole wave
+  const float fs = (j >= 2 && j <= jde) ? fy_v(x, v, rv, ic, k, j) : 0.f;
   ys[wv][lane] = fs;
   __syncthreads();
   if (!jrow) return;
   float fn;
   if (wv < 3) fn = ys[wv + 1][lane];
-  else fn = fy_v(x, v, rv, ic, k, j + 1);  // j + 1 <= jde
-  const float fw = (i >= 2 && i <= ide - 1) ? fx_v(x, v, ru, i, k, j) : 0.f;
+  else fn = (j + 1 >= 2 && j + 1 <= jde) ? fy_v(x, v, rv, ic, k, j + 1) : 0.f;
+  const float fw = (adv && i >= 2 && i <= ide - 1) ? fx_v(x, v, ru, i, k, j) : 0.f;
   const float fe = lane_next(fw, lane);
   if (!own) return;
-  float t = A3(tend, ic, k, j);
-  t -= A2(msfvy, ic, j) * x->rdy * (fn - fs);
-  if (ic >= 2 && ic <= ide - 2) t -= A2(msfvy, ic, j) * x->rdx * (fe - fw);
-  t -= (A2(msfvy, ic, j) / A2(msfvx, ic, j)) * A1(rdzw, k) * (fz_v(x, v, rom, ic, k + 1, j) - fz_v(x, v, rom, ic, k, j));
+  float t = TEND ? 0.f : A3(tend, ic, k, j);
+  if (adv) {
+    t -= A2(msfvy, ic, j) * x->rdy * (fn - fs);
+    if (ic >= 2 && ic <= ide - 2) t -= A2(msfvy, ic, j) * x->rdx * (fe - fw);
+    t -= (A2(msfvy, ic, j) / A2(msfvx, ic, j)) * A1(rdzw, k) * (fz_v(x, v, rom, ic, k + 1, j) - fz_v(x, v, rom, ic, k, j));
+    if (TEND) {
+      t = t - hpg_term(x, 1, ic, k, j);
+      t = cor_curv_v(x, ic, k, j, t);
+    }
+  }
+  if (TEND) t = addtend_v(x, ic, k, j, rk, t);
   A3(tend, ic, k, j) = t;
 }
 // ---------------- w momentum (levels k in [2, kde]) -------------------------
@@ -280,30 +308,54 @@ __device__ __forceinline__ float fz_w(const DevCtx *__restrict__ x, const float 
   return vel * flux3(A3(w, i, k - 2, j), A3(w, i, k - 1, j), A3(w, i, k, j), A3(w, i, k + 1, j), -vel);
 }
 // advect_w (module_advect_em.py:1357-1640); r covers i [1, ide-1], k [2, kde], j [1, jde-1]
-// (face-sharing blocks: x faces i in [2, ide-1], y faces j in [2, jde-1])
+// (face-sharing blocks: x faces i in [2, ide-1], y faces j in [2, jde-1]).
+// TEND: the w tendency of the RK step over k [1, kde]: advection, buoyancy
+// and pressure (pg_buoy_w, reading cqw before k_cqw_inv replaces it) and, below
+// the top, Coriolis/curvature at k [2, kde]; then rk_addtend_dry.
+template <bool TEND>
 __global__ void __launch_bounds__(256) k_advect_w(const DevCtx xv, const float *w, float *tend, const float *ru, const float *rv,
-                                                  const float *rom, R3 r) {
+                                                  const float *rom, R3 r, int rk) {
   const DevCtx *__restrict__ x = &xv;
   __shared__ float ys[4][64];
   GET_FACE_BLOCK(r);
   const float *msftx = FL(msftx), *rdzu = FL(rdn);
   const int ide = x->ide, jde = x->jde;
-  const float fs = (j >= 2 && j <= jde - 1) ? fy_w(x, w, rv, ic, k, j) : 0.f;
+  const bool adv = !TEND || k >= 2;  // whole block
+  const float fs = (adv && j >= 2 && j <= jde - 1) ? fy_w(x, w, rv, ic, k, j) : 0.f;
   ys[wv][lane] = fs;
   __syncthreads();
   if (!jrow) return;
   float fn;
   if (wv < 3) fn = ys[wv + 1][lane];
-  else fn = (j + 1 <= jde - 1) ? fy_w(x, w, rv, ic, k, j + 1) : 0.f;
-  const float fw = (i >= 2 && i <= ide - 1) ? fx_w(x, w, ru, i, k, j) : 0.f;
+  else fn = (adv && j + 1 <= jde - 1) ? fy_w(x, w, rv, ic, k, j + 1) : 0.f;
+  const float fw = (adv && i >= 2 && i <= ide - 1) ? fx_w(x, w, ru, i, k, j) : 0.f;
   const float fe = lane_next(fw, lane);
   if (!own) return;
-  float t = A3(tend, ic, k, j);
-  if (j >= 2 && j <= jde - 2) t -= A2(msftx, ic, j) * x->rdy * (fn - fs);
-  if (ic >= 2 && ic <= ide - 2) t -= A2(msftx, ic, j) * x->rdx * (fe - fw);
-  if (k <= x->kde - 1) t -= A1(rdzu, k) * (fz_w(x, w, rom, ic, k + 1, j) - fz_w(x, w, rom, ic, k, j));
-  else t += 2.f * A1(rdzu, k - 1) * fz_w(x, w, rom, ic, k, j);
+  float t = TEND ? 0.f : A3(tend, ic, k, j);
+  if (adv) {
+    if (j >= 2 && j <= jde - 2) t -= A2(msftx, ic, j) * x->rdy * (fn - fs);
+    if (ic >= 2 && ic <= ide - 2) t -= A2(msftx, ic, j) * x->rdx * (fe - fw);
+    if (k <= x->kde - 1) t -= A1(rdzu, k) * (fz_w(x, w, rom, ic, k + 1, j) - fz_w(x, w, rom, ic, k, j));
+    else t += 2.f * A1(rdzu, k - 1) * fz_w(x, w, rom, ic, k, j);
+    if (TEND) {
+      t = buoy_w(x, ic, k, j, t);
+      if (k <= x->kde - 1) t = cor_curv_w(x, ic, k, j, t);
+    }
+  }
+  if (TEND) t = addtend_w(x, ic, k, j, rk, t);
   A3(tend, ic, k, j) = t;
+}
+
+// the phi tendency of the RK step over k [1, kde]: rhs_ph at k [2, kde] from
+// zero (the cleared array), then rk_addtend_dry; r = i [1, ide-1], k [1, kde],
+// j [1, jde-1]
+__global__ void __launch_bounds__(256) k_tend_ph(const DevCtx xv, R3 r, int rk) {
+  const DevCtx *__restrict__ x = &xv;
+  GET_IKJ(r);
+  float t = 0.f;
+  if (k >= 2) t = rhs_ph_val(x, i, k, j, t);
+  t = addtend_ph(x, i, k, j, rk, t);
+  A3(FL(ph_tend), i, k, j) = t;
 }
 // ---------------- positive-definite scalar advection ------------------------
 // pass 1: high-order minus upwind (antidiffusive) and upwind fluxes

@@ -122,15 +122,13 @@ __global__ void k_ww_cp(const DevCtx xv, R2 r) {
 // rhs_ph (module_big_step_utilities_em.py:2386-2794): vertical advection of
 // phi, the w g term and horizontal phi advection (6th/4th/2nd order with the
 // reference's specified-boundary column pattern).  k in [2, kde].
-__global__ void k_rhs_ph(const DevCtx xv, R3 r) {
-  const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
+// the phi tendency at (i,k,j), k in [2, kde], starting from `out`
+__device__ __forceinline__ float rhs_ph_val(const DevCtx *__restrict__ x, int i, int k, int j, float out) {
   const int ide = x->ide, jde = x->jde, kde = x->kde, kte = kde;
   const float *u = FL(u_2), *v = FL(v_2), *ww = FL(ww), *ph = FL(ph_2), *phb = FL(phb), *w = FL(w_2);
   const float *mut = FL(mut), *muu = FL(muu), *muv = FL(muv), *c1f = FL(c1f), *c2f = FL(c2f);
   const float *fnm = FL(fnm), *fnp = FL(fnp), *rdnw = FL(rdnw), *msfux = FL(msfux), *msfvy = FL(msfvy), *msfty = FL(msfty);
   const float rdx = x->rdx, rdy = x->rdy, g = x->g, cfn = x->cfn, cfn1 = x->cfn1;
-  float out = A3(FL(ph_tend), i, k, j);
   if (k <= kte - 1) {
     float wdk = 0.5f * (A3(ww, i, k, j) + A3(ww, i, k - 1, j)) * A1(rdnw, k - 1) *
                 (A3(ph, i, k, j) - A3(ph, i, k - 1, j) + A3(phb, i, k, j) - A3(phb, i, k - 1, j));
@@ -202,7 +200,12 @@ __global__ void k_rhs_ph(const DevCtx xv, R3 r) {
       }
     }
   }
-  A3(FL(ph_tend), i, k, j) = out;
+  return out;
+}
+__global__ void k_rhs_ph(const DevCtx xv, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
+  GET_IKJ(r);
+  A3(FL(ph_tend), i, k, j) = rhs_ph_val(x, i, k, j, A3(FL(ph_tend), i, k, j));
 }
 
 // horizontal_pressure_gradient (module_big_step_utilities_em.py:1452-1671).
@@ -214,9 +217,9 @@ __device__ __forceinline__ float hpg_dpn(const DevCtx *__restrict__ x, const flo
   if (k >= x->kde) return 0.f;
   return 0.5f * (A1(FL(fnm), k) * (A3(p, i - di, k, j - dj) + A3(p, i, k, j)) + A1(FL(fnp), k) * (A3(p, i - di, k - 1, j - dj) + A3(p, i, k - 1, j)));
 }
-__global__ void k_hpg(const DevCtx xv, int axis, R3 r) {
-  const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
+// the amount the pressure gradient subtracts from the u (axis 0) or v
+// (axis 1) tendency at (i,k,j): cq * dp
+__device__ __forceinline__ float hpg_term(const DevCtx *__restrict__ x, int axis, int i, int k, int j) {
   const float *ph = FL(ph_2), *alt = FL(alt), *p = FL(p), *pb = FL(pb), *al = FL(al), *php = FL(php), *mu = FL(mu_2);
   const float *c1h = FL(c1h), *c2h = FL(c2h), *rdnw = FL(rdnw);
   const int di = axis == 0 ? 1 : 0, dj = axis == 0 ? 0 : 1;
@@ -230,10 +233,40 @@ __global__ void k_hpg(const DevCtx xv, int axis, R3 r) {
   dp = dp + rr * rd * (A3(php, i, k, j) - A3(php, i - di, k, j - dj)) *
                 (A1(rdnw, k) * (hpg_dpn(x, p, i, j, di, dj, k + 1) - hpg_dpn(x, p, i, j, di, dj, k)) -
                  0.5f * (A1(c1h, k) * A2(mu, i - di, j - dj) + A1(c1h, k) * A2(mu, i, j)));
-  if (axis == 0) A3(FL(ru_tend), i, k, j) = A3(FL(ru_tend), i, k, j) - A3(FL(cqu), i, k, j) * dp;
-  else A3(FL(rv_tend), i, k, j) = A3(FL(rv_tend), i, k, j) - A3(FL(cqv), i, k, j) * dp;
+  return (axis == 0 ? A3(FL(cqu), i, k, j) : A3(FL(cqv), i, k, j)) * dp;
+}
+__global__ void k_hpg(const DevCtx xv, int axis, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
+  GET_IKJ(r);
+  if (axis == 0) A3(FL(ru_tend), i, k, j) = A3(FL(ru_tend), i, k, j) - hpg_term(x, 0, i, k, j);
+  else A3(FL(rv_tend), i, k, j) = A3(FL(rv_tend), i, k, j) - hpg_term(x, 1, i, k, j);
 }
 
+// pg_buoy_w (module_big_step_utilities_em.py:1674-1800) at one w level
+// k in [2, kde], added to rwt; reads cqw as it is before the column kernel
+// below replaces it by 1 / (1 + cqw)
+__device__ __forceinline__ float buoy_w(const DevCtx *__restrict__ x, int i, int k, int j, float rwt) {
+  const float *p = FL(p), *mu = FL(mu_2), *mub = FL(mub), *c1f = FL(c1f), *c2f = FL(c2f);
+  const int kde = x->kde;
+  const float gm = (1.f / A2(FL(msfty), i, j)) * x->g;
+  if (k == kde) {
+    const float cq = A3(FL(cqw), i, kde - 1, j);
+    const float cq1 = 1.f / (1.f + cq), cq2 = cq * cq1;
+    return rwt + gm * (cq1 * 2.f * A1(FL(rdnw), kde - 1) * (-A3(p, i, kde - 1, j)) - (A1(c1f, kde) * A2(mu, i, j)) -
+                       cq2 * (A1(c1f, kde) * A2(mub, i, j) + A1(c2f, kde)));
+  }
+  const float cq = A3(FL(cqw), i, k, j);
+  const float cq1 = 1.f / (1.f + cq), cq2 = cq * cq1;
+  return rwt + gm * (cq1 * A1(FL(rdn), k) * (A3(p, i, k, j) - A3(p, i, k - 1, j)) - (A1(c1f, k) * A2(mu, i, j)) -
+                     cq2 * (A1(c1f, k) * A2(mub, i, j) + A1(c2f, k)));
+}
+// cqw := 1 / (1 + cqw) on levels [2, kde-1] (the second half of pg_buoy_w)
+__global__ void k_cqw_inv(const DevCtx xv, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
+  GET_IKJ_P(r);
+  float *cqw = FL(cqw);
+  A3(cqw, i, k, j) = 1.f / (1.f + A3(cqw, i, k, j));
+}
 // pg_buoy_w (module_big_step_utilities_em.py:1674-1800): column kernel
 __global__ void k_pg_buoy_w(const DevCtx xv, R2 r) {
   const DevCtx *__restrict__ x = &xv;
@@ -263,71 +296,104 @@ __device__ __forceinline__ float vxgm(const DevCtx *__restrict__ x, int i, int k
   return 0.5f * (A3(u, i, k, j) + A3(u, i + 1, k, j)) * (A2(FL(msfvx), i, j + 1) - A2(FL(msfvx), i, j)) * x->rdy -
          0.5f * (A3(v, i, k, j) + A3(v, i, k, j + 1)) * (A2(FL(msfuy), i + 1, j) - A2(FL(msfuy), i, j)) * x->rdx;
 }
+__device__ __forceinline__ float cor_curv_u(const DevCtx *__restrict__ x, int i, int k, int j, float val) {
+  const float *rv = FL(rv), *rw = FL(rw), *u = FL(u_2);
+  const float *f = FL(f), *e = FL(e), *cosa = FL(cosa);
+  const float rer = x->reradius;
+  val = val + (A2(FL(msfux), i, j) / A2(FL(msfuy), i, j)) * 0.5f * (A2(f, i, j) + A2(f, i - 1, j)) * 0.25f *
+                  (A3(rv, i - 1, k, j + 1) + A3(rv, i, k, j + 1) + A3(rv, i - 1, k, j) + A3(rv, i, k, j));
+  val = val - 0.5f * (A2(e, i, j) + A2(e, i - 1, j)) * 0.5f * (A2(cosa, i, j) + A2(cosa, i - 1, j)) * 0.25f *
+                  (A3(rw, i - 1, k + 1, j) + A3(rw, i - 1, k, j) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
+  val = val + 0.5f * (vxgm(x, i, k, j) + vxgm(x, i - 1, k, j)) * 0.25f *
+                  (A3(rv, i - 1, k, j + 1) + A3(rv, i, k, j + 1) + A3(rv, i - 1, k, j) + A3(rv, i, k, j)) -
+        A3(u, i, k, j) * rer * 0.25f * (A3(rw, i - 1, k + 1, j) + A3(rw, i - 1, k, j) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
+  return val;
+}
+__device__ __forceinline__ float cor_curv_v(const DevCtx *__restrict__ x, int i, int k, int j, float val) {
+  const float *ru = FL(ru), *rw = FL(rw), *v = FL(v_2);
+  const float *f = FL(f), *e = FL(e), *sina = FL(sina);
+  const float rer = x->reradius;
+  const float r = A2(FL(msfvy), i, j) / A2(FL(msfvx), i, j);
+  val = val - r * 0.5f * (A2(f, i, j) + A2(f, i, j - 1)) * 0.25f *
+                  (A3(ru, i, k, j) + A3(ru, i + 1, k, j) + A3(ru, i, k, j - 1) + A3(ru, i + 1, k, j - 1));
+  val = val + r * 0.5f * (A2(e, i, j) + A2(e, i, j - 1)) * 0.5f * (A2(sina, i, j) + A2(sina, i, j - 1)) * 0.25f *
+                  (A3(rw, i, k + 1, j - 1) + A3(rw, i, k, j - 1) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
+  val = val - 0.5f * (vxgm(x, i, k, j) + vxgm(x, i, k, j - 1)) * 0.25f *
+                  (A3(ru, i, k, j) + A3(ru, i + 1, k, j) + A3(ru, i, k, j - 1) + A3(ru, i + 1, k, j - 1)) -
+        r * A3(v, i, k, j) * rer * 0.25f * (A3(rw, i, k + 1, j - 1) + A3(rw, i, k, j - 1) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
+  return val;
+}
+__device__ __forceinline__ float cor_curv_w(const DevCtx *__restrict__ x, int i, int k, int j, float val) {
+  const float *ru = FL(ru), *rv = FL(rv), *u = FL(u_2), *v = FL(v_2);
+  const float *e = FL(e), *sina = FL(sina), *cosa = FL(cosa), *fzm = FL(fnm), *fzp = FL(fnp);
+  const float rer = x->reradius;
+  const float mr = A2(FL(msftx), i, j) / A2(FL(msfty), i, j);
+  val = val + A2(e, i, j) * (A2(cosa, i, j) * 0.5f * (A1(fzm, k) * (A3(ru, i, k, j) + A3(ru, i + 1, k, j)) + A1(fzp, k) * (A3(ru, i, k - 1, j) + A3(ru, i + 1, k - 1, j))) -
+                             mr * A2(sina, i, j) * 0.5f * (A1(fzm, k) * (A3(rv, i, k, j) + A3(rv, i, k, j + 1)) + A1(fzp, k) * (A3(rv, i, k - 1, j) + A3(rv, i, k - 1, j + 1))));
+  val = val + rer * (0.5f * (A1(fzm, k) * (A3(ru, i, k, j) + A3(ru, i + 1, k, j)) + A1(fzp, k) * (A3(ru, i, k - 1, j) + A3(ru, i + 1, k - 1, j))) *
+                         0.5f * (A1(fzm, k) * (A3(u, i, k, j) + A3(u, i + 1, k, j)) + A1(fzp, k) * (A3(u, i, k - 1, j) + A3(u, i + 1, k - 1, j))) +
+                     mr * 0.5f * (A1(fzm, k) * (A3(rv, i, k, j) + A3(rv, i, k, j + 1)) + A1(fzp, k) * (A3(rv, i, k - 1, j) + A3(rv, i, k - 1, j + 1))) *
+                         0.5f * (A1(fzm, k) * (A3(v, i, k, j) + A3(v, i, k, j + 1)) + A1(fzp, k) * (A3(v, i, k - 1, j) + A3(v, i, k - 1, j + 1))));
+  return val;
+}
 __global__ void k_cor_curv(const DevCtx xv, int comp, R3 r) {
   const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
-  const float *ru = FL(ru), *rv = FL(rv), *rw = FL(rw), *u = FL(u_2), *v = FL(v_2);
-  const float *f = FL(f), *e = FL(e), *sina = FL(sina), *cosa = FL(cosa), *fzm = FL(fnm), *fzp = FL(fnp);
-  const float rer = x->reradius;
-  if (comp == 0) {
-    float val = A3(FL(ru_tend), i, k, j);
-    val = val + (A2(FL(msfux), i, j) / A2(FL(msfuy), i, j)) * 0.5f * (A2(f, i, j) + A2(f, i - 1, j)) * 0.25f *
-                    (A3(rv, i - 1, k, j + 1) + A3(rv, i, k, j + 1) + A3(rv, i - 1, k, j) + A3(rv, i, k, j));
-    val = val - 0.5f * (A2(e, i, j) + A2(e, i - 1, j)) * 0.5f * (A2(cosa, i, j) + A2(cosa, i - 1, j)) * 0.25f *
-                    (A3(rw, i - 1, k + 1, j) + A3(rw, i - 1, k, j) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
-    val = val + 0.5f * (vxgm(x, i, k, j) + vxgm(x, i - 1, k, j)) * 0.25f *
-                    (A3(rv, i - 1, k, j + 1) + A3(rv, i, k, j + 1) + A3(rv, i - 1, k, j) + A3(rv, i, k, j)) -
-          A3(u, i, k, j) * rer * 0.25f * (A3(rw, i - 1, k + 1, j) + A3(rw, i - 1, k, j) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
-    A3(FL(ru_tend), i, k, j) = val;
-  } else if (comp == 1) {
-    const float r = A2(FL(msfvy), i, j) / A2(FL(msfvx), i, j);
-    float val = A3(FL(rv_tend), i, k, j);
-    val = val - r * 0.5f * (A2(f, i, j) + A2(f, i, j - 1)) * 0.25f *
-                    (A3(ru, i, k, j) + A3(ru, i + 1, k, j) + A3(ru, i, k, j - 1) + A3(ru, i + 1, k, j - 1));
-    val = val + r * 0.5f * (A2(e, i, j) + A2(e, i, j - 1)) * 0.5f * (A2(sina, i, j) + A2(sina, i, j - 1)) * 0.25f *
-                    (A3(rw, i, k + 1, j - 1) + A3(rw, i, k, j - 1) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
-    val = val - 0.5f * (vxgm(x, i, k, j) + vxgm(x, i, k, j - 1)) * 0.25f *
-                    (A3(ru, i, k, j) + A3(ru, i + 1, k, j) + A3(ru, i, k, j - 1) + A3(ru, i + 1, k, j - 1)) -
-          r * A3(v, i, k, j) * rer * 0.25f * (A3(rw, i, k + 1, j - 1) + A3(rw, i, k, j - 1) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
-    A3(FL(rv_tend), i, k, j) = val;
-  } else {
-    const float mr = A2(FL(msftx), i, j) / A2(FL(msfty), i, j);
-    float val = A3(FL(rw_tend), i, k, j);
-    val = val + A2(e, i, j) * (A2(cosa, i, j) * 0.5f * (A1(fzm, k) * (A3(ru, i, k, j) + A3(ru, i + 1, k, j)) + A1(fzp, k) * (A3(ru, i, k - 1, j) + A3(ru, i + 1, k - 1, j))) -
-                               mr * A2(sina, i, j) * 0.5f * (A1(fzm, k) * (A3(rv, i, k, j) + A3(rv, i, k, j + 1)) + A1(fzp, k) * (A3(rv, i, k - 1, j) + A3(rv, i, k - 1, j + 1))));
-    val = val + rer * (0.5f * (A1(fzm, k) * (A3(ru, i, k, j) + A3(ru, i + 1, k, j)) + A1(fzp, k) * (A3(ru, i, k - 1, j) + A3(ru, i + 1, k - 1, j))) *
-                           0.5f * (A1(fzm, k) * (A3(u, i, k, j) + A3(u, i + 1, k, j)) + A1(fzp, k) * (A3(u, i, k - 1, j) + A3(u, i + 1, k - 1, j))) +
-                       mr * 0.5f * (A1(fzm, k) * (A3(rv, i, k, j) + A3(rv, i, k, j + 1)) + A1(fzp, k) * (A3(rv, i, k - 1, j) + A3(rv, i, k - 1, j + 1))) *
-                           0.5f * (A1(fzm, k) * (A3(v, i, k, j) + A3(v, i, k, j + 1)) + A1(fzp, k) * (A3(v, i, k - 1, j) + A3(v, i, k - 1, j + 1))));
-    A3(FL(rw_tend), i, k, j) = val;
-  }
+  if (comp == 0) A3(FL(ru_tend), i, k, j) = cor_curv_u(x, i, k, j, A3(FL(ru_tend), i, k, j));
+  else if (comp == 1) A3(FL(rv_tend), i, k, j) = cor_curv_v(x, i, k, j, A3(FL(rv_tend), i, k, j));
+  else A3(FL(rw_tend), i, k, j) = cor_curv_w(x, i, k, j, A3(FL(rw_tend), i, k, j));
 }
 
-// rk_addtend_dry (module_em.py:959-1148); comp 0 u, 1 v, 2 w/ph, 3 theta
+// rk_addtend_dry (module_em.py:959-1148), one component at one point: the
+// physics / boundary tendency (scaled by the map factor) added to the RK
+// tendency t, which is returned; at the first stage the saved tendency is
+// folded into the physics tendency first (written back).
+__device__ __forceinline__ float addtend_u(const DevCtx *__restrict__ x, int i, int k, int j, int rk, float t) {
+  const unsigned o = XI3(i, k, j);
+  const float m = A2(FL(msfuy), i, j);
+  if (rk == 1) elem(FL(ru_tendf), o) = elem(FL(ru_tendf), o) + elem(FL(u_save), o) * m;
+  return t + elem(FL(ru_tendf), o) / m;
+}
+__device__ __forceinline__ float addtend_v(const DevCtx *__restrict__ x, int i, int k, int j, int rk, float t) {
+  const unsigned o = XI3(i, k, j);
+  if (rk == 1) elem(FL(rv_tendf), o) = elem(FL(rv_tendf), o) + elem(FL(v_save), o) * A2(FL(msfvx), i, j);
+  return t + elem(FL(rv_tendf), o) * A2(FL(msfvx_inv), i, j);
+}
+__device__ __forceinline__ float addtend_w(const DevCtx *__restrict__ x, int i, int k, int j, int rk, float t) {
+  const unsigned o = XI3(i, k, j);
+  const float m = A2(FL(msfty), i, j);
+  if (rk == 1) elem(FL(rw_tendf), o) = elem(FL(rw_tendf), o) + elem(FL(w_save), o) * m;
+  return t + elem(FL(rw_tendf), o) / m;
+}
+__device__ __forceinline__ float addtend_ph(const DevCtx *__restrict__ x, int i, int k, int j, int rk, float t) {
+  const unsigned o = XI3(i, k, j);
+  const float m = A2(FL(msfty), i, j);
+  if (rk == 1) elem(FL(ph_tendf), o) = elem(FL(ph_tendf), o) + elem(FL(ph_save), o);
+  return t + elem(FL(ph_tendf), o) / m;
+}
+__device__ __forceinline__ float addtend_t(const DevCtx *__restrict__ x, int i, int k, int j, int rk, float t) {
+  const unsigned o = XI3(i, k, j);
+  const float m = A2(FL(msfty), i, j);
+  const float mass = A1(FL(c1h), k) * A2(FL(mut), i, j) + A1(FL(c2h), k);
+  if (rk == 1) elem(FL(t_tendf), o) = elem(FL(t_tendf), o) + elem(FL(t_save), o);
+  const float heating = mass * elem(FL(h_diabatic), o) / m;
+  if (k == 1) A2(FL(mu_tend), i, j) = A2(FL(mu_tend), i, j) + A2(FL(mu_tendf), i, j);
+  return t + elem(FL(t_tendf), o) / m + heating;
+}
+// comp 0 u, 1 v, 2 w/ph, 3 theta
 __global__ void k_addtend(const DevCtx xv, int comp, int rk, R3 r) {
   const DevCtx *__restrict__ x = &xv;
   GET_IKJ_P(r);
-  const size_t o = XI3(i, k, j);
+  const unsigned o = XI3(i, k, j);
   if (comp == 0) {
-    const float m = A2(FL(msfuy), i, j);
-    if (rk == 1) FL(ru_tendf)[o] = FL(ru_tendf)[o] + FL(u_save)[o] * m;
-    FL(ru_tend)[o] = FL(ru_tend)[o] + FL(ru_tendf)[o] / m;
+    elem(FL(ru_tend), o) = addtend_u(x, i, k, j, rk, elem(FL(ru_tend), o));
   } else if (comp == 1) {
-    if (rk == 1) FL(rv_tendf)[o] = FL(rv_tendf)[o] + FL(v_save)[o] * A2(FL(msfvx), i, j);
-    FL(rv_tend)[o] = FL(rv_tend)[o] + FL(rv_tendf)[o] * A2(FL(msfvx_inv), i, j);
+    elem(FL(rv_tend), o) = addtend_v(x, i, k, j, rk, elem(FL(rv_tend), o));
   } else if (comp == 2) {
-    const float m = A2(FL(msfty), i, j);
-    if (rk == 1) FL(rw_tendf)[o] = FL(rw_tendf)[o] + FL(w_save)[o] * m;
-    FL(rw_tend)[o] = FL(rw_tend)[o] + FL(rw_tendf)[o] / m;
-    if (rk == 1) FL(ph_tendf)[o] = FL(ph_tendf)[o] + FL(ph_save)[o];
-    FL(ph_tend)[o] = FL(ph_tend)[o] + FL(ph_tendf)[o] / m;
+    elem(FL(rw_tend), o) = addtend_w(x, i, k, j, rk, elem(FL(rw_tend), o));
+    elem(FL(ph_tend), o) = addtend_ph(x, i, k, j, rk, elem(FL(ph_tend), o));
   } else {
-    const float m = A2(FL(msfty), i, j);
-    const float mass = A1(FL(c1h), k) * A2(FL(mut), i, j) + A1(FL(c2h), k);
-    if (rk == 1) FL(t_tendf)[o] = FL(t_tendf)[o] + FL(t_save)[o];
-    const float heating = mass * FL(h_diabatic)[o] / m;
-    FL(t_tend)[o] = FL(t_tend)[o] + FL(t_tendf)[o] / m + heating;
-    if (k == 1) A2(FL(mu_tend), i, j) = A2(FL(mu_tend), i, j) + A2(FL(mu_tendf), i, j);
+    elem(FL(t_tend), o) = addtend_t(x, i, k, j, rk, elem(FL(t_tend), o));
   }
 }
 

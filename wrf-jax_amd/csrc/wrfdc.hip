@@ -389,19 +389,19 @@ struct Run {
   }
   void advect_u() const {
     R3 r{2, ide - 1, 1, kde - 1, 1, jde - 1};
-    k_advect_u<<<dim3(face_tiles(r).count()), b3(), 0, s>>>(d, F(F_u_2), F(F_ru_tend), F(F_ru), F(F_rv), F(F_ww), r);
+    k_advect_u<false><<<dim3(face_tiles(r).count()), b3(), 0, s>>>(d, F(F_u_2), F(F_ru_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
   }
   void advect_v() const {
     R3 r{1, ide - 1, 1, kde - 1, 2, jde - 1};
-    k_advect_v<<<dim3(face_tiles(r).count()), b3(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), r);
+    k_advect_v<false><<<dim3(face_tiles(r).count()), b3(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
   }
   void advect_w() const {
     R3 r{1, ide - 1, 2, kde, 1, jde - 1};
-    k_advect_w<<<dim3(face_tiles(r).count()), b3(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), r);
+    k_advect_w<false><<<dim3(face_tiles(r).count()), b3(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
   }
   void advect_scalar(const float *q, float *tend, const float *ru, const float *rv, const float *rom) const {
     R3 m = mass3();
-    k_advect_scalar<<<dim3(face_tiles(m).count()), b3(), 0, s>>>(d, q, tend, ru, rv, rom, m);
+    k_advect_scalar<false><<<dim3(face_tiles(m).count()), b3(), 0, s>>>(d, q, tend, ru, rv, rom, m, 0);
   }
   void advect_scalar_pd(const float *q, const float *q_old, float *tend, double dt) const {
     R3 r = full3();
@@ -432,11 +432,15 @@ struct Run {
     R3 rw{1, ide - 1, 2, kde - 1, 1, jde - 1};
     k_cor_curv<<<g3(rw), b3(), 0, s>>>(d, 2, rw);
   }
-  void rk_tendency() const {
+  // the RK tendencies and the saved tendencies start from zero each stage
+  void clear_tendencies() const {
     const size_t n3 = X->n3, n2 = X->n2;
     fill_many({{F(F_ru_tend), n3}, {F(F_rv_tend), n3}, {F(F_rw_tend), n3}, {F(F_t_tend), n3}, {F(F_ph_tend), n3},
                {F(F_u_save), n3}, {F(F_v_save), n3}, {F(F_w_save), n3}, {F(F_ph_save), n3}, {F(F_t_save), n3},
                {F(F_mu_tend), n2}, {F(F_mu_save), n2}});
+  }
+  void rk_tendency() const {
+    clear_tendencies();
     advect_u();
     advect_v();
     advect_w();
@@ -446,19 +450,50 @@ struct Run {
     pg_buoy_w();
     coriolis_curvature();
   }
+  // rk_tendency + dry_boundary_tendency as the RK step runs them: the
+  // lateral relaxation first (it only writes the physics tendencies, which the
+  // tendency terms do not read), then one kernel per tendency that forms all
+  // its terms in the reference's order and adds the physics tendency
+  // (rk_addtend_dry), then the specified-zone overlay.
+  void fused_tendency() const {
+    const int rk = X->rk;
+    clear_tendencies();
+    if (rk == 1) lateral_relax();
+    const R3 ru{1, ide, 1, kde - 1, 1, jde - 1};
+    k_advect_u<true><<<dim3(face_tiles(ru).count()), b3(), 0, s>>>(d, F(F_u_2), F(F_ru_tend), F(F_ru), F(F_rv), F(F_ww), ru, rk);
+    const R3 rv{1, ide - 1, 1, kde - 1, 1, jde};
+    k_advect_v<true><<<dim3(face_tiles(rv).count()), b3(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), rv, rk);
+    const R3 rw{1, ide - 1, 1, kde, 1, jde - 1};
+    k_advect_w<true><<<dim3(face_tiles(rw).count()), b3(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), rw, rk);
+    const R3 rq{1, ide - 1, 2, kde - 1, 1, jde - 1};
+    k_cqw_inv<<<g3p(rq), b3(), 0, s>>>(d, rq);
+    const R3 m = mass3();
+    k_advect_scalar<true><<<dim3(face_tiles(m).count()), b3(), 0, s>>>(d, F(F_t_2), F(F_t_tend), F(F_ru), F(F_rv), F(F_ww), m, rk);
+    k_tend_ph<<<g3(rw), b3(), 0, s>>>(d, rw, rk);
+    spec_overlays();
+  }
+  // relax_bdy_dry at the first stage (physics tendencies of u, v, ph, theta, mu)
+  void lateral_relax() const {
+    relax(F(F_ru), F(F_ru_tendf), 'u', false, false, bp(F_u_bys, F_u_bye, F_u_bxs, F_u_bxe), bp(F_u_btys, F_u_btye, F_u_btxs, F_u_btxe));
+    relax(F(F_rv), F(F_rv_tendf), 'v', false, false, bp(F_v_bys, F_v_bye, F_v_bxs, F_v_bxe), bp(F_v_btys, F_v_btye, F_v_btxs, F_v_btxe));
+    float *tmp = Scr(S_HAT);
+    R3 r = full3();
+    k_mass_weight<<<g3p(r), b3(), 0, s>>>(d, F(F_ph_2), tmp, 1, r);
+    relax(tmp, F(F_ph_tendf), 'h', true, false, bp(F_ph_bys, F_ph_bye, F_ph_bxs, F_ph_bxe), bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe));
+    k_mass_weight<<<g3p(r), b3(), 0, s>>>(d, F(F_t_2), tmp, 0, r);
+    relax(tmp, F(F_t_tendf), 't', false, false, bp(F_t_bys, F_t_bye, F_t_bxs, F_t_bxe), bp(F_t_btys, F_t_btye, F_t_btxs, F_t_btxe));
+    relax(F(F_mu_2), F(F_mu_tend), 'm', true, true, bp(F_mu_bys, F_mu_bye, F_mu_bxs, F_mu_bxe), bp(F_mu_btys, F_mu_btye, F_mu_btxs, F_mu_btxe));
+  }
+  // spec_bdy_dry: the specified zone's tendencies
+  void spec_overlays() const {
+    spec_overlay(F(F_ru_tend), 'u', false, false, bp(F_u_btys, F_u_btye, F_u_btxs, F_u_btxe));
+    spec_overlay(F(F_rv_tend), 'v', false, false, bp(F_v_btys, F_v_btye, F_v_btxs, F_v_btxe));
+    spec_overlay(F(F_ph_tend), 'h', true, false, bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe));
+    spec_overlay(F(F_t_tend), 't', false, false, bp(F_t_btys, F_t_btye, F_t_btxs, F_t_btxe));
+    spec_overlay(F(F_mu_tend), 'm', true, true, bp(F_mu_btys, F_mu_btye, F_mu_btxs, F_mu_btxe));
+  }
   void dry_boundary_tendency() const {
-    if (X->rk == 1) {
-      relax(F(F_ru), F(F_ru_tendf), 'u', false, false, bp(F_u_bys, F_u_bye, F_u_bxs, F_u_bxe), bp(F_u_btys, F_u_btye, F_u_btxs, F_u_btxe));
-      relax(F(F_rv), F(F_rv_tendf), 'v', false, false, bp(F_v_bys, F_v_bye, F_v_bxs, F_v_bxe), bp(F_v_btys, F_v_btye, F_v_btxs, F_v_btxe));
-      float *tmp = Scr(S_HAT);
-      R3 r = full3();
-      k_mass_weight<<<g3p(r), b3(), 0, s>>>(d, F(F_ph_2), tmp, 1, r);
-      relax(tmp, F(F_ph_tendf), 'h', true, false, bp(F_ph_bys, F_ph_bye, F_ph_bxs, F_ph_bxe), bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe));
-      k_mass_weight<<<g3p(r), b3(), 0, s>>>(d, F(F_t_2), tmp, 0, r);
-      relax(tmp, F(F_t_tendf), 't', false, false, bp(F_t_bys, F_t_bye, F_t_bxs, F_t_bxe), bp(F_t_btys, F_t_btye, F_t_btxs, F_t_btxe));
-      relax(F(F_mu_2), F(F_mu_tend), 'm', true, true, bp(F_mu_bys, F_mu_bye, F_mu_bxs, F_mu_bxe),
-            bp(F_mu_btys, F_mu_btye, F_mu_btxs, F_mu_btxe));
-    }
+    if (X->rk == 1) lateral_relax();
     const int rk = X->rk;
     R3 r0{1, ide, 1, kde - 1, 1, jde - 1};
     k_addtend<<<g3p(r0), b3(), 0, s>>>(d, 0, rk, r0);
@@ -468,11 +503,7 @@ struct Run {
     k_addtend<<<g3p(r2), b3(), 0, s>>>(d, 2, rk, r2);
     R3 r3 = mass3();
     k_addtend<<<g3p(r3), b3(), 0, s>>>(d, 3, rk, r3);
-    spec_overlay(F(F_ru_tend), 'u', false, false, bp(F_u_btys, F_u_btye, F_u_btxs, F_u_btxe));
-    spec_overlay(F(F_rv_tend), 'v', false, false, bp(F_v_btys, F_v_btye, F_v_btxs, F_v_btxe));
-    spec_overlay(F(F_ph_tend), 'h', true, false, bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe));
-    spec_overlay(F(F_t_tend), 't', false, false, bp(F_t_btys, F_t_btye, F_t_btxs, F_t_btxe));
-    spec_overlay(F(F_mu_tend), 'm', true, true, bp(F_mu_btys, F_mu_btye, F_mu_btxs, F_mu_btxe));
+    spec_overlays();
   }
   void small_step_prep() const {
     k_ssp_mass<<<dim3(cdiv(ide, 64), jde), 64, 0, s>>>(d, X->rk);
@@ -668,8 +699,7 @@ struct Run {
       first_rk_step_part1();
       first_rk_step_part2();
     }
-    rk_tendency();
-    dry_boundary_tendency();
+    fused_tendency();
     small_step_prep();
     calc_p_rho(0);
     calc_coef_w();
