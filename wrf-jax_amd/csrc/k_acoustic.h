@@ -28,7 +28,11 @@ __global__ void __launch_bounds__(256) k_uv_substep(const DevCtx xv, float dts, 
   GET_IKJ(r);
   const int ide = x->ide, jde = x->jde;
   const size_t o = XI3(i, k, j);
-  if (j <= jde - 1) {  // u point
+  // both results are formed before either is stored, so no load of the v
+  // point waits behind the store of the u point
+  const bool upt = (j <= jde - 1), vpt = (i <= ide - 1);
+  float un = 0.f, vn = 0.f;
+  if (upt) {  // u point
     if (i >= 2 && i <= ide - 1 && j >= 2 && j <= jde - 2) {
       const float *p = FL(p), *pb = FL(pb), *ph = FL(ph_2), *alt = FL(alt), *al = FL(al), *php = FL(php);
       const float *c1h = FL(c1h), *c2h = FL(c2h), *mu = FL(mu_2), *mudf = FL(mudf);
@@ -46,12 +50,12 @@ __global__ void __launch_bounds__(256) k_uv_substep(const DevCtx xv, float dts, 
       const float vert = A1(FL(rdnw), k) * (uv_iface(x, p, i, j, 1, 0, k + 1) - uv_iface(x, p, i, j, 1, 0, k));
       const float muv_ = 0.5f * (A1(c1h, k) * A2(mu, i - 1, j) + A1(c1h, k) * A2(mu, i, j));
       grad = grad + (ratio * spacing * (A3(php, i, k, j) - A3(php, i - 1, k, j))) * (vert - muv_);
-      FL(u_2)[o] = (adv - dts * FL(cqu)[o] * grad) + A1(c1h, k) * damping;
+      un = (adv - dts * FL(cqu)[o] * grad) + A1(c1h, k) * damping;
     } else {
-      FL(u_2)[o] = FL(u_2)[o] + dts * FL(ru_tend)[o];
+      un = FL(u_2)[o] + dts * FL(ru_tend)[o];
     }
   }
-  if (i <= ide - 1) {  // v point
+  if (vpt) {  // v point
     if (i >= 2 && i <= ide - 2 && j >= 2 && j <= jde - 1) {
       const float *p = FL(p), *pb = FL(pb), *ph = FL(ph_2), *alt = FL(alt), *al = FL(al), *php = FL(php);
       const float *c1h = FL(c1h), *c2h = FL(c2h), *mu = FL(mu_2), *mudf = FL(mudf);
@@ -69,11 +73,13 @@ __global__ void __launch_bounds__(256) k_uv_substep(const DevCtx xv, float dts, 
       const float vert = A1(FL(rdnw), k) * (uv_iface(x, p, i, j, 0, 1, k + 1) - uv_iface(x, p, i, j, 0, 1, k));
       const float muv_ = 0.5f * (A1(c1h, k) * A2(mu, i, j - 1) + A1(c1h, k) * A2(mu, i, j));
       grad = grad + (ratio * spacing * (A3(php, i, k, j) - A3(php, i, k, j - 1))) * (vert - muv_);
-      FL(v_2)[o] = (adv - dts * FL(cqv)[o] * grad) + A1(c1h, k) * damping;
+      vn = (adv - dts * FL(cqv)[o] * grad) + A1(c1h, k) * damping;
     } else {
-      FL(v_2)[o] = FL(v_2)[o] + dts * FL(rv_tend)[o];
+      vn = FL(v_2)[o] + dts * FL(rv_tend)[o];
     }
   }
+  if (upt) FL(u_2)[o] = un;
+  if (vpt) FL(v_2)[o] = vn;
 }
 
 // zero_grad_bdy of w: every destination (the zone ring of the mass grid)
@@ -107,6 +113,7 @@ __global__ void k_zero_grad_w(const DevCtx xv) {
 constexpr int kColWaves = WRFDC_COL_WAVES;
 constexpr int kColGroup = WRFDC_COL_GROUP;
 
+constexpr int kColMaxLev = 152;  // levels the column kernel holds (nz <= 150)
 // bytes of dynamic LDS of k_acoustic_col for kde levels
 __host__ __device__ inline size_t acoustic_col_lds(int kde) { return (size_t)3 * (kde + 1) * 64 * sizeof(float); }
 // levels per wave the kernel is instantiated for (kde <= kColWaves * NPW)
@@ -121,6 +128,9 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
   constexpr int NW = kColWaves, G = kColGroup;
   extern __shared__ float sm[];
   __shared__ float c_mts[64], c_mav[64];
+  // dnw and c1h in LDS for the serial column sum and omega recurrence (their
+  // global loads would otherwise sit on the recurrence's critical path)
+  __shared__ float c_dnw[kColMaxLev], c_c1h[kColMaxLev];
   const int lane = (int)threadIdx.x, wv = (int)threadIdx.y;
   const unsigned gx = (unsigned)(x->ide + 63) / 64u, t = xcd_tile(blockIdx.x, gridDim.x);
   const int i = (int)(t % gx) * 64 + lane + 1, j = (int)(t / gx) + 1;
@@ -144,6 +154,10 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
   const float msfty = A2(FL(msfty), ic, jc), msftx = A2(FL(msftx), ic, jc);
 
   float *ru_m = FL(ru_m), *rv_m = FL(rv_m), *ww_m = FL(ww_m);
+  for (int q = (int)(threadIdx.y * 64 + threadIdx.x); q < kde; q += 64 * NW) {
+    c_dnw[q] = dnw[q];
+    c_c1h[q] = c1h[q];
+  }
   // ---- 1. advance_mu_t: horizontal mass divergence per level -> B0; with
   // the same loads, sumflux of the column's u and v points (levels 1..kde-1)
   if (interior) {
@@ -193,7 +207,7 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
   // ---- 2. column sum, column-mass update and the omega recurrence (wave 0)
   if (wv == 0 && interior) {
     float dmdt = 0.f;
-    for (int k = 1; k <= kend; k++) dmdt = dmdt + A1(dnw, k) * LS(B0, k);
+    for (int k = 1; k <= kend; k++) dmdt = dmdt + c_dnw[k - 1] * LS(B0, k);
     float *mu = FL(mu_2);
     const float old = A2(mu, i, j);
     const float mtend = A2(FL(mu_tend), i, j);
@@ -210,9 +224,9 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
     float prev = A3(ww, i, 1, j);
     LS(B1, 1) = prev;
     for (int k = 2; k <= kend; k++) {
-      float integ = A1(c1h, k - 1) * dmdt + LS(B0, k - 1);
-      integ = integ + A1(c1h, k - 1) * mtend;
-      const float cur = prev - (A1(dnw, k - 1) * integ) / msfty;
+      float integ = c_c1h[k - 2] * dmdt + LS(B0, k - 1);
+      integ = integ + c_c1h[k - 2] * mtend;
+      const float cur = prev - (c_dnw[k - 2] * integ) / msfty;
       LS(B1, k) = cur;
       prev = cur;
     }
@@ -318,7 +332,8 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
     const float *ft = FL(t_tend);
     for (int k = 1 + wv; k <= kend; k += NW) A3(t, i, k, j) = A3(t, i, k, j) + dts * A3(ft, i, k, j);
   }
-  __syncthreads();
+  // (no barrier: phase 5 reads omega (final since the barrier above), phi and
+  // w, and writes B0; phase 4 writes theta, its average and B2 only)
   // ---- 5. advance_w: right-hand side of the phi equation (levels 2..kde) -> B0
   const float *ph = FL(ph_2);
   const float m = A2(FL(mut), ic, jc);
@@ -418,16 +433,19 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
     }
   }
   __syncthreads();
-  // ---- 7. tridiagonal sweeps (wave 0, one column per lane); the LU factors
-  // come from global memory, their loads are independent of the recurrence
-  // (chunks of C levels: the factors of chunk c+1 are requested before the
-  // recurrence of chunk c runs, so their latency overlaps it)
+  // ---- 7. tridiagonal sweeps (wave 0, one column per lane).  Forward: the
+  // LU factors a, alpha come from global memory in chunks of C levels (the
+  // factors of chunk c+1 are requested before the recurrence of chunk c runs,
+  // so their latency overlaps it).  Meanwhile the other waves copy gamma of
+  // every level into B2 (free between phase 6 and phase 9), so the backward
+  // sweep reads it from LDS.
+  const int sk = x->ip;  // element stride between levels
+  const unsigned o1 = XI3(ic, 1, jc);
+  float wprev = 0.f;
   if (wv == 0 && interior) {
     constexpr int C = 8;
-    const float *a = FL(a), *alpha = FL(alpha), *gamma = FL(gamma);
-    const int sk = x->ip;  // element stride between levels
-    const unsigned o1 = XI3(i, 1, j);
-    float wprev = LS(B3, 1);
+    const float *a = FL(a), *alpha = FL(alpha);
+    wprev = LS(B3, 1);
     // forward: levels 2..kde
     {
       const int n = kde - 1;
@@ -461,33 +479,18 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
         }
       }
     }
-    // backward: levels kend..2
-    {
-      const int n = kend - 1;
-      float cg[C], ng[C];
-#pragma unroll
-      for (int q = 0; q < C; q++) {
-        const int k = imax_(kend - q, 2);
-        cg[q] = gamma[o1 + (unsigned)((k - 1) * sk)];
-      }
-      for (int c0 = 0; c0 < n; c0 += C) {
-#pragma unroll
-        for (int q = 0; q < C; q++) {
-          const int k = imax_(kend - c0 - C - q, 2);
-          ng[q] = gamma[o1 + (unsigned)((k - 1) * sk)];
-        }
-#pragma unroll
-        for (int q = 0; q < C; q++) {
-          const int k = kend - c0 - q;
-          if (k >= 2) {
-            const float wl = LS(B3, k) - cg[q] * wprev;
-            LS(B3, k) = wl;
-            wprev = wl;
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < C; q++) cg[q] = ng[q];
-      }
+  } else if (interior) {
+    const float *gamma = FL(gamma);
+    for (int k = 1 + wv; k <= kend; k += NW - 1) LS(B2, k) = gamma[o1 + (unsigned)((k - 1) * sk)];
+  }
+  __syncthreads();
+  // backward: levels kend..2
+  if (wv == 0 && interior) {
+#pragma unroll 8
+    for (int k = kend; k >= 2; k--) {
+      const float wl = LS(B3, k) - LS(B2, k) * wprev;
+      LS(B3, k) = wl;
+      wprev = wl;
     }
   }
   __syncthreads();

@@ -278,7 +278,8 @@ __device__ __forceinline__ float pd_face_low(const PdFace &f, const float *qo, u
 }
 
 // pass 2: tendency from the renormalised faces, then the mass-weighted update
-// (old = q_old after k_species_pd_prep).  Only the interior cells
+// (old = q_old after k_species_pd_prep, which also cleared the physics
+// tendency over the same range: it is added as the +0 it holds).  Only the interior cells
 // i [2, ide-2], j [2, jde-2] (the cells pass 1 limits: the specified zone is
 // one cell wide) take an advective tendency; their antidiffusive fluxes come
 // from pass 1.
@@ -289,7 +290,7 @@ __global__ void __launch_bounds__(256) k_species_pd_update(const DevCtx xv, Spec
   const UpdCoef c = upd_coef(x, i, k, j);
   if (!c.inner) {
     for (int s = 0; s < L.n; s++) {
-      const float tend = 0.f + L.tend[s][o];
+      const float tend = 0.f + 0.f;
       L.out[s][o] = (c.oldm * L.old[s][o] + dt * tend) / c.newm;
     }
     return;
@@ -341,7 +342,7 @@ __global__ void __launch_bounds__(256) k_species_pd_update(const DevCtx xv, Spec
       t += -msftx * (rdy * (yn - ys + ly1 - ly0));
     }
     float tend = t * c.msfty;
-    tend = tend + L.tend[s][o];
+    tend = tend + 0.f;
     L.out[s][o] = (c.oldm * qo[o] + dt * tend) / c.newm;
   }
 }
