@@ -23,63 +23,70 @@
 #include "k_small.h"
 
 // ---------------------------------------------------------------------------
+// vertical pressure difference of advance_uv, rdnw(k) * (P(k+1) - P(k)) with
+// P the interface pressure average of uv_iface, for levels k in [1, kde-1],
+// without branches around its loads: p is read at levels k-1 .. k+2 (clamped
+// into the column) and the surface / interior / top expressions of uv_iface
+// are selected on the level
+__device__ __forceinline__ float uv_vert(const DevCtx *__restrict__ x, const float *p, int i, int j, int di, int dj, int k) {
+  const int kde = x->kde;
+  const int km = imax_(k - 1, 1), kp = imin_(k + 1, kde), kq = imin_(k + 2, kde);
+  const float ppm = A3(p, i, km, j) + A3(p, i - di, km, j - dj);
+  const float pp0 = A3(p, i, k, j) + A3(p, i - di, k, j - dj);
+  const float ppp = A3(p, i, kp, j) + A3(p, i - di, kp, j - dj);
+  const float ppq = A3(p, i, kq, j) + A3(p, i - di, kq, j - dj);
+  // interface k+1 (levels 2 .. kde): interior expression, 0 at the top
+  const int lu = imin_(k + 1, kde - 1);
+  const float up = (k + 1 <= kde - 1) ? 0.5f * (A1(FL(fnm), lu) * ppp + A1(FL(fnp), lu) * pp0) : 0.f;
+  // interface k (levels 1 .. kde-1): surface extrapolation at level 1
+  const int ld = imax_(k, 2);
+  const float dn = (k == 1) ? 0.5f * (x->cf1 * pp0 + x->cf2 * ppp + x->cf3 * ppq)
+                            : 0.5f * (A1(FL(fnm), ld) * pp0 + A1(FL(fnp), ld) * ppm);
+  return A1(FL(rdnw), k) * (up - dn);
+}
+// advance_uv at an interior u point (axis 0: i [2, ide-1], j [2, jde-2]) or v
+// point (axis 1: i [2, ide-2], j [2, jde-1])
+__device__ __forceinline__ float uv_interior(const DevCtx *__restrict__ x, int axis, int i, int k, int j, float dts) {
+  const int di = axis == 0 ? 1 : 0, dj = axis == 0 ? 0 : 1;
+  const unsigned o = XI3(i, k, j);
+  const float *p = FL(p), *pb = FL(pb), *ph = FL(ph_2), *alt = FL(alt), *al = FL(al), *php = FL(php);
+  const float *c1h = FL(c1h), *c2h = FL(c2h), *mu = FL(mu_2), *mudf = FL(mudf);
+  const float spacing = axis == 0 ? x->rdx : x->rdy, grid_length = 1.f / spacing;
+  const float adv = axis == 0 ? elem(FL(u_2), o) + dts * elem(FL(ru_tend), o) : elem(FL(v_2), o) + dts * elem(FL(rv_tend), o);
+  float damping = -1.f * x->emdiv * grid_length * (A2(mudf, i, j) - A2(mudf, i - di, j - dj));
+  damping = axis == 0 ? damping / A2(FL(msfuy), i, j) : damping * A2(FL(msfvx_inv), i, j);
+  const float phi = (A3(ph, i, k + 1, j) - A3(ph, i - di, k + 1, j - dj)) + (A3(ph, i, k, j) - A3(ph, i - di, k, j - dj));
+  const float pres = (A3(alt, i, k, j) + A3(alt, i - di, k, j - dj)) * (A3(p, i, k, j) - A3(p, i - di, k, j - dj));
+  const float base = (A3(al, i, k, j) + A3(al, i - di, k, j - dj)) * (A3(pb, i, k, j) - A3(pb, i - di, k, j - dj));
+  const float term = (phi + pres) + base;
+  const float ratio = axis == 0 ? A2(FL(msfux), i, j) / A2(FL(msfuy), i, j) : A2(FL(msfvy), i, j) / A2(FL(msfvx), i, j);
+  const float m = A1(c1h, k) * (axis == 0 ? A2(FL(muu), i, j) : A2(FL(muv), i, j)) + A1(c2h, k);
+  float grad = (ratio * 0.5f * spacing * m) * term;
+  const float vert = uv_vert(x, p, i, j, di, dj, k);
+  const float muv_ = 0.5f * (A1(c1h, k) * A2(mu, i - di, j - dj) + A1(c1h, k) * A2(mu, i, j));
+  grad = grad + (ratio * spacing * (A3(php, i, k, j) - A3(php, i - di, k, j - dj))) * (vert - muv_);
+  const float cq = axis == 0 ? elem(FL(cqu), o) : elem(FL(cqv), o);
+  return (adv - dts * cq * grad) + A1(c1h, k) * damping;
+}
+// u and v of one acoustic substep (advance_uv plus the specified-zone
+// update).  Every lane evaluates the interior formulas at an in-range point
+// (its own where they apply) and the zone formula, with no branch around the
+// loads, so all of them are in flight together; the results are selected
+// and stored at the end.
 __global__ void __launch_bounds__(256) k_uv_substep(const DevCtx xv, float dts, R3 r) {
   const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const int ide = x->ide, jde = x->jde;
-  const size_t o = XI3(i, k, j);
-  // both results are formed before either is stored, so no load of the v
-  // point waits behind the store of the u point
+  const unsigned o = XI3(i, k, j);
   const bool upt = (j <= jde - 1), vpt = (i <= ide - 1);
-  float un = 0.f, vn = 0.f;
-  if (upt) {  // u point
-    if (i >= 2 && i <= ide - 1 && j >= 2 && j <= jde - 2) {
-      const float *p = FL(p), *pb = FL(pb), *ph = FL(ph_2), *alt = FL(alt), *al = FL(al), *php = FL(php);
-      const float *c1h = FL(c1h), *c2h = FL(c2h), *mu = FL(mu_2), *mudf = FL(mudf);
-      const float spacing = x->rdx, grid_length = 1.f / spacing;
-      const float adv = FL(u_2)[o] + dts * FL(ru_tend)[o];
-      float damping = -1.f * x->emdiv * grid_length * (A2(mudf, i, j) - A2(mudf, i - 1, j));
-      damping = damping / A2(FL(msfuy), i, j);
-      const float phi = (A3(ph, i, k + 1, j) - A3(ph, i - 1, k + 1, j)) + (A3(ph, i, k, j) - A3(ph, i - 1, k, j));
-      const float pres = (A3(alt, i, k, j) + A3(alt, i - 1, k, j)) * (A3(p, i, k, j) - A3(p, i - 1, k, j));
-      const float base = (A3(al, i, k, j) + A3(al, i - 1, k, j)) * (A3(pb, i, k, j) - A3(pb, i - 1, k, j));
-      const float term = (phi + pres) + base;
-      const float ratio = A2(FL(msfux), i, j) / A2(FL(msfuy), i, j);
-      const float m = A1(c1h, k) * A2(FL(muu), i, j) + A1(c2h, k);
-      float grad = (ratio * 0.5f * spacing * m) * term;
-      const float vert = A1(FL(rdnw), k) * (uv_iface(x, p, i, j, 1, 0, k + 1) - uv_iface(x, p, i, j, 1, 0, k));
-      const float muv_ = 0.5f * (A1(c1h, k) * A2(mu, i - 1, j) + A1(c1h, k) * A2(mu, i, j));
-      grad = grad + (ratio * spacing * (A3(php, i, k, j) - A3(php, i - 1, k, j))) * (vert - muv_);
-      un = (adv - dts * FL(cqu)[o] * grad) + A1(c1h, k) * damping;
-    } else {
-      un = FL(u_2)[o] + dts * FL(ru_tend)[o];
-    }
-  }
-  if (vpt) {  // v point
-    if (i >= 2 && i <= ide - 2 && j >= 2 && j <= jde - 1) {
-      const float *p = FL(p), *pb = FL(pb), *ph = FL(ph_2), *alt = FL(alt), *al = FL(al), *php = FL(php);
-      const float *c1h = FL(c1h), *c2h = FL(c2h), *mu = FL(mu_2), *mudf = FL(mudf);
-      const float spacing = x->rdy, grid_length = 1.f / spacing;
-      const float adv = FL(v_2)[o] + dts * FL(rv_tend)[o];
-      float damping = -1.f * x->emdiv * grid_length * (A2(mudf, i, j) - A2(mudf, i, j - 1));
-      damping = damping * A2(FL(msfvx_inv), i, j);
-      const float phi = (A3(ph, i, k + 1, j) - A3(ph, i, k + 1, j - 1)) + (A3(ph, i, k, j) - A3(ph, i, k, j - 1));
-      const float pres = (A3(alt, i, k, j) + A3(alt, i, k, j - 1)) * (A3(p, i, k, j) - A3(p, i, k, j - 1));
-      const float base = (A3(al, i, k, j) + A3(al, i, k, j - 1)) * (A3(pb, i, k, j) - A3(pb, i, k, j - 1));
-      const float term = (phi + pres) + base;
-      const float ratio = A2(FL(msfvy), i, j) / A2(FL(msfvx), i, j);
-      const float m = A1(c1h, k) * A2(FL(muv), i, j) + A1(c2h, k);
-      float grad = (ratio * 0.5f * spacing * m) * term;
-      const float vert = A1(FL(rdnw), k) * (uv_iface(x, p, i, j, 0, 1, k + 1) - uv_iface(x, p, i, j, 0, 1, k));
-      const float muv_ = 0.5f * (A1(c1h, k) * A2(mu, i, j - 1) + A1(c1h, k) * A2(mu, i, j));
-      grad = grad + (ratio * spacing * (A3(php, i, k, j) - A3(php, i, k, j - 1))) * (vert - muv_);
-      vn = (adv - dts * FL(cqv)[o] * grad) + A1(c1h, k) * damping;
-    } else {
-      vn = FL(v_2)[o] + dts * FL(rv_tend)[o];
-    }
-  }
-  if (upt) FL(u_2)[o] = un;
-  if (vpt) FL(v_2)[o] = vn;
+  const bool uin = upt && i >= 2 && i <= ide - 1 && j >= 2 && j <= jde - 2;
+  const bool vin = vpt && i >= 2 && i <= ide - 2 && j >= 2 && j <= jde - 1;
+  const float un_in = uv_interior(x, 0, imin_(imax_(i, 2), ide - 1), k, imin_(imax_(j, 2), jde - 2), dts);
+  const float vn_in = uv_interior(x, 1, imin_(imax_(i, 2), ide - 2), k, imin_(imax_(j, 2), jde - 1), dts);
+  const float un_z = elem(FL(u_2), o) + dts * elem(FL(ru_tend), o);
+  const float vn_z = elem(FL(v_2), o) + dts * elem(FL(rv_tend), o);
+  if (upt) elem(FL(u_2), o) = uin ? un_in : un_z;
+  if (vpt) elem(FL(v_2), o) = vin ? vn_in : vn_z;
 }
 
 // zero_grad_bdy of w: every destination (the zone ring of the mass grid)

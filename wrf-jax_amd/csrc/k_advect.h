@@ -56,7 +56,7 @@ __host__ __device__ inline FaceTiles face_tiles(const R3 &r) {
 #define GET_FACE_BLOCK(r)                                                                \
   const FaceTiles T_ = face_tiles(r);                                                    \
   const unsigned t_ = xcd_tile(blockIdx.x, gridDim.x);                                   \
-  const int lane = (int)threadIdx.x, wv = (int)threadIdx.y;                              \
+  const int lane = (int)threadIdx.x, wv = (int)wave_y();                                 \
   const unsigned u_ = t_ / T_.ni;                                                        \
   const int i = (r).i0 + (int)(t_ % T_.ni) * 63 + lane;                                  \
   const int k = (r).k0 + (int)((u_ / 4u) / T_.nj) * 4 + (int)(u_ % 4u);                  \

@@ -114,6 +114,11 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
   const unsigned xc = b & 7u, idx = b >> 3, q = nb >> 3, rm = nb & 7u;
   return xc * q + (xc < rm ? xc : rm) + idx;
 }
+// threadIdx.y of a block that is 64 threads wide: one value per wavefront.
+// Read through readfirstlane so the compiler knows it is uniform across the
+// wave (indices derived from it stay in scalar registers, 1-D coefficient
+// loads are scalar loads and branches on them are scalar branches).
+__device__ __forceinline__ unsigned wave_y() { return (unsigned)__builtin_amdgcn_readfirstlane((int)threadIdx.y); }
 // Stencil kernels: thread -> (i, k, j) with blockDim (64, 4, 1) (a block is
 // 64 i x 4 levels of one row) and a 1-D grid of
 // ceil(ni/64) * ceil(nk/4) * 4 * ceil(nj/4) blocks.  Tiles run i fastest,
@@ -126,7 +131,7 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
   const unsigned t_ = xcd_tile(blockIdx.x, gridDim.x);                                       \
   const unsigned u_ = t_ / gx_;                                                              \
   const int i = (r).i0 + (int)((t_ % gx_) * 64u + threadIdx.x);                              \
-  const int k = (r).k0 + (int)(((u_ / 4u) % gy_) * 4u + threadIdx.y);                        \
+  const int k = (r).k0 + (int)(((u_ / 4u) % gy_) * 4u + wave_y());                           \
   const int j = (r).j0 + (int)((u_ / 4u / gy_) * 4u + u_ % 4u);                              \
   if (i > (r).i1 || k > (r).k1 || j > (r).j1) return
 // Pointwise kernels (no neighbour in j): the same block shape, a 1-D grid of
@@ -135,7 +140,7 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
   const unsigned gx_ = (unsigned)((r).i1 - (r).i0 + 64) / 64u;                               \
   const unsigned gy_ = (unsigned)((r).k1 - (r).k0 + 4) / 4u;                                 \
   const int i = (r).i0 + (int)((blockIdx.x % gx_) * 64u + threadIdx.x);                      \
-  const int k = (r).k0 + (int)(((blockIdx.x / gx_) % gy_) * 4u + threadIdx.y);               \
+  const int k = (r).k0 + (int)(((blockIdx.x / gx_) % gy_) * 4u + wave_y());                  \
   const int j = (r).j0 + (int)(blockIdx.x / (gx_ * gy_));                                    \
   if (i > (r).i1 || k > (r).k1 || j > (r).j1) return
 // thread -> (i, j) for column kernels: blockDim (64, 1, 1), 1-D grid of
