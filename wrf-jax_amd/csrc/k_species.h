@@ -54,6 +54,47 @@ __device__ __forceinline__ float sfz(const DevCtx *__restrict__ x, const float *
   return vel * flux3(A3(q, i, k - 2, j), A3(q, i, k - 1, j), A3(q, i, k, j), A3(q, i, k + 1, j), -vel);
 }
 
+// The face fluxes above from stencil values already in registers, so a
+// kernel can issue all loads of a point before any branch on the flux order.
+// win6: the 6 values around face f along stride st (slots f-3 .. f+2), each
+// position clamped into [lo, hi] (valid addresses; the slots a lower-order
+// flux does not use are ignored).
+__device__ __forceinline__ void win6(const float *q, unsigned of, int f, int lo, int hi, int st, float w[6]) {
+#pragma unroll
+  for (int n = 0; n < 6; n++) {
+    const int pos = imin_(imax_(f - 3 + n, lo), hi);
+    w[n] = elem(q, of + (unsigned)((pos - f) * st));
+  }
+}
+// the 6 values around the face at element offset of along stride st (slots
+// -3 .. +2), unclamped buffer loads
+__device__ __forceinline__ void win6b(const BufF &b, unsigned of, unsigned st, float w[6]) {
+#pragma unroll
+  for (int n = 0; n < 6; n++) w[n] = bld(b, of + (unsigned)(n - 3) * st);
+}
+// horizontal face flux (sfx / sfy) at face f of an axis whose end index is
+// fe, every order evaluated and the face's own selected (no branch between
+// the window loads and their use)
+__device__ __forceinline__ float hflux_sel(const float w[6], float vel, int f, int fe) {
+  const float h5 = vel * flux5(w[0], w[1], w[2], w[3], w[4], w[5], vel);
+  const float h2 = 0.5f * vel * (w[3] + w[2]);
+  const float h3 = vel * flux3(w[1], w[2], w[3], w[4], vel);
+  return (f >= 4 && f <= fe - 3) ? h5 : ((f == 2 || f == fe - 1) ? h2 : h3);
+}
+// horizontal face flux (sfx / sfy) at face f of an axis whose end index is fe
+__device__ __forceinline__ float hflux_w(const float w[6], float vel, int f, int fe) {
+  if (f >= 4 && f <= fe - 3) return vel * flux5(w[0], w[1], w[2], w[3], w[4], w[5], vel);
+  if (f == 2 || f == fe - 1) return 0.5f * vel * (w[3] + w[2]);
+  return vel * flux3(w[1], w[2], w[3], w[4], vel);
+}
+// vertical face flux (sfz) at level kk from q(kk-2), q(kk-1), q(kk), q(kk+1)
+__device__ __forceinline__ float vflux_w(const DevCtx *__restrict__ x, float qm2, float qm1, float q0, float qp1, float vel, int kk) {
+  const int ktf = x->kde - 1;
+  if (kk <= 1 || kk >= x->kde) return 0.f;
+  if (kk == 2 || kk == ktf) return vel * (A1(FL(fnm), kk) * q0 + A1(FL(fnp), kk) * qm1);
+  return vel * flux3(qm2, qm1, q0, qp1, -vel);
+}
+
 // mass-weighted RK update of one point (rk_update_scalar)
 struct UpdCoef {
   float oldm, newm, msfty;
@@ -84,10 +125,18 @@ __global__ void __launch_bounds__(256) k_species_update(const DevCtx xv, SpecLis
 #define YS(s, w) ys_[((s) * 4 + (w)) * 64 + lane]
   const int ide = x->ide, jde = x->jde, kde = x->kde;
   const float *ru = FL(ru_m), *rv = FL(rv_m), *rom = FL(ww_m);
+  const unsigned nb = (unsigned)x->n3, sj = (unsigned)(kde * x->ip), sk = (unsigned)x->ip;
   {
+    // rows past the range still form their south face (the row below reads
+    // it); its window may fall outside the field, the face is then 0
     const bool yf = (j >= 2 && j <= jde - 1);
     const float vs = yf ? A3(rv, ic, k, j) : 0.f;
-    for (int s = 0; s < L.n; s++) YS(s, wv) = yf ? sfy(x, L.in[s], vs, ic, k, j) : 0.f;
+    const unsigned os = XI3(ic, k, j);
+    for (int s = 0; s < L.n; s++) {
+      float w[6];
+      win6b(bufv(L.in[s], nb), os, sj, w);
+      YS(s, wv) = yf ? hflux_sel(w, vs, j, jde) : 0.f;
+    }
   }
   __syncthreads();
   if (!jrow) return;
@@ -99,22 +148,32 @@ __global__ void __launch_bounds__(256) k_species_update(const DevCtx xv, SpecLis
   const float wt = (k + 1 <= kde - 1) ? A3(rom, ic, k + 1, j) : 0.f, wb = (k >= 2) ? A3(rom, ic, k, j) : 0.f;
   const float msftx = A2(FL(msftx), ic, j), rdzw = A1(FL(rdnw), k);
   const UpdCoef c = upd_coef(x, ic, k, j);
-  const unsigned o = XI3(ic, k, j);
+  const unsigned o = XI3(ic, k, j), ox = XI3(i, k, j);
   for (int s = 0; s < L.n; s++) {
-    const float *q = L.in[s];
-    const float fw = xface ? sfx(x, q, uw, i, k, j) : 0.f;
+    const BufF bq = bufv(L.in[s], nb);
+    // every load of this species first: x stencil of the west face, the
+    // column window k-2 .. k+2 of both vertical faces, tendency and old
+    // value; the top wave also the stencil of its north face
+    float xw[6], yw[6], zw[5];
+    win6b(bq, ox, 1u, xw);
+#pragma unroll
+    for (int n = 0; n < 5; n++) zw[n] = bld(bq, o + (unsigned)(n - 2) * sk);
+    const float tnd = elem(L.tend[s], o);
+    const float old = elem(rk == 1 ? L.in[s] : L.old[s], o);
+    if (wv == 3) win6b(bq, o + sj, sj, yw);
+    // fluxes
+    const float fw = xface ? hflux_sel(xw, uw, i, ide) : 0.f;
     const float fe = lane_next(fw, lane);
     const float fs = YS(s, wv);
     float fn;
     if (wv < 3) fn = YS(s, wv + 1);
-    else fn = yface_n ? sfy(x, q, vn, ic, k, j + 1) : 0.f;
+    else fn = yface_n ? hflux_sel(yw, vn, j + 1, jde) : 0.f;
     float adv = 0.f;
     if (doy) adv -= msftx * x->rdy * (fn - fs);
     if (dox) adv -= msftx * x->rdx * (fe - fw);
-    adv -= rdzw * (sfz(x, q, wt, ic, k + 1, j) - sfz(x, q, wb, ic, k, j));
+    adv -= rdzw * (vflux_w(x, zw[1], zw[2], zw[3], zw[4], wt, k + 1) - vflux_w(x, zw[0], zw[1], zw[2], zw[3], wb, k));
     float tend = c.inner ? adv * c.msfty : 0.f;
-    tend = tend + elem(L.tend[s], o);
-    const float old = (rk == 1) ? elem(q, o) : elem(L.old[s], o);
+    tend = tend + tnd;
     const float out = (c.oldm * old + dt * tend) / c.newm;
     if (own) elem(L.out[s], o) = out;
   }
@@ -175,31 +234,29 @@ __device__ __forceinline__ PdFace pd_face_geom(const DevCtx *__restrict__ x, int
   }
   return f;
 }
-// antidiffusive and upwind flux of one species through a face
-__device__ __forceinline__ void pd_face_flux(const DevCtx *__restrict__ x, const PdFace &f, int axis, const float *q, const float *qo,
-                                             int i, int k, int j, float &anti, float &low) {
-  if (f.mode == 0) {
-    anti = 0.f;
-    low = 0.f;
-    return;
-  }
-  // unit stride of the face normal in the [j][k][i] layout
-  const int st = axis == 0 ? 1 : (axis == 1 ? x->kde * x->ip : x->ip);
-  const unsigned o = XI3(i, k, j);
-  const float *p = q + o, *po = qo + o;
-  low = f.muf * flux_up(po[-st], po[0], f.cr);
-  float high;
-  if (axis == 2) {
-    if (f.mode == 2) high = f.vel * (A1(FL(fnm), k) * p[0] + A1(FL(fnp), k) * p[-st]);
-    else high = f.vel * flux3pd(p[-2 * st], p[-st], p[0], p[st], -f.vel);
-  } else if (f.mode == 5) {
-    high = f.vel * flux5pd(p[-3 * st], p[-2 * st], p[-st], p[0], p[st], p[2 * st], f.vel);
-  } else if (f.mode == 2) {
-    high = 0.5f * f.vel * (p[0] + p[-st]);
-  } else {
-    high = f.vel * flux3pd(p[-2 * st], p[-st], p[0], p[st], f.vel);
-  }
-  anti = high - low;
+// Antidiffusive and upwind flux of one species through a face from values
+// already in registers (all loads of a species are issued first; every
+// order is evaluated and the face's own selected, so no branch separates the
+// loads from their use).  Horizontal: w = q at slots -3 .. +2 of the face,
+// om1 / o0 = q_old on either side.
+__device__ __forceinline__ void pd_hflux_sel(const PdFace &f, const float w[6], float om1, float o0, float &anti, float &low) {
+  const float lo = f.muf * flux_up(om1, o0, f.cr);
+  const float h5 = f.vel * flux5pd(w[0], w[1], w[2], w[3], w[4], w[5], f.vel);
+  const float h2 = 0.5f * f.vel * (w[3] + w[2]);
+  const float h3 = f.vel * flux3pd(w[1], w[2], w[3], w[4], f.vel);
+  const float hi = f.mode == 5 ? h5 : (f.mode == 2 ? h2 : h3);
+  anti = f.mode == 0 ? 0.f : hi - lo;
+  low = f.mode == 0 ? 0.f : lo;
+}
+// vertical face at level kk: q(kk-2 .. kk+1), q_old(kk-1), q_old(kk) and the
+// centred weights fnm(kk), fnp(kk)
+__device__ __forceinline__ void pd_vflux_sel(const PdFace &f, float qm2, float qm1, float q0, float qp1, float om1, float o0, float fm, float fp,
+                                             float &anti, float &low) {
+  const float lo = f.muf * flux_up(om1, o0, f.cr);
+  const float h2 = f.vel * (fm * q0 + fp * qm1);
+  const float h3 = f.vel * flux3pd(qm2, qm1, q0, qp1, -f.vel);
+  anti = f.mode == 0 ? 0.f : (f.mode == 2 ? h2 : h3) - lo;
+  low = f.mode == 0 ? 0.f : lo;
 }
 
 // pass 1: renormalisation factor of every cell (1 where not limited), over
@@ -213,12 +270,19 @@ __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecL
   GET_FACE_BLOCK(r);
 #define YA(s, w) yal_[((s) * 4 + (w)) * 64 + lane]
 #define YL(s, w) yal_[((L.n + (s)) * 4 + (w)) * 64 + lane]
-  const int ide = x->ide, jde = x->jde;
+  const int ide = x->ide, jde = x->jde, kde = x->kde;
+  const unsigned nb = (unsigned)x->n3, sj = (unsigned)(kde * x->ip), sk = (unsigned)x->ip;
   {
+    // rows past the range still form their south face (the row below reads
+    // it); its window may fall outside the field, the face is then 0
     const PdFace fs = pd_face_geom(x, 1, ic, k, j, dt);
+    const unsigned os = XI3(ic, k, j);
     for (int s = 0; s < L.n; s++) {
-      float a, l;
-      pd_face_flux(x, fs, 1, L.in[s], L.old[s], ic, k, j, a, l);
+      const BufF bo = bufv(L.old[s], nb);
+      float w[6], a, l;
+      win6b(bufv(L.in[s], nb), os, sj, w);
+      const float om1 = bld(bo, os - sj), o0 = bld(bo, os);
+      pd_hflux_sel(fs, w, om1, o0, a, l);
       YA(s, wv) = a;
       YL(s, wv) = l;
     }
@@ -234,12 +298,29 @@ __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecL
   const PdFace fx = pd_face_geom(x, 0, i, k, j, dt);
   const PdFace fyn = (wv == 3) ? pd_face_geom(x, 1, ic, k, j + 1, dt) : PdFace{0.f, 0.f, 0.f, 0};
   const PdFace fz0 = pd_face_geom(x, 2, ic, k, j, dt), fz1 = pd_face_geom(x, 2, ic, k + 1, j, dt);
-  const unsigned sj = (unsigned)(x->kde * x->ip);
+  const int k1 = imin_(k + 1, kde);
+  const float fm0 = A1(FL(fnm), k), fp0 = A1(FL(fnp), k), fm1 = A1(FL(fnm), k1), fp1 = A1(FL(fnp), k1);
+  const unsigned ox = XI3(i, k, j);
   const size_t n3 = x->n3;
   for (int s = 0; s < L.n; s++) {
-    const float *q = L.in[s], *qo = L.old[s];
+    const BufF bq = bufv(L.in[s], nb), bo = bufv(L.old[s], nb);
+    // every load of this species first: x stencil of the west face, the
+    // column windows of both vertical faces and the north face stencil (top
+    // wave), with the old values of each
+    float xw[6], yw[6], zw[5], zo[3];
+    win6b(bq, ox, 1u, xw);
+    const float xom = bld(bo, ox - 1u), xo0 = bld(bo, ox);
+#pragma unroll
+    for (int n = 0; n < 5; n++) zw[n] = bld(bq, o + (unsigned)(n - 2) * sk);
+#pragma unroll
+    for (int n = 0; n < 3; n++) zo[n] = bld(bo, o + (unsigned)(n - 1) * sk);
+    float yon = 0.f;
+    if (wv == 3) {
+      win6b(bq, o + sj, sj, yw);
+      yon = bld(bo, o + sj);
+    }
     float ax0, lx0, ay1, ly1, az0, lz0, az1, lz1;
-    pd_face_flux(x, fx, 0, q, qo, i, k, j, ax0, lx0);
+    pd_hflux_sel(fx, xw, xom, xo0, ax0, lx0);
     const float ax1 = lane_next(ax0, lane), lx1 = lane_next(lx0, lane);
     if (!own) continue;
     if (!lim) {
@@ -251,17 +332,17 @@ __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecL
       ay1 = YA(s, wv + 1);
       ly1 = YL(s, wv + 1);
     } else {
-      pd_face_flux(x, fyn, 1, q, qo, ic, k, j + 1, ay1, ly1);
+      pd_hflux_sel(fyn, yw, zo[1], yon, ay1, ly1);
     }
-    pd_face_flux(x, fz0, 2, q, qo, ic, k, j, az0, lz0);
-    pd_face_flux(x, fz1, 2, q, qo, ic, k + 1, j, az1, lz1);
+    pd_vflux_sel(fz0, zw[0], zw[1], zw[2], zw[3], zo[0], zo[1], fm0, fp0, az0, lz0);
+    pd_vflux_sel(fz1, zw[1], zw[2], zw[3], zw[4], zo[1], zo[2], fm1, fp1, az1, lz1);
     float *an = L.anti[s];
     an[o] = ax0;
     an[n3 + o] = ay0;
     an[2 * n3 + o] = az0;
     if (ic == ide - 2) an[o + 1] = ax1;
     if (j == jde - 2) an[n3 + o + sj] = ay1;
-    const float ph_low = mold * qo[o] - dt * (mm * (rdx * (lx1 - lx0) + rdy * (ly1 - ly0)) + msfty * rz * (lz1 - lz0));
+    const float ph_low = mold * zo[1] - dt * (mm * (rdx * (lx1 - lx0) + rdy * (ly1 - ly0)) + msfty * rz * (lz1 - lz0));
     const float flux_out = dt * (mm * (rdx * (fmaxf(0.f, ax1) - fminf(0.f, ax0)) + rdy * (fmaxf(0.f, ay1) - fminf(0.f, ay0))) +
                                  msfty * rz * (fminf(0.f, az1) - fmaxf(0.f, az0)));
     float sc = 1.f;
@@ -270,11 +351,6 @@ __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecL
   }
 #undef YA
 #undef YL
-}
-
-// upwind flux of one species through a face whose geometry is given
-__device__ __forceinline__ float pd_face_low(const PdFace &f, const float *qo, unsigned o, int st) {
-  return f.muf * flux_up(qo[o - st], qo[o], f.cr);
 }
 
 // pass 2: tendency from the renormalised faces, then the mass-weighted update
@@ -301,49 +377,43 @@ __global__ void __launch_bounds__(256) k_species_pd_update(const DevCtx xv, Spec
   const PdFace fx0 = pd_face_geom(x, 0, i, k, j, dt), fx1 = pd_face_geom(x, 0, i + 1, k, j, dt);
   const PdFace fy0 = pd_face_geom(x, 1, i, k, j, dt), fy1 = pd_face_geom(x, 1, i, k, j + 1, dt);
   const unsigned sj = (unsigned)(x->kde * x->ip), sk = (unsigned)x->ip;
+  const unsigned nb = (unsigned)x->n3;
   const size_t n3 = x->n3;
   for (int s = 0; s < L.n; s++) {
-    const float *qo = L.old[s], *sc = L.scale[s] + o, *an = L.anti[s];
+    // x, y and z face fluxes are separate n3-element arrays
+    const BufF bo = bufv(L.old[s], nb), bs = bufv(L.scale[s], nb), bx = bufv(L.anti[s], nb), by = bufv(L.anti[s] + n3, nb),
+               bz = bufv(L.anti[s] + 2 * n3, nb);
+    // every load of this species first (the cell below level 1 and above
+    // level kde-1 is loaded too; its values are not used)
+    const float q0 = bld(bo, o), qw = bld(bo, o - 1u), qe = bld(bo, o + 1u), qs = bld(bo, o - sj), qn = bld(bo, o + sj);
+    const float qb = bld(bo, o - sk), qt = bld(bo, o + sk);
+    const float s0 = bld(bs, o), sw = bld(bs, o - 1u), se = bld(bs, o + 1u), ss = bld(bs, o - sj), sn = bld(bs, o + sj);
+    const float sb = bld(bs, o - sk), st = bld(bs, o + sk);
+    const float ax0 = bld(bx, o), ax1 = bld(bx, o + 1u), ay0 = bld(by, o), ay1 = bld(by, o + sj);
+    const float az0r = bld(bz, o), az1r = bld(bz, o + sk);
     // z faces (no face below level 2 or above level kde-1)
-    float az0 = 0.f, lz0 = 0.f, az1 = 0.f, lz1 = 0.f;
-    if (fz0.mode != 0) {
-      az0 = an[2 * n3 + o];
-      lz0 = pd_face_low(fz0, qo, o, sk);
-    }
-    if (fz1.mode != 0) {
-      az1 = an[2 * n3 + o + sk];
-      lz1 = pd_face_low(fz1, qo, o + sk, sk);
-    }
-    float zt = az1, zb = az0;
-    if (zt < 0.f) zt *= sc[0];
-    else if (zt > 0.f && up_ok) zt *= sc[sk];
-    if (zb > 0.f) zb *= sc[0];
-    else if (zb < 0.f && dn_ok) zb *= sc[-(int)sk];
+    const float az0 = fz0.mode != 0 ? az0r : 0.f, az1 = fz1.mode != 0 ? az1r : 0.f;
+    const float lz0 = fz0.mode != 0 ? fz0.muf * flux_up(qb, q0, fz0.cr) : 0.f;
+    const float lz1 = fz1.mode != 0 ? fz1.muf * flux_up(q0, qt, fz1.cr) : 0.f;
+    const float zt = az1 < 0.f ? az1 * s0 : ((az1 > 0.f && up_ok) ? az1 * st : az1);
+    const float zb = az0 > 0.f ? az0 * s0 : ((az0 < 0.f && dn_ok) ? az0 * sb : az0);
     float t = 0.f;
     t += -rdzw * (zt - zb + lz1 - lz0);
     {
-      const float ax0 = an[o], ax1 = an[o + 1];
-      const float lx0 = pd_face_low(fx0, qo, o, 1), lx1 = pd_face_low(fx1, qo, o + 1, 1);
-      float xe = ax1, xw = ax0;
-      if (xe > 0.f) xe *= sc[0];
-      else if (xe < 0.f) xe *= sc[1];
-      if (xw < 0.f) xw *= sc[0];
-      else if (xw > 0.f) xw *= sc[-1];
+      const float lx0 = fx0.muf * flux_up(qw, q0, fx0.cr), lx1 = fx1.muf * flux_up(q0, qe, fx1.cr);
+      const float xe = ax1 > 0.f ? ax1 * s0 : (ax1 < 0.f ? ax1 * se : ax1);
+      const float xw = ax0 < 0.f ? ax0 * s0 : (ax0 > 0.f ? ax0 * sw : ax0);
       t += -msftx * (rdx * (xe - xw + lx1 - lx0));
     }
     {
-      const float ay0 = an[n3 + o], ay1 = an[n3 + o + sj];
-      const float ly0 = pd_face_low(fy0, qo, o, sj), ly1 = pd_face_low(fy1, qo, o + sj, sj);
-      float yn = ay1, ys = ay0;
-      if (yn > 0.f) yn *= sc[0];
-      else if (yn < 0.f) yn *= sc[sj];
-      if (ys < 0.f) ys *= sc[0];
-      else if (ys > 0.f) ys *= sc[-(int)sj];
+      const float ly0 = fy0.muf * flux_up(qs, q0, fy0.cr), ly1 = fy1.muf * flux_up(q0, qn, fy1.cr);
+      const float yn = ay1 > 0.f ? ay1 * s0 : (ay1 < 0.f ? ay1 * sn : ay1);
+      const float ys = ay0 < 0.f ? ay0 * s0 : (ay0 > 0.f ? ay0 * ss : ay0);
       t += -msftx * (rdy * (yn - ys + ly1 - ly0));
     }
     float tend = t * c.msfty;
     tend = tend + 0.f;
-    L.out[s][o] = (c.oldm * qo[o] + dt * tend) / c.newm;
+    elem(L.out[s], o) = (c.oldm * q0 + dt * tend) / c.newm;
   }
 }
 

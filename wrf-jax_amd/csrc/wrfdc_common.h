@@ -84,6 +84,23 @@ __device__ __forceinline__ float &elem(float *p, unsigned e) { return *reinterpr
 __device__ __forceinline__ const float &elem(const float *p, unsigned e) {
   return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(p) + (e << 2));
 }
+// Buffer-resource loads of one 3-D field: a 128-bit descriptor (base and
+// size in bytes, built in scalar registers from the field's pointer) and a
+// 32-bit byte offset per lane.  The memory unit checks every offset against
+// the size and returns 0 for an offset at or past it without an access; an
+// offset below the base wraps to a large unsigned value and is out of range
+// too.  Stencil windows near the domain edges are loaded unclamped: the
+// slots a boundary-degraded flux does not use may fall outside the field.
+struct BufF {
+  __amdgpu_buffer_rsrc_t r;
+};
+__device__ __forceinline__ BufF bufv(const float *p, unsigned n) {
+  // descriptor word 3: 32-bit data format, untyped access (gfx9 family)
+  return BufF{__builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), (short)0, (int)(n << 2), 0x00020000)};
+}
+__device__ __forceinline__ float bld(const BufF &b, unsigned e) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, (int)(e << 2), 0, 0));
+}
 #define A3(fl, i, k, j) elem(fl, XI3(i, k, j))
 #define A2(fl, i, j) elem(fl, XI2(i, j))
 #define A1(fl, k) ((fl)[(k) - 1])
