@@ -68,36 +68,99 @@ __host__ __device__ inline FaceTiles face_tiles(const R3 &r) {
   const int ic = i <= (r).i1 ? i : (r).i1;                                               \
   const int jc = jrow ? j : (r).j1
 
-// along-j / along-i stencils of a field at fixed (i,k) / (k,j)
-#define QJ(q, i, k, j) A3(q, i, k, j)
-#define QI(q, i, k, j) A3(q, i, k, j)
+// ---------------- face fluxes from loaded stencils ---------------------------
+// Every kernel below issues all loads of a point first (buffer loads,
+// unclamped: the slots a boundary-degraded order does not use may fall
+// outside the field, where they read 0), then evaluates every order of a
+// face and selects the face's own, so no branch separates the loads from
+// their use.  Each selection keeps the arithmetic of the reference's
+// expression for that order.
+
+// the 6 values around the face at element offset of along stride st
+// (slots -3 .. +2: w[3] is the cell on the face's high side)
+__device__ __forceinline__ void win6b(const BufF &b, unsigned of, unsigned st, float w[6]) {
+#pragma unroll
+  for (int n = 0; n < 6; n++) w[n] = bld(b, of + (unsigned)(n - 3) * st);
+}
+// the column window of levels k-2 .. k+2 at element offset o
+__device__ __forceinline__ void win5z(const BufF &b, unsigned o, unsigned sk, float w[5]) {
+#pragma unroll
+  for (int n = 0; n < 5; n++) w[n] = bld(b, o + (unsigned)(n - 2) * sk);
+}
+// scalar and w faces along x / y (fx_s, fy_s, fx_w, fy_w): face velocity
+// vel; 5th order on [4, fe-3], centred at 2 and fe-1, 3rd order between
+__device__ __forceinline__ float hflux_sel(const float w[6], float vel, int f, int fe) {
+  const float h5 = vel * flux5(w[0], w[1], w[2], w[3], w[4], w[5], vel);
+  const float h2 = 0.5f * vel * (w[3] + w[2]);
+  const float h3 = vel * flux3(w[1], w[2], w[3], w[4], vel);
+  return (f >= 4 && f <= fe - 3) ? h5 : ((f == 2 || f == fe - 1) ? h2 : h3);
+}
+// fy_u: rvs = rv(i) + rv(i-1) at the face
+__device__ __forceinline__ float fy_u_sel(const float w[6], float rvs, int j, int jde) {
+  const float vel = 0.5f * rvs;
+  const float h5 = vel * flux5(w[0], w[1], w[2], w[3], w[4], w[5], vel);
+  const float h2 = 0.25f * rvs * (w[3] + w[2]);
+  const float h3 = vel * flux3(w[1], w[2], w[3], w[4], vel);
+  return (j >= 4 && j <= jde - 3) ? h5 : ((j == 2 || j == jde - 1) ? h2 : h3);
+}
+// fx_u: rus = ru(i) + ru(i-1); 5th order on [4, ide-2], upwind-biased
+// second order at the boundary faces 2 and ide
+__device__ __forceinline__ float fx_u_sel(const float w[6], float rus, int i, int ide) {
+  const float vel = 0.5f * rus;
+  const float h5 = vel * flux5(w[0], w[1], w[2], w[3], w[4], w[5], vel);
+  const float h3 = vel * flux3(w[1], w[2], w[3], w[4], vel);
+  const float ub_lo = w[3] < 0.f ? w[3] : w[2];
+  const float h_lo = 0.25f * rus * (w[3] + ub_lo);
+  const float ub_hi = w[2] > 0.f ? w[2] : w[3];
+  const float h_hi = 0.25f * rus * (w[2] + ub_hi);
+  return (i >= 4 && i <= ide - 2) ? h5 : (i == 2 ? h_lo : (i == ide ? h_hi : h3));
+}
+// fy_v: rvs = rv(j) + rv(j-1); 5th order on [4, jde-2], upwind-biased second
+// order at the boundary faces 2 and jde
+__device__ __forceinline__ float fy_v_sel(const float w[6], float rvs, int j, int jde) {
+  const float vel = 0.5f * rvs;
+  const float h5 = vel * flux5(w[0], w[1], w[2], w[3], w[4], w[5], vel);
+  const float h3 = vel * flux3(w[1], w[2], w[3], w[4], vel);
+  const float vb_lo = w[3] < 0.f ? w[3] : w[2];
+  const float h_lo = 0.25f * rvs * (w[3] + vb_lo);
+  const float vb_hi = w[2] > 0.f ? w[2] : w[3];
+  const float h_hi = 0.25f * rvs * (vb_hi + w[2]);
+  return (j >= 4 && j <= jde - 2) ? h5 : (j == 2 ? h_lo : (j == jde ? h_hi : h3));
+}
+// fx_v: rus = ru(j) + ru(j-1); 5th order on [4, ide-3], centred at 2 and ide-1
+__device__ __forceinline__ float fx_v_sel(const float w[6], float rus, int i, int ide) {
+  const float vel = 0.5f * rus;
+  const float h5 = vel * flux5(w[0], w[1], w[2], w[3], w[4], w[5], vel);
+  const float h2 = 0.25f * rus * (w[3] + w[2]);
+  const float h3 = vel * flux3(w[1], w[2], w[3], w[4], vel);
+  return (i >= 4 && i <= ide - 3) ? h5 : ((i == 2 || i == ide - 1) ? h2 : h3);
+}
+// vertical faces of mass-level fields (fz_s, fz_u, fz_v) at level kk from
+// q(kk-2 .. kk+1), the face velocity and the weights fnm(kk), fnp(kk): none
+// at kk <= 1 and kk >= kde, weighted centred at 2 and kde-1, 3rd order between
+__device__ __forceinline__ float zflux_sel(float qm2, float qm1, float q0, float qp1, float vel, int kk, int kde, float fm, float fp) {
+  const float h2 = vel * (fm * q0 + fp * qm1);
+  const float h3 = vel * flux3(qm2, qm1, q0, qp1, -vel);
+  return (kk <= 1 || kk >= kde) ? 0.f : ((kk == 2 || kk == kde - 1) ? h2 : h3);
+}
+// fz_w at w level kk: roms = rom(kk) + rom(kk-1); none at kk <= 1 and
+// kk > kde, centred at 2 and kde, 3rd order between
+__device__ __forceinline__ float zflux_w_sel(float qm2, float qm1, float q0, float qp1, float roms, int kk, int kde) {
+  const float h2 = 0.25f * roms * (q0 + qm1);
+  const float vel = 0.5f * roms;
+  const float h3 = vel * flux3(qm2, qm1, q0, qp1, -vel);
+  return (kk <= 1 || kk > kde) ? 0.f : ((kk == 2 || kk == kde) ? h2 : h3);
+}
+// mass flux at a w level from the values at levels k, k-1, k-2 of the column
+// (extrapolated at the top level kde)
+__device__ __forceinline__ float rvw_sel(const DevCtx *__restrict__ x, float r0, float rm1, float rm2, int k) {
+  const int kde = x->kde, kc = imin_(k, kde - 1);
+  const float top = (2.f - A1(FL(fnm), kde - 1)) * rm1 - A1(FL(fnp), kde - 1) * rm2;
+  const float mid = A1(FL(fnm), kc) * r0 + A1(FL(fnp), kc) * rm1;
+  return k == kde ? top : mid;
+}
 
 // ---------------- scalar (theta, moist, scalars) ----------------------------
-__device__ __forceinline__ float fy_s(const DevCtx *__restrict__ x, const float *q, const float *rv, int i, int k, int j) {
-  const int jde = x->jde;
-  float vel = A3(rv, i, k, j);
-  if (j >= 4 && j <= jde - 3)
-    return vel * flux5(A3(q, i, k, j - 3), A3(q, i, k, j - 2), A3(q, i, k, j - 1), A3(q, i, k, j), A3(q, i, k, j + 1), A3(q, i, k, j + 2), vel);
-  if (j == 2 || j == jde - 1) return 0.5f * vel * (A3(q, i, k, j) + A3(q, i, k, j - 1));
-  return vel * flux3(A3(q, i, k, j - 2), A3(q, i, k, j - 1), A3(q, i, k, j), A3(q, i, k, j + 1), vel);
-}
-__device__ __forceinline__ float fx_s(const DevCtx *__restrict__ x, const float *q, const float *ru, int i, int k, int j) {
-  const int ide = x->ide;
-  float vel = A3(ru, i, k, j);
-  if (i >= 4 && i <= ide - 3)
-    return vel * flux5(A3(q, i - 3, k, j), A3(q, i - 2, k, j), A3(q, i - 1, k, j), A3(q, i, k, j), A3(q, i + 1, k, j), A3(q, i + 2, k, j), vel);
-  if (i == 2 || i == ide - 1) return 0.5f * vel * (A3(q, i, k, j) + A3(q, i - 1, k, j));
-  return vel * flux3(A3(q, i - 2, k, j), A3(q, i - 1, k, j), A3(q, i, k, j), A3(q, i + 1, k, j), vel);
-}
-__device__ __forceinline__ float fz_s(const DevCtx *__restrict__ x, const float *q, const float *rom, int i, int k, int j) {
-  const int ktf = x->kde - 1;
-  if (k <= 1 || k >= x->kde) return 0.f;
-  const float *fzm = FL(fnm), *fzp = FL(fnp);
-  float vel = A3(rom, i, k, j);
-  if (k == 2 || k == ktf) return vel * (A1(fzm, k) * A3(q, i, k, j) + A1(fzp, k) * A3(q, i, k - 1, j));
-  return vel * flux3(A3(q, i, k - 2, j), A3(q, i, k - 1, j), A3(q, i, k, j), A3(q, i, k + 1, j), -vel);
-}
-
 // advect_scalar (module_advect_em.py:90-369): tend -= div(flux), mass points
 // (face-sharing blocks; r = mass points; y faces j in [2, jde-1]).
 // TEND: the theta tendency of the RK step, from zero (the cleared array) plus
@@ -108,242 +171,277 @@ __global__ void __launch_bounds__(256) k_advect_scalar(const DevCtx xv, const fl
   const DevCtx *__restrict__ x = &xv;
   __shared__ float ys[4][64];
   GET_FACE_BLOCK(r);
-  const float *msftx = FL(msftx), *rdzw = FL(rdnw);
-  const int ide = x->ide, jde = x->jde;
-  const float fs = (j >= 2 && j <= jde - 1) ? fy_s(x, q, rv, ic, k, j) : 0.f;
+  const int ide = x->ide, jde = x->jde, kde = x->kde;
+  const unsigned nb = (unsigned)x->n3, sj = (unsigned)(kde * x->ip), sk = (unsigned)x->ip;
+  const BufF bq = bufv(q, nb), bv = bufv(rv, nb);
+  const unsigned o = XI3(ic, k, j);
+  float fs;
+  {
+    float w[6];
+    win6b(bq, o, sj, w);
+    const float f = hflux_sel(w, bld(bv, o), j, jde);
+    fs = (j >= 2 && j <= jde - 1) ? f : 0.f;
+  }
   ys[wv][lane] = fs;
   __syncthreads();
   if (!jrow) return;
-  float fn;
-  if (wv < 3) fn = ys[wv + 1][lane];
-  else fn = (j + 1 <= jde - 1) ? fy_s(x, q, rv, ic, k, j + 1) : 0.f;
-  const float fw = (i >= 2 && i <= ide - 1) ? fx_s(x, q, ru, i, k, j) : 0.f;
+  // every load of the point: x stencil of the west face, column window,
+  // north face stencil (top wave)
+  const BufF bu = bufv(ru, nb), bw = bufv(rom, nb);
+  const unsigned ox = XI3(i, k, j);
+  float wx[6], wz[5], wn[6];
+  win6b(bq, ox, 1u, wx);
+  win5z(bq, o, sk, wz);
+  const float uw = bld(bu, ox), wt = bld(bw, o + sk), wb = bld(bw, o);
+  float vn = 0.f;
+  if (wv == 3) {
+    win6b(bq, o + sj, sj, wn);
+    vn = bld(bv, o + sj);
+  }
+  const int k1 = imin_(k + 1, kde);
+  const float fm0 = A1(FL(fnm), k), fp0 = A1(FL(fnp), k), fm1 = A1(FL(fnm), k1), fp1 = A1(FL(fnp), k1);
+  const float msf = A2(FL(msftx), ic, j), rdz = A1(FL(rdnw), k);
+  const float t0 = TEND ? 0.f : elem(tend, o);
+  const float fw0 = hflux_sel(wx, uw, i, ide);
+  const float fw = (i >= 2 && i <= ide - 1) ? fw0 : 0.f;
   const float fe = lane_next(fw, lane);
   if (!own) return;
-  float t = TEND ? 0.f : A3(tend, ic, k, j);
-  if (j >= 2 && j <= jde - 2) t -= A2(msftx, ic, j) * x->rdy * (fn - fs);
-  if (ic >= 2 && ic <= ide - 2) t -= A2(msftx, ic, j) * x->rdx * (fe - fw);
-  t -= A1(rdzw, k) * (fz_s(x, q, rom, ic, k + 1, j) - fz_s(x, q, rom, ic, k, j));
+  float fn;
+  if (wv < 3) fn = ys[wv + 1][lane];
+  else fn = (j + 1 <= jde - 1) ? hflux_sel(wn, vn, j + 1, jde) : 0.f;
+  float t = t0;
+  if (j >= 2 && j <= jde - 2) t -= msf * x->rdy * (fn - fs);
+  if (ic >= 2 && ic <= ide - 2) t -= msf * x->rdx * (fe - fw);
+  t -= rdz * (zflux_sel(wz[1], wz[2], wz[3], wz[4], wt, k + 1, kde, fm1, fp1) - zflux_sel(wz[0], wz[1], wz[2], wz[3], wb, k, kde, fm0, fp0));
   if (TEND) t = addtend_t(x, ic, k, j, rk, t);
-  A3(tend, ic, k, j) = t;
+  elem(tend, o) = t;
 }
 
 // ---------------- u momentum (u points i in [2, ide-1]) ---------------------
-__device__ __forceinline__ float fy_u(const DevCtx *__restrict__ x, const float *u, const float *rv, int i, int k, int j) {
-  const int jde = x->jde;
-  if (j >= 4 && j <= jde - 3) {
-    float vel = 0.5f * (A3(rv, i, k, j) + A3(rv, i - 1, k, j));
-    return vel * flux5(A3(u, i, k, j - 3), A3(u, i, k, j - 2), A3(u, i, k, j - 1), A3(u, i, k, j), A3(u, i, k, j + 1), A3(u, i, k, j + 2), vel);
-  }
-  if (j == 2 || j == jde - 1) return 0.25f * (A3(rv, i, k, j) + A3(rv, i - 1, k, j)) * (A3(u, i, k, j) + A3(u, i, k, j - 1));
-  float vel = 0.5f * (A3(rv, i, k, j) + A3(rv, i - 1, k, j));
-  return vel * flux3(A3(u, i, k, j - 2), A3(u, i, k, j - 1), A3(u, i, k, j), A3(u, i, k, j + 1), vel);
-}
-__device__ __forceinline__ float fx_u(const DevCtx *__restrict__ x, const float *u, const float *ru, int i, int k, int j) {
-  const int ide = x->ide;
-  if (i >= 4 && i <= ide - 2) {
-    float vel = 0.5f * (A3(ru, i, k, j) + A3(ru, i - 1, k, j));
-    return vel * flux5(A3(u, i - 3, k, j), A3(u, i - 2, k, j), A3(u, i - 1, k, j), A3(u, i, k, j), A3(u, i + 1, k, j), A3(u, i + 2, k, j), vel);
-  }
-  if (i == 2) {
-    float ub = A3(u, i - 1, k, j);
-    if (A3(u, i, k, j) < 0.f) ub = A3(u, i, k, j);
-    return 0.25f * (A3(ru, i, k, j) + A3(ru, i - 1, k, j)) * (A3(u, i, k, j) + ub);
-  }
-  if (i == ide) {
-    float ub = A3(u, i, k, j);
-    if (A3(u, i - 1, k, j) > 0.f) ub = A3(u, i - 1, k, j);
-    return 0.25f * (A3(ru, i, k, j) + A3(ru, i - 1, k, j)) * (A3(u, i - 1, k, j) + ub);
-  }
-  float vel = 0.5f * (A3(ru, i, k, j) + A3(ru, i - 1, k, j));  // i == 3 or ide-1
-  return vel * flux3(A3(u, i - 2, k, j), A3(u, i - 1, k, j), A3(u, i, k, j), A3(u, i + 1, k, j), vel);
-}
-__device__ __forceinline__ float fz_u(const DevCtx *__restrict__ x, const float *u, const float *rom, int i, int k, int j) {
-  const int ktf = x->kde - 1;
-  if (k <= 1 || k >= x->kde) return 0.f;
-  const float *fzm = FL(fnm), *fzp = FL(fnp);
-  if (k == 2 || k == ktf)
-    return 0.5f * (A3(rom, i, k, j) + A3(rom, i - 1, k, j)) * (A1(fzm, k) * A3(u, i, k, j) + A1(fzp, k) * A3(u, i, k - 1, j));
-  float vel = 0.5f * (A3(rom, i - 1, k, j) + A3(rom, i, k, j));
-  return vel * flux3(A3(u, i, k - 2, j), A3(u, i, k - 1, j), A3(u, i, k, j), A3(u, i, k + 1, j), -vel);
-}
 // advect_u (module_advect_em.py:762-1034); r covers i [2, ide-1], j [1, jde-1]
 // (face-sharing blocks: x faces i in [2, ide], y faces j in [2, jde-1]).
 // TEND: the u tendency of the RK step over i [1, ide]: from zero (the cleared
 // array), advection, horizontal pressure gradient and Coriolis/curvature at
 // i [2, ide-1] in the reference's order, then rk_addtend_dry; written once.
+// The pressure-gradient and Coriolis terms are evaluated at every point (at
+// a column clamped into [2, ide-1], so their loads stay in the field) and
+// used where they apply.
 template <bool TEND>
 __global__ void __launch_bounds__(256) k_advect_u(const DevCtx xv, const float *u, float *tend, const float *ru, const float *rv,
                                                   const float *rom, R3 r, int rk) {
   const DevCtx *__restrict__ x = &xv;
   __shared__ float ys[4][64];
   GET_FACE_BLOCK(r);
-  const float *msfux = FL(msfux), *rdzw = FL(rdnw);
-  const int ide = x->ide, jde = x->jde;
+  const int ide = x->ide, jde = x->jde, kde = x->kde;
+  const unsigned nb = (unsigned)x->n3, sj = (unsigned)(kde * x->ip), sk = (unsigned)x->ip;
+  const BufF bq = bufv(u, nb), bv = bufv(rv, nb);
   const bool adv = !TEND || (ic >= 2 && ic <= ide - 1);
-  const float fs = (adv && j >= 2 && j <= jde - 1) ? fy_u(x, u, rv, ic, k, j) : 0.f;
+  const unsigned o = XI3(ic, k, j);
+  float fs;
+  {
+    float w[6];
+    win6b(bq, o, sj, w);
+    const float f = fy_u_sel(w, bld(bv, o) + bld(bv, o - 1u), j, jde);
+    fs = (adv && j >= 2 && j <= jde - 1) ? f : 0.f;
+  }
   ys[wv][lane] = fs;
   __syncthreads();
   if (!jrow) return;
-  float fn;
-  if (wv < 3) fn = ys[wv + 1][lane];
-  else fn = (adv && j + 1 <= jde - 1) ? fy_u(x, u, rv, ic, k, j + 1) : 0.f;
-  const float fw = (i >= 2 && i <= ide) ? fx_u(x, u, ru, i, k, j) : 0.f;
+  const BufF bu = bufv(ru, nb), bw = bufv(rom, nb);
+  const unsigned ox = XI3(i, k, j);
+  float wx[6], wz[5], wn[6];
+  win6b(bq, ox, 1u, wx);
+  win5z(bq, o, sk, wz);
+  const float rus = bld(bu, ox) + bld(bu, ox - 1u);
+  const float romt = bld(bw, o + sk) + bld(bw, o + sk - 1u), romb = bld(bw, o) + bld(bw, o - 1u);
+  float rvn = 0.f;
+  if (wv == 3) {
+    win6b(bq, o + sj, sj, wn);
+    rvn = bld(bv, o + sj) + bld(bv, o + sj - 1u);
+  }
+  const int k1 = imin_(k + 1, kde);
+  const float fm0 = A1(FL(fnm), k), fp0 = A1(FL(fnp), k), fm1 = A1(FL(fnm), k1), fp1 = A1(FL(fnp), k1);
+  const float msf = A2(FL(msfux), ic, j), rdz = A1(FL(rdnw), k);
+  const float t0 = TEND ? 0.f : elem(tend, o);
+  float hp = 0.f, mt = 0.f;
+  float4 cc = make_float4(0.f, 0.f, 0.f, 0.f);
+  float2 tfl = make_float2(0.f, 0.f);
+  if (TEND) {
+    const int ia = imin_(imax_(ic, 2), ide - 1);
+    hp = hpg_term(x, 0, ia, k, j);
+    cc = cor_u_terms(x, ia, k, j);
+    tfl = tendf_load(FL(ru_tendf), FL(u_save), o, rk);
+    mt = A2(FL(msfuy), ic, j);
+  }
+  const float fw0 = fx_u_sel(wx, rus, i, ide);
+  const float fw = (i >= 2 && i <= ide) ? fw0 : 0.f;
   const float fe = lane_next(fw, lane);
   if (!own) return;
-  float t = TEND ? 0.f : A3(tend, ic, k, j);
+  float fn;
+  if (wv < 3) fn = ys[wv + 1][lane];
+  else fn = (adv && j + 1 <= jde - 1) ? fy_u_sel(wn, rvn, j + 1, jde) : 0.f;
+  float t = t0;
   if (adv) {
-    if (j >= 2 && j <= jde - 2) t -= A2(msfux, ic, j) * x->rdy * (fn - fs);
-    t -= A2(msfux, ic, j) * x->rdx * (fe - fw);
-    t -= A1(rdzw, k) * (fz_u(x, u, rom, ic, k + 1, j) - fz_u(x, u, rom, ic, k, j));
+    if (j >= 2 && j <= jde - 2) t -= msf * x->rdy * (fn - fs);
+    t -= msf * x->rdx * (fe - fw);
+    t -= rdz * (zflux_sel(wz[1], wz[2], wz[3], wz[4], 0.5f * romt, k + 1, kde, fm1, fp1) -
+                zflux_sel(wz[0], wz[1], wz[2], wz[3], 0.5f * romb, k, kde, fm0, fp0));
     if (TEND) {
-      t = t - hpg_term(x, 0, ic, k, j);
-      t = cor_curv_u(x, ic, k, j, t);
+      t = t - hp;
+      t = cor_u_add(t, cc);
     }
   }
-  if (TEND) t = addtend_u(x, ic, k, j, rk, t);
-  A3(tend, ic, k, j) = t;
+  if (TEND) t = t + tendf_fold(FL(ru_tendf), o, tfl, mt, rk) / mt;
+  elem(tend, o) = t;
 }
+
 // ---------------- v momentum (v rows j in [2, jde-1]) -----------------------
-__device__ __forceinline__ float fy_v(const DevCtx *__restrict__ x, const float *v, const float *rv, int i, int k, int j) {
-  const int jde = x->jde;
-  if (j >= 4 && j <= jde - 2) {
-    float vel = 0.5f * (A3(rv, i, k, j) + A3(rv, i, k, j - 1));
-    return vel * flux5(A3(v, i, k, j - 3), A3(v, i, k, j - 2), A3(v, i, k, j - 1), A3(v, i, k, j), A3(v, i, k, j + 1), A3(v, i, k, j + 2), vel);
-  }
-  if (j == 2) {
-    float vb = A3(v, i, k, j - 1);
-    if (A3(v, i, k, j) < 0.f) vb = A3(v, i, k, j);
-    return 0.25f * (A3(rv, i, k, j) + A3(rv, i, k, j - 1)) * (A3(v, i, k, j) + vb);
-  }
-  if (j == jde) {
-    float vb = A3(v, i, k, j);
-    if (A3(v, i, k, j - 1) > 0.f) vb = A3(v, i, k, j - 1);
-    return 0.25f * (A3(rv, i, k, j) + A3(rv, i, k, j - 1)) * (vb + A3(v, i, k, j - 1));
-  }
-  float vel = 0.5f * (A3(rv, i, k, j) + A3(rv, i, k, j - 1));  // j == 3 or jde-1
-  return vel * flux3(A3(v, i, k, j - 2), A3(v, i, k, j - 1), A3(v, i, k, j), A3(v, i, k, j + 1), vel);
-}
-__device__ __forceinline__ float fx_v(const DevCtx *__restrict__ x, const float *v, const float *ru, int i, int k, int j) {
-  const int ide = x->ide;
-  if (i >= 4 && i <= ide - 3) {
-    float vel = 0.5f * (A3(ru, i, k, j) + A3(ru, i, k, j - 1));
-    return vel * flux5(A3(v, i - 3, k, j), A3(v, i - 2, k, j), A3(v, i - 1, k, j), A3(v, i, k, j), A3(v, i + 1, k, j), A3(v, i + 2, k, j), vel);
-  }
-  if (i == 2 || i == ide - 1) return 0.25f * (A3(ru, i, k, j) + A3(ru, i, k, j - 1)) * (A3(v, i, k, j) + A3(v, i - 1, k, j));
-  float vel = 0.5f * (A3(ru, i, k, j) + A3(ru, i, k, j - 1));  // i == 3 or ide-2
-  return vel * flux3(A3(v, i - 2, k, j), A3(v, i - 1, k, j), A3(v, i, k, j), A3(v, i + 1, k, j), vel);
-}
-__device__ __forceinline__ float fz_v(const DevCtx *__restrict__ x, const float *v, const float *rom, int i, int k, int j) {
-  const int ktf = x->kde - 1;
-  if (k <= 1 || k >= x->kde) return 0.f;
-  const float *fzm = FL(fnm), *fzp = FL(fnp);
-  if (k == 2 || k == ktf)
-    return 0.5f * (A3(rom, i, k, j) + A3(rom, i, k, j - 1)) * (A1(fzm, k) * A3(v, i, k, j) + A1(fzp, k) * A3(v, i, k - 1, j));
-  float vel = 0.5f * (A3(rom, i, k, j) + A3(rom, i, k, j - 1));
-  return vel * flux3(A3(v, i, k - 2, j), A3(v, i, k - 1, j), A3(v, i, k, j), A3(v, i, k + 1, j), -vel);
-}
 // advect_v (module_advect_em.py:1037-1354); r covers i [1, ide-1], j [2, jde-1]
 // (face-sharing blocks: x faces i in [2, ide-1], y faces j in [2, jde]).
 // TEND: the v tendency of the RK step over j [1, jde] (advection, pressure
-// gradient, Coriolis/curvature at j [2, jde-1], then rk_addtend_dry).
+// gradient, Coriolis/curvature at j [2, jde-1], then rk_addtend_dry); the
+// pressure-gradient and Coriolis terms at a row clamped into [2, jde-1].
 template <bool TEND>
 __global__ void __launch_bounds__(256) k_advect_v(const DevCtx xv, const float *v, float *tend, const float *ru, const float *rv,
                                                   const float *rom, R3 r, int rk) {
   const DevCtx *__restrict__ x = &xv;
   __shared__ float ys[4][64];
   GET_FACE_BLOCK(r);
-  const float *msfvy = FL(msfvy), *msfvx = FL(msfvx), *rdzw = FL(rdnw);
-  const int ide = x->ide, jde = x->jde;
+  const int ide = x->ide, jde = x->jde, kde = x->kde;
+  const unsigned nb = (unsigned)x->n3, sj = (unsigned)(kde * x->ip), sk = (unsigned)x->ip;
+  const BufF bq = bufv(v, nb), bv = bufv(rv, nb);
   const bool adv = !TEND || (j >= 2 && j <= jde - 1);  // whole wave
-  const float fs = (j >= 2 && j <= jde) ? fy_v(x, v, rv, ic, k, j) : 0.f;
+  const unsigned o = XI3(ic, k, j);
+  float fs;
+  {
+    float w[6];
+    win6b(bq, o, sj, w);
+    const float f = fy_v_sel(w, bld(bv, o) + bld(bv, o - sj), j, jde);
+    fs = (j >= 2 && j <= jde) ? f : 0.f;
+  }
   ys[wv][lane] = fs;
   __syncthreads();
   if (!jrow) return;
-  float fn;
-  if (wv < 3) fn = ys[wv + 1][lane];
-  else fn = (j + 1 >= 2 && j + 1 <= jde) ? fy_v(x, v, rv, ic, k, j + 1) : 0.f;
-  const float fw = (adv && i >= 2 && i <= ide - 1) ? fx_v(x, v, ru, i, k, j) : 0.f;
+  const BufF bu = bufv(ru, nb), bw = bufv(rom, nb);
+  const unsigned ox = XI3(i, k, j);
+  float wx[6], wz[5], wn[6];
+  win6b(bq, ox, 1u, wx);
+  win5z(bq, o, sk, wz);
+  const float rus = bld(bu, ox) + bld(bu, ox - sj);
+  const float romt = bld(bw, o + sk) + bld(bw, o + sk - sj), romb = bld(bw, o) + bld(bw, o - sj);
+  float rvn = 0.f;
+  if (wv == 3) {
+    win6b(bq, o + sj, sj, wn);
+    rvn = bld(bv, o + sj) + bld(bv, o);
+  }
+  const int k1 = imin_(k + 1, kde);
+  const float fm0 = A1(FL(fnm), k), fp0 = A1(FL(fnp), k), fm1 = A1(FL(fnm), k1), fp1 = A1(FL(fnp), k1);
+  const float msy = A2(FL(msfvy), ic, j), msx = A2(FL(msfvx), ic, j), rdz = A1(FL(rdnw), k);
+  const float t0 = TEND ? 0.f : elem(tend, o);
+  float hp = 0.f, mvx = 0.f, mvxi = 0.f;
+  float4 cc = make_float4(0.f, 0.f, 0.f, 0.f);
+  float2 tfl = make_float2(0.f, 0.f);
+  if (TEND) {
+    const int ja = imin_(imax_(j, 2), jde - 1);
+    hp = hpg_term(x, 1, ic, k, ja);
+    cc = cor_v_terms(x, ic, k, ja);
+    tfl = tendf_load(FL(rv_tendf), FL(v_save), o, rk);
+    mvx = msx;
+    mvxi = A2(FL(msfvx_inv), ic, j);
+  }
+  const float fw0 = fx_v_sel(wx, rus, i, ide);
+  const float fw = (adv && i >= 2 && i <= ide - 1) ? fw0 : 0.f;
   const float fe = lane_next(fw, lane);
   if (!own) return;
-  float t = TEND ? 0.f : A3(tend, ic, k, j);
+  float fn;
+  if (wv < 3) fn = ys[wv + 1][lane];
+  else fn = (j + 1 >= 2 && j + 1 <= jde) ? fy_v_sel(wn, rvn, j + 1, jde) : 0.f;
+  float t = t0;
   if (adv) {
-    t -= A2(msfvy, ic, j) * x->rdy * (fn - fs);
-    if (ic >= 2 && ic <= ide - 2) t -= A2(msfvy, ic, j) * x->rdx * (fe - fw);
-    t -= (A2(msfvy, ic, j) / A2(msfvx, ic, j)) * A1(rdzw, k) * (fz_v(x, v, rom, ic, k + 1, j) - fz_v(x, v, rom, ic, k, j));
+    t -= msy * x->rdy * (fn - fs);
+    if (ic >= 2 && ic <= ide - 2) t -= msy * x->rdx * (fe - fw);
+    t -= (msy / msx) * rdz *
+         (zflux_sel(wz[1], wz[2], wz[3], wz[4], 0.5f * romt, k + 1, kde, fm1, fp1) - zflux_sel(wz[0], wz[1], wz[2], wz[3], 0.5f * romb, k, kde, fm0, fp0));
     if (TEND) {
-      t = t - hpg_term(x, 1, ic, k, j);
-      t = cor_curv_v(x, ic, k, j, t);
+      t = t - hp;
+      t = cor_v_add(t, cc);
     }
   }
-  if (TEND) t = addtend_v(x, ic, k, j, rk, t);
-  A3(tend, ic, k, j) = t;
+  if (TEND) t = t + tendf_fold(FL(rv_tendf), o, tfl, mvx, rk) * mvxi;
+  elem(tend, o) = t;
 }
+
 // ---------------- w momentum (levels k in [2, kde]) -------------------------
-__device__ __forceinline__ float rvw(const DevCtx *__restrict__ x, const float *rv, int i, int k, int j) {
-  const float *fzm = FL(fnm), *fzp = FL(fnp);
-  if (k == x->kde) return (2.f - A1(fzm, k - 1)) * A3(rv, i, k - 1, j) - A1(fzp, k - 1) * A3(rv, i, k - 2, j);
-  return A1(fzm, k) * A3(rv, i, k, j) + A1(fzp, k) * A3(rv, i, k - 1, j);
-}
-__device__ __forceinline__ float fy_w(const DevCtx *__restrict__ x, const float *w, const float *rv, int i, int k, int j) {
-  const int jde = x->jde;
-  float vel = rvw(x, rv, i, k, j);
-  if (j >= 4 && j <= jde - 3)
-    return vel * flux5(A3(w, i, k, j - 3), A3(w, i, k, j - 2), A3(w, i, k, j - 1), A3(w, i, k, j), A3(w, i, k, j + 1), A3(w, i, k, j + 2), vel);
-  if (j == 2 || j == jde - 1) return 0.5f * vel * (A3(w, i, k, j) + A3(w, i, k, j - 1));
-  return vel * flux3(A3(w, i, k, j - 2), A3(w, i, k, j - 1), A3(w, i, k, j), A3(w, i, k, j + 1), vel);
-}
-__device__ __forceinline__ float fx_w(const DevCtx *__restrict__ x, const float *w, const float *ru, int i, int k, int j) {
-  const int ide = x->ide;
-  float vel = rvw(x, ru, i, k, j);
-  if (i >= 4 && i <= ide - 3)
-    return vel * flux5(A3(w, i - 3, k, j), A3(w, i - 2, k, j), A3(w, i - 1, k, j), A3(w, i, k, j), A3(w, i + 1, k, j), A3(w, i + 2, k, j), vel);
-  if (i == 2 || i == ide - 1) return 0.5f * vel * (A3(w, i, k, j) + A3(w, i - 1, k, j));
-  return vel * flux3(A3(w, i - 2, k, j), A3(w, i - 1, k, j), A3(w, i, k, j), A3(w, i + 1, k, j), vel);
-}
-__device__ __forceinline__ float fz_w(const DevCtx *__restrict__ x, const float *w, const float *rom, int i, int k, int j) {
-  if (k <= 1 || k > x->kde) return 0.f;
-  if (k == 2 || k == x->kde) return 0.25f * (A3(rom, i, k, j) + A3(rom, i, k - 1, j)) * (A3(w, i, k, j) + A3(w, i, k - 1, j));
-  float vel = 0.5f * (A3(rom, i, k, j) + A3(rom, i, k - 1, j));
-  return vel * flux3(A3(w, i, k - 2, j), A3(w, i, k - 1, j), A3(w, i, k, j), A3(w, i, k + 1, j), -vel);
-}
 // advect_w (module_advect_em.py:1357-1640); r covers i [1, ide-1], k [2, kde], j [1, jde-1]
 // (face-sharing blocks: x faces i in [2, ide-1], y faces j in [2, jde-1]).
 // TEND: the w tendency of the RK step over k [1, kde]: advection, buoyancy
 // and pressure (pg_buoy_w, reading cqw before k_cqw_inv replaces it) and, below
-// the top, Coriolis/curvature at k [2, kde]; then rk_addtend_dry.
+// the top, Coriolis/curvature at k [2, kde]; then rk_addtend_dry.  Buoyancy
+// and Coriolis terms at a level clamped into the range they apply to.
 template <bool TEND>
 __global__ void __launch_bounds__(256) k_advect_w(const DevCtx xv, const float *w, float *tend, const float *ru, const float *rv,
                                                   const float *rom, R3 r, int rk) {
   const DevCtx *__restrict__ x = &xv;
   __shared__ float ys[4][64];
   GET_FACE_BLOCK(r);
-  const float *msftx = FL(msftx), *rdzu = FL(rdn);
-  const int ide = x->ide, jde = x->jde;
+  const int ide = x->ide, jde = x->jde, kde = x->kde;
+  const unsigned nb = (unsigned)x->n3, sj = (unsigned)(kde * x->ip), sk = (unsigned)x->ip;
+  const BufF bq = bufv(w, nb), bv = bufv(rv, nb);
   const bool adv = !TEND || k >= 2;  // whole block
-  const float fs = (adv && j >= 2 && j <= jde - 1) ? fy_w(x, w, rv, ic, k, j) : 0.f;
+  const unsigned o = XI3(ic, k, j);
+  float fs;
+  {
+    float wy[6];
+    win6b(bq, o, sj, wy);
+    const float f = hflux_sel(wy, rvw_sel(x, bld(bv, o), bld(bv, o - sk), bld(bv, o - 2u * sk), k), j, jde);
+    fs = (adv && j >= 2 && j <= jde - 1) ? f : 0.f;
+  }
   ys[wv][lane] = fs;
   __syncthreads();
   if (!jrow) return;
-  float fn;
-  if (wv < 3) fn = ys[wv + 1][lane];
-  else fn = (adv && j + 1 <= jde - 1) ? fy_w(x, w, rv, ic, k, j + 1) : 0.f;
-  const float fw = (adv && i >= 2 && i <= ide - 1) ? fx_w(x, w, ru, i, k, j) : 0.f;
+  const BufF bu = bufv(ru, nb), bw = bufv(rom, nb);
+  const unsigned ox = XI3(i, k, j);
+  float wx[6], wz[5], wn[6];
+  win6b(bq, ox, 1u, wx);
+  win5z(bq, o, sk, wz);
+  const float uw = rvw_sel(x, bld(bu, ox), bld(bu, ox - sk), bld(bu, ox - 2u * sk), k);
+  const float rp = bld(bw, o + sk), r0 = bld(bw, o), rm = bld(bw, o - sk);
+  float vn = 0.f;
+  if (wv == 3) {
+    win6b(bq, o + sj, sj, wn);
+    vn = rvw_sel(x, bld(bv, o + sj), bld(bv, o + sj - sk), bld(bv, o + sj - 2u * sk), k);
+  }
+  const float msf = A2(FL(msftx), ic, j), rdz0 = A1(FL(rdn), imin_(k, kde - 1)), rdz1 = A1(FL(rdn), imax_(k - 1, 1));
+  const float t0 = TEND ? 0.f : elem(tend, o);
+  float bu_t = 0.f, mt = 0.f;
+  float2 cc = make_float2(0.f, 0.f), tfl = make_float2(0.f, 0.f);
+  if (TEND) {
+    bu_t = buoy_w_term(x, ic, imax_(k, 2), j);
+    cc = cor_w_terms(x, ic, imin_(imax_(k, 2), kde - 1), j);
+    tfl = tendf_load(FL(rw_tendf), FL(w_save), o, rk);
+    mt = A2(FL(msfty), ic, j);
+  }
+  const float fw0 = hflux_sel(wx, uw, i, ide);
+  const float fw = (adv && i >= 2 && i <= ide - 1) ? fw0 : 0.f;
   const float fe = lane_next(fw, lane);
   if (!own) return;
-  float t = TEND ? 0.f : A3(tend, ic, k, j);
+  float fn;
+  if (wv < 3) fn = ys[wv + 1][lane];
+  else fn = (adv && j + 1 <= jde - 1) ? hflux_sel(wn, vn, j + 1, jde) : 0.f;
+  float t = t0;
   if (adv) {
-    if (j >= 2 && j <= jde - 2) t -= A2(msftx, ic, j) * x->rdy * (fn - fs);
-    if (ic >= 2 && ic <= ide - 2) t -= A2(msftx, ic, j) * x->rdx * (fe - fw);
-    if (k <= x->kde - 1) t -= A1(rdzu, k) * (fz_w(x, w, rom, ic, k + 1, j) - fz_w(x, w, rom, ic, k, j));
-    else t += 2.f * A1(rdzu, k - 1) * fz_w(x, w, rom, ic, k, j);
+    if (j >= 2 && j <= jde - 2) t -= msf * x->rdy * (fn - fs);
+    if (ic >= 2 && ic <= ide - 2) t -= msf * x->rdx * (fe - fw);
+    const float fzb = zflux_w_sel(wz[0], wz[1], wz[2], wz[3], r0 + rm, k, kde);
+    if (k <= kde - 1) t -= rdz0 * (zflux_w_sel(wz[1], wz[2], wz[3], wz[4], rp + r0, k + 1, kde) - fzb);
+    else t += 2.f * rdz1 * fzb;
     if (TEND) {
-      t = buoy_w(x, ic, k, j, t);
-      if (k <= x->kde - 1) t = cor_curv_w(x, ic, k, j, t);
+      t = t + bu_t;
+      if (k <= kde - 1) t = cor_w_add(t, cc);
     }
   }
-  if (TEND) t = addtend_w(x, ic, k, j, rk, t);
-  A3(tend, ic, k, j) = t;
+  if (TEND) t = t + tendf_fold(FL(rw_tendf), o, tfl, mt, rk) / mt;
+  elem(tend, o) = t;
 }
 
 // the phi tendency of the RK step over k [1, kde]: rhs_ph at k [2, kde] from

@@ -210,13 +210,6 @@ __global__ void k_rhs_ph(const DevCtx xv, R3 r) {
 
 // horizontal_pressure_gradient (module_big_step_utilities_em.py:1452-1671).
 // axis 0: u points i [2, ide-1], j [1, jde-1]; axis 1: v points i [1, ide-1], j [2, jde-1].
-__device__ __forceinline__ float hpg_dpn(const DevCtx *__restrict__ x, const float *p, int i, int j, int di, int dj, int k) {
-  if (k == 1)
-    return 0.5f * (x->cf1 * (A3(p, i - di, 1, j - dj) + A3(p, i, 1, j)) + x->cf2 * (A3(p, i - di, 2, j - dj) + A3(p, i, 2, j)) +
-                   x->cf3 * (A3(p, i - di, 3, j - dj) + A3(p, i, 3, j)));
-  if (k >= x->kde) return 0.f;
-  return 0.5f * (A1(FL(fnm), k) * (A3(p, i - di, k, j - dj) + A3(p, i, k, j)) + A1(FL(fnp), k) * (A3(p, i - di, k - 1, j - dj) + A3(p, i, k - 1, j)));
-}
 // the amount the pressure gradient subtracts from the u (axis 0) or v
 // (axis 1) tendency at (i,k,j): cq * dp
 __device__ __forceinline__ float hpg_term(const DevCtx *__restrict__ x, int axis, int i, int k, int j) {
@@ -230,9 +223,21 @@ __device__ __forceinline__ float hpg_term(const DevCtx *__restrict__ x, int axis
              ((A3(ph, i, k + 1, j) - A3(ph, i - di, k + 1, j - dj) + A3(ph, i, k, j) - A3(ph, i - di, k, j - dj)) +
               (A3(alt, i, k, j) + A3(alt, i - di, k, j - dj)) * (A3(p, i, k, j) - A3(p, i - di, k, j - dj)) +
               (A3(al, i, k, j) + A3(al, i - di, k, j - dj)) * (A3(pb, i, k, j) - A3(pb, i - di, k, j - dj)));
+  // level-interpolated pressure sums of both columns at w levels k+1 and k
+  // (none at and above kde; level 1 extrapolated from levels 1-3), from p at
+  // levels k, k+1 and k-1 (level 3 at k = 1), every form evaluated and the
+  // level's selected
+  const int kde = x->kde, kp = k + 1;
+  const BufF bp = bufv(p, (unsigned)x->n3);
+  const unsigned sk = (unsigned)x->ip, oa = XI3(i - di, k, j - dj), ob = XI3(i, k, j);
+  const unsigned ol = k == 1 ? 2u * sk : 0u - sk;
+  const float a0 = bld(bp, oa), a1 = bld(bp, oa + sk), a2 = bld(bp, oa + ol);
+  const float b0 = bld(bp, ob), b1 = bld(bp, ob + sk), b2 = bld(bp, ob + ol);
+  const float dpn1 = kp >= kde ? 0.f : 0.5f * (A1(FL(fnm), kp) * (a1 + b1) + A1(FL(fnp), kp) * (a0 + b0));
+  const float dpn0 = k == 1 ? 0.5f * (x->cf1 * (a0 + b0) + x->cf2 * (a1 + b1) + x->cf3 * (a2 + b2))
+                            : (k >= kde ? 0.f : 0.5f * (A1(FL(fnm), k) * (a0 + b0) + A1(FL(fnp), k) * (a2 + b2)));
   dp = dp + rr * rd * (A3(php, i, k, j) - A3(php, i - di, k, j - dj)) *
-                (A1(rdnw, k) * (hpg_dpn(x, p, i, j, di, dj, k + 1) - hpg_dpn(x, p, i, j, di, dj, k)) -
-                 0.5f * (A1(c1h, k) * A2(mu, i - di, j - dj) + A1(c1h, k) * A2(mu, i, j)));
+                (A1(rdnw, k) * (dpn1 - dpn0) - 0.5f * (A1(c1h, k) * A2(mu, i - di, j - dj) + A1(c1h, k) * A2(mu, i, j)));
   return (axis == 0 ? A3(FL(cqu), i, k, j) : A3(FL(cqv), i, k, j)) * dp;
 }
 __global__ void k_hpg(const DevCtx xv, int axis, R3 r) {
@@ -245,21 +250,19 @@ __global__ void k_hpg(const DevCtx xv, int axis, R3 r) {
 // pg_buoy_w (module_big_step_utilities_em.py:1674-1800) at one w level
 // k in [2, kde], added to rwt; reads cqw as it is before the column kernel
 // below replaces it by 1 / (1 + cqw)
-__device__ __forceinline__ float buoy_w(const DevCtx *__restrict__ x, int i, int k, int j, float rwt) {
+__device__ __forceinline__ float buoy_w_term(const DevCtx *__restrict__ x, int i, int k, int j) {
   const float *p = FL(p), *mu = FL(mu_2), *mub = FL(mub), *c1f = FL(c1f), *c2f = FL(c2f);
-  const int kde = x->kde;
+  const int kde = x->kde, kc = imin_(k, kde - 1);  // cqw and p level: k, kde-1 at the top
   const float gm = (1.f / A2(FL(msfty), i, j)) * x->g;
-  if (k == kde) {
-    const float cq = A3(FL(cqw), i, kde - 1, j);
-    const float cq1 = 1.f / (1.f + cq), cq2 = cq * cq1;
-    return rwt + gm * (cq1 * 2.f * A1(FL(rdnw), kde - 1) * (-A3(p, i, kde - 1, j)) - (A1(c1f, kde) * A2(mu, i, j)) -
-                       cq2 * (A1(c1f, kde) * A2(mub, i, j) + A1(c2f, kde)));
-  }
-  const float cq = A3(FL(cqw), i, k, j);
+  const float cq = A3(FL(cqw), i, kc, j);
   const float cq1 = 1.f / (1.f + cq), cq2 = cq * cq1;
-  return rwt + gm * (cq1 * A1(FL(rdn), k) * (A3(p, i, k, j) - A3(p, i, k - 1, j)) - (A1(c1f, k) * A2(mu, i, j)) -
-                     cq2 * (A1(c1f, k) * A2(mub, i, j) + A1(c2f, k)));
+  const float pk = A3(p, i, kc, j), pkm = A3(p, i, k - 1, j);
+  const float top = gm * (cq1 * 2.f * A1(FL(rdnw), kde - 1) * (-pk) - (A1(c1f, kde) * A2(mu, i, j)) -
+                          cq2 * (A1(c1f, kde) * A2(mub, i, j) + A1(c2f, kde)));
+  const float mid = gm * (cq1 * A1(FL(rdn), kc) * (pk - pkm) - (A1(c1f, k) * A2(mu, i, j)) - cq2 * (A1(c1f, k) * A2(mub, i, j) + A1(c2f, k)));
+  return k == kde ? top : mid;
 }
+__device__ __forceinline__ float buoy_w(const DevCtx *__restrict__ x, int i, int k, int j, float rwt) { return rwt + buoy_w_term(x, i, k, j); }
 // cqw := 1 / (1 + cqw) on levels [2, kde-1] (the second half of pg_buoy_w)
 __global__ void k_cqw_inv(const DevCtx xv, R3 r) {
   const DevCtx *__restrict__ x = &xv;
@@ -296,46 +299,66 @@ __device__ __forceinline__ float vxgm(const DevCtx *__restrict__ x, int i, int k
   return 0.5f * (A3(u, i, k, j) + A3(u, i + 1, k, j)) * (A2(FL(msfvx), i, j + 1) - A2(FL(msfvx), i, j)) * x->rdy -
          0.5f * (A3(v, i, k, j) + A3(v, i, k, j + 1)) * (A2(FL(msfuy), i + 1, j) - A2(FL(msfuy), i, j)) * x->rdx;
 }
-__device__ __forceinline__ float cor_curv_u(const DevCtx *__restrict__ x, int i, int k, int j, float val) {
+// The terms of each component as values of their own (a kernel can issue
+// their loads with its others) and their accumulation in the reference's
+// order.
+__device__ __forceinline__ float4 cor_u_terms(const DevCtx *__restrict__ x, int i, int k, int j) {
   const float *rv = FL(rv), *rw = FL(rw), *u = FL(u_2);
   const float *f = FL(f), *e = FL(e), *cosa = FL(cosa);
   const float rer = x->reradius;
-  val = val + (A2(FL(msfux), i, j) / A2(FL(msfuy), i, j)) * 0.5f * (A2(f, i, j) + A2(f, i - 1, j)) * 0.25f *
-                  (A3(rv, i - 1, k, j + 1) + A3(rv, i, k, j + 1) + A3(rv, i - 1, k, j) + A3(rv, i, k, j));
-  val = val - 0.5f * (A2(e, i, j) + A2(e, i - 1, j)) * 0.5f * (A2(cosa, i, j) + A2(cosa, i - 1, j)) * 0.25f *
-                  (A3(rw, i - 1, k + 1, j) + A3(rw, i - 1, k, j) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
-  val = val + 0.5f * (vxgm(x, i, k, j) + vxgm(x, i - 1, k, j)) * 0.25f *
-                  (A3(rv, i - 1, k, j + 1) + A3(rv, i, k, j + 1) + A3(rv, i - 1, k, j) + A3(rv, i, k, j)) -
-        A3(u, i, k, j) * rer * 0.25f * (A3(rw, i - 1, k + 1, j) + A3(rw, i - 1, k, j) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
-  return val;
+  float4 c;
+  c.x = (A2(FL(msfux), i, j) / A2(FL(msfuy), i, j)) * 0.5f * (A2(f, i, j) + A2(f, i - 1, j)) * 0.25f *
+        (A3(rv, i - 1, k, j + 1) + A3(rv, i, k, j + 1) + A3(rv, i - 1, k, j) + A3(rv, i, k, j));
+  c.y = 0.5f * (A2(e, i, j) + A2(e, i - 1, j)) * 0.5f * (A2(cosa, i, j) + A2(cosa, i - 1, j)) * 0.25f *
+        (A3(rw, i - 1, k + 1, j) + A3(rw, i - 1, k, j) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
+  c.z = 0.5f * (vxgm(x, i, k, j) + vxgm(x, i - 1, k, j)) * 0.25f * (A3(rv, i - 1, k, j + 1) + A3(rv, i, k, j + 1) + A3(rv, i - 1, k, j) + A3(rv, i, k, j));
+  c.w = A3(u, i, k, j) * rer * 0.25f * (A3(rw, i - 1, k + 1, j) + A3(rw, i - 1, k, j) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
+  return c;
 }
-__device__ __forceinline__ float cor_curv_v(const DevCtx *__restrict__ x, int i, int k, int j, float val) {
+__device__ __forceinline__ float cor_u_add(float val, float4 c) {
+  val = val + c.x;
+  val = val - c.y;
+  return val + c.z - c.w;
+}
+__device__ __forceinline__ float cor_curv_u(const DevCtx *__restrict__ x, int i, int k, int j, float val) { return cor_u_add(val, cor_u_terms(x, i, k, j)); }
+__device__ __forceinline__ float4 cor_v_terms(const DevCtx *__restrict__ x, int i, int k, int j) {
   const float *ru = FL(ru), *rw = FL(rw), *v = FL(v_2);
   const float *f = FL(f), *e = FL(e), *sina = FL(sina);
   const float rer = x->reradius;
   const float r = A2(FL(msfvy), i, j) / A2(FL(msfvx), i, j);
-  val = val - r * 0.5f * (A2(f, i, j) + A2(f, i, j - 1)) * 0.25f *
-                  (A3(ru, i, k, j) + A3(ru, i + 1, k, j) + A3(ru, i, k, j - 1) + A3(ru, i + 1, k, j - 1));
-  val = val + r * 0.5f * (A2(e, i, j) + A2(e, i, j - 1)) * 0.5f * (A2(sina, i, j) + A2(sina, i, j - 1)) * 0.25f *
-                  (A3(rw, i, k + 1, j - 1) + A3(rw, i, k, j - 1) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
-  val = val - 0.5f * (vxgm(x, i, k, j) + vxgm(x, i, k, j - 1)) * 0.25f *
-                  (A3(ru, i, k, j) + A3(ru, i + 1, k, j) + A3(ru, i, k, j - 1) + A3(ru, i + 1, k, j - 1)) -
-        r * A3(v, i, k, j) * rer * 0.25f * (A3(rw, i, k + 1, j - 1) + A3(rw, i, k, j - 1) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
-  return val;
+  float4 c;
+  c.x = r * 0.5f * (A2(f, i, j) + A2(f, i, j - 1)) * 0.25f * (A3(ru, i, k, j) + A3(ru, i + 1, k, j) + A3(ru, i, k, j - 1) + A3(ru, i + 1, k, j - 1));
+  c.y = r * 0.5f * (A2(e, i, j) + A2(e, i, j - 1)) * 0.5f * (A2(sina, i, j) + A2(sina, i, j - 1)) * 0.25f *
+        (A3(rw, i, k + 1, j - 1) + A3(rw, i, k, j - 1) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
+  c.z = 0.5f * (vxgm(x, i, k, j) + vxgm(x, i, k, j - 1)) * 0.25f * (A3(ru, i, k, j) + A3(ru, i + 1, k, j) + A3(ru, i, k, j - 1) + A3(ru, i + 1, k, j - 1));
+  c.w = r * A3(v, i, k, j) * rer * 0.25f * (A3(rw, i, k + 1, j - 1) + A3(rw, i, k, j - 1) + A3(rw, i, k + 1, j) + A3(rw, i, k, j));
+  return c;
 }
-__device__ __forceinline__ float cor_curv_w(const DevCtx *__restrict__ x, int i, int k, int j, float val) {
+__device__ __forceinline__ float cor_v_add(float val, float4 c) {
+  val = val - c.x;
+  val = val + c.y;
+  return val - c.z - c.w;
+}
+__device__ __forceinline__ float cor_curv_v(const DevCtx *__restrict__ x, int i, int k, int j, float val) { return cor_v_add(val, cor_v_terms(x, i, k, j)); }
+__device__ __forceinline__ float2 cor_w_terms(const DevCtx *__restrict__ x, int i, int k, int j) {
   const float *ru = FL(ru), *rv = FL(rv), *u = FL(u_2), *v = FL(v_2);
   const float *e = FL(e), *sina = FL(sina), *cosa = FL(cosa), *fzm = FL(fnm), *fzp = FL(fnp);
   const float rer = x->reradius;
   const float mr = A2(FL(msftx), i, j) / A2(FL(msfty), i, j);
-  val = val + A2(e, i, j) * (A2(cosa, i, j) * 0.5f * (A1(fzm, k) * (A3(ru, i, k, j) + A3(ru, i + 1, k, j)) + A1(fzp, k) * (A3(ru, i, k - 1, j) + A3(ru, i + 1, k - 1, j))) -
-                             mr * A2(sina, i, j) * 0.5f * (A1(fzm, k) * (A3(rv, i, k, j) + A3(rv, i, k, j + 1)) + A1(fzp, k) * (A3(rv, i, k - 1, j) + A3(rv, i, k - 1, j + 1))));
-  val = val + rer * (0.5f * (A1(fzm, k) * (A3(ru, i, k, j) + A3(ru, i + 1, k, j)) + A1(fzp, k) * (A3(ru, i, k - 1, j) + A3(ru, i + 1, k - 1, j))) *
-                         0.5f * (A1(fzm, k) * (A3(u, i, k, j) + A3(u, i + 1, k, j)) + A1(fzp, k) * (A3(u, i, k - 1, j) + A3(u, i + 1, k - 1, j))) +
-                     mr * 0.5f * (A1(fzm, k) * (A3(rv, i, k, j) + A3(rv, i, k, j + 1)) + A1(fzp, k) * (A3(rv, i, k - 1, j) + A3(rv, i, k - 1, j + 1))) *
-                         0.5f * (A1(fzm, k) * (A3(v, i, k, j) + A3(v, i, k, j + 1)) + A1(fzp, k) * (A3(v, i, k - 1, j) + A3(v, i, k - 1, j + 1))));
-  return val;
+  float2 c;
+  c.x = A2(e, i, j) * (A2(cosa, i, j) * 0.5f * (A1(fzm, k) * (A3(ru, i, k, j) + A3(ru, i + 1, k, j)) + A1(fzp, k) * (A3(ru, i, k - 1, j) + A3(ru, i + 1, k - 1, j))) -
+                       mr * A2(sina, i, j) * 0.5f * (A1(fzm, k) * (A3(rv, i, k, j) + A3(rv, i, k, j + 1)) + A1(fzp, k) * (A3(rv, i, k - 1, j) + A3(rv, i, k - 1, j + 1))));
+  c.y = rer * (0.5f * (A1(fzm, k) * (A3(ru, i, k, j) + A3(ru, i + 1, k, j)) + A1(fzp, k) * (A3(ru, i, k - 1, j) + A3(ru, i + 1, k - 1, j))) *
+                   0.5f * (A1(fzm, k) * (A3(u, i, k, j) + A3(u, i + 1, k, j)) + A1(fzp, k) * (A3(u, i, k - 1, j) + A3(u, i + 1, k - 1, j))) +
+               mr * 0.5f * (A1(fzm, k) * (A3(rv, i, k, j) + A3(rv, i, k, j + 1)) + A1(fzp, k) * (A3(rv, i, k - 1, j) + A3(rv, i, k - 1, j + 1))) *
+                   0.5f * (A1(fzm, k) * (A3(v, i, k, j) + A3(v, i, k, j + 1)) + A1(fzp, k) * (A3(v, i, k - 1, j) + A3(v, i, k - 1, j + 1))));
+  return c;
 }
+__device__ __forceinline__ float cor_w_add(float val, float2 c) {
+  val = val + c.x;
+  return val + c.y;
+}
+__device__ __forceinline__ float cor_curv_w(const DevCtx *__restrict__ x, int i, int k, int j, float val) { return cor_w_add(val, cor_w_terms(x, i, k, j)); }
 __global__ void k_cor_curv(const DevCtx xv, int comp, R3 r) {
   const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
@@ -348,37 +371,51 @@ __global__ void k_cor_curv(const DevCtx xv, int comp, R3 r) {
 // physics / boundary tendency (scaled by the map factor) added to the RK
 // tendency t, which is returned; at the first stage the saved tendency is
 // folded into the physics tendency first (written back).
+// In two parts, so a kernel can issue the loads with its others:
+// tendf_load reads the physics tendency and, at the first stage, the saved
+// one; tendf_fold adds the saved tendency times its weight at the first
+// stage (written back; only the thread owning the point may call it) and
+// returns the physics tendency.
+__device__ __forceinline__ float2 tendf_load(const float *tf, const float *sv, unsigned o, int rk) {
+  return make_float2(elem(tf, o), rk == 1 ? elem(sv, o) : 0.f);
+}
+__device__ __forceinline__ float tendf_fold(float *tf, unsigned o, float2 ld, float w, int rk) {
+  float v = ld.x;
+  if (rk == 1) {
+    v = ld.x + ld.y * w;
+    elem(tf, o) = v;
+  }
+  return v;
+}
 __device__ __forceinline__ float addtend_u(const DevCtx *__restrict__ x, int i, int k, int j, int rk, float t) {
   const unsigned o = XI3(i, k, j);
   const float m = A2(FL(msfuy), i, j);
-  if (rk == 1) elem(FL(ru_tendf), o) = elem(FL(ru_tendf), o) + elem(FL(u_save), o) * m;
-  return t + elem(FL(ru_tendf), o) / m;
+  return t + tendf_fold(FL(ru_tendf), o, tendf_load(FL(ru_tendf), FL(u_save), o, rk), m, rk) / m;
 }
 __device__ __forceinline__ float addtend_v(const DevCtx *__restrict__ x, int i, int k, int j, int rk, float t) {
   const unsigned o = XI3(i, k, j);
-  if (rk == 1) elem(FL(rv_tendf), o) = elem(FL(rv_tendf), o) + elem(FL(v_save), o) * A2(FL(msfvx), i, j);
-  return t + elem(FL(rv_tendf), o) * A2(FL(msfvx_inv), i, j);
+  return t + tendf_fold(FL(rv_tendf), o, tendf_load(FL(rv_tendf), FL(v_save), o, rk), A2(FL(msfvx), i, j), rk) * A2(FL(msfvx_inv), i, j);
 }
 __device__ __forceinline__ float addtend_w(const DevCtx *__restrict__ x, int i, int k, int j, int rk, float t) {
   const unsigned o = XI3(i, k, j);
   const float m = A2(FL(msfty), i, j);
-  if (rk == 1) elem(FL(rw_tendf), o) = elem(FL(rw_tendf), o) + elem(FL(w_save), o) * m;
-  return t + elem(FL(rw_tendf), o) / m;
+  return t + tendf_fold(FL(rw_tendf), o, tendf_load(FL(rw_tendf), FL(w_save), o, rk), m, rk) / m;
 }
+// (the saved phi and theta tendencies carry no weight: x * 1 is x)
 __device__ __forceinline__ float addtend_ph(const DevCtx *__restrict__ x, int i, int k, int j, int rk, float t) {
   const unsigned o = XI3(i, k, j);
   const float m = A2(FL(msfty), i, j);
-  if (rk == 1) elem(FL(ph_tendf), o) = elem(FL(ph_tendf), o) + elem(FL(ph_save), o);
-  return t + elem(FL(ph_tendf), o) / m;
+  return t + tendf_fold(FL(ph_tendf), o, tendf_load(FL(ph_tendf), FL(ph_save), o, rk), 1.f, rk) / m;
 }
+// theta: also the diabatic heating, and the column's mu tendency at k = 1
 __device__ __forceinline__ float addtend_t(const DevCtx *__restrict__ x, int i, int k, int j, int rk, float t) {
   const unsigned o = XI3(i, k, j);
   const float m = A2(FL(msfty), i, j);
   const float mass = A1(FL(c1h), k) * A2(FL(mut), i, j) + A1(FL(c2h), k);
-  if (rk == 1) elem(FL(t_tendf), o) = elem(FL(t_tendf), o) + elem(FL(t_save), o);
+  const float tf = tendf_fold(FL(t_tendf), o, tendf_load(FL(t_tendf), FL(t_save), o, rk), 1.f, rk);
   const float heating = mass * elem(FL(h_diabatic), o) / m;
   if (k == 1) A2(FL(mu_tend), i, j) = A2(FL(mu_tend), i, j) + A2(FL(mu_tendf), i, j);
-  return t + elem(FL(t_tendf), o) / m + heating;
+  return t + tf / m + heating;
 }
 // comp 0 u, 1 v, 2 w/ph, 3 theta
 __global__ void k_addtend(const DevCtx xv, int comp, int rk, R3 r) {

@@ -31,70 +31,6 @@ struct SpecList {
   int n;
 };
 
-// scalar face fluxes with the face velocity given (same arithmetic as
-// fy_s / fx_s / fz_s of k_advect.h)
-__device__ __forceinline__ float sfy(const DevCtx *__restrict__ x, const float *q, float vel, int i, int k, int j) {
-  const int jde = x->jde;
-  if (j >= 4 && j <= jde - 3)
-    return vel * flux5(A3(q, i, k, j - 3), A3(q, i, k, j - 2), A3(q, i, k, j - 1), A3(q, i, k, j), A3(q, i, k, j + 1), A3(q, i, k, j + 2), vel);
-  if (j == 2 || j == jde - 1) return 0.5f * vel * (A3(q, i, k, j) + A3(q, i, k, j - 1));
-  return vel * flux3(A3(q, i, k, j - 2), A3(q, i, k, j - 1), A3(q, i, k, j), A3(q, i, k, j + 1), vel);
-}
-__device__ __forceinline__ float sfx(const DevCtx *__restrict__ x, const float *q, float vel, int i, int k, int j) {
-  const int ide = x->ide;
-  if (i >= 4 && i <= ide - 3)
-    return vel * flux5(A3(q, i - 3, k, j), A3(q, i - 2, k, j), A3(q, i - 1, k, j), A3(q, i, k, j), A3(q, i + 1, k, j), A3(q, i + 2, k, j), vel);
-  if (i == 2 || i == ide - 1) return 0.5f * vel * (A3(q, i, k, j) + A3(q, i - 1, k, j));
-  return vel * flux3(A3(q, i - 2, k, j), A3(q, i - 1, k, j), A3(q, i, k, j), A3(q, i + 1, k, j), vel);
-}
-__device__ __forceinline__ float sfz(const DevCtx *__restrict__ x, const float *q, float vel, int i, int k, int j) {
-  const int ktf = x->kde - 1;
-  if (k <= 1 || k >= x->kde) return 0.f;
-  if (k == 2 || k == ktf) return vel * (A1(FL(fnm), k) * A3(q, i, k, j) + A1(FL(fnp), k) * A3(q, i, k - 1, j));
-  return vel * flux3(A3(q, i, k - 2, j), A3(q, i, k - 1, j), A3(q, i, k, j), A3(q, i, k + 1, j), -vel);
-}
-
-// The face fluxes above from stencil values already in registers, so a
-// kernel can issue all loads of a point before any branch on the flux order.
-// win6: the 6 values around face f along stride st (slots f-3 .. f+2), each
-// position clamped into [lo, hi] (valid addresses; the slots a lower-order
-// flux does not use are ignored).
-__device__ __forceinline__ void win6(const float *q, unsigned of, int f, int lo, int hi, int st, float w[6]) {
-#pragma unroll
-  for (int n = 0; n < 6; n++) {
-    const int pos = imin_(imax_(f - 3 + n, lo), hi);
-    w[n] = elem(q, of + (unsigned)((pos - f) * st));
-  }
-}
-// the 6 values around the face at element offset of along stride st (slots
-// -3 .. +2), unclamped buffer loads
-__device__ __forceinline__ void win6b(const BufF &b, unsigned of, unsigned st, float w[6]) {
-#pragma unroll
-  for (int n = 0; n < 6; n++) w[n] = bld(b, of + (unsigned)(n - 3) * st);
-}
-// horizontal face flux (sfx / sfy) at face f of an axis whose end index is
-// fe, every order evaluated and the face's own selected (no branch between
-// the window loads and their use)
-__device__ __forceinline__ float hflux_sel(const float w[6], float vel, int f, int fe) {
-  const float h5 = vel * flux5(w[0], w[1], w[2], w[3], w[4], w[5], vel);
-  const float h2 = 0.5f * vel * (w[3] + w[2]);
-  const float h3 = vel * flux3(w[1], w[2], w[3], w[4], vel);
-  return (f >= 4 && f <= fe - 3) ? h5 : ((f == 2 || f == fe - 1) ? h2 : h3);
-}
-// horizontal face flux (sfx / sfy) at face f of an axis whose end index is fe
-__device__ __forceinline__ float hflux_w(const float w[6], float vel, int f, int fe) {
-  if (f >= 4 && f <= fe - 3) return vel * flux5(w[0], w[1], w[2], w[3], w[4], w[5], vel);
-  if (f == 2 || f == fe - 1) return 0.5f * vel * (w[3] + w[2]);
-  return vel * flux3(w[1], w[2], w[3], w[4], vel);
-}
-// vertical face flux (sfz) at level kk from q(kk-2), q(kk-1), q(kk), q(kk+1)
-__device__ __forceinline__ float vflux_w(const DevCtx *__restrict__ x, float qm2, float qm1, float q0, float qp1, float vel, int kk) {
-  const int ktf = x->kde - 1;
-  if (kk <= 1 || kk >= x->kde) return 0.f;
-  if (kk == 2 || kk == ktf) return vel * (A1(FL(fnm), kk) * q0 + A1(FL(fnp), kk) * qm1);
-  return vel * flux3(qm2, qm1, q0, qp1, -vel);
-}
-
 // mass-weighted RK update of one point (rk_update_scalar)
 struct UpdCoef {
   float oldm, newm, msfty;
@@ -149,6 +85,8 @@ __global__ void __launch_bounds__(256) k_species_update(const DevCtx xv, SpecLis
   const float msftx = A2(FL(msftx), ic, j), rdzw = A1(FL(rdnw), k);
   const UpdCoef c = upd_coef(x, ic, k, j);
   const unsigned o = XI3(ic, k, j), ox = XI3(i, k, j);
+  const int k1 = imin_(k + 1, kde);
+  const float fm0 = A1(FL(fnm), k), fp0 = A1(FL(fnp), k), fm1 = A1(FL(fnm), k1), fp1 = A1(FL(fnp), k1);
   for (int s = 0; s < L.n; s++) {
     const BufF bq = bufv(L.in[s], nb);
     // every load of this species first: x stencil of the west face, the
@@ -156,8 +94,7 @@ __global__ void __launch_bounds__(256) k_species_update(const DevCtx xv, SpecLis
     // value; the top wave also the stencil of its north face
     float xw[6], yw[6], zw[5];
     win6b(bq, ox, 1u, xw);
-#pragma unroll
-    for (int n = 0; n < 5; n++) zw[n] = bld(bq, o + (unsigned)(n - 2) * sk);
+    win5z(bq, o, sk, zw);
     const float tnd = elem(L.tend[s], o);
     const float old = elem(rk == 1 ? L.in[s] : L.old[s], o);
     if (wv == 3) win6b(bq, o + sj, sj, yw);
@@ -171,7 +108,7 @@ __global__ void __launch_bounds__(256) k_species_update(const DevCtx xv, SpecLis
     float adv = 0.f;
     if (doy) adv -= msftx * x->rdy * (fn - fs);
     if (dox) adv -= msftx * x->rdx * (fe - fw);
-    adv -= rdzw * (vflux_w(x, zw[1], zw[2], zw[3], zw[4], wt, k + 1) - vflux_w(x, zw[0], zw[1], zw[2], zw[3], wb, k));
+    adv -= rdzw * (zflux_sel(zw[1], zw[2], zw[3], zw[4], wt, k + 1, kde, fm1, fp1) - zflux_sel(zw[0], zw[1], zw[2], zw[3], wb, k, kde, fm0, fp0));
     float tend = c.inner ? adv * c.msfty : 0.f;
     tend = tend + tnd;
     const float out = (c.oldm * old + dt * tend) / c.newm;
@@ -310,8 +247,7 @@ __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecL
     float xw[6], yw[6], zw[5], zo[3];
     win6b(bq, ox, 1u, xw);
     const float xom = bld(bo, ox - 1u), xo0 = bld(bo, ox);
-#pragma unroll
-    for (int n = 0; n < 5; n++) zw[n] = bld(bq, o + (unsigned)(n - 2) * sk);
+    win5z(bq, o, sk, zw);
 #pragma unroll
     for (int n = 0; n < 3; n++) zo[n] = bld(bo, o + (unsigned)(n - 1) * sk);
     float yon = 0.f;
