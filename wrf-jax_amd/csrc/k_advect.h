@@ -87,6 +87,28 @@ __device__ __forceinline__ void win5z(const BufF &b, unsigned o, unsigned sk, fl
 #pragma unroll
   for (int n = 0; n < 5; n++) w[n] = bld(b, o + (unsigned)(n - 2) * sk);
 }
+// Holds loaded values in registers at this point (an empty asm statement
+// that reads them): their loads complete here, so the compiler cannot sink
+// them into the flux-order branch that uses them.  Placed after the last
+// load of a group (the statement also keeps later loads after it).
+__device__ __forceinline__ void pin6(const float w[6]) { asm volatile("" : : "v"(w[0]), "v"(w[1]), "v"(w[2]), "v"(w[3]), "v"(w[4]), "v"(w[5])); }
+__device__ __forceinline__ void pin5(const float w[5]) { asm volatile("" : : "v"(w[0]), "v"(w[1]), "v"(w[2]), "v"(w[3]), "v"(w[4])); }
+// Two ways to pick a face's flux order once its stencil is in registers:
+// *_sel evaluates every order and selects (no branch: kernels with many
+// loads in flight keep their registers), *_br branches on the order (only
+// the face's own order is evaluated: the species and scalar kernels, whose
+// loads are pinned first).  The order of a face in j or k is uniform over a
+// wave; in i it differs only in the waves at the domain edges.
+__device__ __forceinline__ float hflux_br(const float w[6], float vel, int f, int fe) {
+  if (f >= 4 && f <= fe - 3) return vel * flux5(w[0], w[1], w[2], w[3], w[4], w[5], vel);
+  if (f == 2 || f == fe - 1) return 0.5f * vel * (w[3] + w[2]);
+  return vel * flux3(w[1], w[2], w[3], w[4], vel);
+}
+__device__ __forceinline__ float zflux_br(float qm2, float qm1, float q0, float qp1, float vel, int kk, int kde, float fm, float fp) {
+  if (kk <= 1 || kk >= kde) return 0.f;
+  if (kk == 2 || kk == kde - 1) return vel * (fm * q0 + fp * qm1);
+  return vel * flux3(qm2, qm1, q0, qp1, -vel);
+}
 // scalar and w faces along x / y (fx_s, fy_s, fx_w, fy_w): face velocity
 // vel; 5th order on [4, fe-3], centred at 2 and fe-1, 3rd order between
 __device__ __forceinline__ float hflux_sel(const float w[6], float vel, int f, int fe) {
@@ -179,7 +201,9 @@ __global__ void __launch_bounds__(256) k_advect_scalar(const DevCtx xv, const fl
   {
     float w[6];
     win6b(bq, o, sj, w);
-    const float f = hflux_sel(w, bld(bv, o), j, jde);
+    const float vs = bld(bv, o);
+    pin6(w);
+    const float f = hflux_br(w, vs, j, jde);
     fs = (j >= 2 && j <= jde - 1) ? f : 0.f;
   }
   ys[wv][lane] = fs;
@@ -202,17 +226,20 @@ __global__ void __launch_bounds__(256) k_advect_scalar(const DevCtx xv, const fl
   const float fm0 = A1(FL(fnm), k), fp0 = A1(FL(fnp), k), fm1 = A1(FL(fnm), k1), fp1 = A1(FL(fnp), k1);
   const float msf = A2(FL(msftx), ic, j), rdz = A1(FL(rdnw), k);
   const float t0 = TEND ? 0.f : elem(tend, o);
-  const float fw0 = hflux_sel(wx, uw, i, ide);
+  pin6(wx);
+  pin5(wz);
+  if (wv == 3) pin6(wn);
+  const float fw0 = hflux_br(wx, uw, i, ide);
   const float fw = (i >= 2 && i <= ide - 1) ? fw0 : 0.f;
   const float fe = lane_next(fw, lane);
   if (!own) return;
   float fn;
   if (wv < 3) fn = ys[wv + 1][lane];
-  else fn = (j + 1 <= jde - 1) ? hflux_sel(wn, vn, j + 1, jde) : 0.f;
+  else fn = (j + 1 <= jde - 1) ? hflux_br(wn, vn, j + 1, jde) : 0.f;
   float t = t0;
   if (j >= 2 && j <= jde - 2) t -= msf * x->rdy * (fn - fs);
   if (ic >= 2 && ic <= ide - 2) t -= msf * x->rdx * (fe - fw);
-  t -= rdz * (zflux_sel(wz[1], wz[2], wz[3], wz[4], wt, k + 1, kde, fm1, fp1) - zflux_sel(wz[0], wz[1], wz[2], wz[3], wb, k, kde, fm0, fp0));
+  t -= rdz * (zflux_br(wz[1], wz[2], wz[3], wz[4], wt, k + 1, kde, fm1, fp1) - zflux_br(wz[0], wz[1], wz[2], wz[3], wb, k, kde, fm0, fp0));
   if (TEND) t = addtend_t(x, ic, k, j, rk, t);
   elem(tend, o) = t;
 }

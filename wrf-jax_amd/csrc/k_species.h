@@ -71,7 +71,8 @@ __global__ void __launch_bounds__(256) k_species_update(const DevCtx xv, SpecLis
     for (int s = 0; s < L.n; s++) {
       float w[6];
       win6b(bufv(L.in[s], nb), os, sj, w);
-      YS(s, wv) = yf ? hflux_sel(w, vs, j, jde) : 0.f;
+      pin6(w);
+      YS(s, wv) = yf ? hflux_br(w, vs, j, jde) : 0.f;
     }
   }
   __syncthreads();
@@ -97,18 +98,23 @@ __global__ void __launch_bounds__(256) k_species_update(const DevCtx xv, SpecLis
     win5z(bq, o, sk, zw);
     const float tnd = elem(L.tend[s], o);
     const float old = elem(rk == 1 ? L.in[s] : L.old[s], o);
-    if (wv == 3) win6b(bq, o + sj, sj, yw);
+    if (wv == 3) {
+      win6b(bq, o + sj, sj, yw);
+      pin6(yw);
+    }
+    pin6(xw);
+    pin5(zw);
     // fluxes
-    const float fw = xface ? hflux_sel(xw, uw, i, ide) : 0.f;
+    const float fw = xface ? hflux_br(xw, uw, i, ide) : 0.f;
     const float fe = lane_next(fw, lane);
     const float fs = YS(s, wv);
     float fn;
     if (wv < 3) fn = YS(s, wv + 1);
-    else fn = yface_n ? hflux_sel(yw, vn, j + 1, jde) : 0.f;
+    else fn = yface_n ? hflux_br(yw, vn, j + 1, jde) : 0.f;
     float adv = 0.f;
     if (doy) adv -= msftx * x->rdy * (fn - fs);
     if (dox) adv -= msftx * x->rdx * (fe - fw);
-    adv -= rdzw * (zflux_sel(zw[1], zw[2], zw[3], zw[4], wt, k + 1, kde, fm1, fp1) - zflux_sel(zw[0], zw[1], zw[2], zw[3], wb, k, kde, fm0, fp0));
+    adv -= rdzw * (zflux_br(zw[1], zw[2], zw[3], zw[4], wt, k + 1, kde, fm1, fp1) - zflux_br(zw[0], zw[1], zw[2], zw[3], wb, k, kde, fm0, fp0));
     float tend = c.inner ? adv * c.msfty : 0.f;
     tend = tend + tnd;
     const float out = (c.oldm * old + dt * tend) / c.newm;
@@ -172,18 +178,21 @@ __device__ __forceinline__ PdFace pd_face_geom(const DevCtx *__restrict__ x, int
   return f;
 }
 // Antidiffusive and upwind flux of one species through a face from values
-// already in registers (all loads of a species are issued first; every
-// order is evaluated and the face's own selected, so no branch separates the
-// loads from their use).  Horizontal: w = q at slots -3 .. +2 of the face,
-// om1 / o0 = q_old on either side.
-__device__ __forceinline__ void pd_hflux_sel(const PdFace &f, const float w[6], float om1, float o0, float &anti, float &low) {
-  const float lo = f.muf * flux_up(om1, o0, f.cr);
-  const float h5 = f.vel * flux5pd(w[0], w[1], w[2], w[3], w[4], w[5], f.vel);
-  const float h2 = 0.5f * f.vel * (w[3] + w[2]);
-  const float h3 = f.vel * flux3pd(w[1], w[2], w[3], w[4], f.vel);
-  const float hi = f.mode == 5 ? h5 : (f.mode == 2 ? h2 : h3);
-  anti = f.mode == 0 ? 0.f : hi - lo;
-  low = f.mode == 0 ? 0.f : lo;
+// already in registers (every load of a species is issued and pinned first,
+// then the order branches).  Horizontal: w = q at slots -3 .. +2 of the
+// face, om1 / o0 = q_old on either side.
+__device__ __forceinline__ void pd_hflux_br(const PdFace &f, const float w[6], float om1, float o0, float &anti, float &low) {
+  if (f.mode == 0) {
+    anti = 0.f;
+    low = 0.f;
+    return;
+  }
+  low = f.muf * flux_up(om1, o0, f.cr);
+  float high;
+  if (f.mode == 5) high = f.vel * flux5pd(w[0], w[1], w[2], w[3], w[4], w[5], f.vel);
+  else if (f.mode == 2) high = 0.5f * f.vel * (w[3] + w[2]);
+  else high = f.vel * flux3pd(w[1], w[2], w[3], w[4], f.vel);
+  anti = high - low;
 }
 // vertical face at level kk: q(kk-2 .. kk+1), q_old(kk-1), q_old(kk) and the
 // centred weights fnm(kk), fnp(kk)
@@ -219,7 +228,8 @@ __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecL
       float w[6], a, l;
       win6b(bufv(L.in[s], nb), os, sj, w);
       const float om1 = bld(bo, os - sj), o0 = bld(bo, os);
-      pd_hflux_sel(fs, w, om1, o0, a, l);
+      pin6(w);
+      pd_hflux_br(fs, w, om1, o0, a, l);
       YA(s, wv) = a;
       YL(s, wv) = l;
     }
@@ -254,9 +264,12 @@ __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecL
     if (wv == 3) {
       win6b(bq, o + sj, sj, yw);
       yon = bld(bo, o + sj);
+      pin6(yw);
     }
+    pin6(xw);
+    pin5(zw);
     float ax0, lx0, ay1, ly1, az0, lz0, az1, lz1;
-    pd_hflux_sel(fx, xw, xom, xo0, ax0, lx0);
+    pd_hflux_br(fx, xw, xom, xo0, ax0, lx0);
     const float ax1 = lane_next(ax0, lane), lx1 = lane_next(lx0, lane);
     if (!own) continue;
     if (!lim) {
@@ -268,7 +281,7 @@ __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecL
       ay1 = YA(s, wv + 1);
       ly1 = YL(s, wv + 1);
     } else {
-      pd_hflux_sel(fyn, yw, zo[1], yon, ay1, ly1);
+      pd_hflux_br(fyn, yw, zo[1], yon, ay1, ly1);
     }
     pd_vflux_sel(fz0, zw[0], zw[1], zw[2], zw[3], zo[0], zo[1], fm0, fp0, az0, lz0);
     pd_vflux_sel(fz1, zw[1], zw[2], zw[3], zw[4], zo[1], zo[2], fm1, fp1, az1, lz1);
