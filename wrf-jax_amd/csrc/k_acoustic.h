@@ -109,16 +109,21 @@ __global__ void k_zero_grad_w(const DevCtx xv) {
 }
 
 // ---------------------------------------------------------------------------
-// waves per block of k_acoustic_col and levels per register group (loads of
-// one group are issued together)
+// waves per block of k_acoustic_col, and levels per register group of each
+// phase (the loads of one group are issued together).  The group sizes trade
+// memory round trips against registers: phases with few loads per level take
+// larger groups (phase 3: 2 loads per level, all 8 levels of a wave at once;
+// phases 5, 6, 10: 4 levels), phases 1, 4 and 9 stay at 2 so the kernel keeps
+// 6 waves per SIMD (at most 80 VGPRs, the LDS limit being 3 blocks per CU).
 #ifndef WRFDC_COL_WAVES
 #define WRFDC_COL_WAVES 8
 #endif
 #ifndef WRFDC_COL_GROUP
 #define WRFDC_COL_GROUP 2
 #endif
+constexpr int kColG1 = WRFDC_COL_GROUP, kColG3 = 4 * WRFDC_COL_GROUP, kColG4 = WRFDC_COL_GROUP, kColG5 = 2 * WRFDC_COL_GROUP,
+              kColG6 = 2 * WRFDC_COL_GROUP, kColG9 = WRFDC_COL_GROUP, kColG10 = 2 * WRFDC_COL_GROUP;
 constexpr int kColWaves = WRFDC_COL_WAVES;
-constexpr int kColGroup = WRFDC_COL_GROUP;
 
 constexpr int kColMaxLev = 152;  // levels the column kernel holds (nz <= 150)
 // bytes of dynamic LDS of k_acoustic_col for kde levels
@@ -132,7 +137,8 @@ inline int acoustic_col_npw(int kde) {
 template <int NPW>
 __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv, float dts, int first, int fin, float ns) {
   const DevCtx *__restrict__ x = &xv;
-  constexpr int NW = kColWaves, G = kColGroup;
+  constexpr int NW = kColWaves;
+  constexpr int G1 = kColG1, G3 = kColG3, G4 = kColG4, G5 = kColG5, G6 = kColG6, G9 = kColG9, G10 = kColG10;
   extern __shared__ float sm[];
   __shared__ float c_mts[64], c_mav[64];
   // dnw and c1h in LDS for the serial column sum and omega recurrence (their
@@ -174,10 +180,10 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
     const float vx0 = A2(FL(msfvx_inv), i, j), vx1 = A2(FL(msfvx_inv), i, j + 1);
     const float mm = msftx * msfty;
 #pragma unroll
-    for (int gq = 0; gq < NPW; gq += G) {
-      float r_ue[G], r_uw[G], r_use[G], r_usw[G], r_vn[G], r_vs[G], r_vsn[G], r_vss[G], r_rum[G], r_rvm[G];
+    for (int gq = 0; gq < NPW; gq += G1) {
+      float r_ue[G1], r_uw[G1], r_use[G1], r_usw[G1], r_vn[G1], r_vs[G1], r_vsn[G1], r_vss[G1], r_rum[G1], r_rvm[G1];
 #pragma unroll
-      for (int q = 0; q < G; q++) {
+      for (int q = 0; q < G1; q++) {
         const int k = imin_(LEV(gq + q), kend);
         r_rum[q] = first ? 0.f : A3(ru_m, i, k, j);
         r_rvm[q] = first ? 0.f : A3(rv_m, i, k, j);
@@ -191,7 +197,7 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
         r_vss[q] = A3(vs, i, k, j);
       }
 #pragma unroll
-      for (int q = 0; q < G; q++) {
+      for (int q = 0; q < G1; q++) {
         const int k = LEV(gq + q);
         if (k > kend) break;
         const float c1 = A1(c1h, k), c2 = A1(c2h, k);
@@ -251,16 +257,16 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
   if (interior) {
     const float *ww_1 = FL(ww_1);
 #pragma unroll
-    for (int gq = 0; gq < NPW; gq += G) {
-      float r[G], r_wwm[G];
+    for (int gq = 0; gq < NPW; gq += G3) {
+      float r[G3], r_wwm[G3];
 #pragma unroll
-      for (int q = 0; q < G; q++) {
+      for (int q = 0; q < G3; q++) {
         const int k = imin_(LEV(gq + q), kend);
         r[q] = A3(ww_1, i, k, j);
         r_wwm[q] = first ? 0.f : A3(ww_m, i, k, j);
       }
 #pragma unroll
-      for (int q = 0; q < G; q++) {
+      for (int q = 0; q < G3; q++) {
         const int k = LEV(gq + q);
         if (k > kend) break;
         const float vv = LS(B1, k) - r[q];
@@ -280,19 +286,17 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
     }
   }
   __syncthreads();
-  // ---- 4. theta (advance_mu_t) and its acoustic average (advance_w) -> B2;
-  // the stage-start and new theta of each level stay in registers for
-  // calc_p_rho (phase 10 handles the same levels in the same wave)
-  float c_t1[NPW], c_t2[NPW];
+  // ---- 4. theta (advance_mu_t) and its acoustic average (advance_w) -> B2
+  // (phase 10 reads the new theta back: the same thread wrote it)
   if (interior) {
     const float mts = c_mts[lane], mav = c_mav[lane];
     const float *ft = FL(t_tend);
     float *t = FL(t_2), *tav_out = FL(t_2ave);
 #pragma unroll
-    for (int gq = 0; gq < NPW; gq += G) {
-      float r_t1[G], r_t1m[G], r_t1p[G], r_t1e[G], r_t1w[G], r_t1n[G], r_t1s[G], r_ue[G], r_uw[G], r_vn[G], r_vs[G], r_t[G], r_ft[G];
+    for (int gq = 0; gq < NPW; gq += G4) {
+      float r_t1[G4], r_t1m[G4], r_t1p[G4], r_t1e[G4], r_t1w[G4], r_t1n[G4], r_t1s[G4], r_ue[G4], r_uw[G4], r_vn[G4], r_vs[G4], r_t[G4], r_ft[G4];
 #pragma unroll
-      for (int q = 0; q < G; q++) {
+      for (int q = 0; q < G4; q++) {
         const int k = imin_(LEV(gq + q), kend);
         r_t1[q] = A3(t_1, i, k, j);
         r_t1m[q] = A3(t_1, i, imax_(k - 1, 1), j);
@@ -309,7 +313,7 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
         r_ft[q] = A3(ft, i, k, j);
       }
 #pragma unroll
-      for (int q = 0; q < G; q++) {
+      for (int q = 0; q < G4; q++) {
         const int k = LEV(gq + q);
         if (k > kend) break;
         const float wlo = (k >= 2) ? LS(B1, k) * (A1(fnm, k) * r_t1[q] + A1(fnp, k) * r_t1m[q]) : 0.f;
@@ -326,8 +330,6 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
         const float vert = A1(rdnw, k) * (whi - wlo);
         const float tnew = ta - (dts * msfty) * (horiz + vert);
         A3(t, i, k, j) = tnew;
-        c_t1[gq + q] = r_t1[q];
-        c_t2[gq + q] = tnew;
         float tav = 0.5f * (ope * tnew + ome * tk);
         tav = (tav + (A1(c1h, k) * mav) * t0) / ((A1(c1h, k) * mts + A1(c2h, k)) * (t0 + r_t1[q]));
         A3(tav_out, i, k, j) = tav;
@@ -347,10 +349,10 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
   if (interior) {
     const float *ph_1 = FL(ph_save), *phb = FL(phb), *ph_tend = FL(ph_tend), *w = FL(w_2);
 #pragma unroll
-    for (int gq = 0; gq < NPW; gq += G) {
-      float r_pt[G], r_w[G], r_p1[G], r_p1m[G], r_p1p[G], r_pb[G], r_pbm[G], r_pbp[G], r_ph[G];
+    for (int gq = 0; gq < NPW; gq += G5) {
+      float r_pt[G5], r_w[G5], r_p1[G5], r_p1m[G5], r_p1p[G5], r_pb[G5], r_pbm[G5], r_pbp[G5], r_ph[G5];
 #pragma unroll
-      for (int q = 0; q < G; q++) {
+      for (int q = 0; q < G5; q++) {
         const int k = imax_(imin_(LEV(gq + q), kde), 2);
         r_pt[q] = A3(ph_tend, i, k, j);
         r_w[q] = A3(w, i, k, j);
@@ -363,7 +365,7 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
         r_ph[q] = A3(ph, i, k, j);
       }
 #pragma unroll
-      for (int q = 0; q < G; q++) {
+      for (int q = 0; q < G5; q++) {
         const int k = LEV(gq + q);
         if (k > kde) break;
         if (k == 1) {
@@ -387,10 +389,10 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
     const float *w = FL(w_2), *rw_tend = FL(rw_tend), *c2a = FL(c2a), *cqw = FL(cqw), *alt = FL(alt);
     const float mav = c_mav[lane], msft_inv = 1.f / msfty;
 #pragma unroll
-    for (int gq = 0; gq < NPW; gq += G) {
-      float r_c2a[G], r_c2am[G], r_alt[G], r_altm[G], r_ph[G], r_phm[G], r_php[G], r_w[G], r_rwt[G], r_cqw[G];
+    for (int gq = 0; gq < NPW; gq += G6) {
+      float r_c2a[G6], r_c2am[G6], r_alt[G6], r_altm[G6], r_ph[G6], r_phm[G6], r_php[G6], r_w[G6], r_rwt[G6], r_cqw[G6];
 #pragma unroll
-      for (int q = 0; q < G; q++) {
+      for (int q = 0; q < G6; q++) {
         const int k = imax_(imin_(LEV(gq + q), kde), 2);
         r_c2a[q] = A3(c2a, i, imin_(k, kend), j);
         r_c2am[q] = A3(c2a, i, k - 1, j);
@@ -404,7 +406,7 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
         r_cqw[q] = A3(cqw, i, k, j);
       }
 #pragma unroll
-      for (int q = 0; q < G; q++) {
+      for (int q = 0; q < G6; q++) {
         const int k = LEV(gq + q);
         if (k > kde) break;
         float wl;
@@ -524,17 +526,17 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
     const float mu_old = mts - dts * mtend;
     const float *ph_tend = FL(ph_tend), *ph_save = FL(ph_save);
 #pragma unroll
-    for (int gq = 0; gq < NPW; gq += G) {
-      float r_ph[G], r_pt[G], r_ps[G];
+    for (int gq = 0; gq < NPW; gq += G9) {
+      float r_ph[G9], r_pt[G9], r_ps[G9];
 #pragma unroll
-      for (int q = 0; q < G; q++) {
+      for (int q = 0; q < G9; q++) {
         const int k = imin_(LEV(gq + q), kde);
         r_ph[q] = A3(php2, i, k, j);
         r_pt[q] = A3(ph_tend, i, k, j);
         r_ps[q] = A3(ph_save, i, k, j);
       }
 #pragma unroll
-      for (int q = 0; q < G; q++) {
+      for (int q = 0; q < G9; q++) {
         const int k = LEV(gq + q);
         if (k > kde) break;
         const float den = A1(c1f, k) * mts + A1(c2f, k);
@@ -558,10 +560,10 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
     const float cmu_u = upt ? A2(FL(muu), i, j) : 0.f, cmsf_u = upt ? A2(FL(msfuy), i, j) : 1.f;
     const float cmu_v = vpt ? A2(FL(muv), i, j) : 0.f, cmsf_v = vpt ? A2(FL(msfvx_inv), i, j) : 1.f;
 #pragma unroll
-    for (int gq = 0; gq < NPW; gq += G) {
-      float r_rum[G], r_rvm[G], r_wwm[G], r_u[G], r_v[G], r_ww[G], r_us[G], r_vs[G], r_ww1[G];
+    for (int gq = 0; gq < NPW; gq += G9) {
+      float r_rum[G9], r_rvm[G9], r_wwm[G9], r_u[G9], r_v[G9], r_ww[G9], r_us[G9], r_vs[G9], r_ww1[G9];
 #pragma unroll
-      for (int q = 0; q < G; q++) {
+      for (int q = 0; q < G9; q++) {
         const int k = imin_(LEV(gq + q), kde);
         const size_t o = XI3(i, k, j);
         r_rum[q] = first ? 0.f : ru_m[o];
@@ -575,7 +577,7 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
         r_ww1[q] = (fin && mass) ? FL(ww_1)[o] : 0.f;
       }
 #pragma unroll
-      for (int q = 0; q < G; q++) {
+      for (int q = 0; q < G9; q++) {
         const int k = LEV(gq + q);
         if (k > kde) break;
         const size_t o = XI3(i, k, j);
@@ -611,19 +613,19 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
     const float *alt = FL(alt), *t_2 = FL(t_2), *c2a = FL(c2a);
     float *al = FL(al), *p = FL(p), *pm1 = FL(pm1);
 #pragma unroll
-    for (int gq = 0; gq < NPW; gq += G) {
-      float r_alt[G], r_t1[G], r_t2[G], r_c2a[G], r_pm1[G];
+    for (int gq = 0; gq < NPW; gq += G10) {
+      float r_alt[G10], r_t1[G10], r_t2[G10], r_c2a[G10], r_pm1[G10];
 #pragma unroll
-      for (int q = 0; q < G; q++) {
+      for (int q = 0; q < G10; q++) {
         const size_t o = XI3(i, imin_(LEV(gq + q), kend), j);
         r_alt[q] = alt[o];
-        r_t1[q] = interior ? c_t1[gq + q] : t_1[o];
-        r_t2[q] = interior ? c_t2[gq + q] : t_2[o];
+        r_t1[q] = t_1[o];
+        r_t2[q] = t_2[o];
         r_c2a[q] = c2a[o];
         r_pm1[q] = pm1[o];
       }
 #pragma unroll
-      for (int q = 0; q < G; q++) {
+      for (int q = 0; q < G10; q++) {
         const int k = LEV(gq + q);
         if (k > kend) break;
         const size_t o = XI3(i, k, j);

@@ -49,39 +49,71 @@ __global__ void k_rk_prep3(const DevCtx xv, R3 r) {
   const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const int ide = x->ide, jde = x->jde, kde = x->kde;
-  const float *u = FL(u_2), *v = FL(v_2), *w = FL(w_2), *muu = FL(muu), *muv = FL(muv), *mut = FL(mut);
-  const float *c1h = FL(c1h), *c2h = FL(c2h), *c1f = FL(c1f), *c2f = FL(c2f);
-  const size_t o = XI3(i, k, j);
+  const unsigned o = XI3(i, k, j), o2 = XI2(i, j);
+  const unsigned ok1 = k < kde ? o + (unsigned)x->ip : o;  // level k+1 (k at the top, not used there)
+  const int kh = imin_(k, kde - 1);
+  const float c1h = A1(FL(c1h), kh), c2h = A1(FL(c2h), kh), c1f = A1(FL(c1f), k), c2f = A1(FL(c2f), k);
+  // every load of the point first (the moist species follow in groups)
+  const float u = elem(FL(u_2), o), v = elem(FL(v_2), o), w = elem(FL(w_2), o);
+  const float muu = elem(FL(muu), o2), muv = elem(FL(muv), o2), mut = elem(FL(mut), o2);
+  const float msfuy = elem(FL(msfuy), o2), msfvxi = elem(FL(msfvx_inv), o2), msfty = elem(FL(msfty), o2);
+  const float al = elem(FL(al), o), alb = elem(FL(alb), o);
+  const float phb0 = elem(FL(phb), o), phb1 = elem(FL(phb), ok1), ph0 = elem(FL(ph_2), o), ph1 = elem(FL(ph_2), ok1);
+  pin1(u);
+  pin1(v);
+  pin1(w);
+  pin1(al);
+  pin1(alb);
+  pin1(phb0);
+  pin1(phb1);
+  pin1(ph0);
+  pin1(ph1);
   // ru / rv / rw
-  FL(ru)[o] = (j <= jde - 1 && k <= kde - 1) ? A3(u, i, k, j) * (A1(c1h, k) * A2(muu, i, j) + A1(c2h, k)) / A2(FL(msfuy), i, j) : 0.f;
-  FL(rv)[o] = (i <= ide - 1 && k <= kde - 1) ? A3(v, i, k, j) * (A1(c1h, k) * A2(muv, i, j) + A1(c2h, k)) * A2(FL(msfvx_inv), i, j) : 0.f;
-  FL(rw)[o] = (i <= ide - 1 && j <= jde - 1) ? A3(w, i, k, j) * (A1(c1f, k) * A2(mut, i, j) + A1(c2f, k)) / A2(FL(msfty), i, j) : 0.f;
+  elem(FL(ru), o) = (j <= jde - 1 && k <= kde - 1) ? u * (c1h * muu + c2h) / msfuy : 0.f;
+  elem(FL(rv), o) = (i <= ide - 1 && k <= kde - 1) ? v * (c1h * muv + c2h) * msfvxi : 0.f;
+  elem(FL(rw), o) = (i <= ide - 1 && j <= jde - 1) ? w * (c1f * mut + c2f) / msfty : 0.f;
   // alt / php on mass points
   const bool mass = (i <= ide - 1 && j <= jde - 1 && k <= kde - 1);
-  FL(alt)[o] = mass ? A3(FL(al), i, k, j) + A3(FL(alb), i, k, j) : 0.f;
-  if (i == ide || j == jde) FL(ww)[o] = 0.f;
-  FL(php)[o] = mass ? 0.5f * (A3(FL(phb), i, k, j) + A3(FL(phb), i, k + 1, j) + A3(FL(ph_2), i, k, j) + A3(FL(ph_2), i, k + 1, j)) : 0.f;
+  elem(FL(alt), o) = mass ? al + alb : 0.f;
+  if (i == ide || j == jde) elem(FL(ww), o) = 0.f;
+  elem(FL(php), o) = mass ? 0.5f * (phb0 + phb1 + ph0 + ph1) : 0.f;
   // cqu / cqv / cqw; neighbour i-1 / j-1 at the low edge wraps to the last
   // stored index like the reference's negative indexing
   float cu = 0.f, cv = 0.f, cw = 0.f;
   if (k <= kde - 1) {
     const int im = (i == 1) ? ide : i - 1;
     const int jm = (j == 1) ? jde : j - 1;
+    const unsigned om = o, oi = XI3(im, k, j), oj = XI3(i, k, jm), ok = XI3(i, imax_(k - 1, 1), j);
     float su = 0.f, sv = 0.f, sw = 0.f;
-    for (int s = 0; s < x->nmoist; s++) {
-      const float *q = SP(moist, s);
-      float q0 = A3(q, i, k, j);
-      su = su + q0 + A3(q, im, k, j);
-      sv = sv + q0 + A3(q, i, k, jm);
-      if (k > 1) sw = sw + q0 + A3(q, i, k - 1, j);
+    // species in groups of 4, every load of a group issued first (species
+    // past the last load the last one again); sums in species order
+    const int nm = x->nmoist;
+    for (int s0 = 0; s0 < nm; s0 += 4) {
+      float q0[4], qi[4], qj[4], qk[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const float *qp = SP(moist, imin_(s0 + q, nm - 1));
+        q0[q] = elem(qp, om);
+        qi[q] = elem(qp, oi);
+        qj[q] = elem(qp, oj);
+        qk[q] = elem(qp, ok);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        if (s0 + q < nm) {
+          su = su + q0[q] + qi[q];
+          sv = sv + q0[q] + qj[q];
+          if (k > 1) sw = sw + q0[q] + qk[q];
+        }
+      }
     }
     if (j < jde) cu = 1.f / (1.f + 0.5f * su);
     if (i < ide) cv = 1.f / (1.f + 0.5f * sv);
     if (i < ide && j < jde && k > 1) cw = 0.5f * sw;
   }
-  FL(cqu)[o] = cu;
-  FL(cqv)[o] = cv;
-  FL(cqw)[o] = cw;
+  elem(FL(cqu), o) = cu;
+  elem(FL(cqv), o) = cv;
+  elem(FL(cqw), o) = cw;
 }
 
 // calc_ww_cp (module_big_step_utilities_em.py:655-803): column kernel over

@@ -55,49 +55,68 @@ __global__ void k_ssp_mass(const DevCtx xv, int rk) {
 // small_step_prep, 3-D part (module_small_step_em.py:420-717): save the
 // stage-start state, form the acoustic perturbation variables and c2a.
 // Full extent i [1, ide], k [1, kde], j [1, jde].
-__global__ void k_ssp_3d(const DevCtx xv, int rk, R3 r) {
+// Every field of the point is loaded first (all addresses are points of the
+// full-extent fields), then the stores of the staggerings the point belongs
+// to; RK1: the first RK stage (stage-start copies, perturbations zero).
+template <bool RK1>
+__global__ void k_ssp_3d(const DevCtx xv, R3 r) {
   const DevCtx *__restrict__ x = &xv;
   GET_IKJ_P(r);
   const int ide = x->ide, jde = x->jde, kde = x->kde;
-  const size_t o = XI3(i, k, j);
-  const float *c1h = FL(c1h), *c2h = FL(c2h), *c1f = FL(c1f), *c2f = FL(c2f);
-  if (k <= kde - 1 && i <= ide - 1 && j <= jde - 1)
-    FL(c2a)[o] = (x->cpovcv * (FL(pb)[o] + FL(p)[o])) / FL(alt)[o];
-  if (j <= jde - 1 && k <= kde - 1) {  // u points
-    const float old = FL(u_2)[o];
-    if (rk == 1) FL(u_1)[o] = old;
-    FL(u_save)[o] = old;
-    FL(u_2)[o] = (rk == 1) ? 0.f
-                           : ((A1(c1h, k) * A2(FL(muus), i, j) + A1(c2h, k)) * FL(u_1)[o] - (A1(c1h, k) * A2(FL(muu), i, j) + A1(c2h, k)) * old) /
-                                 A2(FL(msfuy), i, j);
+  const unsigned o = XI3(i, k, j), o2 = XI2(i, j);
+  const bool up = (j <= jde - 1 && k <= kde - 1), vp = (i <= ide - 1 && k <= kde - 1), mp = (i <= ide - 1 && j <= jde - 1);
+  const bool tp = mp && k <= kde - 1;
+  const int kh = imin_(k, kde - 1);  // in-range index of the half-level coefficients
+  const float c1h = A1(FL(c1h), kh), c2h = A1(FL(c2h), kh), c1f = A1(FL(c1f), k), c2f = A1(FL(c2f), k);
+  const float pb = elem(FL(pb), o), pp = elem(FL(p), o), alt = elem(FL(alt), o);
+  const float u2 = elem(FL(u_2), o), v2 = elem(FL(v_2), o), t2 = elem(FL(t_2), o), w2 = elem(FL(w_2), o), ph2 = elem(FL(ph_2), o);
+  const float wwv = elem(FL(ww), o);
+  float u1 = u2, v1 = v2, t1 = t2, w1 = w2, ph1 = ph2;
+  float muus = 0.f, muu = 0.f, msfuy = 1.f, muvs = 0.f, muv = 0.f, msfvxi = 1.f, mts = 0.f, mt = 0.f, msfty = 1.f;
+  if (!RK1) {
+    u1 = elem(FL(u_1), o);
+    v1 = elem(FL(v_1), o);
+    t1 = elem(FL(t_1), o);
+    w1 = elem(FL(w_1), o);
+    ph1 = elem(FL(ph_1), o);
+    muus = elem(FL(muus), o2);
+    muu = elem(FL(muu), o2);
+    msfuy = elem(FL(msfuy), o2);
+    muvs = elem(FL(muvs), o2);
+    muv = elem(FL(muv), o2);
+    msfvxi = elem(FL(msfvx_inv), o2);
+    mts = elem(FL(muts), o2);
+    mt = elem(FL(mut), o2);
+    msfty = elem(FL(msfty), o2);
   }
-  if (i <= ide - 1 && k <= kde - 1) {  // v points
-    const float old = FL(v_2)[o];
-    if (rk == 1) FL(v_1)[o] = old;
-    FL(v_save)[o] = old;
-    FL(v_2)[o] = (rk == 1) ? 0.f
-                           : ((A1(c1h, k) * A2(FL(muvs), i, j) + A1(c2h, k)) * FL(v_1)[o] - (A1(c1h, k) * A2(FL(muv), i, j) + A1(c2h, k)) * old) *
-                                 A2(FL(msfvx_inv), i, j);
+  pin1(pb);
+  pin1(pp);
+  pin1(alt);
+  if (tp) elem(FL(c2a), o) = (x->cpovcv * (pb + pp)) / alt;
+  if (up) {  // u points
+    if (RK1) elem(FL(u_1), o) = u2;
+    elem(FL(u_save), o) = u2;
+    elem(FL(u_2), o) = RK1 ? 0.f : ((c1h * muus + c2h) * u1 - (c1h * muu + c2h) * u2) / msfuy;
   }
-  if (i <= ide - 1 && j <= jde - 1) {
-    const float mts = A2(FL(muts), i, j), mt = A2(FL(mut), i, j);
+  if (vp) {  // v points
+    if (RK1) elem(FL(v_1), o) = v2;
+    elem(FL(v_save), o) = v2;
+    elem(FL(v_2), o) = RK1 ? 0.f : ((c1h * muvs + c2h) * v1 - (c1h * muv + c2h) * v2) * msfvxi;
+  }
+  if (mp) {
     if (k <= kde - 1) {  // theta
-      const float old = FL(t_2)[o];
-      if (rk == 1) FL(t_1)[o] = old;
-      FL(t_save)[o] = old;
-      FL(t_2)[o] = (rk == 1) ? 0.f : (A1(c1h, k) * mts + A1(c2h, k)) * FL(t_1)[o] - (A1(c1h, k) * mt + A1(c2h, k)) * old;
+      if (RK1) elem(FL(t_1), o) = t2;
+      elem(FL(t_save), o) = t2;
+      elem(FL(t_2), o) = RK1 ? 0.f : (c1h * mts + c2h) * t1 - (c1h * mt + c2h) * t2;
     }
     // w / ph / ww levels
-    const float wold = FL(w_2)[o];
-    if (rk == 1) FL(w_1)[o] = wold;
-    FL(w_save)[o] = wold;
-    FL(w_2)[o] = (rk == 1) ? 0.f
-                           : ((A1(c1f, k) * mts + A1(c2f, k)) * FL(w_1)[o] - (A1(c1f, k) * mt + A1(c2f, k)) * wold) / A2(FL(msfty), i, j);
-    const float pold = FL(ph_2)[o];
-    if (rk == 1) FL(ph_1)[o] = pold;
-    FL(ph_save)[o] = pold;
-    FL(ph_2)[o] = FL(ph_1)[o] - pold;
-    FL(ww_1)[o] = FL(ww)[o];
+    if (RK1) elem(FL(w_1), o) = w2;
+    elem(FL(w_save), o) = w2;
+    elem(FL(w_2), o) = RK1 ? 0.f : ((c1f * mts + c2f) * w1 - (c1f * mt + c2f) * w2) / msfty;
+    if (RK1) elem(FL(ph_1), o) = ph2;
+    elem(FL(ph_save), o) = ph2;
+    elem(FL(ph_2), o) = ph1 - ph2;
+    elem(FL(ww_1), o) = wwv;
   }
 }
 
@@ -390,33 +409,43 @@ __global__ void k_sumflux(const DevCtx xv, int first, int fin, float ns, R3 r) {
 
 // small_step_finish (module_small_step_em.py:1119-1332): recouple the
 // acoustic perturbations onto the stage-start state.  Full extent.
-__global__ void k_ssf(const DevCtx xv, int final_step, float dts, float nsm, R3 r) {
+// (loads first, as k_ssp_3d; FINAL: the last RK stage, which also removes
+// the diabatic heating)
+template <bool FINAL>
+__global__ void k_ssf(const DevCtx xv, float dts, float nsm, R3 r) {
   const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const int ide = x->ide, jde = x->jde, kde = x->kde;
-  const size_t o = XI3(i, k, j);
-  const float *c1h = FL(c1h), *c2h = FL(c2h), *c1f = FL(c1f), *c2f = FL(c2f);
+  const unsigned o = XI3(i, k, j), o2 = XI2(i, j);
+  const int kh = imin_(k, kde - 1);
+  const float c1h = A1(FL(c1h), kh), c2h = A1(FL(c2h), kh), c1f = A1(FL(c1f), k), c2f = A1(FL(c2f), k);
+  const float v2 = elem(FL(v_2), o), vs = elem(FL(v_save), o), u2 = elem(FL(u_2), o), us = elem(FL(u_save), o);
+  const float w2 = elem(FL(w_2), o), ws = elem(FL(w_save), o), ph2 = elem(FL(ph_2), o), phs = elem(FL(ph_save), o);
+  const float wwv = elem(FL(ww), o), ww1 = elem(FL(ww_1), o), t2 = elem(FL(t_2), o), ts = elem(FL(t_save), o);
+  const float hd = FINAL ? elem(FL(h_diabatic), o) : 0.f;
+  const float muv = elem(FL(muv), o2), muvs = elem(FL(muvs), o2), msfvx = elem(FL(msfvx), o2);
+  const float muu = elem(FL(muu), o2), muus = elem(FL(muus), o2), msfuy = elem(FL(msfuy), o2);
+  const float mt = elem(FL(mut), o2), mts = elem(FL(muts), o2), msfty = elem(FL(msfty), o2);
   if (i <= ide - 1 && k <= kde - 1) {
-    const float old = A1(c1h, k) * A2(FL(muv), i, j) + A1(c2h, k), nw = A1(c1h, k) * A2(FL(muvs), i, j) + A1(c2h, k);
-    FL(v_2)[o] = (A2(FL(msfvx), i, j) * FL(v_2)[o] + FL(v_save)[o] * old) / nw;
+    const float old = c1h * muv + c2h, nw = c1h * muvs + c2h;
+    elem(FL(v_2), o) = (msfvx * v2 + vs * old) / nw;
   }
   if (j <= jde - 1 && k <= kde - 1) {
-    const float old = A1(c1h, k) * A2(FL(muu), i, j) + A1(c2h, k), nw = A1(c1h, k) * A2(FL(muus), i, j) + A1(c2h, k);
-    FL(u_2)[o] = (A2(FL(msfuy), i, j) * FL(u_2)[o] + FL(u_save)[o] * old) / nw;
+    const float old = c1h * muu + c2h, nw = c1h * muus + c2h;
+    elem(FL(u_2), o) = (msfuy * u2 + us * old) / nw;
   }
   if (i <= ide - 1 && j <= jde - 1) {
-    const float mt = A2(FL(mut), i, j), mts = A2(FL(muts), i, j);
     {
-      const float old = A1(c1f, k) * mt + A1(c2f, k), nw = A1(c1f, k) * mts + A1(c2f, k);
-      FL(w_2)[o] = (A2(FL(msfty), i, j) * FL(w_2)[o] + FL(w_save)[o] * old) / nw;
-      FL(ph_2)[o] = FL(ph_2)[o] + FL(ph_save)[o];
-      FL(ww)[o] = FL(ww)[o] + FL(ww_1)[o];
+      const float old = c1f * mt + c2f, nw = c1f * mts + c2f;
+      elem(FL(w_2), o) = (msfty * w2 + ws * old) / nw;
+      elem(FL(ph_2), o) = ph2 + phs;
+      elem(FL(ww), o) = wwv + ww1;
     }
     if (k <= kde - 1) {
-      const float old = A1(c1h, k) * mt + A1(c2h, k), nw = A1(c1h, k) * mts + A1(c2h, k);
-      float tt = FL(t_2)[o];
-      if (final_step) tt = tt - ((dts * nsm) * old) * FL(h_diabatic)[o];
-      FL(t_2)[o] = (tt + FL(t_save)[o] * old) / nw;
+      const float old = c1h * mt + c2h, nw = c1h * mts + c2h;
+      float tt = t2;
+      if (FINAL) tt = tt - ((dts * nsm) * old) * hd;
+      elem(FL(t_2), o) = (tt + ts * old) / nw;
     }
     if (k == 1) A2(FL(mu_2), i, j) = A2(FL(mu_2), i, j) + A2(FL(mu_save), i, j);
   }

@@ -508,7 +508,8 @@ struct Run {
   void small_step_prep() const {
     k_ssp_mass<<<dim3(cdiv(ide, 64), jde), 64, 0, s>>>(d, X->rk);
     R3 r = full3();
-    k_ssp_3d<<<g3p(r), b3(), 0, s>>>(d, X->rk, r);
+    if (X->rk == 1) k_ssp_3d<true><<<g3p(r), b3(), 0, s>>>(d, r);
+    else k_ssp_3d<false><<<g3p(r), b3(), 0, s>>>(d, r);
   }
   void calc_p_rho(int step) const {
     R3 m = mass3();
@@ -558,7 +559,8 @@ struct Run {
   void small_step_finish() const {
     k_mu_uv<<<dim3(cdiv(ide, 64), jde), 64, 0, s>>>(d, F(F_muts), nullptr, F(F_muus), F(F_muvs), nullptr);
     R3 r = full3();
-    k_ssf<<<g3(r), b3(), 0, s>>>(d, X->rk == 3, (float)X->dts_rk, (float)X->nsmall, r);
+    if (X->rk == 3) k_ssf<true><<<g3(r), b3(), 0, s>>>(d, (float)X->dts_rk, (float)X->nsmall, r);
+    else k_ssf<false><<<g3(r), b3(), 0, s>>>(d, (float)X->dts_rk, (float)X->nsmall, r);
   }
   void calc_p_rho_phi() const {
     R3 m = mass3();

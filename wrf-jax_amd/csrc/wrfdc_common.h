@@ -101,6 +101,12 @@ __device__ __forceinline__ BufF bufv(const float *p, unsigned n) {
 __device__ __forceinline__ float bld(const BufF &b, unsigned e) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, (int)(e << 2), 0, 0));
 }
+// Holds a loaded value in a register at this point (an empty asm statement
+// that reads it): its load completes here, so the compiler cannot sink the
+// load into a branch that uses the value later.  Placed after the last load
+// of a group, it costs one wait for the whole group (and keeps later loads
+// after it).
+__device__ __forceinline__ void pin1(float v) { asm volatile("" : : "v"(v)); }
 #define A3(fl, i, k, j) elem(fl, XI3(i, k, j))
 #define A2(fl, i, j) elem(fl, XI2(i, j))
 #define A1(fl, k) ((fl)[(k) - 1])
