@@ -65,8 +65,7 @@ __host__ __device__ inline FaceTiles face_tiles(const R3 &r) {
   if (k > (r).k1) return; /* whole block */                                              \
   const bool jrow = (j <= (r).j1);                                                       \
   const bool own = (lane < 63 && i <= (r).i1 && jrow);                                   \
-  const int ic = i <= (r).i1 ? i : (r).i1;                                               \
-  const int jc = jrow ? j : (r).j1
+  const int ic = i <= (r).i1 ? i : (r).i1
 
 // ---------------- face fluxes from loaded stencils ---------------------------
 // Every kernel below issues all loads of a point first (buffer loads,
@@ -477,10 +476,14 @@ __global__ void __launch_bounds__(256) k_advect_w(const DevCtx xv, const float *
 __global__ void __launch_bounds__(256) k_tend_ph(const DevCtx xv, R3 r, int rk) {
   const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
+  const unsigned o = XI3(i, k, j);
+  // rk_addtend_dry's loads go out first (addtend_ph in two parts)
+  const float2 tfl = tendf_load(FL(ph_tendf), FL(ph_save), o, rk);
+  const float m = A2(FL(msfty), i, j);
   float t = 0.f;
   if (k >= 2) t = rhs_ph_val(x, i, k, j, t);
-  t = addtend_ph(x, i, k, j, rk, t);
-  A3(FL(ph_tend), i, k, j) = t;
+  t = t + tendf_fold(FL(ph_tendf), o, tfl, 1.f, rk) / m;
+  elem(FL(ph_tend), o) = t;
 }
 // ---------------- positive-definite scalar advection ------------------------
 // pass 1: high-order minus upwind (antidiffusive) and upwind fluxes

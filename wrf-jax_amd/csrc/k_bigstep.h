@@ -157,20 +157,77 @@ __global__ void k_ww_cp(const DevCtx xv, R2 r) {
 // the phi tendency at (i,k,j), k in [2, kde], starting from `out`
 __device__ __forceinline__ float rhs_ph_val(const DevCtx *__restrict__ x, int i, int k, int j, float out) {
   const int ide = x->ide, jde = x->jde, kde = x->kde, kte = kde;
-  const float *u = FL(u_2), *v = FL(v_2), *ww = FL(ww), *ph = FL(ph_2), *phb = FL(phb), *w = FL(w_2);
-  const float *mut = FL(mut), *muu = FL(muu), *muv = FL(muv), *c1f = FL(c1f), *c2f = FL(c2f);
-  const float *fnm = FL(fnm), *fnp = FL(fnp), *rdnw = FL(rdnw), *msfux = FL(msfux), *msfvy = FL(msfvy), *msfty = FL(msfty);
+  const float *c1f = FL(c1f), *c2f = FL(c2f), *fnm = FL(fnm), *fnp = FL(fnp), *rdnw = FL(rdnw);
   const float rdx = x->rdx, rdy = x->rdy, g = x->g, cfn = x->cfn, cfn1 = x->cfn1;
+  const unsigned nb = (unsigned)x->n3, o = XI3(i, k, j), o2 = XI2(i, j), sk = (unsigned)x->ip, sj = (unsigned)(kde * x->ip);
+  // every load of the point first (buffer loads: the levels k-2 .. k+1 and
+  // the stencil windows may fall outside the field where they are not used)
+  const BufF bu = bufv(FL(u_2), nb), bv = bufv(FL(v_2), nb), bww = bufv(FL(ww), nb), bph = bufv(FL(ph_2), nb), bpb = bufv(FL(phb), nb);
+  const float wwm = bld(bww, o - sk), ww0 = bld(bww, o), wwp = bld(bww, o + sk);
+  const float phm = bld(bph, o - sk), php1 = bld(bph, o + sk), pbm = bld(bpb, o - sk), pbp1 = bld(bpb, o + sk);
+  float phy[7], pby[7], phx[7], pbx[7];  // ph, phb at j-3 .. j+3 and i-3 .. i+3 (slot 3: the point)
+#pragma unroll
+  for (int n = 0; n < 7; n++) {
+    phy[n] = bld(bph, o + (unsigned)(n - 3) * sj);
+    pby[n] = bld(bpb, o + (unsigned)(n - 3) * sj);
+    phx[n] = bld(bph, o + (unsigned)(n - 3));
+    pbx[n] = bld(bpb, o + (unsigned)(n - 3));
+  }
+  float vn[3], vs[3], ue[3], uw[3];  // v at rows j+1, j and u at columns i+1, i; levels k, k-1, k-2
+#pragma unroll
+  for (int n = 0; n < 3; n++) {
+    vn[n] = bld(bv, o + sj - (unsigned)n * sk);
+    vs[n] = bld(bv, o - (unsigned)n * sk);
+    ue[n] = bld(bu, o + 1u - (unsigned)n * sk);
+    uw[n] = bld(bu, o - (unsigned)n * sk);
+  }
+  const float wk = elem(FL(w_2), o), mut = elem(FL(mut), o2), msfty = elem(FL(msfty), o2);
+  // column coefficients (the top level's rdnw index clamped: not used there)
+  const float rdnw0 = A1(rdnw, k - 1), rdnw1 = A1(rdnw, imin_(k, kde - 1)), fnm0 = A1(fnm, k), fnp0 = A1(fnp, k);
+  const float c1fk = A1(c1f, k), c2fk = A1(c2f, k);
+  const float muv1 = A2(FL(muv), i, j + 1), muv0 = elem(FL(muv), o2), msfvy1 = A2(FL(msfvy), i, j + 1), msfvy0 = elem(FL(msfvy), o2);
+  const float muu1 = A2(FL(muu), i + 1, j), muu0 = elem(FL(muu), o2), msfux1 = A2(FL(msfux), i + 1, j), msfux0 = elem(FL(msfux), o2);
+  pin1(wk);
+  pin1(mut);
+  pin1(msfty);
+  pin1(muv1);
+  pin1(muv0);
+  pin1(msfvy1);
+  pin1(msfvy0);
+  pin1(muu1);
+  pin1(muu0);
+  pin1(msfux1);
+  pin1(msfux0);
+  pin1(wwm);
+  pin1(ww0);
+  pin1(wwp);
+  pin1(phm);
+  pin1(php1);
+  pin1(pbm);
+  pin1(pbp1);
+#pragma unroll
+  for (int n = 0; n < 7; n++) {
+    pin1(phy[n]);
+    pin1(pby[n]);
+    pin1(phx[n]);
+    pin1(pbx[n]);
+  }
+#pragma unroll
+  for (int n = 0; n < 3; n++) {
+    pin1(vn[n]);
+    pin1(vs[n]);
+    pin1(ue[n]);
+    pin1(uw[n]);
+  }
+  const float ph0 = phy[3], pb0 = pby[3];
   if (k <= kte - 1) {
-    float wdk = 0.5f * (A3(ww, i, k, j) + A3(ww, i, k - 1, j)) * A1(rdnw, k - 1) *
-                (A3(ph, i, k, j) - A3(ph, i, k - 1, j) + A3(phb, i, k, j) - A3(phb, i, k - 1, j));
-    float wdk1 = 0.5f * (A3(ww, i, k + 1, j) + A3(ww, i, k, j)) * A1(rdnw, k) *
-                 (A3(ph, i, k + 1, j) - A3(ph, i, k, j) + A3(phb, i, k + 1, j) - A3(phb, i, k, j));
-    out += -(A1(fnm, k) * wdk1 + A1(fnp, k) * wdk);
+    float wdk = 0.5f * (ww0 + wwm) * rdnw0 * (ph0 - phm + pb0 - pbm);
+    float wdk1 = 0.5f * (wwp + ww0) * rdnw1 * (php1 - ph0 + pbp1 - pb0);
+    out += -(fnm0 * wdk1 + fnp0 * wdk);
   } else {
     out = 0.f;
   }
-  out += (A1(c1f, k) * A2(mut, i, j) + A1(c2f, k)) * g * A3(w, i, k, j) / A2(msfty, i, j);
+  out += (c1fk * mut + c2fk) * g * wk / msfty;
   // y direction
   {
     int mode = 0;  // 0 none, 2 second, 4 fourth, 6 sixth
@@ -180,26 +237,24 @@ __device__ __forceinline__ float rhs_ph_val(const DevCtx *__restrict__ x, int i,
     if (mode) {
       float va1, va0, fac;
       if (k < kte) {
-        va1 = (A1(c1f, k) * A2(muv, i, j + 1) + A1(c2f, k)) * (A3(v, i, k, j + 1) + A3(v, i, k - 1, j + 1)) * A2(msfvy, i, j + 1);
-        va0 = (A1(c1f, k) * A2(muv, i, j) + A1(c2f, k)) * (A3(v, i, k, j) + A3(v, i, k - 1, j)) * A2(msfvy, i, j);
+        va1 = (c1fk * muv1 + c2fk) * (vn[0] + vn[1]) * msfvy1;
+        va0 = (c1fk * muv0 + c2fk) * (vs[0] + vs[1]) * msfvy0;
         fac = 0.25f;
       } else {
-        va1 = (A1(c1f, k) * A2(muv, i, j + 1) + A1(c2f, k)) * (cfn * A3(v, i, k - 1, j + 1) + cfn1 * A3(v, i, k - 2, j + 1)) * A2(msfvy, i, j + 1);
-        va0 = (A1(c1f, k) * A2(muv, i, j) + A1(c2f, k)) * (cfn * A3(v, i, k - 1, j) + cfn1 * A3(v, i, k - 2, j)) * A2(msfvy, i, j);
+        va1 = (c1fk * muv1 + c2fk) * (cfn * vn[1] + cfn1 * vn[2]) * msfvy1;
+        va0 = (c1fk * muv0 + c2fk) * (cfn * vs[1] + cfn1 * vs[2]) * msfvy0;
         fac = 0.5f;
       }
       if (mode == 2) {
-        float term = va1 * (A3(phb, i, k, j + 1) - A3(phb, i, k, j) + A3(ph, i, k, j + 1) - A3(ph, i, k, j)) +
-                     va0 * (A3(phb, i, k, j) - A3(phb, i, k, j - 1) + A3(ph, i, k, j) - A3(ph, i, k, j - 1));
-        out += -(fac * rdy / A2(msfty, i, j)) * term;
+        float term = va1 * (pby[4] - pb0 + phy[4] - ph0) + va0 * (pb0 - pby[2] + ph0 - phy[2]);
+        out += -(fac * rdy / msfty) * term;
       } else {
         float ca = mode == 6 ? 45.f : 8.f, cb = mode == 6 ? -9.f : -1.f, cc = mode == 6 ? 1.f : 0.f;
         float scale = mode == 6 ? (1.f / 60.f) : (1.f / 12.f);
-        int mo = mode == 6 ? 3 : 2;
-        float grad = ca * (A3(ph, i, k, j + 1) - A3(ph, i, k, j - 1)) + cb * (A3(ph, i, k, j + 2) - A3(ph, i, k, j - 2)) +
-                     cc * (A3(ph, i, k, j + mo) - A3(ph, i, k, j - mo)) + ca * (A3(phb, i, k, j + 1) - A3(phb, i, k, j - 1)) +
-                     cb * (A3(phb, i, k, j + 2) - A3(phb, i, k, j - 2)) + cc * (A3(phb, i, k, j + mo) - A3(phb, i, k, j - mo));
-        out += -(fac * rdy / A2(msfty, i, j)) * (va1 + va0) * scale * grad;
+        const int mo = mode == 6 ? 3 : 2;
+        float grad = ca * (phy[4] - phy[2]) + cb * (phy[5] - phy[1]) + cc * (phy[3 + mo] - phy[3 - mo]) + ca * (pby[4] - pby[2]) +
+                     cb * (pby[5] - pby[1]) + cc * (pby[3 + mo] - pby[3 - mo]);
+        out += -(fac * rdy / msfty) * (va1 + va0) * scale * grad;
       }
     }
   }
@@ -212,23 +267,21 @@ __device__ __forceinline__ float rhs_ph_val(const DevCtx *__restrict__ x, int i,
     if (mode) {
       float ua1, ua0, fac;
       if (k < kte) {
-        ua1 = (A1(c1f, k) * A2(muu, i + 1, j) + A1(c2f, k)) * (A3(u, i + 1, k, j) + A3(u, i + 1, k - 1, j)) * A2(msfux, i + 1, j);
-        ua0 = (A1(c1f, k) * A2(muu, i, j) + A1(c2f, k)) * (A3(u, i, k, j) + A3(u, i, k - 1, j)) * A2(msfux, i, j);
+        ua1 = (c1fk * muu1 + c2fk) * (ue[0] + ue[1]) * msfux1;
+        ua0 = (c1fk * muu0 + c2fk) * (uw[0] + uw[1]) * msfux0;
         fac = 0.25f;
       } else {
-        ua1 = (A1(c1f, k) * A2(muu, i + 1, j) + A1(c2f, k)) * (cfn * A3(u, i + 1, k - 1, j) + cfn1 * A3(u, i + 1, k - 2, j)) * A2(msfux, i + 1, j);
-        ua0 = (A1(c1f, k) * A2(muu, i, j) + A1(c2f, k)) * (cfn * A3(u, i, k - 1, j) + cfn1 * A3(u, i, k - 2, j)) * A2(msfux, i, j);
+        ua1 = (c1fk * muu1 + c2fk) * (cfn * ue[1] + cfn1 * ue[2]) * msfux1;
+        ua0 = (c1fk * muu0 + c2fk) * (cfn * uw[1] + cfn1 * uw[2]) * msfux0;
         fac = 0.5f;
       }
       if (mode == 2) {
-        float term = ua1 * (A3(phb, i + 1, k, j) - A3(phb, i, k, j) + A3(ph, i + 1, k, j) - A3(ph, i, k, j)) +
-                     ua0 * (A3(phb, i, k, j) - A3(phb, i - 1, k, j) + A3(ph, i, k, j) - A3(ph, i - 1, k, j));
-        out += -(fac * rdx / A2(msfty, i, j)) * term;
+        float term = ua1 * (pbx[4] - pb0 + phx[4] - ph0) + ua0 * (pb0 - pbx[2] + ph0 - phx[2]);
+        out += -(fac * rdx / msfty) * term;
       } else {
-        float grad = 45.f * (A3(ph, i + 1, k, j) - A3(ph, i - 1, k, j)) - 9.f * (A3(ph, i + 2, k, j) - A3(ph, i - 2, k, j)) +
-                     1.f * (A3(ph, i + 3, k, j) - A3(ph, i - 3, k, j)) + 45.f * (A3(phb, i + 1, k, j) - A3(phb, i - 1, k, j)) -
-                     9.f * (A3(phb, i + 2, k, j) - A3(phb, i - 2, k, j)) + 1.f * (A3(phb, i + 3, k, j) - A3(phb, i - 3, k, j));
-        out += -(fac * rdx / A2(msfty, i, j)) * (ua1 + ua0) * (1.f / 60.f) * grad;
+        float grad = 45.f * (phx[4] - phx[2]) - 9.f * (phx[5] - phx[1]) + 1.f * (phx[6] - phx[0]) + 45.f * (pbx[4] - pbx[2]) -
+                     9.f * (pbx[5] - pbx[1]) + 1.f * (pbx[6] - pbx[0]);
+        out += -(fac * rdx / msfty) * (ua1 + ua0) * (1.f / 60.f) * grad;
       }
     }
   }

@@ -323,20 +323,19 @@ __global__ void k_hdiff_w(const DevCtx xv, const float *t1, const float *t2, R3 
 // formed once per point for all of them.  Every value is the reference's
 // expression in the reference's operation order.
 
-// average of columns A (the face's own cell) and B (its upwind neighbour)
-// at level k of the face (k = 1: extrapolated to the surface; k = kde:
-// extrapolated to the model top)
-__device__ __forceinline__ float hs_hav(const DevCtx *__restrict__ x, const float *var, int ia, int ja, int ib, int jb, int k) {
-  const int ktf = x->kde - 1, ktes1 = x->kde - 1, ktes2 = x->kde - 2;
-  if (k == 1)
-    return 0.5f * (x->cf1 * A3(var, ia, 1, ja) + x->cf2 * A3(var, ia, 2, ja) + x->cf3 * A3(var, ia, 3, ja) + x->cf1 * A3(var, ib, 1, jb) +
-                   x->cf2 * A3(var, ib, 2, jb) + x->cf3 * A3(var, ib, 3, jb));
-  if (k == ktf + 1) {
-    const float dnw1 = A1(FL(dnw), ktes1), dn1 = A1(FL(dn), ktes1);
-    return 0.5f * (A3(var, ia, ktes1, ja) + (A3(var, ia, ktes1, ja) - A3(var, ia, ktes2, ja)) * 0.5f * dnw1 / dn1 + A3(var, ib, ktes1, jb) +
-                   (A3(var, ib, ktes1, jb) - A3(var, ib, ktes2, jb)) * 0.5f * dnw1 / dn1);
-  }
-  return 0.5f * (A1(FL(fnm), k) * (A3(var, ib, k, jb) + A3(var, ia, k, ja)) + A1(FL(fnp), k) * (A3(var, ib, k - 1, jb) + A3(var, ia, k - 1, ja)));
+// The level averages of a scalar on the two columns A and B of a face, from
+// the values at levels k-1, k, k+1, k+2 of each column (qa[0..3], qb[0..3]):
+// at level k (surface extrapolation at k = 1) and at level k+1 (the top
+// extrapolation at kde); hs_hav's three forms with the same arithmetic.
+__device__ __forceinline__ float hs_hav_dn(const DevCtx *__restrict__ x, const float qa[4], const float qb[4], int k, float fm, float fp) {
+  if (k == 1) return 0.5f * (x->cf1 * qa[1] + x->cf2 * qa[2] + x->cf3 * qa[3] + x->cf1 * qb[1] + x->cf2 * qb[2] + x->cf3 * qb[3]);
+  return 0.5f * (fm * (qb[1] + qa[1]) + fp * (qb[0] + qa[0]));
+}
+__device__ __forceinline__ float hs_hav_up(const DevCtx *__restrict__ x, const float qa[4], const float qb[4], int k, float fm, float fp,
+                                           float dnw1, float dn1) {
+  if (k + 1 == x->kde)
+    return 0.5f * (qa[1] + (qa[1] - qa[0]) * 0.5f * dnw1 / dn1 + qb[1] + (qb[1] - qb[0]) * 0.5f * dnw1 / dn1);
+  return 0.5f * (fm * (qb[2] + qa[2]) + fp * (qb[1] + qa[1]));
 }
 
 // scalar-independent part of one face at level k: -msf * xk * rho (c),
@@ -379,13 +378,31 @@ __global__ void __launch_bounds__(256) k_hdiff_s_faces(const DevCtx xv, HdsList 
   HsFace a{0.f, 0.f, 0.f}, b{0.f, 0.f, 0.f};
   if (f1) a = hs_face(x, FL(zx), A2(FL(msfux), i, j), i, j, i - 1, j, k);
   if (f2) b = hs_face(x, FL(zy), A2(FL(msfvy), i, j), i, j, i, j - 1, k);
+  const int kde = x->kde, k1 = imin_(k + 1, kde);
+  const float fm0 = A1(FL(fnm), k), fp0 = A1(FL(fnp), k), fm1 = A1(FL(fnm), k1), fp1 = A1(FL(fnp), k1);
+  const float dnw1 = A1(FL(dnw), kde - 1), dn1 = A1(FL(dn), kde - 1);
+  const unsigned nb = (unsigned)x->n3, sk = (unsigned)x->ip, sj = (unsigned)(kde * x->ip);
   for (int s = 0; s < L.n; s++) {
-    const float *q = L.var[s];
-    const float qc = A3(q, i, k, j);
-    if (f1)
-      elem(L.h1[s], o) = hs_flux(a, x->rdx, qc, A3(q, i - 1, k, j), hs_hav(x, q, i, j, i - 1, j, k + 1), hs_hav(x, q, i, j, i - 1, j, k));
-    if (f2)
-      elem(L.h2[s], o) = hs_flux(b, x->rdy, qc, A3(q, i, k, j - 1), hs_hav(x, q, i, j, i, j - 1, k + 1), hs_hav(x, q, i, j, i, j - 1, k));
+    // levels k-1 .. k+2 of the point's column and of its west and south
+    // neighbours, all loaded first (buffer loads: the levels outside the
+    // column read 0 and are not used)
+    const BufF bq = bufv(L.var[s], nb);
+    float qc[4], qw[4], qs[4];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      const unsigned on = o + (unsigned)(n - 1) * sk;
+      qc[n] = bld(bq, on);
+      qw[n] = bld(bq, on - 1u);
+      qs[n] = bld(bq, on - sj);
+    }
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      pin1(qc[n]);
+      pin1(qw[n]);
+      pin1(qs[n]);
+    }
+    if (f1) elem(L.h1[s], o) = hs_flux(a, x->rdx, qc[1], qw[1], hs_hav_up(x, qc, qw, k, fm1, fp1, dnw1, dn1), hs_hav_dn(x, qc, qw, k, fm0, fp0));
+    if (f2) elem(L.h2[s], o) = hs_flux(b, x->rdy, qc[1], qs[1], hs_hav_up(x, qc, qs, k, fm1, fp1, dnw1, dn1), hs_hav_dn(x, qc, qs, k, fm0, fp0));
   }
 }
 
@@ -406,15 +423,29 @@ __global__ void __launch_bounds__(256) k_hdiff_s_tend(const DevCtx xv, HdsList L
   const bool lo = (k != 1), hi = (k + 1 != ktf + 1);  // H*A(k), H*A(k+1) nonzero
   const float fm0 = A1(fnm, k), fp0 = A1(fnp, k), fm1 = A1(fnm, k + 1), fp1 = A1(fnp, k + 1);
   const unsigned o = XI3(i, k, j), sk = (unsigned)x->ip, sj = (unsigned)(x->kde * x->ip);
+  const unsigned nb = (unsigned)x->n3;
   for (int s = 0; s < L.n; s++) {
-    const float *h1 = L.h1[s], *h2 = L.h2[s];
-    const float a0 = elem(h1, o), a1 = elem(h1, o + 1), b0 = elem(h2, o), b1 = elem(h2, o + sj);
-    const float h1a0 = lo ? 0.5f * (fm0 * (a1 + a0) + fp0 * (elem(h1, o + 1 - sk) + elem(h1, o - sk))) : 0.f;
-    const float h1a1 = hi ? 0.5f * (fm1 * (elem(h1, o + 1 + sk) + elem(h1, o + sk)) + fp1 * (a1 + a0)) : 0.f;
-    const float h2a0 = lo ? 0.5f * (fm0 * (b1 + b0) + fp0 * (elem(h2, o + sj - sk) + elem(h2, o - sk))) : 0.f;
-    const float h2a1 = hi ? 0.5f * (fm1 * (elem(h2, o + sj + sk) + elem(h2, o + sk)) + fp1 * (b1 + b0)) : 0.f;
-    const float inc = fac * (mrdx * (a1 - a0) + mrdy * (b1 - b0) - cx * (h1a1 - h1a0) * rz - cy * (h2a1 - h2a0) * rz);
+    // every load of this scalar first (buffer loads: the level below 1 and
+    // above kde-1 read 0 and are not used)
+    const BufF b1 = bufv(L.h1[s], nb), b2 = bufv(L.h2[s], nb);
     float *t = L.tend[s];
-    elem(t, o) = elem(t, o) + inc;
+    const float a0 = bld(b1, o), a1 = bld(b1, o + 1u), b0 = bld(b2, o), b1v = bld(b2, o + sj);
+    const float a0m = bld(b1, o - sk), a1m = bld(b1, o + 1u - sk), a0p = bld(b1, o + sk), a1p = bld(b1, o + 1u + sk);
+    const float b0m = bld(b2, o - sk), b1m = bld(b2, o + sj - sk), b0p = bld(b2, o + sk), b1p = bld(b2, o + sj + sk);
+    const float t0 = elem(t, o);
+    pin1(a0m);
+    pin1(a1m);
+    pin1(a0p);
+    pin1(a1p);
+    pin1(b0m);
+    pin1(b1m);
+    pin1(b0p);
+    pin1(b1p);
+    const float h1a0 = lo ? 0.5f * (fm0 * (a1 + a0) + fp0 * (a1m + a0m)) : 0.f;
+    const float h1a1 = hi ? 0.5f * (fm1 * (a1p + a0p) + fp1 * (a1 + a0)) : 0.f;
+    const float h2a0 = lo ? 0.5f * (fm0 * (b1v + b0) + fp0 * (b1m + b0m)) : 0.f;
+    const float h2a1 = hi ? 0.5f * (fm1 * (b1p + b0p) + fp1 * (b1v + b0)) : 0.f;
+    const float inc = fac * (mrdx * (a1 - a0) + mrdy * (b1v - b0) - cx * (h1a1 - h1a0) * rz - cy * (h2a1 - h2a0) * rz);
+    elem(t, o) = t0 + inc;
   }
 }

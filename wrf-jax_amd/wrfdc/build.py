@@ -19,7 +19,7 @@ ARCH = "gfx950"
 
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
-    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Werror=uninitialized",
 ]
 
 
