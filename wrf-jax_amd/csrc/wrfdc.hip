@@ -44,11 +44,34 @@ __global__ void k_mass_weight(const DevCtx xv, const float *fl, float *out, int 
   A3(out, i, k, j) = v;
 }
 
-// dst = src on mass points (initialize_moist_old)
-__global__ void k_copy_mass(const DevCtx xv, float *dst, const float *src, R3 r) {
+// dst[c] = src[c] over the points of r, for every pair of the list in one
+// launch (initialize_moist_old, q_old <- q, the species buffer copies).
+// Block (64, 4): 256 consecutive i of 4 levels; grid (i segments x level
+// groups, rows, pairs).  Each thread copies 4 consecutive i with one 16-byte
+// load and store where all 4 lie in r and start on a 16-byte boundary (rows
+// start 64-byte aligned), else element by element.
+constexpr int kMaxCopy = 2 * kMaxSpec;
+struct CopyList {
+  float *dst[kMaxCopy];
+  const float *src[kMaxCopy];
+  int n;
+};
+__global__ void __launch_bounds__(256) k_copy_batch(const DevCtx xv, CopyList L, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ_P(r);
-  A3(dst, i, k, j) = A3(src, i, k, j);
+  const unsigned gi = (unsigned)(r.i1 - r.i0 + 256) / 256u;
+  const int i0 = r.i0 + (int)(blockIdx.x % gi) * 256 + 4 * (int)threadIdx.x;
+  const int k = r.k0 + (int)(blockIdx.x / gi) * 4 + (int)threadIdx.y;
+  const int j = r.j0 + (int)blockIdx.y;
+  if (k > r.k1 || i0 > r.i1) return;
+  float *dst = L.dst[blockIdx.z];
+  const float *src = L.src[blockIdx.z];
+  const unsigned o = XI3(i0, k, j);
+  if (i0 + 3 <= r.i1 && (o & 3u) == 0) {
+    *reinterpret_cast<float4 *>(dst + o) = *reinterpret_cast<const float4 *>(src + o);
+  } else {
+    const int ie = imin_(i0 + 3, r.i1);
+    for (int ii = i0; ii <= ie; ii++) elem(dst, o + (unsigned)(ii - i0)) = elem(src, o + (unsigned)(ii - i0));
+  }
 }
 
 // FP64 checksums over mass points: [0] sum(mu_2 + mub), [1] sum(t_2),
@@ -603,12 +626,27 @@ struct Run {
     copy_species(R3{1, ide, 1, kde - 1, 1, jde}, alt);
     use_species(alt);
   }
+  // one launch of k_copy_batch over r for the pairs of L
+  void copy_batch(const CopyList &L, R3 r) const {
+    if (L.n == 0) return;
+    const unsigned gi = (unsigned)(r.i1 - r.i0 + 256) / 256u, gk = (unsigned)(r.k1 - r.k0 + 4) / 4u;
+    k_copy_batch<<<dim3(gi * gk, (unsigned)(r.j1 - r.j0 + 1), (unsigned)L.n), dim3(64, 4), 0, s>>>(d, L, r);
+  }
   void copy_species(R3 r, bool to_alt) const {
     const size_t n = X->n3;
     const float *sm = to_alt ? user_moist : alt_moist, *ss = to_alt ? user_scalar : alt_scalar;
     float *dm = to_alt ? alt_moist : user_moist, *ds = to_alt ? alt_scalar : user_scalar;
-    for (int q = 0; q < X->c.nmoist; q++) k_copy_mass<<<g3p(r), b3(), 0, s>>>(d, dm + q * n, sm + q * n, r);
-    for (int q = 0; q < X->c.nscalar; q++) k_copy_mass<<<g3p(r), b3(), 0, s>>>(d, ds + q * n, ss + q * n, r);
+    CopyList L;
+    memset(&L, 0, sizeof(L));
+    for (int q = 0; q < X->c.nmoist; q++) {
+      L.dst[L.n] = dm + q * n;
+      L.src[L.n++] = sm + q * n;
+    }
+    for (int q = 0; q < X->c.nscalar; q++) {
+      L.dst[L.n] = ds + q * n;
+      L.src[L.n++] = ss + q * n;
+    }
+    copy_batch(L, r);
   }
   void fill_copy(float *dst, const float *src, size_t n) const {
     unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
@@ -617,11 +655,19 @@ struct Run {
   // q_old <- q on mass points (the reference's rk1 update sets it; for the
   // moist species the step start has already done so)
   void sync_old(bool moist, bool scalar) const {
-    R3 m = mass3();
+    CopyList L;
+    memset(&L, 0, sizeof(L));
     if (moist)
-      for (int q = 0; q < X->c.nmoist; q++) k_copy_mass<<<g3p(m), b3(), 0, s>>>(d, Sp(F_moist_old, q), Sp(F_moist, q), m);
+      for (int q = 0; q < X->c.nmoist; q++) {
+        L.dst[L.n] = Sp(F_moist_old, q);
+        L.src[L.n++] = Sp(F_moist, q);
+      }
     if (scalar)
-      for (int q = 0; q < X->c.nscalar; q++) k_copy_mass<<<g3p(m), b3(), 0, s>>>(d, Sp(F_scalar_old, q), Sp(F_scalar, q), m);
+      for (int q = 0; q < X->c.nscalar; q++) {
+        L.dst[L.n] = Sp(F_scalar_old, q);
+        L.src[L.n++] = Sp(F_scalar, q);
+      }
+    copy_batch(L, mass3());
   }
   // Buffers: rk1 reads the stencils from q_old (== q) and writes q in place;
   // rk2 reads q and writes the second buffer, which later kernels of the step
@@ -727,16 +773,14 @@ struct Run {
   void step() const {
     R3 m = mass3();
     if (species_in_alt()) species_to(false);
+    // initialize_moist_old (module_big_step_utilities_em.py:935-975); the
+    // scalars' q_old <- q of the first-stage update is done here as well
+    // (one copy launch), then the moist q_old's boundary update
+    sync_old(true, true);
     {
-      // initialize_moist_old (module_big_step_utilities_em.py:935-975); the
-      // scalars' q_old <- q of the first-stage update is done here as well
       BcBatch bc(this);
-      for (int q = 0; q < X->c.nmoist; q++) {
-        k_copy_mass<<<g3p(m), b3(), 0, s>>>(d, Sp(F_moist_old, q), Sp(F_moist, q), m);
-        bc.add3(Sp(F_moist_old, q), 'p');
-      }
+      for (int q = 0; q < X->c.nmoist; q++) bc.add3(Sp(F_moist_old, q), 'p');
     }
-    sync_old(false, true);
     old_synced = true;
     for (int rk = 1; rk <= 3; rk++) {
       set_rk(rk);
