@@ -135,7 +135,7 @@ inline int acoustic_col_npw(int kde) {
 }
 
 template <int NPW>
-__global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv, float dts, int first, int fin, float ns) {
+__global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv, float dts, int first, int fin, float ns, int store_tav) {
   const DevCtx *__restrict__ x = &xv;
   constexpr int NW = kColWaves;
   constexpr int G1 = kColG1, G3 = kColG3, G4 = kColG4, G5 = kColG5, G6 = kColG6, G9 = kColG9, G10 = kColG10;
@@ -332,7 +332,7 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
         A3(t, i, k, j) = tnew;
         float tav = 0.5f * (ope * tnew + ome * tk);
         tav = (tav + (A1(c1h, k) * mav) * t0) / ((A1(c1h, k) * mts + A1(c2h, k)) * (t0 + r_t1[q]));
-        A3(tav_out, i, k, j) = tav;
+        if (store_tav) A3(tav_out, i, k, j) = tav;
         LS(B2, k) = tav;
       }
     }
@@ -679,15 +679,17 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
 }
 
 // host launcher: instantiation by levels per wave
-inline void launch_acoustic_col(const DevCtx &d, hipStream_t s, float dts, int first, int fin, float ns) {
+// store_tav: write the acoustic theta average to t_2ave (only its last
+// substep's value is read outside the kernel)
+inline void launch_acoustic_col(const DevCtx &d, hipStream_t s, float dts, int first, int fin, float ns, int store_tav) {
   const dim3 grid((unsigned)((d.ide + 63) / 64) * (unsigned)d.jde), block(64, kColWaves);
   const size_t lds = acoustic_col_lds(d.kde);
   switch (acoustic_col_npw(d.kde)) {
-    case 4: k_acoustic_col<4><<<grid, block, lds, s>>>(d, dts, first, fin, ns); break;
-    case 8: k_acoustic_col<8><<<grid, block, lds, s>>>(d, dts, first, fin, ns); break;
-    case 12: k_acoustic_col<12><<<grid, block, lds, s>>>(d, dts, first, fin, ns); break;
-    case 16: k_acoustic_col<16><<<grid, block, lds, s>>>(d, dts, first, fin, ns); break;
-    default: k_acoustic_col<20><<<grid, block, lds, s>>>(d, dts, first, fin, ns); break;
+    case 4: k_acoustic_col<4><<<grid, block, lds, s>>>(d, dts, first, fin, ns, store_tav); break;
+    case 8: k_acoustic_col<8><<<grid, block, lds, s>>>(d, dts, first, fin, ns, store_tav); break;
+    case 12: k_acoustic_col<12><<<grid, block, lds, s>>>(d, dts, first, fin, ns, store_tav); break;
+    case 16: k_acoustic_col<16><<<grid, block, lds, s>>>(d, dts, first, fin, ns, store_tav); break;
+    default: k_acoustic_col<20><<<grid, block, lds, s>>>(d, dts, first, fin, ns, store_tav); break;
   }
 }
 // raise the dynamic-LDS limit of the instantiation kde needs (beyond 64 KB)

@@ -528,12 +528,17 @@ __global__ void k_calc_p_rho_phi(const DevCtx xv, R3 r) {
   const float pfu = A1(FL(c3f), k + 1) * m + A1(FL(c4f), k + 1) + ptop;
   const float pfd = A1(FL(c3f), k) * m + A1(FL(c4f), k) + ptop;
   const float phm = A1(FL(c3h), k) * m + A1(FL(c4h), k) + ptop;
+  // every load first (the stores below could alias them as far as the
+  // compiler knows)
+  const unsigned o = XI3(i, k, j), o1 = o + (unsigned)x->ip;
+  const float ph1 = elem(ph, o1), ph0 = elem(ph, o), phb1 = elem(phb, o1), phb0 = elem(phb, o), albv = elem(alb, o);
+  const float qv = elem(SP(moist, 0), o), tv = elem(t, o), pbv = elem(pb, o);
   // log and pow in FP64, rounded once to FP32 (the correctly rounded FP32
   // values in all but vanishingly rare cases)
   const float lg = (float)log((double)(pfd / pfu));
-  const float alv = (A3(ph, i, k + 1, j) - A3(ph, i, k, j) + A3(phb, i, k + 1, j) - A3(phb, i, k, j)) / phm / lg - A3(alb, i, k, j);
-  A3(FL(al), i, k, j) = alv;
-  const float qvf = 1.f + x->rvovrd * A3(SP(moist, 0), i, k, j);
-  const float temp = (x->r_d * (x->t0 + A3(t, i, k, j)) * qvf) / (x->p0 * (alv + A3(alb, i, k, j)));
-  A3(FL(p), i, k, j) = (float)pow((double)temp, (double)x->cpovcv) * x->p0 - A3(pb, i, k, j);
+  const float alv = (ph1 - ph0 + phb1 - phb0) / phm / lg - albv;
+  elem(FL(al), o) = alv;
+  const float qvf = 1.f + x->rvovrd * qv;
+  const float temp = (x->r_d * (x->t0 + tv) * qvf) / (x->p0 * (alv + albv));
+  elem(FL(p), o) = (float)pow((double)temp, (double)x->cpovcv) * x->p0 - pbv;
 }

@@ -125,19 +125,24 @@ __global__ void k_ssp_3d(const DevCtx xv, R3 r) {
 __global__ void k_calc_p_rho(const DevCtx xv, int step, R3 r) {
   const DevCtx *__restrict__ x = &xv;
   GET_IKJ_P(r);
-  const size_t o = XI3(i, k, j);
-  const float mass = A1(FL(c1h), k) * A2(FL(muts), i, j) + A1(FL(c2h), k);
-  const float dry = A1(FL(c1h), k) * A2(FL(mu_2), i, j);
-  const float alt = FL(alt)[o], t1 = FL(t_save)[o];
-  const float bracket = alt * dry + A1(FL(rdnw), k) * (A3(FL(ph_2), i, k + 1, j) - FL(ph_2)[o]);
+  const unsigned o = XI3(i, k, j);
+  // every load first (the stores below could alias them as far as the
+  // compiler knows)
+  const float muts = A2(FL(muts), i, j), mu2 = A2(FL(mu_2), i, j);
+  const float alt = elem(FL(alt), o), t1 = elem(FL(t_save), o), t2 = elem(FL(t_2), o), c2a = elem(FL(c2a), o);
+  const float ph1 = elem(FL(ph_2), o + (unsigned)x->ip), ph0 = elem(FL(ph_2), o);
+  const float pm1 = step == 0 ? 0.f : elem(FL(pm1), o);
+  const float mass = A1(FL(c1h), k) * muts + A1(FL(c2h), k);
+  const float dry = A1(FL(c1h), k) * mu2;
+  const float bracket = alt * dry + A1(FL(rdnw), k) * (ph1 - ph0);
   const float alv = (-1.f / mass) * bracket;
-  const float tdel = FL(t_2)[o] - dry * t1;
+  const float tdel = t2 - dry * t1;
   const float frac = (alt * tdel) / (mass * (x->t0 + t1));
-  const float pr = FL(c2a)[o] * (frac - alv);
-  FL(al)[o] = alv;
-  if (step == 0) FL(p)[o] = pr;
-  else FL(p)[o] = pr + x->smdiv * (pr - FL(pm1)[o]);
-  FL(pm1)[o] = pr;
+  const float pr = c2a * (frac - alv);
+  elem(FL(al), o) = alv;
+  if (step == 0) elem(FL(p), o) = pr;
+  else elem(FL(p), o) = pr + x->smdiv * (pr - pm1);
+  elem(FL(pm1), o) = pr;
 }
 
 // calc_coef_w (module_small_step_em.py:1359-1586): LU factors of the

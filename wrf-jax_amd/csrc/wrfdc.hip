@@ -569,13 +569,14 @@ struct Run {
     k_zero_grad_rows<<<dim3(cdiv(ide - 1, 64), kde, 2), 64, 0, s>>>(d, F(F_w_2));
     k_zero_grad_cols<<<dim3(cdiv(jde - 1, 64), kde, 2), 64, 0, s>>>(d, F(F_w_2));
   }
-  // one fused acoustic substep: u/v, the column kernel, zero_grad_bdy of w
-  void acoustic_substep(int it) const {
+  // one fused acoustic substep: u/v, the column kernel, zero_grad_bdy of w;
+  // all_outputs: also the intermediate t_2ave (a substep called on its own)
+  void acoustic_substep(int it, bool all_outputs) const {
     X->iteration = it;
     const float dts = (float)X->dts_rk;
     R3 r{1, ide, 1, kde - 1, 1, jde};
     k_uv_substep<<<g3(r), b3(), 0, s>>>(d, dts, r);
-    launch_acoustic_col(d, s, dts, it == 1, it == X->nsmall, (float)X->nsmall);
+    launch_acoustic_col(d, s, dts, it == 1, it == X->nsmall, (float)X->nsmall, all_outputs || it == X->nsmall);
     Z z = zd('w', true, false);
     k_zero_grad_w<<<dim3(cdiv(zone_n(z), 64), kde), 64, 0, s>>>(d);
   }
@@ -755,7 +756,7 @@ struct Run {
         .add3(F(F_ru_tend), 'u').add3(F(F_rv_tend), 'v').add3(F(F_ph_2), 'w').add3(F(F_al), 'p').add3(F(F_p), 'p')
         .add3(F(F_t_1), 'p').add3(F(F_t_save), 't').add2(F(F_mu_1), 't').add2(F(F_mu_2), 't').add2(F(F_mudf), 't')
         .flush();
-    for (int it = 1; it <= X->nsmall; it++) acoustic_substep(it);
+    for (int it = 1; it <= X->nsmall; it++) acoustic_substep(it, false);
     small_step_finish();
     if (X->rk == 3)
       BcBatch(this).add3(F(F_ru_m), 'u').add3(F(F_rv_m), 'v').add3(F(F_ww_m), 'w').add2(F(F_mut), 't').add2(F(F_muts), 't').flush();
@@ -1071,12 +1072,12 @@ int wrfdc_stage(wrfdc_ctx *x, const char *name, int rk_step, int iteration) {
         R3 r{1, run.ide, 1, run.kde - 1, 1, run.jde};
         k_uv_substep<<<Run::g3(r), Run::b3(), 0, x->s>>>(x->h, dts, r);
       } else {
-        launch_acoustic_col(x->h, x->s, dts, x->iteration == 1, x->iteration == x->nsmall, (float)x->nsmall);
+        launch_acoustic_col(x->h, x->s, dts, x->iteration == 1, x->iteration == x->nsmall, (float)x->nsmall, 1);
       }
     }
     else if (n == "acoustic_substep") {
       if (iteration < 1) return set_err(x, WRFDC_EINVAL, "acoustic_substep needs iteration >= 1");
-      run.acoustic_substep(iteration);
+      run.acoustic_substep(iteration, true);
     }
     else if (n == "small_step_finish") run.small_step_finish();
     else if (n == "calc_p_rho_phi") run.calc_p_rho_phi();
