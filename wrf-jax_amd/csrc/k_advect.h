@@ -38,15 +38,19 @@ __device__ __forceinline__ float lane_next(float v, int lane) {
 // top wave also forms its own north face), one level k.  Each x face flux is
 // evaluated once and each y face 5/4 times instead of twice; every thread
 // still owns one point, so the grid keeps one wave per 63 points.
-// Tile order: i fastest, then 4 consecutive levels, then rows, then groups
-// of 4 levels, XCD-contiguous: the neighbour rows of a y stencil and the
+// Tile order: i fastest, then kFaceLG consecutive levels, then rows, then
+// groups of kFaceLG levels, XCD-contiguous: the neighbour rows of a y stencil and the
 // neighbour levels of a z stencil are read by the same XCD shortly before.
+#ifndef WRFDC_FACE_LG
+#define WRFDC_FACE_LG 4
+#endif
+constexpr unsigned kFaceLG = WRFDC_FACE_LG;  // levels per level group of the tile order
 struct FaceTiles {
-  unsigned ni, nj, nkc;  // tiles along i, row groups, level groups of 4
-  __host__ __device__ unsigned count() const { return ni * nj * nkc * 4u; }
+  unsigned ni, nj, nkc;  // tiles along i, row groups, level groups
+  __host__ __device__ unsigned count() const { return ni * nj * nkc * kFaceLG; }
 };
 __host__ __device__ inline FaceTiles face_tiles(const R3 &r) {
-  return FaceTiles{(unsigned)(r.i1 - r.i0 + 63) / 63u, (unsigned)(r.j1 - r.j0 + 4) / 4u, (unsigned)(r.k1 - r.k0 + 4) / 4u};
+  return FaceTiles{(unsigned)(r.i1 - r.i0 + 63) / 63u, (unsigned)(r.j1 - r.j0 + 4) / 4u, (unsigned)(r.k1 - r.k0 + kFaceLG) / kFaceLG};
 }
 // thread -> lane, wave (row in the block), column i, level k, row j and the
 // block's first row j0; own: the lane holds a point of r; ic: i clamped into
@@ -59,8 +63,8 @@ __host__ __device__ inline FaceTiles face_tiles(const R3 &r) {
   const int lane = (int)threadIdx.x, wv = (int)wave_y();                                 \
   const unsigned u_ = t_ / T_.ni;                                                        \
   const int i = (r).i0 + (int)(t_ % T_.ni) * 63 + lane;                                  \
-  const int k = (r).k0 + (int)((u_ / 4u) / T_.nj) * 4 + (int)(u_ % 4u);                  \
-  const int j0 = (r).j0 + (int)((u_ / 4u) % T_.nj) * 4;                                  \
+  const int k = (r).k0 + (int)((u_ / kFaceLG) / T_.nj) * (int)kFaceLG + (int)(u_ % kFaceLG); \
+  const int j0 = (r).j0 + (int)((u_ / kFaceLG) % T_.nj) * 4;                             \
   const int j = j0 + wv;                                                                 \
   if (k > (r).k1) return; /* whole block */                                              \
   const bool jrow = (j <= (r).j1);                                                       \

@@ -455,15 +455,24 @@ struct Run {
     R3 rw{1, ide - 1, 2, kde - 1, 1, jde - 1};
     k_cor_curv<<<g3(rw), b3(), 0, s>>>(d, 2, rw);
   }
-  // the RK tendencies and the saved tendencies start from zero each stage
-  void clear_tendencies() const {
+  // the RK tendencies and the saved tendencies start from zero each stage.
+  // saves = false (later stages of the step): the saved tendencies keep
+  // their values: nothing reads them before small_step_prep writes them
+  // again over the same points, and nothing writes the points outside (zero
+  // since the first stage's clear) - the first stage's addtend fold is their
+  // only earlier reader.
+  void clear_tendencies(bool saves) const {
     const size_t n3 = X->n3, n2 = X->n2;
-    fill_many({{F(F_ru_tend), n3}, {F(F_rv_tend), n3}, {F(F_rw_tend), n3}, {F(F_t_tend), n3}, {F(F_ph_tend), n3},
-               {F(F_u_save), n3}, {F(F_v_save), n3}, {F(F_w_save), n3}, {F(F_ph_save), n3}, {F(F_t_save), n3},
-               {F(F_mu_tend), n2}, {F(F_mu_save), n2}});
+    if (saves)
+      fill_many({{F(F_ru_tend), n3}, {F(F_rv_tend), n3}, {F(F_rw_tend), n3}, {F(F_t_tend), n3}, {F(F_ph_tend), n3},
+                 {F(F_u_save), n3}, {F(F_v_save), n3}, {F(F_w_save), n3}, {F(F_ph_save), n3}, {F(F_t_save), n3},
+                 {F(F_mu_tend), n2}, {F(F_mu_save), n2}});
+    else
+      fill_many({{F(F_ru_tend), n3}, {F(F_rv_tend), n3}, {F(F_rw_tend), n3}, {F(F_t_tend), n3}, {F(F_ph_tend), n3},
+                 {F(F_mu_tend), n2}});
   }
   void rk_tendency() const {
-    clear_tendencies();
+    clear_tendencies(true);
     advect_u();
     advect_v();
     advect_w();
@@ -480,7 +489,7 @@ struct Run {
   // (rk_addtend_dry), then the specified-zone overlay.
   void fused_tendency() const {
     const int rk = X->rk;
-    clear_tendencies();
+    clear_tendencies(rk == 1);
     if (rk == 1) lateral_relax();
     const R3 ru{1, ide, 1, kde - 1, 1, jde - 1};
     k_advect_u<true><<<dim3(face_tiles(ru).count()), b3(), 0, s>>>(d, F(F_u_2), F(F_ru_tend), F(F_ru), F(F_rv), F(F_ww), ru, rk);
