@@ -105,6 +105,49 @@ def test_one_step_parity(domain_for, shape):
     assert not bad, f"fields above rel-L2 {REL_L2_STEP} after one RK3 step: {bad}"
 
 
+def test_boundary_refresh(domain_for):
+    """Time-varying lateral forcing across a boundary-record change, as the
+    em_real loop drives it (src/wrf_jax/trajectory.py:1504-1526 loads the next
+    wrfbdy record and resets dtbc to 0): three steps relaxing towards edge
+    values that drift with nonzero tendencies, then the next record (edge
+    values advanced over the interval, new tendencies) with dtbc restarting
+    at 0, three more steps.  Every registry field against the oracle."""
+    from oracle import Oracle
+    from wrfdc.fields import FIELD_SPECS
+
+    print("[boundary refresh] shape=(28, 24, 12), 3 + 3 steps", flush=True)
+    case = _case(28, 24, 12, nmoist=6, nscalar=2, hydrometeors=True, hill_height=500.0)
+    A = {k: v.copy() for k, v in case.arrays.items()}
+    dt = case.config.dt
+    rng = np.random.default_rng(20260)
+    bnames = [n for n, kind in FIELD_SPECS if kind in "XYxy" and "_b" in n and "_bt" not in n]
+    assert bnames
+    for n in bnames:  # record 1: the edge state drifts by up to +-5 % over 600 s
+        t = n.replace("_b", "_bt", 1)
+        A[t] = (A[n] * rng.uniform(-0.05, 0.05, A[n].shape) / 600.0).astype(np.float32)
+    assert any(np.abs(A[n.replace("_b", "_bt", 1)]).max() > 0 for n in bnames)
+    dom = domain_for(case.config)
+    o = Oracle(case.config, A)
+    dom.upload(A)
+    o.step(3, 0.0)
+    dom.step(3, 0.0)
+    # record 2: edge values advanced over the three steps, halved tendencies
+    nxt = {}
+    for n in bnames:
+        t = n.replace("_b", "_bt", 1)
+        nxt[n] = (A[n] + A[t] * np.float32(3 * dt)).astype(np.float32)
+        nxt[t] = (A[t] * np.float32(0.5)).astype(np.float32)
+    dom.upload(nxt)
+    for k, v in nxt.items():
+        o.arrays[k][...] = v
+    o.step(3, 0.0)
+    dom.step(3, 0.0)
+    got = dom.download()
+    bad = compare(got, o.arrays, tol=REL_L2_STEP, skip=STEP_DIAGNOSTICS)
+    o.close()
+    assert not bad, f"fields above rel-L2 {REL_L2_STEP} after a boundary refresh: {bad}"
+
+
 def test_hundred_steps_nrmse(domain_for):
     from oracle import Oracle
 

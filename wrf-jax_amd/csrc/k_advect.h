@@ -97,11 +97,13 @@ __device__ __forceinline__ void win5z(const BufF &b, unsigned o, unsigned sk, fl
 __device__ __forceinline__ void pin6(const float w[6]) { asm volatile("" : : "v"(w[0]), "v"(w[1]), "v"(w[2]), "v"(w[3]), "v"(w[4]), "v"(w[5])); }
 __device__ __forceinline__ void pin5(const float w[5]) { asm volatile("" : : "v"(w[0]), "v"(w[1]), "v"(w[2]), "v"(w[3]), "v"(w[4])); }
 // Two ways to pick a face's flux order once its stencil is in registers:
-// *_sel evaluates every order and selects (no branch: kernels with many
-// loads in flight keep their registers), *_br branches on the order (only
-// the face's own order is evaluated: the species and scalar kernels, whose
-// loads are pinned first).  The order of a face in j or k is uniform over a
-// wave; in i it differs only in the waves at the domain edges.
+// *_sel evaluates the 5th-order flux and, unless `fast` (a wave-uniform
+// flag: every lane's face takes the 5th order) says it is the face's own for
+// the whole wave, every other order and selects (no divergent branch:
+// kernels with many loads in flight keep their registers); *_br branches on
+// the order (only the face's own order is evaluated: the species and scalar
+// kernels, whose loads are pinned first).  The order of a face in j or k is
+// uniform over a wave; in i it differs only in the waves at the domain edges.
 __device__ __forceinline__ float hflux_br(const float w[6], float vel, int f, int fe) {
   if (f >= 4 && f <= fe - 3) return vel * flux5(w[0], w[1], w[2], w[3], w[4], w[5], vel);
   if (f == 2 || f == fe - 1) return 0.5f * vel * (w[3] + w[2]);
@@ -114,25 +116,28 @@ __device__ __forceinline__ float zflux_br(float qm2, float qm1, float q0, float 
 }
 // scalar and w faces along x / y (fx_s, fy_s, fx_w, fy_w): face velocity
 // vel; 5th order on [4, fe-3], centred at 2 and fe-1, 3rd order between
-__device__ __forceinline__ float hflux_sel(const float w[6], float vel, int f, int fe) {
+__device__ __forceinline__ float hflux_sel(const float w[6], float vel, int f, int fe, bool fast) {
   const float h5 = vel * flux5(w[0], w[1], w[2], w[3], w[4], w[5], vel);
+  if (fast) return h5;
   const float h2 = 0.5f * vel * (w[3] + w[2]);
   const float h3 = vel * flux3(w[1], w[2], w[3], w[4], vel);
   return (f >= 4 && f <= fe - 3) ? h5 : ((f == 2 || f == fe - 1) ? h2 : h3);
 }
 // fy_u: rvs = rv(i) + rv(i-1) at the face
-__device__ __forceinline__ float fy_u_sel(const float w[6], float rvs, int j, int jde) {
+__device__ __forceinline__ float fy_u_sel(const float w[6], float rvs, int j, int jde, bool fast) {
   const float vel = 0.5f * rvs;
   const float h5 = vel * flux5(w[0], w[1], w[2], w[3], w[4], w[5], vel);
+  if (fast) return h5;
   const float h2 = 0.25f * rvs * (w[3] + w[2]);
   const float h3 = vel * flux3(w[1], w[2], w[3], w[4], vel);
   return (j >= 4 && j <= jde - 3) ? h5 : ((j == 2 || j == jde - 1) ? h2 : h3);
 }
 // fx_u: rus = ru(i) + ru(i-1); 5th order on [4, ide-2], upwind-biased
 // second order at the boundary faces 2 and ide
-__device__ __forceinline__ float fx_u_sel(const float w[6], float rus, int i, int ide) {
+__device__ __forceinline__ float fx_u_sel(const float w[6], float rus, int i, int ide, bool fast) {
   const float vel = 0.5f * rus;
   const float h5 = vel * flux5(w[0], w[1], w[2], w[3], w[4], w[5], vel);
+  if (fast) return h5;
   const float h3 = vel * flux3(w[1], w[2], w[3], w[4], vel);
   const float ub_lo = w[3] < 0.f ? w[3] : w[2];
   const float h_lo = 0.25f * rus * (w[3] + ub_lo);
@@ -142,9 +147,10 @@ __device__ __forceinline__ float fx_u_sel(const float w[6], float rus, int i, in
 }
 // fy_v: rvs = rv(j) + rv(j-1); 5th order on [4, jde-2], upwind-biased second
 // order at the boundary faces 2 and jde
-__device__ __forceinline__ float fy_v_sel(const float w[6], float rvs, int j, int jde) {
+__device__ __forceinline__ float fy_v_sel(const float w[6], float rvs, int j, int jde, bool fast) {
   const float vel = 0.5f * rvs;
   const float h5 = vel * flux5(w[0], w[1], w[2], w[3], w[4], w[5], vel);
+  if (fast) return h5;
   const float h3 = vel * flux3(w[1], w[2], w[3], w[4], vel);
   const float vb_lo = w[3] < 0.f ? w[3] : w[2];
   const float h_lo = 0.25f * rvs * (w[3] + vb_lo);
@@ -153,9 +159,10 @@ __device__ __forceinline__ float fy_v_sel(const float w[6], float rvs, int j, in
   return (j >= 4 && j <= jde - 2) ? h5 : (j == 2 ? h_lo : (j == jde ? h_hi : h3));
 }
 // fx_v: rus = ru(j) + ru(j-1); 5th order on [4, ide-3], centred at 2 and ide-1
-__device__ __forceinline__ float fx_v_sel(const float w[6], float rus, int i, int ide) {
+__device__ __forceinline__ float fx_v_sel(const float w[6], float rus, int i, int ide, bool fast) {
   const float vel = 0.5f * rus;
   const float h5 = vel * flux5(w[0], w[1], w[2], w[3], w[4], w[5], vel);
+  if (fast) return h5;
   const float h2 = 0.25f * rus * (w[3] + w[2]);
   const float h3 = vel * flux3(w[1], w[2], w[3], w[4], vel);
   return (i >= 4 && i <= ide - 3) ? h5 : ((i == 2 || i == ide - 1) ? h2 : h3);
@@ -271,7 +278,7 @@ __global__ void __launch_bounds__(256) k_advect_u(const DevCtx xv, const float *
   {
     float w[6];
     win6b(bq, o, sj, w);
-    const float f = fy_u_sel(w, bld(bv, o) + bld(bv, o - 1u), j, jde);
+    const float f = fy_u_sel(w, bld(bv, o) + bld(bv, o - 1u), j, jde, j >= 4 && j <= jde - 3);
     fs = (adv && j >= 2 && j <= jde - 1) ? f : 0.f;
   }
   ys[wv][lane] = fs;
@@ -303,13 +310,13 @@ __global__ void __launch_bounds__(256) k_advect_u(const DevCtx xv, const float *
     tfl = tendf_load(FL(ru_tendf), FL(u_save), o, rk);
     mt = A2(FL(msfuy), ic, j);
   }
-  const float fw0 = fx_u_sel(wx, rus, i, ide);
+  const float fw0 = fx_u_sel(wx, rus, i, ide, i - lane >= 4 && i - lane + 63 <= ide - 2);
   const float fw = (i >= 2 && i <= ide) ? fw0 : 0.f;
   const float fe = lane_next(fw, lane);
   if (!own) return;
   float fn;
   if (wv < 3) fn = ys[wv + 1][lane];
-  else fn = (adv && j + 1 <= jde - 1) ? fy_u_sel(wn, rvn, j + 1, jde) : 0.f;
+  else fn = (adv && j + 1 <= jde - 1) ? fy_u_sel(wn, rvn, j + 1, jde, j + 1 >= 4 && j + 1 <= jde - 3) : 0.f;
   float t = t0;
   if (adv) {
     if (j >= 2 && j <= jde - 2) t -= msf * x->rdy * (fn - fs);
@@ -346,7 +353,7 @@ __global__ void __launch_bounds__(256) k_advect_v(const DevCtx xv, const float *
   {
     float w[6];
     win6b(bq, o, sj, w);
-    const float f = fy_v_sel(w, bld(bv, o) + bld(bv, o - sj), j, jde);
+    const float f = fy_v_sel(w, bld(bv, o) + bld(bv, o - sj), j, jde, j >= 4 && j <= jde - 2);
     fs = (j >= 2 && j <= jde) ? f : 0.f;
   }
   ys[wv][lane] = fs;
@@ -379,13 +386,13 @@ __global__ void __launch_bounds__(256) k_advect_v(const DevCtx xv, const float *
     mvx = msx;
     mvxi = A2(FL(msfvx_inv), ic, j);
   }
-  const float fw0 = fx_v_sel(wx, rus, i, ide);
+  const float fw0 = fx_v_sel(wx, rus, i, ide, i - lane >= 4 && i - lane + 63 <= ide - 3);
   const float fw = (adv && i >= 2 && i <= ide - 1) ? fw0 : 0.f;
   const float fe = lane_next(fw, lane);
   if (!own) return;
   float fn;
   if (wv < 3) fn = ys[wv + 1][lane];
-  else fn = (j + 1 >= 2 && j + 1 <= jde) ? fy_v_sel(wn, rvn, j + 1, jde) : 0.f;
+  else fn = (j + 1 >= 2 && j + 1 <= jde) ? fy_v_sel(wn, rvn, j + 1, jde, j + 1 >= 4 && j + 1 <= jde - 2) : 0.f;
   float t = t0;
   if (adv) {
     t -= msy * x->rdy * (fn - fs);
@@ -423,7 +430,7 @@ __global__ void __launch_bounds__(256) k_advect_w(const DevCtx xv, const float *
   {
     float wy[6];
     win6b(bq, o, sj, wy);
-    const float f = hflux_sel(wy, rvw_sel(x, bld(bv, o), bld(bv, o - sk), bld(bv, o - 2u * sk), k), j, jde);
+    const float f = hflux_sel(wy, rvw_sel(x, bld(bv, o), bld(bv, o - sk), bld(bv, o - 2u * sk), k), j, jde, j >= 4 && j <= jde - 3);
     fs = (adv && j >= 2 && j <= jde - 1) ? f : 0.f;
   }
   ys[wv][lane] = fs;
@@ -451,13 +458,13 @@ __global__ void __launch_bounds__(256) k_advect_w(const DevCtx xv, const float *
     tfl = tendf_load(FL(rw_tendf), FL(w_save), o, rk);
     mt = A2(FL(msfty), ic, j);
   }
-  const float fw0 = hflux_sel(wx, uw, i, ide);
+  const float fw0 = hflux_sel(wx, uw, i, ide, i - lane >= 4 && i - lane + 63 <= ide - 3);
   const float fw = (adv && i >= 2 && i <= ide - 1) ? fw0 : 0.f;
   const float fe = lane_next(fw, lane);
   if (!own) return;
   float fn;
   if (wv < 3) fn = ys[wv + 1][lane];
-  else fn = (adv && j + 1 <= jde - 1) ? hflux_sel(wn, vn, j + 1, jde) : 0.f;
+  else fn = (adv && j + 1 <= jde - 1) ? hflux_sel(wn, vn, j + 1, jde, j + 1 >= 4 && j + 1 <= jde - 3) : 0.f;
   float t = t0;
   if (adv) {
     if (j >= 2 && j <= jde - 2) t -= msf * x->rdy * (fn - fs);
