@@ -118,37 +118,70 @@ __global__ void k_rk_prep3(const DevCtx xv, R3 r) {
 
 // calc_ww_cp (module_big_step_utilities_em.py:655-803): column kernel over
 // mass columns.  Omega is the bottom-up integral of the coupled horizontal
-// mass divergence minus its column share; the divergence of level k-1 is
-// recomputed in the second pass instead of being held in a per-thread array.
-__device__ __forceinline__ float ww_divv(const DevCtx *__restrict__ x, int i, int k, int j, float muu_i, float muu_ip, float muv_j,
-                                         float muv_jp) {
-  const float *u = FL(u_2), *v = FL(v_2);
-  const float c1 = A1(FL(c1h), k), c2 = A1(FL(c2h), k);
-  float xf = (c1 * muu_ip + c2) * A3(u, i + 1, k, j) / A2(FL(msfuy), i + 1, j) - (c1 * muu_i + c2) * A3(u, i, k, j) / A2(FL(msfuy), i, j);
-  float yf = (c1 * muv_jp + c2) * A3(v, i, k, j + 1) * A2(FL(msfvx_inv), i, j + 1) - (c1 * muv_j + c2) * A3(v, i, k, j) * A2(FL(msfvx_inv), i, j);
-  return A2(FL(msftx), i, j) * A1(FL(dnw), k) * (x->rdx * xf + x->rdy * yf);
-}
-__global__ void k_ww_cp(const DevCtx xv, R2 r) {
+// mass divergence minus its column share.
+// One thread per mass column (block 64): the divergence of every level is
+// formed once, 8 levels of loads at a time, and kept in LDS (dynamic,
+// (kde - 1) x 64 floats); the column sum and the integration then run in the
+// reference's level order, and the integration's stores never hold up a
+// load.
+__global__ void __launch_bounds__(64) k_ww_cp(const DevCtx xv, R2 r) {
   const DevCtx *__restrict__ x = &xv;
+  constexpr int kMaxLev = 152;  // kde <= 151 (wrfdc_create checks nz <= 150)
+  __shared__ float s_c1h[kMaxLev], s_c2h[kMaxLev], s_dnw[kMaxLev];
+  extern __shared__ float s_div_[];  // (kde - 1) x 64: the divergence of every level
+#define s_div(kk) s_div_[(kk) * 64 + lane]
+  const int kde = x->kde, ktf = kde - 1;
+  for (int q = (int)threadIdx.x; q < kde; q += 64) {
+    s_c1h[q] = FL(c1h)[q];
+    s_c2h[q] = FL(c2h)[q];
+    s_dnw[q] = FL(dnw)[q];
+  }
+  __syncthreads();
   GET_IJ(r);
-  const float *mup = FL(mu_2), *mub = FL(mub);
+  const float *mup = FL(mu_2), *mub = FL(mub), *u = FL(u_2), *v = FL(v_2);
   float *ww = FL(ww);
-  const int ktf = x->kde - 1;
+  const int lane = (int)threadIdx.x;
   const int im = (i - 1 < 1) ? i : i - 1, jm = (j - 1 < 1) ? j : j - 1;
   const float muu_i = 0.5f * (A2(mup, i, j) + A2(mub, i, j) + A2(mup, im, j) + A2(mub, im, j));
   const float muu_ip = 0.5f * (A2(mup, i + 1, j) + A2(mub, i + 1, j) + A2(mup, i, j) + A2(mub, i, j));
   const float muv_j = 0.5f * (A2(mup, i, j) + A2(mub, i, j) + A2(mup, i, jm) + A2(mub, i, jm));
   const float muv_jp = 0.5f * (A2(mup, i, j + 1) + A2(mub, i, j + 1) + A2(mup, i, j) + A2(mub, i, j));
+  const float fu1 = A2(FL(msfuy), i + 1, j), fu0 = A2(FL(msfuy), i, j), fv1 = A2(FL(msfvx_inv), i, j + 1), fv0 = A2(FL(msfvx_inv), i, j);
+  const float mtx = A2(FL(msftx), i, j);
+  const unsigned sk = (unsigned)x->ip, sj = (unsigned)(kde * x->ip), o1 = XI3(i, 1, j);
   float dmdt = 0.f;
-  for (int k = 1; k <= ktf; k++) dmdt += ww_divv(x, i, k, j, muu_i, muu_ip, muv_j, muv_jp);
+  for (int k0 = 1; k0 <= ktf; k0 += 8) {
+    float ue[8], uw[8], vn[8], vs[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const unsigned o = o1 + (unsigned)(imin_(k0 + q, ktf) - 1) * sk;
+      ue[q] = elem(u, o + 1u);
+      uw[q] = elem(u, o);
+      vn[q] = elem(v, o + sj);
+      vs[q] = elem(v, o);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int k = k0 + q;
+      if (k > ktf) break;
+      // ww_divv with the loaded values (same arithmetic)
+      const float c1 = s_c1h[k - 1], c2 = s_c2h[k - 1];
+      const float xf = (c1 * muu_ip + c2) * ue[q] / fu1 - (c1 * muu_i + c2) * uw[q] / fu0;
+      const float yf = (c1 * muv_jp + c2) * vn[q] * fv1 - (c1 * muv_j + c2) * vs[q] * fv0;
+      const float dv = mtx * s_dnw[k - 1] * (x->rdx * xf + x->rdy * yf);
+      s_div(k - 1) = dv;
+      dmdt += dv;
+    }
+  }
   A3(ww, i, 1, j) = 0.f;
   float prev = 0.f;
   for (int k = 2; k <= ktf; k++) {
-    const float cur = prev - A1(FL(dnw), k - 1) * A1(FL(c1h), k - 1) * dmdt - ww_divv(x, i, k - 1, j, muu_i, muu_ip, muv_j, muv_jp);
-    A3(ww, i, k, j) = cur;
+    const float cur = prev - s_dnw[k - 2] * s_c1h[k - 2] * dmdt - s_div(k - 2);
+    elem(ww, o1 + (unsigned)(k - 1) * sk) = cur;
     prev = cur;
   }
   A3(ww, i, x->kde, j) = 0.f;
+#undef s_div
 }
 
 // rhs_ph (module_big_step_utilities_em.py:2386-2794): vertical advection of

@@ -149,42 +149,72 @@ __global__ void k_calc_p_rho(const DevCtx xv, int step, R3 r) {
 // implicit w-phi system, one thread per mass column.
 __global__ void k_calc_coef_w(const DevCtx xv, float dts, R2 r) {
   const DevCtx *__restrict__ x = &xv;
+  // the column coefficients in LDS: read per level inside the recurrence, and
+  // from global memory every such read would wait behind the previous level's
+  // stores (the compiler cannot prove they do not alias)
+  constexpr int kMaxLev = 152;  // kde <= 151 (wrfdc_create checks nz <= 150)
+  __shared__ float c1h[kMaxLev], c2h[kMaxLev], c1f[kMaxLev], c2f[kMaxLev], rdn[kMaxLev], rdnw[kMaxLev];
+  const int kde = x->kde;
+  for (int q = (int)threadIdx.x; q < kde; q += (int)blockDim.x) {
+    c1h[q] = FL(c1h)[q];
+    c2h[q] = FL(c2h)[q];
+    c1f[q] = FL(c1f)[q];
+    c2f[q] = FL(c2f)[q];
+    rdn[q] = FL(rdn)[q];
+    rdnw[q] = FL(rdnw)[q];
+  }
+  __syncthreads();
   GET_IJ(r);
   float *a = FL(a), *alpha = FL(alpha), *gamma = FL(gamma);
-  const float *cqw = FL(cqw), *c2a = FL(c2a), *c1h = FL(c1h), *c2h = FL(c2h), *c1f = FL(c1f), *c2f = FL(c2f);
-  const float *rdn = FL(rdn), *rdnw = FL(rdnw);
-  const int kde = x->kde;
+  const float *cqw = FL(cqw), *c2a = FL(c2a);
   const float lid = x->top_lid ? 0.f : 1.f;
   float cof = 0.5f * dts * x->g * (1.f + x->epssm);
   cof = cof * cof;
   const float m = A2(FL(mut), i, j);
-  A3(a, i, 2, j) = 0.f;
-  A3(a, i, kde, j) = (-2.f * cof * (A1(rdnw, kde - 1) * A1(rdnw, kde - 1)) * A3(c2a, i, kde - 1, j) * lid) /
+  const float c2a_top = A3(c2a, i, kde - 1, j);
+  const float atop = (-2.f * cof * (A1(rdnw, kde - 1) * A1(rdnw, kde - 1)) * c2a_top * lid) /
                      ((A1(c1h, kde - 1) * m + A1(c2h, kde - 1)) * (A1(c1f, kde - 1) * m + A1(c2f, kde - 1)));
+  A3(a, i, 2, j) = 0.f;
+  A3(a, i, kde, j) = atop;
   A3(gamma, i, 1, j) = 0.f;
   float gprev = 0.f;
-  for (int k = 2; k <= kde - 1; k++) {
-    const float mh = A1(c1h, k) * m + A1(c2h, k), mf = A1(c1f, k) * m + A1(c2f, k), mhb = A1(c1h, k - 1) * m + A1(c2h, k - 1);
-    const float mfa = A1(c1f, k + 1) * m + A1(c2f, k + 1);
-    float ak = 0.f;
-    if (k >= 3) {
-      ak = (-A3(cqw, i, k, j) * cof * A1(rdn, k) * A1(rdnw, k - 1) * A3(c2a, i, k - 1, j)) /
-           ((A1(c1h, k - 1) * m + A1(c2h, k - 1)) * (A1(c1f, k - 1) * m + A1(c2f, k - 1)));
-      A3(a, i, k, j) = ak;
+  // levels in chunks of 8: the chunk's cqw and c2a are read before its
+  // stores (which the compiler cannot move later loads past), so a column
+  // waits for memory once per chunk instead of once per level
+  for (int k0 = 2; k0 <= kde - 1; k0 += 8) {
+    float r_cqw[8], r_c2a[8], r_c2am[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int k = imin_(k0 + q, kde - 1);
+      r_cqw[q] = A3(cqw, i, k, j);
+      r_c2a[q] = A3(c2a, i, k, j);
+      r_c2am[q] = A3(c2a, i, k - 1, j);
     }
-    const float bu = (A1(rdnw, k) * A3(c2a, i, k, j)) / (mh * mf);
-    const float bl = (A1(rdnw, k - 1) * A3(c2a, i, k - 1, j)) / (mhb * mf);
-    const float b = 1.f + A3(cqw, i, k, j) * cof * A1(rdn, k) * (bu + bl);
-    const float c = (-A3(cqw, i, k, j) * cof * A1(rdn, k) * A1(rdnw, k) * A3(c2a, i, k, j)) / (mh * mfa);
-    const float al_ = 1.f / (b - ak * gprev);
-    const float ga = c * al_;
-    A3(alpha, i, k, j) = al_;
-    A3(gamma, i, k, j) = ga;
-    gprev = ga;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int k = k0 + q;
+      if (k > kde - 1) break;
+      const float mh = A1(c1h, k) * m + A1(c2h, k), mf = A1(c1f, k) * m + A1(c2f, k), mhb = A1(c1h, k - 1) * m + A1(c2h, k - 1);
+      const float mfa = A1(c1f, k + 1) * m + A1(c2f, k + 1);
+      float ak = 0.f;
+      if (k >= 3) {
+        ak = (-r_cqw[q] * cof * A1(rdn, k) * A1(rdnw, k - 1) * r_c2am[q]) / ((A1(c1h, k - 1) * m + A1(c2h, k - 1)) * (A1(c1f, k - 1) * m + A1(c2f, k - 1)));
+        A3(a, i, k, j) = ak;
+      }
+      const float bu = (A1(rdnw, k) * r_c2a[q]) / (mh * mf);
+      const float bl = (A1(rdnw, k - 1) * r_c2am[q]) / (mhb * mf);
+      const float b = 1.f + r_cqw[q] * cof * A1(rdn, k) * (bu + bl);
+      const float c = (-r_cqw[q] * cof * A1(rdn, k) * A1(rdnw, k) * r_c2a[q]) / (mh * mfa);
+      const float al_ = 1.f / (b - ak * gprev);
+      const float ga = c * al_;
+      A3(alpha, i, k, j) = al_;
+      A3(gamma, i, k, j) = ga;
+      gprev = ga;
+    }
   }
-  const float btop = 1.f + (2.f * cof * (A1(rdnw, kde - 1) * A1(rdnw, kde - 1)) * A3(c2a, i, kde - 1, j)) /
+  const float btop = 1.f + (2.f * cof * (A1(rdnw, kde - 1) * A1(rdnw, kde - 1)) * c2a_top) /
                                ((A1(c1h, kde - 1) * m + A1(c2h, kde - 1)) * (A1(c1f, kde) * m + A1(c2f, kde)));
-  A3(alpha, i, kde, j) = 1.f / (btop - A3(a, i, kde, j) * gprev);
+  A3(alpha, i, kde, j) = 1.f / (btop - atop * gprev);
   A3(gamma, i, kde, j) = 0.f;
 }
 

@@ -326,7 +326,7 @@ struct Run {
     R3 r = full3();
     k_rk_prep3<<<g3(r), b3(), 0, s>>>(d, r);
     R2 c = mass2();
-    k_ww_cp<<<g2(c), 64, 0, s>>>(d, c);
+    k_ww_cp<<<g2(c), 64, sizeof(float) * 64 * (size_t)(kde - 1), s>>>(d, c);
   }
   void first_rk_step_part1() const {
     const size_t n3 = X->n3;
