@@ -155,7 +155,19 @@ __global__ void k_zero_grad_cols(const DevCtx xv, float *w) {
 // ---- relaxation zone (relax_bdytend_core.py:27-294) -----------------------
 // blockIdx.z = side*(rz-1) + (bd-1), side 0..3 = south, north, west, east.
 // fld is the mass-coupled field, tend receives the increment.
-__global__ void k_relax(const DevCtx xv, const float *fld, float *tend, int ibe, int jbe, int kt, int is2d,
+// the relaxed field at one point: the field itself (mw 0), or mass_weight's
+// coupled value (module_bc_em.py:561-624) formed on the spot, the field times
+// c1*mut + c2 (mw 1: half levels, c1h/c2h, k <= kde-1; mw 2: w levels,
+// c1f/c2f), zero outside the coupled range
+__device__ __forceinline__ float relax_fv(const DevCtx *__restrict__ x, const float *fld, int ii, int k, int jj, int is2d, int mw) {
+  if (is2d) return A2(fld, ii, jj);
+  const float f = A3(fld, ii, k, jj);
+  if (mw == 0) return f;
+  const bool in = (ii <= x->ide - 1 && jj <= x->jde - 1 && (mw == 2 || k <= x->kde - 1));
+  const float *c1 = mw == 2 ? FL(c1f) : FL(c1h), *c2 = mw == 2 ? FL(c2f) : FL(c2h);
+  return in ? f * (A1(c1, k) * A2(FL(mut), ii, jj) + A1(c2, k)) : 0.f;
+}
+__global__ void k_relax(const DevCtx xv, const float *fld, float *tend, int ibe, int jbe, int kt, int is2d, int mw,
                         BPtrs b, BPtrs bt, float dt_bc) {
   const DevCtx *__restrict__ x = &xv;
   const int a = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -183,7 +195,7 @@ __global__ void k_relax(const DevCtx xv, const float *fld, float *tend, int ibe,
     along = j;
   }
   (void)along;
-#define FV(ii_, jj_) (is2d ? A2(fld, ii_, jj_) : A3(fld, ii_, k, jj_))
+#define FV(ii_, jj_) relax_fv(x, fld, ii_, k, jj_, is2d, mw)
 #define BD(e_, a_, bb_) (bval(x, b.e, e_, a_, k, bb_, is2d) + dt_bc * bval(x, bt.e, e_, a_, k, bb_, is2d))
   float f0, f1, f2, f3, f4;
   if (side < 2) {

@@ -30,20 +30,6 @@
 // small helper kernels local to the orchestration
 // ---------------------------------------------------------------------------
 
-// field times its column mass (mass_weight, module_bc_em.py:561-624), zero
-// outside the coupled range; full extent.  stag_w selects w levels and c1f/c2f.
-__global__ void k_mass_weight(const DevCtx xv, const float *fl, float *out, int stag_w, R3 r) {
-  const DevCtx *__restrict__ x = &xv;
-  GET_IKJ_P(r);
-  const bool in = (i <= x->ide - 1 && j <= x->jde - 1 && (stag_w || k <= x->kde - 1));
-  float v = 0.f;
-  if (in) {
-    const float *c1 = stag_w ? FL(c1f) : FL(c1h), *c2 = stag_w ? FL(c2f) : FL(c2h);
-    v = A3(fl, i, k, j) * (A1(c1, k) * A2(FL(mut), i, j) + A1(c2, k));
-  }
-  A3(out, i, k, j) = v;
-}
-
 // dst[c] = src[c] over the points of r, for every pair of the list in one
 // launch (initialize_moist_old, q_old <- q, the species buffer copies).
 // Block (64, 4): 256 consecutive i of 4 levels; grid (i segments x level
@@ -301,11 +287,14 @@ struct Run {
     Z z = zd(var, var == 'm' || var == 'h', is2d);
     k_spec_update<<<dim3(cdiv(zone_n(z), 64), z.kt), 64, 0, s>>>(d, fl, tend, (float)dt, z.ibe, z.jbe, z.kt, is2d ? 1 : 0);
   }
-  void relax(const float *fld, float *tend, char var, bool ktop, bool is2d, BPtrs b, BPtrs bt) const {
+  // relax_bdytend of fld (mw 0), or of its mass-weighted value (mw 1: half
+  // levels, 2: w levels), formed by the relaxation kernel on the zone points
+  void relax(const float *fld, float *tend, char var, bool ktop, bool is2d, BPtrs b, BPtrs bt, int mw = 0) const {
     Z z = zd(var, ktop, is2d);
     const int rz = X->c.relax_zone;
     const int along = z.ibe > z.jbe ? z.ibe : z.jbe;
-    k_relax<<<dim3(cdiv(along, 64), z.kt, 4 * (rz - 1)), 64, 0, s>>>(d, fld, tend, z.ibe, z.jbe, z.kt, is2d ? 1 : 0, b, bt, (float)X->dtbc);
+    k_relax<<<dim3(cdiv(along, 64), z.kt, 4 * (rz - 1)), 64, 0, s>>>(d, fld, tend, z.ibe, z.jbe, z.kt, is2d ? 1 : 0, mw, b, bt,
+                                                                   (float)X->dtbc);
   }
   void spec_final(float *fl, char var, bool is2d, const float *mu, const float *msf, const float *c1, const float *c2, BPtrs b,
                   BPtrs bt) const {
@@ -508,12 +497,9 @@ struct Run {
   void lateral_relax() const {
     relax(F(F_ru), F(F_ru_tendf), 'u', false, false, bp(F_u_bys, F_u_bye, F_u_bxs, F_u_bxe), bp(F_u_btys, F_u_btye, F_u_btxs, F_u_btxe));
     relax(F(F_rv), F(F_rv_tendf), 'v', false, false, bp(F_v_bys, F_v_bye, F_v_bxs, F_v_bxe), bp(F_v_btys, F_v_btye, F_v_btxs, F_v_btxe));
-    float *tmp = Scr(S_HAT);
-    R3 r = full3();
-    k_mass_weight<<<g3p(r), b3(), 0, s>>>(d, F(F_ph_2), tmp, 1, r);
-    relax(tmp, F(F_ph_tendf), 'h', true, false, bp(F_ph_bys, F_ph_bye, F_ph_bxs, F_ph_bxe), bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe));
-    k_mass_weight<<<g3p(r), b3(), 0, s>>>(d, F(F_t_2), tmp, 0, r);
-    relax(tmp, F(F_t_tendf), 't', false, false, bp(F_t_bys, F_t_bye, F_t_bxs, F_t_bxe), bp(F_t_btys, F_t_btye, F_t_btxs, F_t_btxe));
+    relax(F(F_ph_2), F(F_ph_tendf), 'h', true, false, bp(F_ph_bys, F_ph_bye, F_ph_bxs, F_ph_bxe), bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe),
+          2);
+    relax(F(F_t_2), F(F_t_tendf), 't', false, false, bp(F_t_bys, F_t_bye, F_t_bxs, F_t_bxe), bp(F_t_btys, F_t_btye, F_t_btxs, F_t_btxe), 1);
     relax(F(F_mu_2), F(F_mu_tend), 'm', true, true, bp(F_mu_bys, F_mu_bye, F_mu_bxs, F_mu_bxe), bp(F_mu_btys, F_mu_btye, F_mu_btxs, F_mu_btxe));
   }
   // spec_bdy_dry: the specified zone's tendencies
@@ -610,10 +596,7 @@ struct Run {
     else advect_scalar(q, adv, F(F_ru_m), F(F_rv_m), F(F_ww_m));
     R3 m = mass3();
     if (is_qv && X->rk == 1) {
-      float *tmp = Scr(S_HAT);
-      R3 r = full3();
-      k_mass_weight<<<g3p(r), b3(), 0, s>>>(d, q, tmp, 0, r);
-      relax(tmp, q_tend, 'q', false, false, bp(F_qv_bys, F_qv_bye, F_qv_bxs, F_qv_bxe), bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe));
+      relax(q, q_tend, 'q', false, false, bp(F_qv_bys, F_qv_bye, F_qv_bxs, F_qv_bxe), bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe), 1);
       spec_overlay(q_tend, 't', false, false, bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe));
     }
     k_update_scalar<<<g3p(m), b3(), 0, s>>>(d, q, q_old, q_tend, X->rk, (float)X->dt_rk, m);
@@ -691,10 +674,8 @@ struct Run {
       if (!old_synced) sync_old(true, true);
       old_synced = true;
       // relax_bdy_scalar + spec_bdy_scalar of QV into its tendency
-      float *tmp = Scr(S_HAT);
-      R3 r = full3();
-      k_mass_weight<<<g3p(r), b3(), 0, s>>>(d, Sp(F_moist, 0), tmp, 0, r);
-      relax(tmp, Sp(F_moist_tend, 0), 'q', false, false, bp(F_qv_bys, F_qv_bye, F_qv_bxs, F_qv_bxe), bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe));
+      relax(Sp(F_moist, 0), Sp(F_moist_tend, 0), 'q', false, false, bp(F_qv_bys, F_qv_bye, F_qv_bxs, F_qv_bxe),
+            bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe), 1);
       spec_overlay(Sp(F_moist_tend, 0), 't', false, false, bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe));
     }
     if (rk == 3 && !species_in_alt()) species_to(true);

@@ -58,7 +58,7 @@ enum {
 };
 
 // Library-owned scratch arrays (3-D, same layout as fields).
-enum { S_T1, S_T2, S_FQX, S_FQY, S_FQZ, S_FQXL, S_FQYL, S_FQZL, S_SCALE, S_HAT, S_HAT2, S_VXGM, NSCRATCH };
+enum { S_T1, S_T2, S_FQX, S_FQY, S_FQZ, S_FQXL, S_FQYL, S_FQZL, S_SCALE, S_HAT2, S_VXGM, NSCRATCH };
 
 // View of one domain (dimensions, scalars, pointers), passed to every kernel
 // by value (about 1.7 KB of kernel arguments).
