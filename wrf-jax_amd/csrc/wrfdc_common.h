@@ -179,14 +179,18 @@ __device__ __forceinline__ float sgn1(float v) { return v >= 0.f ? 1.f : -1.f; }
 // x / c correctly rounded (IEEE RN), for an integer constant c (12 or 60)
 // with rc = RN(1/c).  Checked bitwise against x / c for every finite FP32 x,
 // for c = 12 and c = 60.
-//  - A quotient of at least 2^-125 is one FMA correction of x * rc.  The
-//    sign is taken from x, which only changes the result for x = -0.
-//  - A nonzero x below c * 2^-125 has X = |x| * 2^149, an integer below
-//    2^30, and a quotient n * 2^-149 with n = X / c rounded to nearest, ties
-//    to even (n <= 2^24, so exact).  n comes from two rounded estimates,
-//    each corrected by the exact remainder X - n * c (one FMA), then the
-//    half-way cases; every constant is an inline operand or c / rc, so the
-//    rare path takes no scalar registers of its own.
+//  - The quotient is x * rc with one FMA correction: t = x * rc * c - x
+//    rounded once, then x * rc - t * rc rounded once.  Written with t (not
+//    x - q * c), x = -0 gives -0 with no sign fix-up.
+//  - That is exact wherever its result is not subnormal, and a subnormal
+//    result (one class test) sends the lane to the rare path, which also
+//    covers every x whose true quotient is subnormal.
+//  - The rare path: a nonzero x below c * 2^-125 has X = |x| * 2^149, an
+//    integer below 2^30, and a quotient n * 2^-149 with n = X / c rounded to
+//    nearest, ties to even (n <= 2^24, so exact).  n comes from two rounded
+//    estimates, each corrected by the exact remainder X - n * c (one FMA),
+//    then the half-way cases; every constant is an inline operand or c / rc,
+//    so the rare path takes no scalar registers of its own.
 __device__ __forceinline__ float div_tiny(float xv, float c, float rc) {
   const float X = __builtin_ldexpf(__builtin_ldexpf(__builtin_ldexpf(fabsf(xv), 64), 64), 21);
   float n = __builtin_rintf(X * rc);
@@ -201,9 +205,9 @@ __device__ __forceinline__ float div_tiny(float xv, float c, float rc) {
 }
 __device__ __forceinline__ float div_const(float xv, float c, float rc) {
   const float q = xv * rc;
-  const float e = __builtin_fmaf(-q, c, xv);
-  float q2 = copysignf(__builtin_fmaf(e, rc, q), xv);
-  if (__builtin_expect(fabsf(xv) < c * 0x1p-125f && xv != 0.f, 0)) q2 = div_tiny(xv, c, rc);
+  const float t = __builtin_fmaf(q, c, -xv);
+  float q2 = __builtin_fmaf(-t, rc, q);
+  if (__builtin_expect(__builtin_isfpclass(q2, 0x0090), 0)) q2 = div_tiny(xv, c, rc);  // +-subnormal
   return q2;
 }
 __device__ __forceinline__ int imin_(int a, int b) { return a < b ? a : b; }
