@@ -250,7 +250,7 @@ __global__ void __launch_bounds__(256) k_advect_scalar(const DevCtx xv, const fl
   if (j >= 2 && j <= jde - 2) t -= msf * x->rdy * (fn - fs);
   if (ic >= 2 && ic <= ide - 2) t -= msf * x->rdx * (fe - fw);
   t -= rdz * (zflux_br(wz[1], wz[2], wz[3], wz[4], wt, k + 1, kde, fm1, fp1) - zflux_br(wz[0], wz[1], wz[2], wz[3], wb, k, kde, fm0, fp0));
-  if (TEND) t = addtend_t(x, ic, k, j, rk, t);
+  if (TEND) t = addtend_t(x, ic, k, j, rk, t, rk > 1);
   elem(tend, o) = t;
 }
 

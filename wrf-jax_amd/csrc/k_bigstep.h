@@ -526,13 +526,15 @@ __device__ __forceinline__ float addtend_ph(const DevCtx *__restrict__ x, int i,
   return t + tendf_fold(FL(ph_tendf), o, tendf_load(FL(ph_tendf), FL(ph_save), o, rk), 1.f, rk) / m;
 }
 // theta: also the diabatic heating, and the column's mu tendency at k = 1
-__device__ __forceinline__ float addtend_t(const DevCtx *__restrict__ x, int i, int k, int j, int rk, float t) {
+// mu_zero: mu_tend is known to be 0 here (a later stage of the fused step,
+// which does not clear it) and is formed as 0 + mu_tendf
+__device__ __forceinline__ float addtend_t(const DevCtx *__restrict__ x, int i, int k, int j, int rk, float t, bool mu_zero = false) {
   const unsigned o = XI3(i, k, j);
   const float m = A2(FL(msfty), i, j);
   const float mass = A1(FL(c1h), k) * A2(FL(mut), i, j) + A1(FL(c2h), k);
   const float tf = tendf_fold(FL(t_tendf), o, tendf_load(FL(t_tendf), FL(t_save), o, rk), 1.f, rk);
   const float heating = mass * elem(FL(h_diabatic), o) / m;
-  if (k == 1) A2(FL(mu_tend), i, j) = A2(FL(mu_tend), i, j) + A2(FL(mu_tendf), i, j);
+  if (k == 1) A2(FL(mu_tend), i, j) = (mu_zero ? 0.f : A2(FL(mu_tend), i, j)) + A2(FL(mu_tendf), i, j);
   return t + tf / m + heating;
 }
 // comp 0 u, 1 v, 2 w/ph, 3 theta

@@ -445,20 +445,20 @@ struct Run {
     k_cor_curv<<<g3(rw), b3(), 0, s>>>(d, 2, rw);
   }
   // the RK tendencies and the saved tendencies start from zero each stage.
-  // saves = false (later stages of the step): the saved tendencies keep
-  // their values: nothing reads them before small_step_prep writes them
-  // again over the same points, and nothing writes the points outside (zero
-  // since the first stage's clear) - the first stage's addtend fold is their
-  // only earlier reader.
-  void clear_tendencies(bool saves) const {
+  // The first stage clears every tendency and saved tendency.  Later stages
+  // of the fused step (first = false) clear nothing: the fused tendency
+  // kernels write every point of their ranges from 0 (the points outside
+  // stay 0 from the first stage's clear), the zone overlay writes its
+  // points, mu_tend is formed as 0 + mu_tendf by the theta kernel at rk > 1,
+  // and the saved tendencies keep their values - nothing reads them before
+  // small_step_prep writes them again over the same points, and the first
+  // stage's addtend fold is their only earlier reader.
+  void clear_tendencies(bool first) const {
     const size_t n3 = X->n3, n2 = X->n2;
-    if (saves)
+    if (first)
       fill_many({{F(F_ru_tend), n3}, {F(F_rv_tend), n3}, {F(F_rw_tend), n3}, {F(F_t_tend), n3}, {F(F_ph_tend), n3},
                  {F(F_u_save), n3}, {F(F_v_save), n3}, {F(F_w_save), n3}, {F(F_ph_save), n3}, {F(F_t_save), n3},
                  {F(F_mu_tend), n2}, {F(F_mu_save), n2}});
-    else
-      fill_many({{F(F_ru_tend), n3}, {F(F_rv_tend), n3}, {F(F_rw_tend), n3}, {F(F_t_tend), n3}, {F(F_ph_tend), n3},
-                 {F(F_mu_tend), n2}});
   }
   void rk_tendency() const {
     clear_tendencies(true);
