@@ -80,7 +80,7 @@ __device__ __forceinline__ bool zone_point(int t, int ibe, int jbe, int &i, int 
   if (t < nj) { i = ibe; j = 2 + t; edge = 3; return true; }
   return false;
 }
-__device__ __forceinline__ float bval(const DevCtx *__restrict__ x, const float *b4[4], int e, int a, int k, int b, int is2d) {
+__device__ __forceinline__ float bval(const DevCtx *__restrict__ x, const float *const b4[4], int e, int a, int k, int b, int is2d) {
   if (is2d) return (e < 2) ? BY2(b4[e], a, b) : BX2(b4[e], a, b);
   return (e < 2) ? BY(b4[e], a, k, b) : BX(b4[e], a, k, b);
 }
@@ -99,6 +99,28 @@ __global__ void k_spec_overlay(const DevCtx xv, float *tend, int ibe, int jbe, i
   float v = bval(x, bt.e, e, a, k, 1, is2d);
   if (is2d) A2(tend, i, j) = v;
   else A3(tend, i, k, j) = v;
+}
+
+// several overlays in one launch (blockIdx.z selects the field): the five
+// dry tendencies of one stage
+constexpr int kOverlayMax = 5;
+struct OverlayList {
+  float *tend[kOverlayMax];
+  BPtrs bt[kOverlayMax];
+  int ibe[kOverlayMax], jbe[kOverlayMax], kt[kOverlayMax], is2d[kOverlayMax];
+};
+__global__ void k_spec_overlay_batch(const DevCtx xv, OverlayList L) {
+  const DevCtx *__restrict__ x = &xv;
+  const int f = (int)blockIdx.z;
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int k = 1 + (int)blockIdx.y;
+  int i, j, e;
+  if (k > L.kt[f] || !zone_point(t, L.ibe[f], L.jbe[f], i, j, e)) return;
+  const int a = (e < 2) ? i : j;
+  const int is2d = L.is2d[f];
+  const float v = bval(x, L.bt[f].e, e, a, k, 1, is2d);
+  if (is2d) A2(L.tend[f], i, j) = v;
+  else A3(L.tend[f], i, k, j) = v;
 }
 
 // spec_bdyupdate: field += dt * tend on the zone
@@ -153,7 +175,7 @@ __global__ void k_zero_grad_cols(const DevCtx xv, float *w) {
 }
 
 // ---- relaxation zone (relax_bdytend_core.py:27-294) -----------------------
-// blockIdx.z = side*(rz-1) + (bd-1), side 0..3 = south, north, west, east.
+// side 0..3 = south, north, west, east; bd = 1 .. rz-1 rows into the zone.
 // fld is the mass-coupled field, tend receives the increment.
 // the relaxed field at one point: the field itself (mw 0), or mass_weight's
 // coupled value (module_bc_em.py:561-624) formed on the spot, the field times
@@ -167,14 +189,27 @@ __device__ __forceinline__ float relax_fv(const DevCtx *__restrict__ x, const fl
   const float *c1 = mw == 2 ? FL(c1f) : FL(c1h), *c2 = mw == 2 ? FL(c2f) : FL(c2h);
   return in ? f * (A1(c1, k) * A2(FL(mut), ii, jj) + A1(c2, k)) : 0.f;
 }
-__global__ void k_relax(const DevCtx xv, const float *fld, float *tend, int ibe, int jbe, int kt, int is2d, int mw,
-                        BPtrs b, BPtrs bt, float dt_bc) {
+// the fields of one launch: blockIdx.z = field * 4*(rz-1) + side*(rz-1) + bd-1
+constexpr int kRelaxMax = 5;
+struct RelaxList {
+  const float *fld[kRelaxMax];
+  float *tend[kRelaxMax];
+  BPtrs b[kRelaxMax], bt[kRelaxMax];
+  int ibe[kRelaxMax], jbe[kRelaxMax], kt[kRelaxMax], is2d[kRelaxMax], mw[kRelaxMax];
+};
+__global__ void k_relax(const DevCtx xv, RelaxList R, float dt_bc) {
   const DevCtx *__restrict__ x = &xv;
   const int a = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = 1 + (int)blockIdx.y;
   const int rz = x->relax_zone;
-  const int side = (int)blockIdx.z / (rz - 1);
-  const int bd = 1 + (int)blockIdx.z % (rz - 1);
+  const int zs = 4 * (rz - 1);
+  const int fi = (int)blockIdx.z / zs, zz = (int)blockIdx.z % zs;
+  const float *fld = R.fld[fi];
+  float *tend = R.tend[fi];
+  const int ibe = R.ibe[fi], jbe = R.jbe[fi], kt = R.kt[fi], is2d = R.is2d[fi], mw = R.mw[fi];
+  const BPtrs &b = R.b[fi], &bt = R.bt[fi];
+  const int side = zz / (rz - 1);
+  const int bd = 1 + zz % (rz - 1);
   if (k > kt) return;
   int i, j, iin, iout, jin, jout, along;
   if (side < 2) {

@@ -289,12 +289,40 @@ struct Run {
   }
   // relax_bdytend of fld (mw 0), or of its mass-weighted value (mw 1: half
   // levels, 2: w levels), formed by the relaxation kernel on the zone points
+  // (RelaxBatch: several fields in one launch)
+  struct RelaxBatch {
+    const Run *r;
+    RelaxList L;
+    int n = 0, along = 1, kmax = 1;
+    explicit RelaxBatch(const Run *run) : r(run) { memset(&L, 0, sizeof(L)); }
+    RelaxBatch &add(const float *fld, float *tend, char var, bool ktop, bool is2d, BPtrs b, BPtrs bt, int mw = 0) {
+      if (n == kRelaxMax) flush();
+      const Z z = r->zd(var, ktop, is2d);
+      L.fld[n] = fld;
+      L.tend[n] = tend;
+      L.b[n] = b;
+      L.bt[n] = bt;
+      L.ibe[n] = z.ibe;
+      L.jbe[n] = z.jbe;
+      L.kt[n] = z.kt;
+      L.is2d[n] = is2d ? 1 : 0;
+      L.mw[n] = mw;
+      along = std::max(along, std::max(z.ibe, z.jbe));
+      kmax = std::max(kmax, z.kt);
+      n++;
+      return *this;
+    }
+    void flush() {
+      if (n == 0) return;
+      const int rz = r->X->c.relax_zone;
+      k_relax<<<dim3(cdiv(along, 64), kmax, n * 4 * (rz - 1)), 64, 0, r->s>>>(r->d, L, (float)r->X->dtbc);
+      n = 0;
+      along = kmax = 1;
+    }
+    ~RelaxBatch() { flush(); }
+  };
   void relax(const float *fld, float *tend, char var, bool ktop, bool is2d, BPtrs b, BPtrs bt, int mw = 0) const {
-    Z z = zd(var, ktop, is2d);
-    const int rz = X->c.relax_zone;
-    const int along = z.ibe > z.jbe ? z.ibe : z.jbe;
-    k_relax<<<dim3(cdiv(along, 64), z.kt, 4 * (rz - 1)), 64, 0, s>>>(d, fld, tend, z.ibe, z.jbe, z.kt, is2d ? 1 : 0, mw, b, bt,
-                                                                   (float)X->dtbc);
+    RelaxBatch(this).add(fld, tend, var, ktop, is2d, b, bt, mw);
   }
   void spec_final(float *fl, char var, bool is2d, const float *mu, const float *msf, const float *c1, const float *c2, BPtrs b,
                   BPtrs bt) const {
@@ -495,20 +523,39 @@ struct Run {
   }
   // relax_bdy_dry at the first stage (physics tendencies of u, v, ph, theta, mu)
   void lateral_relax() const {
-    relax(F(F_ru), F(F_ru_tendf), 'u', false, false, bp(F_u_bys, F_u_bye, F_u_bxs, F_u_bxe), bp(F_u_btys, F_u_btye, F_u_btxs, F_u_btxe));
-    relax(F(F_rv), F(F_rv_tendf), 'v', false, false, bp(F_v_bys, F_v_bye, F_v_bxs, F_v_bxe), bp(F_v_btys, F_v_btye, F_v_btxs, F_v_btxe));
-    relax(F(F_ph_2), F(F_ph_tendf), 'h', true, false, bp(F_ph_bys, F_ph_bye, F_ph_bxs, F_ph_bxe), bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe),
-          2);
-    relax(F(F_t_2), F(F_t_tendf), 't', false, false, bp(F_t_bys, F_t_bye, F_t_bxs, F_t_bxe), bp(F_t_btys, F_t_btye, F_t_btxs, F_t_btxe), 1);
-    relax(F(F_mu_2), F(F_mu_tend), 'm', true, true, bp(F_mu_bys, F_mu_bye, F_mu_bxs, F_mu_bxe), bp(F_mu_btys, F_mu_btye, F_mu_btxs, F_mu_btxe));
+    RelaxBatch(this)
+        .add(F(F_ru), F(F_ru_tendf), 'u', false, false, bp(F_u_bys, F_u_bye, F_u_bxs, F_u_bxe), bp(F_u_btys, F_u_btye, F_u_btxs, F_u_btxe))
+        .add(F(F_rv), F(F_rv_tendf), 'v', false, false, bp(F_v_bys, F_v_bye, F_v_bxs, F_v_bxe), bp(F_v_btys, F_v_btye, F_v_btxs, F_v_btxe))
+        .add(F(F_ph_2), F(F_ph_tendf), 'h', true, false, bp(F_ph_bys, F_ph_bye, F_ph_bxs, F_ph_bxe),
+             bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe), 2)
+        .add(F(F_t_2), F(F_t_tendf), 't', false, false, bp(F_t_bys, F_t_bye, F_t_bxs, F_t_bxe), bp(F_t_btys, F_t_btye, F_t_btxs, F_t_btxe), 1)
+        .add(F(F_mu_2), F(F_mu_tend), 'm', true, true, bp(F_mu_bys, F_mu_bye, F_mu_bxs, F_mu_bxe),
+             bp(F_mu_btys, F_mu_btye, F_mu_btxs, F_mu_btxe));
   }
   // spec_bdy_dry: the specified zone's tendencies
+  // spec_bdytend of u, v, ph, theta and mu in one launch
   void spec_overlays() const {
-    spec_overlay(F(F_ru_tend), 'u', false, false, bp(F_u_btys, F_u_btye, F_u_btxs, F_u_btxe));
-    spec_overlay(F(F_rv_tend), 'v', false, false, bp(F_v_btys, F_v_btye, F_v_btxs, F_v_btxe));
-    spec_overlay(F(F_ph_tend), 'h', true, false, bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe));
-    spec_overlay(F(F_t_tend), 't', false, false, bp(F_t_btys, F_t_btye, F_t_btxs, F_t_btxe));
-    spec_overlay(F(F_mu_tend), 'm', true, true, bp(F_mu_btys, F_mu_btye, F_mu_btxs, F_mu_btxe));
+    OverlayList L;
+    memset(&L, 0, sizeof(L));
+    int n = 0, nmax = 0, kmax = 1;
+    auto add = [&](float *tend, char var, bool ktop, bool is2d, BPtrs bt) {
+      const Z z = zd(var, ktop, is2d);
+      L.tend[n] = tend;
+      L.bt[n] = bt;
+      L.ibe[n] = z.ibe;
+      L.jbe[n] = z.jbe;
+      L.kt[n] = z.kt;
+      L.is2d[n] = is2d ? 1 : 0;
+      nmax = zone_n(z) > nmax ? zone_n(z) : nmax;
+      kmax = z.kt > kmax ? z.kt : kmax;
+      n++;
+    };
+    add(F(F_ru_tend), 'u', false, false, bp(F_u_btys, F_u_btye, F_u_btxs, F_u_btxe));
+    add(F(F_rv_tend), 'v', false, false, bp(F_v_btys, F_v_btye, F_v_btxs, F_v_btxe));
+    add(F(F_ph_tend), 'h', true, false, bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe));
+    add(F(F_t_tend), 't', false, false, bp(F_t_btys, F_t_btye, F_t_btxs, F_t_btxe));
+    add(F(F_mu_tend), 'm', true, true, bp(F_mu_btys, F_mu_btye, F_mu_btxs, F_mu_btxe));
+    k_spec_overlay_batch<<<dim3(cdiv(nmax, 64), kmax, n), 64, 0, s>>>(d, L);
   }
   void dry_boundary_tendency() const {
     if (X->rk == 1) lateral_relax();
