@@ -128,6 +128,8 @@ class _Dataset:
 
     def attrs(self) -> dict:
         out = {}
+        # scipy keeps the global attributes in this dict (also exposed as
+        # attributes of the file object)
         for name, value in self._nc._attributes.items():
             if isinstance(value, bytes):
                 value = value.decode("ascii", "replace")
@@ -138,8 +140,9 @@ class _Dataset:
 
     def size(self, dim: str) -> int:
         n = self._nc.dimensions[dim]
-        if n is None:  # the record (Time) dimension
-            return int(self._nc._recs)
+        if n is None:  # the record (Time) dimension: the record count of its variables
+            recs = [v.shape[0] for v in self._nc.variables.values() if v.dimensions and v.dimensions[0] == dim]
+            return int(max(recs)) if recs else 0
         return int(n)
 
     def require(self, name: str):
