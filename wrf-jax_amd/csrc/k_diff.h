@@ -215,10 +215,9 @@ __global__ void k_smag(const DevCtx xv, R3 r) {
 // titau (module_diffusion_em.py:894-1316): stress component with rho and the
 // diffusivity averaged to the component's point; zero outside [i0,i1]x[j0,j1]
 // (and at k = 1, kde for the 13 / 23 components).  kind 11, 12, 13, 23.
-__global__ void k_titau(const DevCtx xv, float *tt, const float *xk, const float *defor, int kind, int i0, int i1, int j0,
-                        int j1, R3 r) {
-  const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
+// one stress component at (i, k, j): zero outside [i0, i1] x [j0, j1] x [1, kde-1]
+__device__ __forceinline__ float titau_val(const DevCtx *__restrict__ x, const float *xk, const float *defor, int kind, int i0, int i1,
+                                           int j0, int j1, int i, int k, int j) {
   const float *rho = FL(rho);
   const int ktf = x->kde - 1;
   float val = 0.f;
@@ -237,7 +236,21 @@ __global__ void k_titau(const DevCtx xv, float *tt, const float *xk, const float
       val = -xa * A3(defor, i, k, j);
     }
   }
-  A3(tt, i, k, j) = val;
+  return val;
+}
+// the two stress components one diffusion kernel reads, in one pass
+struct TitauPair {
+  float *tt[2];
+  const float *xk[2], *defor[2];
+  int kind[2], i0[2], i1[2], j0[2], j1[2];
+};
+__global__ void k_titau(const DevCtx xv, TitauPair P, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
+  GET_IKJ(r);
+  const float v0 = titau_val(x, P.xk[0], P.defor[0], P.kind[0], P.i0[0], P.i1[0], P.j0[0], P.j1[0], i, k, j);
+  const float v1 = titau_val(x, P.xk[1], P.defor[1], P.kind[1], P.i0[1], P.i1[1], P.j0[1], P.j1[1], i, k, j);
+  A3(P.tt[0], i, k, j) = v0;
+  A3(P.tt[1], i, k, j) = v1;
 }
 
 // horizontal_diffusion_u_2 (module_diffusion_em.py:2302-2532):

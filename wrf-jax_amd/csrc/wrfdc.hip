@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <initializer_list>
 #include <utility>
@@ -377,9 +378,24 @@ struct Run {
     R3 r{2, ide - 2, 1, kde - 1, 2, jde - 2};
     k_smag<<<g3(r), b3(), 0, s>>>(d, r);
   }
-  void titau(float *tt, int xk, int defor, int kind, int i0, int i1, int j0, int j1) const {
+  // two stress components (t1, t2 of one diffusion kernel) in one launch;
+  // each entry: xk, defor, kind, i0, i1, j0, j1
+  void titau(float *t1, float *t2, std::array<int, 7> a, std::array<int, 7> b) const {
+    TitauPair P;
+    P.tt[0] = t1;
+    P.tt[1] = t2;
+    const std::array<int, 7> *e[2] = {&a, &b};
+    for (int n = 0; n < 2; n++) {
+      P.xk[n] = F((*e[n])[0]);
+      P.defor[n] = F((*e[n])[1]);
+      P.kind[n] = (*e[n])[2];
+      P.i0[n] = (*e[n])[3];
+      P.i1[n] = (*e[n])[4];
+      P.j0[n] = (*e[n])[5];
+      P.j1[n] = (*e[n])[6];
+    }
     R3 r = full3();
-    k_titau<<<g3(r), b3(), 0, s>>>(d, tt, F(xk), F(defor), kind, i0, i1, j0, j1, r);
+    k_titau<<<g3(r), b3(), 0, s>>>(d, P, r);
   }
   // horizontal_diffusion_s of theta and every species: faces, then tendencies
   void hdiff_scalars() const {
@@ -402,16 +418,13 @@ struct Run {
   }
   void horizontal_diffusion_2() const {
     float *t1 = Scr(S_T1), *t2 = Scr(S_T2);
-    titau(t1, F_xkmh, F_defor11, 11, 1, ide - 1, 2, jde - 1);
-    titau(t2, F_xkmh, F_defor12, 12, 2, ide - 1, 2, jde);
+    titau(t1, t2, {F_xkmh, F_defor11, 11, 1, ide - 1, 2, jde - 1}, {F_xkmh, F_defor12, 12, 2, ide - 1, 2, jde});
     R3 ru{2, ide - 1, 1, kde - 1, 2, jde - 2};
     k_hdiff_u<<<g3(ru), b3(), 0, s>>>(d, t1, t2, ru);
-    titau(t1, F_xkmh, F_defor12, 12, 2, ide, 2, jde - 1);
-    titau(t2, F_xkmh, F_defor22, 11, 2, ide - 1, 1, jde - 1);
+    titau(t1, t2, {F_xkmh, F_defor12, 12, 2, ide, 2, jde - 1}, {F_xkmh, F_defor22, 11, 2, ide - 1, 1, jde - 1});
     R3 rv{2, ide - 2, 1, kde - 1, 2, jde - 1};
     k_hdiff_v<<<g3(rv), b3(), 0, s>>>(d, t1, t2, rv);
-    titau(t1, F_xkmv, F_defor13, 13, 2, ide, 2, jde - 2);
-    titau(t2, F_xkmv, F_defor23, 23, 2, ide - 2, 2, jde);
+    titau(t1, t2, {F_xkmv, F_defor13, 13, 2, ide, 2, jde - 2}, {F_xkmv, F_defor23, 23, 2, ide - 2, 2, jde});
     R3 rw{2, ide - 2, 2, kde - 1, 2, jde - 2};
     k_hdiff_w<<<g3(rw), b3(), 0, s>>>(d, t1, t2, rw);
     hdiff_scalars();
