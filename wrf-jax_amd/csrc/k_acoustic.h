@@ -45,14 +45,14 @@ __device__ __forceinline__ float uv_vert(const DevCtx *__restrict__ x, const flo
   return A1(FL(rdnw), k) * (up - dn);
 }
 // advance_uv at an interior u point (axis 0: i [2, ide-1], j [2, jde-2]) or v
-// point (axis 1: i [2, ide-2], j [2, jde-1])
-__device__ __forceinline__ float uv_interior(const DevCtx *__restrict__ x, int axis, int i, int k, int j, float dts) {
+// point (axis 1: i [2, ide-2], j [2, jde-1]); adv = u + dts * ru_tend (v +
+// dts * rv_tend) of that point, formed by the caller
+__device__ __forceinline__ float uv_interior(const DevCtx *__restrict__ x, int axis, int i, int k, int j, float dts, float adv) {
   const int di = axis == 0 ? 1 : 0, dj = axis == 0 ? 0 : 1;
   const unsigned o = XI3(i, k, j);
   const float *p = FL(p), *pb = FL(pb), *ph = FL(ph_2), *alt = FL(alt), *al = FL(al), *php = FL(php);
   const float *c1h = FL(c1h), *c2h = FL(c2h), *mu = FL(mu_2), *mudf = FL(mudf);
   const float spacing = axis == 0 ? x->rdx : x->rdy, grid_length = 1.f / spacing;
-  const float adv = axis == 0 ? elem(FL(u_2), o) + dts * elem(FL(ru_tend), o) : elem(FL(v_2), o) + dts * elem(FL(rv_tend), o);
   float damping = -1.f * x->emdiv * grid_length * (A2(mudf, i, j) - A2(mudf, i - di, j - dj));
   damping = axis == 0 ? damping / A2(FL(msfuy), i, j) : damping * A2(FL(msfvx_inv), i, j);
   const float phi = (A3(ph, i, k + 1, j) - A3(ph, i - di, k + 1, j - dj)) + (A3(ph, i, k, j) - A3(ph, i - di, k, j - dj));
@@ -81,10 +81,13 @@ __global__ void __launch_bounds__(256) k_uv_substep(const DevCtx xv, float dts, 
   const bool upt = (j <= jde - 1), vpt = (i <= ide - 1);
   const bool uin = upt && i >= 2 && i <= ide - 1 && j >= 2 && j <= jde - 2;
   const bool vin = vpt && i >= 2 && i <= ide - 2 && j >= 2 && j <= jde - 1;
-  const float un_in = uv_interior(x, 0, imin_(imax_(i, 2), ide - 1), k, imin_(imax_(j, 2), jde - 2), dts);
-  const float vn_in = uv_interior(x, 1, imin_(imax_(i, 2), ide - 2), k, imin_(imax_(j, 2), jde - 1), dts);
+  // u + dts * ru_tend at the point itself: the zone value, and the start of
+  // the interior formula (whose result is kept only where the point is
+  // interior, i.e. where the clamped point is the point itself)
   const float un_z = elem(FL(u_2), o) + dts * elem(FL(ru_tend), o);
   const float vn_z = elem(FL(v_2), o) + dts * elem(FL(rv_tend), o);
+  const float un_in = uv_interior(x, 0, imin_(imax_(i, 2), ide - 1), k, imin_(imax_(j, 2), jde - 2), dts, un_z);
+  const float vn_in = uv_interior(x, 1, imin_(imax_(i, 2), ide - 2), k, imin_(imax_(j, 2), jde - 1), dts, vn_z);
   if (upt) elem(FL(u_2), o) = uin ? un_in : un_z;
   if (vpt) elem(FL(v_2), o) = vin ? vn_in : vn_z;
 }
