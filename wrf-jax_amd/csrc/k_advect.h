@@ -450,10 +450,10 @@ __global__ void __launch_bounds__(256) k_advect_w(const DevCtx xv, const float *
   }
   const float msf = A2(FL(msftx), ic, j), rdz0 = A1(FL(rdn), imin_(k, kde - 1)), rdz1 = A1(FL(rdn), imax_(k - 1, 1));
   const float t0 = TEND ? 0.f : elem(tend, o);
-  float bu_t = 0.f, mt = 0.f;
+  float bu_t = 0.f, mt = 0.f, cq1 = 0.f;
   float2 cc = make_float2(0.f, 0.f), tfl = make_float2(0.f, 0.f);
   if (TEND) {
-    bu_t = buoy_w_term(x, ic, imax_(k, 2), j);
+    bu_t = buoy_w_term(x, ic, imax_(k, 2), j, &cq1);
     cc = cor_w_terms(x, ic, imin_(imax_(k, 2), kde - 1), j);
     tfl = tendf_load(FL(rw_tendf), FL(w_save), o, rk);
     mt = A2(FL(msfty), ic, j);
@@ -479,6 +479,10 @@ __global__ void __launch_bounds__(256) k_advect_w(const DevCtx xv, const float *
   }
   if (TEND) t = t + tendf_fold(FL(rw_tendf), o, tfl, mt, rk) / mt;
   elem(tend, o) = t;
+  // pg_buoy_w's cqw := 1 / (1 + cqw) at this point (read only by this
+  // thread here; level kde-1, which the top level's thread also reads, is
+  // replaced after this kernel)
+  if (TEND && k >= 2 && k <= kde - 2) elem(FL(cqw), o) = cq1;
 }
 
 // the phi tendency of the RK step over k [1, kde]: rhs_ph at k [2, kde] from

@@ -368,12 +368,15 @@ __global__ void k_hpg(const DevCtx xv, int axis, R3 r) {
 // pg_buoy_w (module_big_step_utilities_em.py:1674-1800) at one w level
 // k in [2, kde], added to rwt; reads cqw as it is before the column kernel
 // below replaces it by 1 / (1 + cqw)
-__device__ __forceinline__ float buoy_w_term(const DevCtx *__restrict__ x, int i, int k, int j) {
+// (cq1_out: 1 / (1 + cqw) at level min(k, kde-1), the value pg_buoy_w then
+// stores in cqw)
+__device__ __forceinline__ float buoy_w_term(const DevCtx *__restrict__ x, int i, int k, int j, float *cq1_out = nullptr) {
   const float *p = FL(p), *mu = FL(mu_2), *mub = FL(mub), *c1f = FL(c1f), *c2f = FL(c2f);
   const int kde = x->kde, kc = imin_(k, kde - 1);  // cqw and p level: k, kde-1 at the top
   const float gm = (1.f / A2(FL(msfty), i, j)) * x->g;
   const float cq = A3(FL(cqw), i, kc, j);
   const float cq1 = 1.f / (1.f + cq), cq2 = cq * cq1;
+  if (cq1_out) *cq1_out = cq1;
   const float pk = A3(p, i, kc, j), pkm = A3(p, i, k - 1, j);
   const float top = gm * (cq1 * 2.f * A1(FL(rdnw), kde - 1) * (-pk) - (A1(c1f, kde) * A2(mu, i, j)) -
                           cq2 * (A1(c1f, kde) * A2(mub, i, j) + A1(c2f, kde)));
@@ -381,7 +384,9 @@ __device__ __forceinline__ float buoy_w_term(const DevCtx *__restrict__ x, int i
   return k == kde ? top : mid;
 }
 __device__ __forceinline__ float buoy_w(const DevCtx *__restrict__ x, int i, int k, int j, float rwt) { return rwt + buoy_w_term(x, i, k, j); }
-// cqw := 1 / (1 + cqw) on levels [2, kde-1] (the second half of pg_buoy_w)
+// cqw := 1 / (1 + cqw) (the second half of pg_buoy_w): levels [2, kde-1] in
+// the stage API; in the fused step only level kde-1 (the w kernel's top
+// level reads it, so the fused w kernel replaces levels [2, kde-2] itself)
 __global__ void k_cqw_inv(const DevCtx xv, R3 r) {
   const DevCtx *__restrict__ x = &xv;
   GET_IKJ_P(r);

@@ -527,7 +527,7 @@ struct Run {
     k_advect_v<true><<<dim3(face_tiles(rv).count()), b3(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), rv, rk);
     const R3 rw{1, ide - 1, 1, kde, 1, jde - 1};
     k_advect_w<true><<<dim3(face_tiles(rw).count()), b3(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), rw, rk);
-    const R3 rq{1, ide - 1, 2, kde - 1, 1, jde - 1};
+    const R3 rq{1, ide - 1, kde - 1, kde - 1, 1, jde - 1};  // levels [2, kde-2]: k_advect_w<true>
     k_cqw_inv<<<g3p(rq), b3(), 0, s>>>(d, rq);
     const R3 m = mass3();
     k_advect_scalar<true><<<dim3(face_tiles(m).count()), b3(), 0, s>>>(d, F(F_t_2), F(F_t_tend), F(F_ru), F(F_rv), F(F_ww), m, rk);
