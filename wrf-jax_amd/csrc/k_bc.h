@@ -279,11 +279,23 @@ __global__ void k_flow_dep_cols(const DevCtx xv, float *q, const float *u) {
 }
 
 // spec_bdy_final (spec_bdy_final.py:27-191)
-__global__ void k_spec_final(const DevCtx xv, float *fl, int ibe, int jbe, int kt, int is2d, int msfc, int muc,
-                             const float *mu, const float *msf, const float *c1, const float *c2, BPtrs b, BPtrs bt, float dt_bc) {
+// the fields of one launch (blockIdx.z selects the field)
+constexpr int kFinalMax = 6;
+struct FinalList {
+  float *fl[kFinalMax];
+  const float *mu[kFinalMax], *msf[kFinalMax], *c1[kFinalMax], *c2[kFinalMax];
+  BPtrs b[kFinalMax], bt[kFinalMax];
+  int ibe[kFinalMax], jbe[kFinalMax], kt[kFinalMax], is2d[kFinalMax], msfc[kFinalMax], muc[kFinalMax];
+};
+__global__ void k_spec_final(const DevCtx xv, FinalList L, float dt_bc) {
   const DevCtx *__restrict__ x = &xv;
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = 1 + (int)blockIdx.y;
+  const int f = (int)blockIdx.z;
+  float *fl = L.fl[f];
+  const float *mu = L.mu[f], *msf = L.msf[f], *c1 = L.c1[f], *c2 = L.c2[f];
+  const BPtrs &b = L.b[f], &bt = L.bt[f];
+  const int ibe = L.ibe[f], jbe = L.jbe[f], kt = L.kt[f], is2d = L.is2d[f], msfc = L.msfc[f], muc = L.muc[f];
   int i, j, e;
   if (k > kt || !zone_point(t, ibe, jbe, i, j, e)) return;
   const int a = (e < 2) ? i : j;

@@ -325,14 +325,43 @@ struct Run {
   void relax(const float *fld, float *tend, char var, bool ktop, bool is2d, BPtrs b, BPtrs bt, int mw = 0) const {
     RelaxBatch(this).add(fld, tend, var, ktop, is2d, b, bt, mw);
   }
-  void spec_final(float *fl, char var, bool is2d, const float *mu, const float *msf, const float *c1, const float *c2, BPtrs b,
-                  BPtrs bt) const {
-    Z z = zd(var, var == 'm' || var == 'h' || var == 'w', is2d);
-    const int msfc = (var == 'u' || var == 'v' || var == 'w');
-    const int muc = (var != 'm');
-    k_spec_final<<<dim3(cdiv(zone_n(z), 64), z.kt), 64, 0, s>>>(d, fl, z.ibe, z.jbe, z.kt, is2d ? 1 : 0, msfc, muc, mu, msf, c1, c2, b,
-                                                               bt, (float)X->dtbc);
-  }
+  // spec_bdy_final of several fields in one launch (the fields of one batch
+  // must not read each other)
+  struct FinalBatch {
+    const Run *r;
+    FinalList L;
+    int n = 0, nmax = 1, kmax = 1;
+    explicit FinalBatch(const Run *run) : r(run) { memset(&L, 0, sizeof(L)); }
+    FinalBatch &add(float *fl, char var, bool is2d, const float *mu, const float *msf, const float *c1, const float *c2, BPtrs b,
+                    BPtrs bt) {
+      if (n == kFinalMax) flush();
+      const Z z = r->zd(var, var == 'm' || var == 'h' || var == 'w', is2d);
+      L.fl[n] = fl;
+      L.mu[n] = mu;
+      L.msf[n] = msf;
+      L.c1[n] = c1;
+      L.c2[n] = c2;
+      L.b[n] = b;
+      L.bt[n] = bt;
+      L.ibe[n] = z.ibe;
+      L.jbe[n] = z.jbe;
+      L.kt[n] = z.kt;
+      L.is2d[n] = is2d ? 1 : 0;
+      L.msfc[n] = (var == 'u' || var == 'v' || var == 'w');
+      L.muc[n] = (var != 'm');
+      nmax = std::max(nmax, zone_n(z));
+      kmax = std::max(kmax, z.kt);
+      n++;
+      return *this;
+    }
+    void flush() {
+      if (n == 0) return;
+      k_spec_final<<<dim3(cdiv(nmax, 64), kmax, n), 64, 0, r->s>>>(r->d, L, (float)r->X->dtbc);
+      n = 0;
+      nmax = kmax = 1;
+    }
+    ~FinalBatch() { flush(); }
+  };
   void edge_copy(float *f0, float *f1, int axis, int dst, int src_, int kend) const {
     const int n = axis == 0 ? jde : ide;
     k_edge_copy<<<dim3(cdiv(n, 64), kend), 64, 0, s>>>(d, f0, f1, axis, dst, src_, kend);
@@ -841,18 +870,21 @@ struct Run {
         .add3(F(F_u_1), 'u').add3(F(F_u_2), 'u').add3(F(F_v_1), 'v').add3(F(F_v_2), 'v').add3(F(F_w_1), 'w')
         .add3(F(F_w_2), 'w').add3(F(F_t_1), 'p').add3(F(F_t_2), 'p').add3(F(F_ph_1), 'w').add3(F(F_ph_2), 'w')
         .add2(F(F_mu_1), 't').add2(F(F_mu_2), 't').add3(Sp(F_moist, 0), 'p').flush();
-    spec_final(F(F_u_2), 'u', false, F(F_muus), F(F_msfuy), F(F_c1h), F(F_c2h), bp(F_u_bys, F_u_bye, F_u_bxs, F_u_bxe),
-               bp(F_u_btys, F_u_btye, F_u_btxs, F_u_btxe));
-    spec_final(F(F_v_2), 'v', false, F(F_muvs), F(F_msfvx), F(F_c1h), F(F_c2h), bp(F_v_bys, F_v_bye, F_v_bxs, F_v_bxe),
-               bp(F_v_btys, F_v_btye, F_v_btxs, F_v_btxe));
-    spec_final(F(F_t_2), 't', false, F(F_muts), F(F_msfty), F(F_c1h), F(F_c2h), bp(F_t_bys, F_t_bye, F_t_bxs, F_t_bxe),
-               bp(F_t_btys, F_t_btye, F_t_btxs, F_t_btxe));
-    spec_final(F(F_ph_2), 'h', false, F(F_muts), F(F_msfty), F(F_c1f), F(F_c2f), bp(F_ph_bys, F_ph_bye, F_ph_bxs, F_ph_bxe),
-               bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe));
-    spec_final(F(F_mu_2), 'm', true, F(F_muts), F(F_msfty), F(F_c1h), F(F_c2h), bp(F_mu_bys, F_mu_bye, F_mu_bxs, F_mu_bxe),
-               bp(F_mu_btys, F_mu_btye, F_mu_btxs, F_mu_btxe));
-    spec_final(Sp(F_moist, 0), 't', false, F(F_muts), F(F_msfty), F(F_c1h), F(F_c2h), bp(F_qv_bys, F_qv_bye, F_qv_bxs, F_qv_bxe),
-               bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe));
+    // the six fields read only coupling masses and map factors, none of them
+    // another's result, so one launch serves all
+    FinalBatch(this)
+        .add(F(F_u_2), 'u', false, F(F_muus), F(F_msfuy), F(F_c1h), F(F_c2h), bp(F_u_bys, F_u_bye, F_u_bxs, F_u_bxe),
+             bp(F_u_btys, F_u_btye, F_u_btxs, F_u_btxe))
+        .add(F(F_v_2), 'v', false, F(F_muvs), F(F_msfvx), F(F_c1h), F(F_c2h), bp(F_v_bys, F_v_bye, F_v_bxs, F_v_bxe),
+             bp(F_v_btys, F_v_btye, F_v_btxs, F_v_btxe))
+        .add(F(F_t_2), 't', false, F(F_muts), F(F_msfty), F(F_c1h), F(F_c2h), bp(F_t_bys, F_t_bye, F_t_bxs, F_t_bxe),
+             bp(F_t_btys, F_t_btye, F_t_btxs, F_t_btxe))
+        .add(F(F_ph_2), 'h', false, F(F_muts), F(F_msfty), F(F_c1f), F(F_c2f), bp(F_ph_bys, F_ph_bye, F_ph_bxs, F_ph_bxe),
+             bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe))
+        .add(F(F_mu_2), 'm', true, F(F_muts), F(F_msfty), F(F_c1h), F(F_c2h), bp(F_mu_bys, F_mu_bye, F_mu_bxs, F_mu_bxe),
+             bp(F_mu_btys, F_mu_btye, F_mu_btxs, F_mu_btxe))
+        .add(Sp(F_moist, 0), 't', false, F(F_muts), F(F_msfty), F(F_c1h), F(F_c2h), bp(F_qv_bys, F_qv_bye, F_qv_bxs, F_qv_bxe),
+             bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe));
     k_set_w_surface<<<dim3(cdiv(ide - 1, 64), jde - 1), 64, 0, s>>>(d);
   }
 
