@@ -31,10 +31,18 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+# A rebuilt copy of the restatement (the mutation tests build one per planted
+# slip) is loaded instead of oracle/_build when this variable names its folder.
+BUILD_ENV = "WRFDC_ORACLE_BUILD"
+
+
 def _lib(precision: str) -> ctypes.CDLL:
     if precision not in _LIBS:
-        path = os.path.join(HERE, "_build", f"liboracle_{precision}.so")
+        build_dir = os.environ.get(BUILD_ENV) or os.path.join(HERE, "_build")
+        path = os.path.join(build_dir, f"liboracle_{precision}.so")
         if not os.path.exists(path):
+            if os.environ.get(BUILD_ENV):
+                raise FileNotFoundError(path)
             build()
         lib = ctypes.CDLL(path)
         lib.o_create.restype = ctypes.c_void_p
