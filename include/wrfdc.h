@@ -16,7 +16,8 @@
  *   wrfdc_stage      one StageFn callback of strict_solve_callbacks()
  *                                                src/wrf_jax/runtime.py:16235-16301
  *   wrfdc_checksums  assert_state_finite         src/wrf_jax/data_io.py:222-228
- *                    (plus FP64 mass / theta / moisture checksums)
+ *                    (plus FP64 conservation checksums: column mass, theta,
+ *                    moisture, dry-air mass and total energy)
  */
 #ifndef WRFDC_H
 #define WRFDC_H
@@ -31,6 +32,12 @@ extern "C" {
 #define WRFDC_EUNBOUND -3  /* a required field has no device buffer bound */
 #define WRFDC_EHIP -4      /* HIP runtime error */
 #define WRFDC_ENONFINITE -5 /* NaN/Inf found by wrfdc_checksums (FloatingPointError) */
+
+/* wrfdc_checksums output, FP64 sums over the mass points:
+ * [0] sum(mu_2 + mub), [1] sum(t_2), [2] sum of all moist species,
+ * [3] number of non-finite prognostic values, [4] dry-air mass (kg),
+ * [5] total energy: sum of dm * (kinetic + cv*T + geopotential) (J). */
+#define WRFDC_NCHECKSUMS 6
 
 /* Scalar configuration; field order and types match the Python mirror
  * wrfdc.config.CConfig.  Dimensions are mass-point counts. */
@@ -67,8 +74,10 @@ int wrfdc_bind(wrfdc_ctx *ctx, const char *name, void *device_ptr);
 int wrfdc_set_stream(wrfdc_ctx *ctx, void *hip_stream);
 int wrfdc_step(wrfdc_ctx *ctx, int nsteps, double dtbc0);
 int wrfdc_stage(wrfdc_ctx *ctx, const char *name, int rk_step, int iteration);
-int wrfdc_checksums(wrfdc_ctx *ctx, double out[4]);
+int wrfdc_checksums(wrfdc_ctx *ctx, double out[WRFDC_NCHECKSUMS]);
 int wrfdc_sync(wrfdc_ctx *ctx);
+/* Message of the last failure on ctx; with ctx NULL, of the last failure of
+ * wrfdc_create (or another entry point without a context) on this thread. */
 const char *wrfdc_last_error(wrfdc_ctx *ctx);
 
 #ifdef __cplusplus

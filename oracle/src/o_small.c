@@ -255,7 +255,9 @@ void o_advance_mu_t(OCtx *x) {
   real dvdxi[512], wdtn[512];
   for (int j = 2; j <= JDE - 2; j++)
     for (int i = 2; i <= IDE - 2; i++) {
-      real dmdt = 0;
+      /* column mass tendency: the FP32 level terms summed in FP64 and
+       * rounded once, as the GPU kernels do */
+      double dmdt_acc = 0;
       for (int k = 1; k <= k_end; k++) {
         real c1 = A1(c1h, k), c2 = A1(c2h, k);
         real vn = A3(v, i, k, j + 1) + ((c1 * A2(muv, i, j + 1) + c2) * A3(v_1, i, k, j + 1)) * A2(msfvx_inv, i, j + 1);
@@ -264,8 +266,9 @@ void o_advance_mu_t(OCtx *x) {
         real uw = A3(u, i, k, j) + ((c1 * A2(muu, i, j) + c2) * A3(u_1, i, k, j)) / A2(msfuy, i, j);
         real hf = rdy * (vn - vs) + rdx * (ue - uw);
         dvdxi[k] = (A2(msftx, i, j) * A2(msfty, i, j)) * hf;
-        dmdt = dmdt + A1(dnw, k) * dvdxi[k];
+        dmdt_acc += (double)(A1(dnw, k) * dvdxi[k]);
       }
+      real dmdt = (real)dmdt_acc;
       real old = A2(mu, i, j);
       real tend = dmdt + A2(mu_tend, i, j);
       real nmu = old + dts * tend;

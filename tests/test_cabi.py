@@ -81,6 +81,20 @@ def test_create_rejects_bad_configs(lib, over, code):
     assert not ctx.value
 
 
+def test_create_failure_message_reaches_python(lib):
+    """wrfdc_create has no context to hold its message: wrfdc_last_error(NULL)
+    returns it (per thread), and raise_for puts it into the exception."""
+    rc, ctx = _create(lib, nx=8)
+    assert rc == _lib.WRFDC_EINVAL
+    assert b"domain too small" in lib.wrfdc_last_error(None)
+    with pytest.raises(ValueError, match="domain too small"):
+        _lib.raise_for(rc, lib, None, "wrfdc_create")
+    rc, ctx = _create(lib, top_lid=1)
+    assert rc == _lib.WRFDC_ENOTIMPL
+    with pytest.raises(NotImplementedError, match="wrfdc_create: .+"):
+        _lib.raise_for(rc, lib, None, "wrfdc_create")
+
+
 def test_error_mapping_uses_reference_exception_types(lib):
     with pytest.raises(ValueError):
         _lib.raise_for(_lib.WRFDC_EINVAL, lib, None, "x")

@@ -1,0 +1,215 @@
+"""GPU parity at the benchmark workloads and at every column-kernel shape.
+
+* the 100x100x45 plain case (6 moist species): one RK3 step against the
+  FP32 oracle (every field within relative L2 1e-5) and 100 steps (every
+  prognostic field within NRMSE 1e-3);
+* the 400x400x60 case (8 species, PD advection): one step against the
+  oracle, and 100 GPU steps checked through properties that do not need the
+  oracle -- a finite state, dry-air mass and total energy that drift by
+  less than stated bounds, species bounded below, winds bounded;
+* one step at each instantiation of the acoustic column kernel (levels per
+  wave 4, 8, 12, 16, 20: kde up to 32, 64, 96, 128, 151), and at a width
+  whose x-face tiles take the all-5th-order path of the flux kernels.
+"""
+
+import numpy as np
+import pytest
+
+from parity import NRMSE_LONG, REL_L2_STEP, STEP_DIAGNOSTICS, compare, nrmse
+
+pytestmark = pytest.mark.gpu
+
+PROGNOSTIC = ("u_2", "v_2", "w_2", "t_2", "ph_2", "mu_2", "p", "al", "moist", "scalar")
+
+
+def _domain(config, gpu):
+    from wrfdc.device import Domain
+
+    return Domain(config, gpu)
+
+
+def _one_step(case, gpu, nsteps=1):
+    from oracle import Oracle
+
+    dom = _domain(case.config, gpu)
+    dom.upload(case.arrays)
+    dom.step(nsteps, 0.0)
+    got = dom.download()
+    dom.close()
+    o = Oracle(case.config, case.arrays)
+    o.step(nsteps, 0.0)
+    ref = o.copy()
+    o.close()
+    return got, ref
+
+
+def test_c0_one_step(gpu):
+    from wrfdc.cases import named_case
+
+    case = named_case("c0")
+    print("[workload] c0 100x100x45, 1 step vs oracle", flush=True)
+    got, ref = _one_step(case, gpu)
+    bad = compare(got, ref, tol=REL_L2_STEP, skip=STEP_DIAGNOSTICS)
+    assert not bad, f"fields above rel-L2 {REL_L2_STEP}: {bad}"
+
+
+@pytest.mark.timeout(900)
+def test_c0_hundred_steps(gpu):
+    from wrfdc.cases import named_case
+
+    case = named_case("c0")
+    print("[workload] c0 100x100x45, 100 steps vs oracle", flush=True)
+    got, ref = _one_step(case, gpu, nsteps=100)
+    nx, ny, nz = case.config.nx, case.config.ny, case.config.nz
+    errs = {}
+    for n in PROGNOSTIC:
+        if n not in ref or ref[n].size == 0:
+            continue
+        a, b = got[n], ref[n]
+        if a.ndim == 4:
+            a, b = a[:, :ny, :nz, :nx], b[:, :ny, :nz, :nx]
+        elif a.ndim == 3:
+            a, b = a[:ny + 1, :nz + 1, :nx + 1], b[:ny + 1, :nz + 1, :nx + 1]
+        else:
+            a, b = a[:ny, :nx], b[:ny, :nx]
+        errs[n] = nrmse(a, b)
+    print("[workload] c0 NRMSE after 100 steps:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    assert all(np.isfinite(v) and v <= NRMSE_LONG for v in errs.values()), errs
+
+
+@pytest.mark.timeout(900)
+def test_c1_one_step(gpu):
+    from wrfdc.cases import named_case
+
+    case = named_case("c1")
+    print("[workload] c1 400x400x60, 1 step vs oracle", flush=True)
+    got, ref = _one_step(case, gpu)
+    bad = compare(got, ref, tol=REL_L2_STEP, skip=STEP_DIAGNOSTICS)
+    assert not bad, f"fields above rel-L2 {REL_L2_STEP}: {bad}"
+
+
+@pytest.mark.timeout(600)
+def test_c1_hundred_steps_properties(gpu):
+    """100 steps (30 simulated minutes) of the 400x400x60 case on the GPU.
+    Dry-air mass changes only through the lateral boundary zone (the
+    boundary values are the initial edge state, tendencies zero), total
+    energy by the work of the boundary relaxation and the diffusion, so
+    both stay within 1e-4 of their start; no value turns non-finite; the
+    winds stay below 60 m/s; the species stay above -2e-3 of their maximum.
+    The reference's scheme is positive definite only in the last RK stage's
+    advective update (the first two stages advect with the unlimited 5th-order
+    fluxes, and the limited update's low-order flux is not itself positive
+    where the mass flux diverges), so small negatives appear over the hill:
+    the FP32 oracle on a 100x100x60 version of this case reaches -1.0e-6
+    against a maximum of 1.2e-3 (qs, level 2 over the hill crest) after 10
+    steps."""
+    from wrfdc.cases import named_case
+
+    case = named_case("c1")
+    cfg = case.config
+    print("[workload] c1 400x400x60, 100 GPU steps, property checks", flush=True)
+    dom = _domain(cfg, gpu)
+    dom.upload(case.arrays)
+    ck0 = dom.checksums()
+    for blk in range(10):
+        dom.step(10, blk * 10 * cfg.dt)
+        ck = dom.checksums()  # raises FloatingPointError on NaN/Inf
+        print(f"[workload] c1 after {10 * (blk + 1)} steps: dry mass {ck[4]:.9e} kg, energy {ck[5]:.9e} J", flush=True)
+    got = dom.download(["u_2", "v_2", "w_2", "moist", "scalar"])
+    dom.close()
+    assert ck[3] == 0
+    assert abs(ck[4] - ck0[4]) / ck0[4] < 1e-4
+    assert abs(ck[5] - ck0[5]) / ck0[5] < 1e-4
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    for n in ("moist", "scalar"):
+        q = got[n][:, :ny, :nz, :nx].astype(np.float64)
+        scale = np.abs(q).reshape(q.shape[0], -1).max(axis=1)
+        assert np.all(q.reshape(q.shape[0], -1).min(axis=1) >= -2e-3 * scale), n
+    for n in ("u_2", "v_2", "w_2"):
+        assert np.abs(got[n][:ny, :nz, :nx]).max() < 60.0, n
+
+
+COLUMN_SHAPES = [
+    (136, 20, 40),   # kde 41: 8 levels per wave; x tiles with all faces 5th order
+    (24, 20, 70),    # kde 71: 12 levels per wave
+    (20, 16, 100),   # kde 101: 16 levels per wave
+    (16, 14, 130),   # kde 131: 20 levels per wave
+]
+
+
+@pytest.mark.parametrize("shape", COLUMN_SHAPES, ids=[f"{a}x{b}x{c}" for a, b, c in COLUMN_SHAPES])
+def test_column_kernel_shapes_one_step(gpu, shape):
+    from wrfdc.cases import make_case
+
+    nx, ny, nz = shape
+    case = make_case(nx, ny, nz, nmoist=6, nscalar=2, hydrometeors=True, hill_height=500.0)
+    print(f"[workload] shape {shape}, 1 step vs oracle", flush=True)
+    got, ref = _one_step(case, gpu)
+    bad = compare(got, ref, tol=REL_L2_STEP, skip=STEP_DIAGNOSTICS)
+    assert not bad, f"fields above rel-L2 {REL_L2_STEP}: {bad}"
+
+
+@pytest.mark.parametrize("shape", COLUMN_SHAPES[:2], ids=[f"{a}x{b}x{c}" for a, b, c in COLUMN_SHAPES[:2]])
+def test_column_kernel_shapes_acoustic_substep(gpu, shape):
+    """The fused acoustic substep alone (rk 3, substep 2) from a stepped state."""
+    from oracle import Oracle
+    from wrfdc.cases import make_case
+
+    nx, ny, nz = shape
+    case = make_case(nx, ny, nz, nmoist=6, nscalar=2, hydrometeors=True, hill_height=500.0)
+    o = Oracle(case.config, case.arrays)
+    o.step(1, 0.0)
+    state = o.copy()
+    o.stage("acoustic_substep", 3, 2)
+    ref = o.copy()
+    o.close()
+    dom = _domain(case.config, gpu)
+    dom.upload(state)
+    dom.stage("acoustic_substep", 3, 2)
+    got = dom.download()
+    dom.close()
+    bad = compare(got, ref, tol=REL_L2_STEP)
+    assert not bad, f"fields above rel-L2 {REL_L2_STEP}: {bad}"
+
+
+# ---------------------------------------------------------------------------
+# FP64 conservation checksums
+# ---------------------------------------------------------------------------
+def _uniform_state(U, V, W):
+    from wrfdc.cases import make_case
+
+    case = make_case(28, 24, 12, nmoist=1, nscalar=0)
+    A = {k: v.copy() for k, v in case.arrays.items()}
+    cfg = case.config
+    mu0, theta0, p_tot, phi0 = 9.0e4, 300.0, 7.0e4, 2.0e4
+    A["mub"][:] = mu0
+    A["mu_2"][:] = 0.0
+    for n, val in (("u_2", U), ("v_2", V), ("w_2", W), ("t_2", theta0 - cfg.t0), ("pb", p_tot), ("p", 0.0),
+                   ("phb", phi0), ("ph_2", 0.0), ("msftx", 1.0), ("msfty", 1.0)):
+        A[n][:] = val
+    # closed form from the FP32 values the GPU holds, evaluated in FP64
+    f = lambda v: np.float64(np.float32(v))  # noqa: E731
+    kinetic = 0.5 * (f(U) ** 2 + f(V) ** 2 + f(W) ** 2)
+    cv = f(cfg.r_d) / (f(cfg.cpovcv) - 1.0)
+    temp = f(cfg.t0) * (f(p_tot) / f(cfg.p0)) ** (1.0 - 1.0 / f(cfg.cpovcv))
+    nz = cfg.nz
+    c1h, c2h, dnw = (A[n][:nz].astype(np.float64) for n in ("c1h", "c2h", "dnw"))
+    col = float(np.sum((c1h * f(mu0) + c2h) * -dnw))
+    mass = cfg.nx * cfg.ny * col * f(cfg.dx) * f(cfg.dy) / f(cfg.g)
+    return cfg, A, mass, mass * (kinetic + cv * temp + f(phi0))
+
+
+@pytest.mark.parametrize("wind", [(0.0, 0.0, 0.0), (10.0, -4.0, 0.5)], ids=["rest", "uniform-wind"])
+def test_energy_checksum_closed_form(gpu, wind):
+    """A horizontally and vertically uniform state (column mass mu, theta,
+    total pressure, geopotential and winds constant): the dry-air mass is
+    nx*ny*dx*dy/g * sum_k (c1h*mu + c2h)*(-dnw) and the total energy that
+    mass times 0.5*|V|^2 + cv*theta*(p/p0)^(R/cp) + phi (at rest: no kinetic
+    part)."""
+    cfg, A, mass, energy = _uniform_state(*wind)
+    dom = _domain(cfg, gpu)
+    dom.upload(A)
+    ck = dom.checksums()
+    dom.close()
+    assert ck[4] == pytest.approx(mass, rel=1e-12)
+    assert ck[5] == pytest.approx(energy, rel=1e-12)

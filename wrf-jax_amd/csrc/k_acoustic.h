@@ -222,8 +222,10 @@ __global__ void __launch_bounds__(64 * kColWaves) k_acoustic_col(const DevCtx xv
   __syncthreads();
   // ---- 2. column sum, column-mass update and the omega recurrence (wave 0)
   if (wv == 0 && interior) {
-    float dmdt = 0.f;
-    for (int k = 1; k <= kend; k++) dmdt = dmdt + c_dnw[k - 1] * LS(B0, k);
+    // column mass tendency: FP32 level terms summed in FP64, rounded once
+    double dmdt_acc = 0.0;
+    for (int k = 1; k <= kend; k++) dmdt_acc += (double)(c_dnw[k - 1] * LS(B0, k));
+    const float dmdt = (float)dmdt_acc;
     float *mu = FL(mu_2);
     const float old = A2(mu, i, j);
     const float mtend = A2(FL(mu_tend), i, j);

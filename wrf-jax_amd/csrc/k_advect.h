@@ -453,7 +453,10 @@ __global__ void __launch_bounds__(256) k_advect_w(const DevCtx xv, const float *
   float bu_t = 0.f, mt = 0.f, cq1 = 0.f;
   float2 cc = make_float2(0.f, 0.f), tfl = make_float2(0.f, 0.f);
   if (TEND) {
-    bu_t = buoy_w_term(x, ic, imax_(k, 2), j, &cq1);
+    // level 1 has no buoyancy term (its result is discarded); it reads the
+    // top level's inputs, which no block of this launch writes, instead of
+    // level 2's cqw, which the level-2 block replaces
+    bu_t = buoy_w_term(x, ic, k < 2 ? kde : k, j, &cq1);
     cc = cor_w_terms(x, ic, imin_(imax_(k, 2), kde - 1), j);
     tfl = tendf_load(FL(rw_tendf), FL(w_save), o, rk);
     mt = A2(FL(msfty), ic, j);

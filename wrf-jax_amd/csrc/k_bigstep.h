@@ -573,8 +573,12 @@ __global__ void k_calc_p_rho_phi(const DevCtx xv, R3 r) {
   const unsigned o = XI3(i, k, j), o1 = o + (unsigned)x->ip;
   const float ph1 = elem(ph, o1), ph0 = elem(ph, o), phb1 = elem(phb, o1), phb0 = elem(phb, o), albv = elem(alb, o);
   const float qv = elem(SP(moist, 0), o), tv = elem(t, o), pbv = elem(pb, o);
-  // log and pow in FP64, rounded once to FP32 (the correctly rounded FP32
-  // values in all but vanishingly rare cases)
+  // log and pow rounded once to FP32 from an FP64 evaluation: the correctly
+  // rounded FP32 values in all but vanishingly rare cases.  p and al are
+  // small differences from the base state (p = p0*temp^cpovcv - pb), which
+  // magnify an ulp of the total about a thousandfold: the device library's
+  // FP32 logf / powf (within 1-2 ulp) put p and al 4.5e-5 and 2.8e-5 (relative
+  // L2) away from the correctly rounded result after one call.
   const float lg = (float)log((double)(pfd / pfu));
   const float alv = (ph1 - ph0 + phb1 - phb0) / phm / lg - albv;
   elem(FL(al), o) = alv;

@@ -281,8 +281,10 @@ __global__ void k_advance_mu_t(const DevCtx xv, float dts, R2 r) {
   const float *ww_1 = FL(ww_1), *t_1 = FL(t_save), *ft = FL(t_tend), *u = FL(u_2), *v = FL(v_2);
   const float *c1h = FL(c1h), *dnw = FL(dnw), *fnm = FL(fnm), *fnp = FL(fnp), *rdnw = FL(rdnw);
   const int k_end = x->kde - 1;
-  float dmdt = 0.f;
-  for (int k = 1; k <= k_end; k++) dmdt = dmdt + A1(dnw, k) * mut_dvdxi(x, i, k, j);
+  // column mass tendency: FP32 level terms summed in FP64, rounded once
+  double dmdt_acc = 0.0;
+  for (int k = 1; k <= k_end; k++) dmdt_acc += (double)(A1(dnw, k) * mut_dvdxi(x, i, k, j));
+  const float dmdt = (float)dmdt_acc;
   float *mu = FL(mu_2);
   const float old = A2(mu, i, j);
   const float mtend = A2(FL(mu_tend), i, j);

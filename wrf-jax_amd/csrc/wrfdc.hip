@@ -61,66 +61,84 @@ __global__ void __launch_bounds__(256) k_copy_batch(const DevCtx xv, CopyList L,
   }
 }
 
-// FP64 checksums over mass points: [0] sum(mu_2 + mub), [1] sum(t_2),
-// [2] sum of all moist species, [3] count of non-finite prognostic values.
+// FP64 checksums over mass points (WRFDC_NCHECKSUMS values):
+// [0] sum(mu_2 + mub), [1] sum(t_2), [2] sum of all moist species,
+// [3] count of non-finite prognostic values,
+// [4] dry-air mass, sum of dm = (c1h*mu_t + c2h) * (-dnw) * dx*dy / (g*msftx*msfty)  (kg),
+// [5] total energy, sum of dm * (kinetic + cv*T + geopotential)  (J), with the
+//     winds averaged to the mass point, T = theta * ((pb+p)/p0)^(R/cp),
+//     cv = R/(cp/cv - 1) and the geopotential averaged from the two w levels.
 // Pass 1 writes one partial per block (vector stores, no atomics).
 #define CK_BLOCK 256
+#define CK_N 6
 __global__ void __launch_bounds__(CK_BLOCK) k_checksum_partial(const DevCtx xv, double *part) {
   const DevCtx *__restrict__ x = &xv;
   const int ni = x->ide - 1, nk = x->kde - 1, nj = x->jde - 1;
   const size_t ncol = (size_t)ni * nj, npt = ncol * nk;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  const double cv = (double)x->r_d / ((double)x->cpovcv - 1.0);
+  const double rcp = 1.0 - 1.0 / (double)x->cpovcv;  // R/cp
+  const double area_g = (double)x->dx * (double)x->dy / (double)x->g;
+  double s[CK_N] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < npt; t += (size_t)gridDim.x * blockDim.x) {
     const int i = 1 + (int)(t % ni);
     const int k = 1 + (int)((t / ni) % nk);
     const int j = 1 + (int)(t / ((size_t)ni * nk));
     const size_t o = XI3(i, k, j);
     const float tv = FL(t_2)[o], uv = FL(u_2)[o], vv = FL(v_2)[o], wv = FL(w_2)[o], pv = FL(ph_2)[o];
-    s1 += (double)tv;
+    s[1] += (double)tv;
     float nf = 0.f;
     nf += isfinite(tv) ? 0.f : 1.f;
     nf += isfinite(uv) ? 0.f : 1.f;
     nf += isfinite(vv) ? 0.f : 1.f;
     nf += isfinite(wv) ? 0.f : 1.f;
     nf += isfinite(pv) ? 0.f : 1.f;
-    for (int s = 0; s < x->nmoist; s++) {
-      const float q = SP(moist, s)[o];
-      s2 += (double)q;
+    for (int sp = 0; sp < x->nmoist; sp++) {
+      const float q = SP(moist, sp)[o];
+      s[2] += (double)q;
       nf += isfinite(q) ? 0.f : 1.f;
     }
+    const double mt = (double)A2(FL(mu_2), i, j) + (double)A2(FL(mub), i, j);
     if (k == 1) {
-      const float m = A2(FL(mu_2), i, j);
-      s0 += (double)m + (double)A2(FL(mub), i, j);
-      nf += isfinite(m) ? 0.f : 1.f;
+      s[0] += mt;
+      nf += isfinite(A2(FL(mu_2), i, j)) ? 0.f : 1.f;
     }
-    s3 += (double)nf;
+    s[3] += (double)nf;
+    const double dm = ((double)A1(FL(c1h), k) * mt + (double)A1(FL(c2h), k)) * -(double)A1(FL(dnw), k) * area_g /
+                      ((double)A2(FL(msftx), i, j) * (double)A2(FL(msfty), i, j));
+    const double ub = 0.5 * ((double)uv + (double)A3(FL(u_2), i + 1, k, j));
+    const double vb = 0.5 * ((double)vv + (double)A3(FL(v_2), i, k, j + 1));
+    const double wb = 0.5 * ((double)wv + (double)A3(FL(w_2), i, k + 1, j));
+    const double theta = (double)x->t0 + (double)tv;
+    const double ptot = (double)A3(FL(pb), i, k, j) + (double)A3(FL(p), i, k, j);
+    const double temp = theta * pow(ptot / (double)x->p0, rcp);
+    const double phi = 0.5 * ((double)pv + (double)A3(FL(phb), i, k, j) + (double)A3(FL(ph_2), i, k + 1, j) +
+                              (double)A3(FL(phb), i, k + 1, j));
+    s[4] += dm;
+    s[5] += dm * (0.5 * (ub * ub + vb * vb + wb * wb) + cv * temp + phi);
   }
-  __shared__ double red[4][CK_BLOCK];
-  red[0][threadIdx.x] = s0;
-  red[1][threadIdx.x] = s1;
-  red[2][threadIdx.x] = s2;
-  red[3][threadIdx.x] = s3;
+  __shared__ double red[CK_N][CK_BLOCK];
+  for (int c = 0; c < CK_N; c++) red[c][threadIdx.x] = s[c];
   __syncthreads();
   for (int w = CK_BLOCK / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w)
-      for (int c = 0; c < 4; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+      for (int c = 0; c < CK_N; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x < 4) part[(size_t)blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < CK_N) part[(size_t)blockIdx.x * CK_N + threadIdx.x] = red[threadIdx.x][0];
 }
 __global__ void __launch_bounds__(CK_BLOCK) k_checksum_final(const double *part, int nblocks, double *out) {
-  __shared__ double red[4][CK_BLOCK];
-  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  __shared__ double red[CK_N][CK_BLOCK];
+  double s[CK_N] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int b = threadIdx.x; b < nblocks; b += CK_BLOCK)
-    for (int c = 0; c < 4; c++) s[c] += part[(size_t)b * 4 + c];
-  for (int c = 0; c < 4; c++) red[c][threadIdx.x] = s[c];
+    for (int c = 0; c < CK_N; c++) s[c] += part[(size_t)b * CK_N + c];
+  for (int c = 0; c < CK_N; c++) red[c][threadIdx.x] = s[c];
   __syncthreads();
   for (int w = CK_BLOCK / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w)
-      for (int c = 0; c < 4; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+      for (int c = 0; c < CK_N; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x < 4) out[threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < CK_N) out[threadIdx.x] = red[threadIdx.x][0];
 }
 
 // ---------------------------------------------------------------------------
@@ -150,7 +168,7 @@ struct wrfdc_ctx {
   float *spec_scale = nullptr;  // positive-definite renormalisation factors, one 3-D array per species
   float *spec_anti = nullptr;   // positive-definite antidiffusive face fluxes, three 3-D arrays per species
   float *hds = nullptr;         // horizontal scalar diffusion face fluxes, two 3-D arrays per diffused scalar
-  double *ck = nullptr;  // kCkBlocks * 4 partials + 4 results
+  double *ck = nullptr;  // kCkBlocks * CK_N partials + CK_N results
   int rk = 1, nsmall = 1, iteration = 1;
   double dt_rk = 0.0, dts_rk = 0.0, dtbc = 0.0;
   std::string err;
@@ -917,8 +935,13 @@ int num_sound_steps(const wrfdc_config &c) {
   return n > 4 ? n : 4;
 }
 
+// message of the last failure of an entry point that has no context
+// (wrfdc_create, the context-free entry points), per host thread
+thread_local std::string g_thread_err;
+
 int set_err(wrfdc_ctx *x, int code, const std::string &msg) {
   if (x) x->err = msg;
+  else g_thread_err = msg;
   return code;
 }
 
@@ -978,14 +1001,11 @@ const char *wrfdc_field_name(int index) { return (index >= 0 && index < NFIELDS)
 int wrfdc_field_kind(int index) { return (index >= 0 && index < NFIELDS) ? (int)kKinds[index] : 0; }
 
 int wrfdc_create(const wrfdc_config *cfg, wrfdc_ctx **out) {
-  if (!cfg || !out) return WRFDC_EINVAL;
+  if (!cfg || !out) return set_err(nullptr, WRFDC_EINVAL, "cfg and out must not be null");
   *out = nullptr;
   std::string msg;
   int rc = validate(*cfg, msg);
-  if (rc != WRFDC_OK) {
-    fprintf(stderr, "wrfdc_create: %s\n", msg.c_str());
-    return rc;
-  }
+  if (rc != WRFDC_OK) return set_err(nullptr, rc, msg);
   wrfdc_ctx *x = new wrfdc_ctx();
   x->c = *cfg;
   x->ide = cfg->nx + 1;
@@ -1042,14 +1062,14 @@ int wrfdc_create(const wrfdc_config *cfg, wrfdc_ctx **out) {
   }
   try {
     hchk(hipMalloc((void **)&x->scratch, sizeof(float) * x->n3 * NSCRATCH), "hipMalloc scratch");
-    hchk(hipMalloc((void **)&x->ck, sizeof(double) * (kCkBlocks * 4 + 4)), "hipMalloc checksum");
+    hchk(hipMalloc((void **)&x->ck, sizeof(double) * (kCkBlocks * CK_N + CK_N)), "hipMalloc checksum");
     hchk(hipMalloc((void **)&x->spec_alt, sizeof(float) * x->n3 * (size_t)(cfg->nmoist + cfg->nscalar)), "hipMalloc species");
     hchk(hipMalloc((void **)&x->spec_scale, sizeof(float) * x->n3 * (size_t)(cfg->nmoist + cfg->nscalar)), "hipMalloc species scale");
     if (cfg->pd_moist || cfg->pd_scalar)
       hchk(hipMalloc((void **)&x->spec_anti, sizeof(float) * x->n3 * 3 * (size_t)(cfg->nmoist + cfg->nscalar)), "hipMalloc species fluxes");
     hchk(hipMalloc((void **)&x->hds, sizeof(float) * x->n3 * 2 * (size_t)(1 + cfg->nmoist + cfg->nscalar)), "hipMalloc diffusion fluxes");
   } catch (const HipError &e) {
-    fprintf(stderr, "wrfdc_create: %s: %s\n", e.what, hipGetErrorString(e.e));
+    set_err(nullptr, WRFDC_EHIP, std::string(e.what) + ": " + hipGetErrorString(e.e));
     if (x->scratch) (void)hipFree(x->scratch);
     if (x->ck) (void)hipFree(x->ck);
     if (x->spec_alt) (void)hipFree(x->spec_alt);
@@ -1063,9 +1083,8 @@ int wrfdc_create(const wrfdc_config *cfg, wrfdc_ctx **out) {
   {
     hipError_t e = acoustic_col_allow_lds(x->kde);
     if (e != hipSuccess) {
-      fprintf(stderr, "wrfdc_create: LDS size for nz=%d: %s\n", cfg->nz, hipGetErrorString(e));
       wrfdc_destroy(x);
-      return WRFDC_EHIP;
+      return set_err(nullptr, WRFDC_EHIP, "LDS size for nz=" + std::to_string(cfg->nz) + ": " + hipGetErrorString(e));
     }
   }
   *out = x;
@@ -1173,12 +1192,13 @@ int wrfdc_stage(wrfdc_ctx *x, const char *name, int rk_step, int iteration) {
   });
 }
 
-int wrfdc_checksums(wrfdc_ctx *x, double out[4]) {
+int wrfdc_checksums(wrfdc_ctx *x, double out[WRFDC_NCHECKSUMS]) {
+  static_assert(CK_N == WRFDC_NCHECKSUMS, "checksum count");
   if (!out) return set_err(x, WRFDC_EINVAL, "out is null");
   int rc = guarded(x, [&]() {
     k_checksum_partial<<<kCkBlocks, CK_BLOCK, 0, x->s>>>(x->h, x->ck);
-    k_checksum_final<<<1, CK_BLOCK, 0, x->s>>>(x->ck, kCkBlocks, x->ck + kCkBlocks * 4);
-    hchk(hipMemcpyAsync(out, x->ck + kCkBlocks * 4, sizeof(double) * 4, hipMemcpyDeviceToHost, x->s), "copy checksums");
+    k_checksum_final<<<1, CK_BLOCK, 0, x->s>>>(x->ck, kCkBlocks, x->ck + kCkBlocks * CK_N);
+    hchk(hipMemcpyAsync(out, x->ck + kCkBlocks * CK_N, sizeof(double) * CK_N, hipMemcpyDeviceToHost, x->s), "copy checksums");
     hchk(hipStreamSynchronize(x->s), "synchronize checksums");
     return WRFDC_OK;
   });
@@ -1194,6 +1214,6 @@ int wrfdc_sync(wrfdc_ctx *x) {
   return WRFDC_OK;
 }
 
-const char *wrfdc_last_error(wrfdc_ctx *x) { return x ? x->err.c_str() : "null context"; }
+const char *wrfdc_last_error(wrfdc_ctx *x) { return x ? x->err.c_str() : g_thread_err.c_str(); }
 
 }  // extern "C"

@@ -69,7 +69,8 @@ def raise_for(rc: int, lib: ctypes.CDLL, ctx, what: str) -> None:
     """Map a negative return code onto the reference's exception types."""
     if rc == WRFDC_OK:
         return
-    msg = lib.wrfdc_last_error(ctx).decode() if ctx else ""
+    raw = lib.wrfdc_last_error(ctx)  # ctx None: the failure of a context-free call on this thread
+    msg = raw.decode() if raw else ""
     text = f"{what}: {msg}" if msg else what
     if rc == WRFDC_EINVAL:
         raise ValueError(text)

@@ -17,6 +17,9 @@ from . import _lib
 from .config import DycoreConfig
 from .fields import FIELD_SPECS, Dims
 
+# values returned by wrfdc_checksums (WRFDC_NCHECKSUMS in include/wrfdc.h)
+NCHECKSUMS = 6
+
 
 def _torch():
     import torch
@@ -93,8 +96,9 @@ class Domain:
         self._check(self.lib.wrfdc_stage(self.ctx, name.encode(), int(rk_step), int(iteration)), f"wrfdc_stage({name})")
 
     def checksums(self) -> np.ndarray:
-        """FP64 [sum(mu_2+mub), sum(t_2), sum(moist), n_nonfinite]; raises FloatingPointError on NaN/Inf."""
-        out = (ctypes.c_double * 4)()
+        """FP64 [sum(mu_2+mub), sum(t_2), sum(moist), n_nonfinite, dry-air mass (kg),
+        total energy (J)]; raises FloatingPointError on NaN/Inf."""
+        out = (ctypes.c_double * NCHECKSUMS)()
         self._check(self.lib.wrfdc_checksums(self.ctx, out), "wrfdc_checksums")
         return np.array(list(out))
 

@@ -171,7 +171,7 @@ DYNAMIC_ARRAY_NAMES = tuple(sorted(reference_name(n) for n, _ in FIELD_SPECS))
 @dataclass
 class SolveResult:
     """Outcome of one HIP step: FP64 checksums [sum(mu_2+mub), sum(t_2),
-    sum(moist), n_nonfinite] when requested."""
+    sum(moist), n_nonfinite, dry-air mass, total energy] when requested."""
 
     checksums: Any = None
 
