@@ -75,6 +75,21 @@ int wrfdc_set_stream(wrfdc_ctx *ctx, void *hip_stream);
 int wrfdc_step(wrfdc_ctx *ctx, int nsteps, double dtbc0);
 int wrfdc_stage(wrfdc_ctx *ctx, const char *name, int rk_step, int iteration);
 int wrfdc_checksums(wrfdc_ctx *ctx, double out[WRFDC_NCHECKSUMS]);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory or wait on the host (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the checksums of wrfdc_checksums
+ * without a context, an allocation or a synchronisation.  fields holds the
+ * device pointer of every registry field in wrfdc_field_name order; the ones
+ * the sums read must be non-null (u_2 v_2 w_2 t_2 ph_2 p pb phb mu_2 mub c1h
+ * c2h dnw msftx msfty moist).  Two kernels on hip_stream write
+ * WRFDC_NCHECKSUMS doubles to out_dev (device memory); non-finite values are
+ * counted in out_dev[3], not reported by the return code.  The partial sums
+ * live in a fixed 24 KiB device array of the library (512 blocks x 6
+ * doubles), one per GPU, shared by every call on that GPU: calls that overlap
+ * on one GPU race.  Any domain wrfdc_create accepts fits (fewer than 2^30
+ * elements per 3-D field); configurations it rejects are rejected here with
+ * the same codes, messages via wrfdc_last_error(NULL). */
+int wrfdc_checksums_noalloc(const wrfdc_config *cfg, void *const *fields, double *out_dev, void *hip_stream);
 int wrfdc_sync(wrfdc_ctx *ctx);
 /* Message of the last failure on ctx; with ctx NULL, of the last failure of
  * wrfdc_create (or another entry point without a context) on this thread. */

@@ -14,6 +14,7 @@
 #include <array>
 #include <cstring>
 #include <initializer_list>
+#include <mutex>
 #include <utility>
 #include <string>
 #include <vector>
@@ -156,6 +157,13 @@ static const char kKinds[] = {
 };
 
 static const int kCkBlocks = 512;
+
+// Partial sums of wrfdc_checksums_noalloc: one fixed array per GPU (each
+// device that loads the library holds its own copy until the process ends).
+// Host code never passes this name to a kernel: it is a host-side stand-in.
+// The device address comes from hipGetSymbolAddress, resolved once per
+// device with that device current (noalloc_scratch below).
+static __device__ __align__(256) double scratch[kCkBlocks * CK_N];
 
 struct wrfdc_ctx {
   wrfdc_config c;
@@ -869,7 +877,6 @@ struct Run {
   }
 
   void step() const {
-    R3 m = mass3();
     if (species_in_alt()) species_to(false);
     // initialize_moist_old (module_big_step_utilities_em.py:935-975); the
     // scalars' q_old <- q of the first-stage update is done here as well
@@ -965,6 +972,77 @@ int validate(const wrfdc_config &c, std::string &msg) {
   return WRFDC_OK;
 }
 
+// Device address of `scratch` on the current device: resolved with
+// hipGetSymbolAddress on a device's first call and kept per device.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+int noalloc_scratch(double **out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_err(nullptr, WRFDC_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return set_err(nullptr, WRFDC_EINVAL, "device index beyond " + std::to_string(kMaxDevices));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_err(nullptr, WRFDC_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return WRFDC_OK;
+}
+
+// Dimensions, scalars and relaxation weights of a DevCtx (no pointers).
+void fill_devctx(const wrfdc_config &c, DevCtx &h) {
+  const wrfdc_config *cfg = &c;
+  memset(&h, 0, sizeof(h));
+  h.ide = cfg->nx + 1;
+  h.jde = cfg->ny + 1;
+  h.kde = cfg->nz + 1;
+  h.ip = cfg->ip;
+  h.nmoist = cfg->nmoist;
+  h.nscalar = cfg->nscalar;
+  h.spec_zone = cfg->spec_zone;
+  h.relax_zone = cfg->relax_zone;
+  h.spec_bdy_width = cfg->spec_bdy_width;
+  h.top_lid = cfg->top_lid;
+  h.n3 = (size_t)h.jde * h.kde * h.ip;
+  h.n2 = (size_t)h.jde * h.ip;
+  h.rdx = (float)(1.0 / cfg->dx);
+  h.rdy = (float)(1.0 / cfg->dy);
+  h.dx = (float)cfg->dx;
+  h.dy = (float)cfg->dy;
+  h.dt = (float)cfg->dt;
+  h.g = (float)cfg->g;
+  h.t0 = (float)cfg->t0;
+  h.p0 = (float)cfg->p0;
+  h.ptop = (float)cfg->ptop;
+  h.r_d = (float)cfg->r_d;
+  h.cpovcv = (float)cfg->cpovcv;
+  h.rvovrd = (float)cfg->rvovrd;
+  h.epssm = (float)cfg->epssm;
+  h.smdiv = (float)cfg->smdiv;
+  h.emdiv = (float)cfg->emdiv;
+  h.c_s = (float)cfg->c_s;
+  h.prandtl = (float)cfg->prandtl;
+  h.cf1 = (float)cfg->cf1;
+  h.cf2 = (float)cfg->cf2;
+  h.cf3 = (float)cfg->cf3;
+  h.cfn = (float)cfg->cfn;
+  h.cfn1 = (float)cfg->cfn1;
+  h.reradius = (float)cfg->reradius;
+  // lbc_fcx_gcx (boundary/lbc_fcx_gcx.py:10-56), spec_exp = 0, in FP32
+  for (int b = cfg->spec_zone + 1; b <= cfg->spec_zone + cfg->relax_zone && b <= cfg->spec_bdy_width; b++) {
+    const float dist = (float)(cfg->spec_zone + cfg->relax_zone - b);
+    const float den = (float)(cfg->relax_zone - 1);
+    h.fcx[b] = ((0.1f / (float)cfg->dt) * dist) / den;
+    h.gcx[b] = (((1.0f / (float)cfg->dt) / 50.0f) * dist) / den;
+  }
+}
+
 int ensure_ready(wrfdc_ctx *x) {
   for (int f = 0; f < NFIELDS; f++) {
     if (x->h.F[f]) continue;
@@ -1016,50 +1094,7 @@ int wrfdc_create(const wrfdc_config *cfg, wrfdc_ctx **out) {
   x->n2 = (size_t)x->jde * x->ip;
   x->ns = num_sound_steps(*cfg);
   x->dtbc = cfg->dtbc;
-  DevCtx &h = x->h;
-  memset(&h, 0, sizeof(h));
-  h.ide = x->ide;
-  h.jde = x->jde;
-  h.kde = x->kde;
-  h.ip = x->ip;
-  h.nmoist = cfg->nmoist;
-  h.nscalar = cfg->nscalar;
-  h.spec_zone = cfg->spec_zone;
-  h.relax_zone = cfg->relax_zone;
-  h.spec_bdy_width = cfg->spec_bdy_width;
-  h.top_lid = cfg->top_lid;
-  h.n3 = x->n3;
-  h.n2 = x->n2;
-  h.rdx = (float)(1.0 / cfg->dx);
-  h.rdy = (float)(1.0 / cfg->dy);
-  h.dx = (float)cfg->dx;
-  h.dy = (float)cfg->dy;
-  h.dt = (float)cfg->dt;
-  h.g = (float)cfg->g;
-  h.t0 = (float)cfg->t0;
-  h.p0 = (float)cfg->p0;
-  h.ptop = (float)cfg->ptop;
-  h.r_d = (float)cfg->r_d;
-  h.cpovcv = (float)cfg->cpovcv;
-  h.rvovrd = (float)cfg->rvovrd;
-  h.epssm = (float)cfg->epssm;
-  h.smdiv = (float)cfg->smdiv;
-  h.emdiv = (float)cfg->emdiv;
-  h.c_s = (float)cfg->c_s;
-  h.prandtl = (float)cfg->prandtl;
-  h.cf1 = (float)cfg->cf1;
-  h.cf2 = (float)cfg->cf2;
-  h.cf3 = (float)cfg->cf3;
-  h.cfn = (float)cfg->cfn;
-  h.cfn1 = (float)cfg->cfn1;
-  h.reradius = (float)cfg->reradius;
-  // lbc_fcx_gcx (boundary/lbc_fcx_gcx.py:10-56), spec_exp = 0, in FP32
-  for (int b = cfg->spec_zone + 1; b <= cfg->spec_zone + cfg->relax_zone && b <= cfg->spec_bdy_width; b++) {
-    const float dist = (float)(cfg->spec_zone + cfg->relax_zone - b);
-    const float den = (float)(cfg->relax_zone - 1);
-    h.fcx[b] = ((0.1f / (float)cfg->dt) * dist) / den;
-    h.gcx[b] = (((1.0f / (float)cfg->dt) / 50.0f) * dist) / den;
-  }
+  fill_devctx(*cfg, x->h);
   try {
     hchk(hipMalloc((void **)&x->scratch, sizeof(float) * x->n3 * NSCRATCH), "hipMalloc scratch");
     hchk(hipMalloc((void **)&x->ck, sizeof(double) * (kCkBlocks * CK_N + CK_N)), "hipMalloc checksum");
@@ -1079,7 +1114,7 @@ int wrfdc_create(const wrfdc_config *cfg, wrfdc_ctx **out) {
     delete x;
     return WRFDC_EHIP;
   }
-  for (int i = 0; i < NSCRATCH; i++) h.S[i] = x->scratch + (size_t)i * x->n3;
+  for (int i = 0; i < NSCRATCH; i++) x->h.S[i] = x->scratch + (size_t)i * x->n3;
   {
     hipError_t e = acoustic_col_allow_lds(x->kde);
     if (e != hipSuccess) {
@@ -1204,6 +1239,29 @@ int wrfdc_checksums(wrfdc_ctx *x, double out[WRFDC_NCHECKSUMS]) {
   });
   if (rc != WRFDC_OK) return rc;
   if (out[3] > 0.0) return set_err(x, WRFDC_ENONFINITE, "non-finite values in the prognostic state");
+  return WRFDC_OK;
+}
+
+int wrfdc_checksums_noalloc(const wrfdc_config *cfg, void *const *fields, double *out_dev, void *hip_stream) {
+  if (!cfg || !fields || !out_dev) return set_err(nullptr, WRFDC_EINVAL, "cfg, fields and out_dev must not be null");
+  std::string msg;
+  int rc = validate(*cfg, msg);
+  if (rc != WRFDC_OK) return set_err(nullptr, rc, msg);
+  DevCtx h;
+  fill_devctx(*cfg, h);
+  for (int f = 0; f < NFIELDS; f++) h.F[f] = static_cast<float *>(fields[f]);
+  static const int needed[] = {F_u_2, F_v_2, F_w_2, F_t_2, F_ph_2, F_p, F_pb, F_phb, F_mu_2, F_mub,
+                               F_c1h, F_c2h, F_dnw, F_msftx, F_msfty, F_moist};
+  for (int f : needed)
+    if (!h.F[f]) return set_err(nullptr, WRFDC_EUNBOUND, std::string("field not given: ") + kNames[f]);
+  double *part = nullptr;
+  rc = noalloc_scratch(&part);
+  if (rc != WRFDC_OK) return rc;
+  hipStream_t s = static_cast<hipStream_t>(hip_stream);
+  k_checksum_partial<<<kCkBlocks, CK_BLOCK, 0, s>>>(h, part);
+  k_checksum_final<<<1, CK_BLOCK, 0, s>>>(part, kCkBlocks, out_dev);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err(nullptr, WRFDC_EHIP, std::string("checksum kernels: ") + hipGetErrorString(e));
   return WRFDC_OK;
 }
 
