@@ -232,6 +232,13 @@ def test_input_errors(tmp_path):
     nl.write_text(nl.read_text().replace("diff_opt = 1", "diff_opt = 2").replace("moist_adv_opt = 1", "moist_adv_opt = 2"))
     with pytest.raises(NotImplementedError, match="moist_adv_opt"):
         dycore_config(load_case(tmp_path / "diff"))
+    nl.write_text(nl.read_text().replace("moist_adv_opt = 2", "moist_adv_opt = 1"))
+    for line, key in ((" spec_exp = 0.33,", "spec_exp"), (" have_bcs_moist = .true.,", "have_bcs_moist")):
+        nl.write_text(nl.read_text().replace(" specified = .true.,", " specified = .true.,\n" + line))
+        with pytest.raises(NotImplementedError, match=key):
+            dycore_config(load_case(tmp_path / "diff"))
+        nl.write_text(nl.read_text().replace(line + "\n", ""))
+    dycore_config(load_case(tmp_path / "diff"))
     h5 = tmp_path / "h5"
     h5.mkdir()
     (h5 / "wrfinput_d01").write_bytes(b"\x89HDF\r\n\x1a\n" + bytes(64))

@@ -152,6 +152,13 @@ def dycore_config(case: CaseData) -> DycoreConfig:
         "h_sca_adv_order": int(_option(case, "H_SCA_ADV_ORDER", "dynamics", "h_sca_adv_order", 5)),
         "v_sca_adv_order": int(_option(case, "V_SCA_ADV_ORDER", "dynamics", "v_sca_adv_order", 3)),
         "top_lid": bool(_option(case, "TOP_LID", "dynamics", "top_lid", False)),
+        # relaxation-zone sponge exponent (runtime.py:3742; lbc_fcx_gcx scales
+        # the weights by exp(-(b - spec_zone - 1) * spec_exp)) and lateral
+        # boundary data for species beyond QV (runtime.py:3855-3856): the
+        # path implements spec_exp 0 and QV-only boundary data
+        "spec_exp": float(_option(case, "SPEC_EXP", "bdy_control", "spec_exp", 0.0)),
+        "have_bcs_moist": bool(_option(case, "HAVE_BCS_MOIST", "bdy_control", "have_bcs_moist", False)),
+        "have_bcs_scalar": bool(_option(case, "HAVE_BCS_SCALAR", "bdy_control", "have_bcs_scalar", False)),
     }
     out = DycoreConfig(
         nx=nx, ny=ny, nz=nz, ip=Dims(nx=nx, ny=ny, nz=nz, nmoist=nmoist, nscalar=nscalar).ip, nmoist=nmoist, nscalar=nscalar,
