@@ -91,6 +91,61 @@ int wrfdc_checksums(wrfdc_ctx *ctx, double out[WRFDC_NCHECKSUMS]);
  * the same codes, messages via wrfdc_last_error(NULL). */
 int wrfdc_checksums_noalloc(const wrfdc_config *cfg, void *const *fields, double *out_dev, void *hip_stream);
 int wrfdc_sync(wrfdc_ctx *ctx);
+/* ---- learned eddy viscosity (north-star extension of smag_km, the km of
+ * calculate_km_kh, src/wrf_jax/dyn_em/module_diffusion_em_fast.py:773-1083;
+ * no counterpart interface in the reference) ----
+ * A 55 -> 64 -> 64 -> 1 network (bf16 MFMA hidden layers, SiLU, softplus
+ * output) gives km from the 3x3x3 stencils of log|D| and N^2 and log(dx).
+ * Parameters: one FP32 vector theta of WRFDC_MLP_NPARAMS (W1[64][64], b1,
+ * W2[64][64], b2, w3, b3) and its bf16 weight copies wbf (W1, W2, W2^T;
+ * WRFDC_MLP_WBF_SIZE bf16 values), both in device memory. */
+#define WRFDC_MLP_NPARAMS 8385
+#define WRFDC_MLP_WBF_SIZE 12288
+#define WRFDC_MLP_FEATURES 64 /* bf16 per training row (55 used) */
+#define WRFDC_MLP_TARGETS 6   /* floats per training row: t1 t2 s1 s2 divisor km_smag */
+typedef struct wrfdc_closure_norm {
+  float d_shift, d_scale; /* log|D| features: (log(|D| + 1e-7) - d_shift) / d_scale */
+  float n_shift, n_scale; /* N^2 features: (N^2 - n_shift) / n_scale */
+  float dx_feature;       /* log(dx / 1000 m) */
+  float kref;             /* km = kref * softplus(network output) */
+} wrfdc_closure_norm;
+
+/* Use the network for km in the step's first-RK-stage diffusion (the slope
+ * divisor and the 10 l cap still apply; a non-finite output falls back to
+ * Smagorinsky).  theta NULL switches back to Smagorinsky.  The pointers are
+ * kept, not copied. */
+int wrfdc_set_closure(wrfdc_ctx *ctx, const float *theta, const void *wbf, const wrfdc_closure_norm *norm);
+/* Training rows of the current state, one per point of the smag2d_km range
+ * (i 2..nx-1, k 1..nz, j 2..ny-1; row = ((j-2)*nz + k-1)*(nx-2) + i-2):
+ * features (bf16 [n][64]) and target columns 2..5 (s1 = (D11-D22)/2,
+ * s2 = D12, slope divisor, Smagorinsky km) of rows_t ([n][6]).  Runs the
+ * diffusion metrics, deformation, N^2 and Smagorinsky kernels first. */
+long long wrfdc_closure_row_count(const wrfdc_config *cfg);
+int wrfdc_closure_rows(wrfdc_ctx *ctx, const wrfdc_closure_norm *norm, void *rows_x, float *rows_t);
+/* Columns 0..1 of the rows of a coarse grid (nx_coarse x ny_coarse, the fine
+ * grid divided by ratio): t1 = (tau11 - tau22)/2 and t2 = tau12, the
+ * horizontal sub-grid stresses of the fine domain's winds over each coarse
+ * cell. */
+int wrfdc_subgrid_stress(wrfdc_ctx *fine, int nx_coarse, int ny_coarse, int ratio, float *rows_t);
+/* Block means of the fine domain's dynamic fields (winds on their faces,
+ * w, theta, geopotential, pressure, inverse density, column mass, terrain
+ * and moisture) into the coarse domain's buffers, on the coarse stream. */
+int wrfdc_coarse_grain(wrfdc_ctx *fine, wrfdc_ctx *coarse, int ratio);
+/* Training: gradient of the loss sum over rows of
+ * ((t1 + K s1)^2 + (t2 + K s2)^2) * lossnorm, K = kref*softplus(z)/divisor
+ * (work: wrfdc_mlp_work_floats(nrows) floats; grad: WRFDC_MLP_NPARAMS;
+ * loss: 1 float), or that gradient followed by one Adam step on theta and
+ * wbf (moments adam_m, adam_v of WRFDC_MLP_NPARAMS, step >= 1).  All device
+ * pointers, stream-ordered, no allocation. */
+long long wrfdc_mlp_work_floats(long long nrows);
+int wrfdc_mlp_pack(const float *theta, void *wbf, void *hip_stream);
+int wrfdc_mlp_grad(const float *theta, const void *wbf, const void *rows_x, const float *rows_t, long long nrows, float lossnorm, float kref,
+                   float *work, float *grad, float *loss, void *hip_stream);
+int wrfdc_mlp_train_step(float *theta, void *wbf, const void *rows_x, const float *rows_t, long long nrows, float lossnorm, float kref,
+                         float *work, float *adam_m, float *adam_v, int step, float lr, float *loss, void *hip_stream);
+/* Network output z (before softplus) of each row. */
+int wrfdc_mlp_forward(const float *theta, const void *wbf, const void *rows_x, long long nrows, float *z, void *hip_stream);
+
 /* Message of the last failure on ctx; with ctx NULL, of the last failure of
  * wrfdc_create (or another entry point without a context) on this thread. */
 const char *wrfdc_last_error(wrfdc_ctx *ctx);

@@ -44,7 +44,7 @@ typedef OREAL real;
   X(advect_tend, '3') X(rho, '3') X(z, '3') X(rdz, '3') X(rdzw, '3')         \
   X(zx, '3') X(zy, '3') X(div, '3') X(defor11, '3') X(defor22, '3')          \
   X(defor33, '3') X(defor12, '3') X(defor13, '3') X(defor23, '3')            \
-  X(xkmh, '3') X(xkmv, '3') X(xkhh, '3') X(xkhv, '3')                        \
+  X(xkmh, '3') X(xkmv, '3') X(xkhh, '3') X(xkhv, '3') X(bn2, '3')            \
   X(mu_1, '2') X(mu_2, '2') X(mub, '2') X(mut, '2') X(muu, '2') X(muv, '2')  \
   X(muts, '2') X(muus, '2') X(muvs, '2') X(muave, '2') X(mudf, '2')          \
   X(mu_tend, '2') X(mu_tendf, '2') X(mu_save, '2') X(ht, '2') X(msfux, '2')  \

@@ -285,6 +285,113 @@ void o_smag2d_km(OCtx *x) {
       }
 }
 
+/* phy_prep (module_big_step_utilities_em.py:1823-2081, use_theta_m 0): full
+ * theta, pressure and temperature at mass points and the surface values of
+ * pressure and temperature on w level 1 (linear in height through mass
+ * levels 1 and 2).  Arrays are full 3-D temporaries. */
+static void phy_prep_thermo(OCtx *x, real *th, real *pp, real *tt, real *p8w1, real *t8w1) {
+  real *t2 = FLD(t_2), *p = FLD(p), *pb = FLD(pb), *ph = FLD(ph_2), *phb = FLD(phb);
+  real t0 = (real)x->c.t0, g = (real)x->c.g;
+  real rcp = (real)(287.0 / (7.0 * 287.0 / 2.0)), p1000 = (real)100000.0;
+  for (int j = 1; j <= JDE - 1; j++)
+    for (int k = 1; k <= KDE - 1; k++)
+      for (int i = 1; i <= IDE - 1; i++) {
+        real thv = A3(t2, i, k, j) + t0;
+        real pv = A3(p, i, k, j) + A3(pb, i, k, j);
+        real pi = (real)pow((double)(pv / p1000), (double)rcp);
+        A3(th, i, k, j) = thv;
+        A3(pp, i, k, j) = pv;
+        A3(tt, i, k, j) = thv * pi;
+      }
+  for (int j = 1; j <= JDE - 1; j++)
+    for (int i = 1; i <= IDE - 1; i++) {
+      real zw1 = (A3(phb, i, 1, j) + A3(ph, i, 1, j)) / g, zw2 = (A3(phb, i, 2, j) + A3(ph, i, 2, j)) / g;
+      real zw3 = (A3(phb, i, 3, j) + A3(ph, i, 3, j)) / g;
+      real z1 = (real)0.5 * (zw1 + zw2), z2 = (real)0.5 * (zw2 + zw3);
+      real w1 = (zw1 - z2) / (z1 - z2), w2 = (real)1 - w1;
+      A2(p8w1, i, j) = w1 * A3(pp, i, 1, j) + w2 * A3(pp, i, 2, j);
+      A2(t8w1, i, j) = w1 * A3(tt, i, 1, j) + w2 * A3(tt, i, 2, j);
+    }
+}
+
+/* Brunt-Vaisala frequency squared of calculate_km_kh
+ * (module_diffusion_em_fast.py:933-1005; calculate_n2,
+ * module_diffusion_em.py:1319-1499): moist where the air is saturated
+ * (qv >= qvs or qc >= 1e-5), dry otherwise; level 1 uses the surface values,
+ * level ktf repeats ktf-1.  i [2, ide-2], j [2, jde-2]; qc and qi are moist
+ * species 1 and 3 when present. */
+void o_calc_n2(OCtx *x) {
+  real *bn2 = FLD(bn2), *rdz = FLD(rdz), *rdzw = FLD(rdzw);
+  real *th = (real *)calloc(x->n3, sizeof(real)), *pp = (real *)calloc(x->n3, sizeof(real)), *tt = (real *)calloc(x->n3, sizeof(real));
+  real *qvs = (real *)calloc(x->n3, sizeof(real)), *qt = (real *)calloc(x->n3, sizeof(real));
+  real *p8w1 = (real *)calloc(x->n2, sizeof(real)), *t8w1 = (real *)calloc(x->n2, sizeof(real));
+  phy_prep_thermo(x, th, pp, tt, p8w1, t8w1);
+  const real *qv = SPC(moist, 0);
+  const real *qc = x->c.nmoist > 1 ? SPC(moist, 1) : 0;
+  const real *qi = x->c.nmoist > 3 ? SPC(moist, 3) : 0;
+  real G = (real)9.81, R_D = (real)287.0, CP = (real)(7.0 * 287.0 / 2.0), XLV = (real)2.5e6, RV = (real)461.6;
+  real SVP1 = (real)0.6112, SVP2 = (real)17.67, SVP3 = (real)29.65, SVPT0 = (real)273.15, EP_2 = (real)(287.0 / 461.6);
+  real cf1 = (real)x->c.cf1, cf2 = (real)x->c.cf2, cf3 = (real)x->c.cf3, P1000MB = (real)100000.0;
+  real qcmin = (real)0.00001, c161 = (real)1.61;
+  int ktf = KDE - 1;
+  for (int j = 2; j <= JDE - 2; j++)
+    for (int k = 1; k <= ktf; k++)
+      for (int i = 2; i <= IDE - 2; i++) {
+        real tv = A3(tt, i, k, j), tc = tv - SVPT0;
+        real es = (real)1000.0 * SVP1 * (real)exp((double)(SVP2 * tc / (tv - SVP3)));
+        A3(qvs, i, k, j) = EP_2 * es / (A3(pp, i, k, j) - es);
+        real q = A3(qv, i, k, j);
+        if (qc) q = q + A3(qc, i, k, j);
+        if (qi) q = q + A3(qi, i, k, j);
+        A3(qt, i, k, j) = q;
+      }
+#define QC(i, k, j) (qc ? A3(qc, i, k, j) : (real)0)
+  for (int j = 2; j <= JDE - 2; j++)
+    for (int i = 2; i <= IDE - 2; i++) {
+      for (int k = 2; k <= ktf - 1; k++) {
+        real tmpdz = (real)1.0 / A3(rdz, i, k, j) + (real)1.0 / A3(rdz, i, k + 1, j);
+        int sat = (A3(qv, i, k, j) >= A3(qvs, i, k, j)) || (QC(i, k, j) >= qcmin);
+        real tk = A3(tt, i, k, j), thk = A3(th, i, k, j), xlvqv = XLV * A3(qv, i, k, j);
+        real val;
+        if (sat) {
+          real coefa = ((real)1.0 + xlvqv / R_D / tk) / ((real)1.0 + XLV * xlvqv / CP / RV / tk / tk) / thk;
+          real thetaep1 = A3(th, i, k + 1, j) * ((real)1.0 + XLV * A3(qvs, i, k + 1, j) / CP / A3(tt, i, k + 1, j));
+          real thetaem1 = A3(th, i, k - 1, j) * ((real)1.0 + XLV * A3(qvs, i, k - 1, j) / CP / A3(tt, i, k - 1, j));
+          val = G * (coefa * (thetaep1 - thetaem1) / tmpdz - (A3(qt, i, k + 1, j) - A3(qt, i, k - 1, j)) / tmpdz);
+        } else {
+          val = G * ((A3(th, i, k + 1, j) - A3(th, i, k - 1, j)) / thk / tmpdz +
+                     c161 * (A3(qv, i, k + 1, j) - A3(qv, i, k - 1, j)) / tmpdz - (A3(qt, i, k + 1, j) - A3(qt, i, k - 1, j)) / tmpdz);
+        }
+        A3(bn2, i, k, j) = val;
+      }
+      {
+        real tmpdz_m = (real)1.0 / A3(rdz, i, 2, j) + (real)0.5 / A3(rdzw, i, 1, j);
+        real thsfc = A2(t8w1, i, j) / (real)pow((double)(A2(p8w1, i, j) / P1000MB), (double)(R_D / CP));
+        int sat = (A3(qv, i, 1, j) >= A3(qvs, i, 1, j)) || (QC(i, 1, j) >= qcmin);
+        real qvsfc_sat = cf1 * A3(qvs, i, 1, j) + cf2 * A3(qvs, i, 2, j) + cf3 * A3(qvs, i, 3, j);
+        real t1 = A3(tt, i, 1, j), th1 = A3(th, i, 1, j), xlvqv = XLV * A3(qv, i, 1, j);
+        real qtsfc = cf1 * A3(qv, i, 1, j) + cf2 * A3(qv, i, 2, j) + cf3 * A3(qv, i, 3, j);
+        if (qc) qtsfc = qtsfc + cf1 * A3(qc, i, 1, j) + cf2 * A3(qc, i, 2, j) + cf3 * A3(qc, i, 3, j);
+        if (qi) qtsfc = qtsfc + cf1 * A3(qi, i, 1, j) + cf2 * A3(qi, i, 2, j) + cf3 * A3(qi, i, 3, j);
+        real val;
+        if (sat) {
+          real coefa = ((real)1.0 + xlvqv / R_D / t1) / ((real)1.0 + XLV * xlvqv / CP / RV / t1 / t1) / th1;
+          real thetaep1 = A3(th, i, 2, j) * ((real)1.0 + XLV * A3(qvs, i, 2, j) / CP / A3(tt, i, 2, j));
+          real thetaesfc = thsfc * ((real)1.0 + XLV * qvsfc_sat / CP / A2(t8w1, i, j));
+          val = G * (coefa * (thetaep1 - thetaesfc) / tmpdz_m - (A3(qt, i, 2, j) - qtsfc) / tmpdz_m);
+        } else {
+          real qvsfc = cf1 * A3(qv, i, 1, j) + cf2 * A3(qv, i, 2, j) + cf3 * A3(qv, i, 3, j);
+          real tmpdz_d = (real)1.0 / A3(rdzw, i, 1, j);
+          val = G * ((A3(th, i, 2, j) - th1) / th1 / tmpdz_d + c161 * (A3(qv, i, 2, j) - qvsfc) / tmpdz_d - (A3(qt, i, 2, j) - qtsfc) / tmpdz_d);
+        }
+        A3(bn2, i, 1, j) = val;
+      }
+      A3(bn2, i, ktf, j) = A3(bn2, i, ktf - 1, j);
+    }
+#undef QC
+  free(th); free(pp); free(tt); free(qvs); free(qt); free(p8w1); free(t8w1);
+}
+
 /* titau helpers (module_diffusion_em.py:894-1316): stress components with
  * rho and diffusivity averaged to the component's point.  Output arrays are
  * full 3-D, zero outside the computed range. */

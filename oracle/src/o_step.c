@@ -51,6 +51,7 @@ void o_phy_prep_rho(OCtx *x);
 void o_compute_diff_metrics(OCtx *x);
 void o_cal_deform_and_div(OCtx *x);
 void o_smag2d_km(OCtx *x);
+void o_calc_n2(OCtx *x);
 void o_horizontal_diffusion_2(OCtx *x);
 
 static const char *o_names[] = {
@@ -177,6 +178,9 @@ void o_first_rk_step_part2(OCtx *x) {
   o_bc3(x, FLD(zx), 'e');
   o_bc3(x, FLD(zy), 'f');
   o_cal_deform_and_div(x);
+  /* calculate_km_kh, km_opt 4: N^2 (an output no later stage reads here)
+   * and the Smagorinsky diffusivities */
+  o_calc_n2(x);
   o_smag2d_km(x);
   /* phy_bc (module_diffusion_em.py:180-363) */
   o_bc3(x, FLD(xkmh), 't');
@@ -398,6 +402,8 @@ int o_run_stage(OCtx *x, const char *name, int rk_step, int iteration) {
   else if (IS("compute_diff_metrics")) o_compute_diff_metrics(x);
   else if (IS("cal_deform_and_div")) o_cal_deform_and_div(x);
   else if (IS("smag2d_km")) o_smag2d_km(x);
+  else if (IS("calculate_n2")) o_calc_n2(x);
+  else if (IS("calculate_km_kh")) { o_calc_n2(x); o_smag2d_km(x); }
   else if (IS("horizontal_diffusion_2")) o_horizontal_diffusion_2(x);
   else if (IS("rk_tendency")) o_rk_tendency(x);
   else if (IS("advect_u")) o_advect_u(x, FLD(u_2), FLD(ru_tend), FLD(ru), FLD(rv), FLD(ww));

@@ -32,10 +32,12 @@ def nrmse(a: np.ndarray, b: np.ndarray) -> float:
     return rmse / rng
 
 
-# Diagnostic output of the reference's species stage (the last species'
-# advective tendency, runtime.py:14830); no later stage reads it and the fused
-# species kernels of the step do not write it.
-STEP_DIAGNOSTICS = ("advect_tend",)
+# Diagnostic outputs no later stage of the step reads, which the fused step
+# does not write: the last species' advective tendency of the reference's
+# species stage (runtime.py:14830), and calculate_km_kh's N^2 with km_opt 4
+# (module_diffusion_em_fast.py:933-1005; the step computes it only when the
+# learned closure takes it as an input).
+STEP_DIAGNOSTICS = ("advect_tend", "bn2")
 
 
 def compare(gpu: dict, ref: dict, names=None, tol: float = REL_L2_STEP, skip=()) -> dict:

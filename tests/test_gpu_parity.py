@@ -53,7 +53,8 @@ def domain_for(gpu):
 
 STAGES = [
     ("rk_step_prep", 1, 0), ("first_rk_step_part1", 1, 0), ("compute_diff_metrics", 1, 0),
-    ("cal_deform_and_div", 1, 0), ("smag2d_km", 1, 0), ("horizontal_diffusion_2", 1, 0),
+    ("cal_deform_and_div", 1, 0), ("smag2d_km", 1, 0), ("calculate_n2", 1, 0), ("calculate_km_kh", 1, 0),
+    ("horizontal_diffusion_2", 1, 0),
     ("first_rk_step_part2", 1, 0), ("advect_u", 2, 0), ("advect_v", 2, 0), ("advect_w", 2, 0),
     ("advect_scalar", 2, 0), ("advect_scalar_pd", 3, 0), ("rhs_ph", 2, 0),
     ("horizontal_pressure_gradient", 2, 0), ("pg_buoy_w", 2, 0), ("rk_tendency", 2, 0),

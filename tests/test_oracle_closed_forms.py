@@ -595,3 +595,82 @@ def test_relax_zone_davies_weights():
             got = out["ru_tendf"][mid_j - 1][:, :kt, i - 1]  # (j, k)
             expect = f1 * (-q[mid_j - 1, i - 1]) - f2 * (-4.0 * a)
             np.testing.assert_allclose(got, np.broadcast_to(expect[:, None], got.shape), rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# Brunt-Vaisala frequency (calculate_km_kh's N^2)
+# ---------------------------------------------------------------------------
+def _column_state(theta_of_z, qv_of_z, qc=0.0, nz=12):
+    """Flat columns with geopotential levels 0, 500, 1000, ... m (mass levels
+    midway), theta(z), qv(z), a constant cloud water qc, and a pressure that
+    falls with a 8 km scale height; metrics from compute_diff_metrics."""
+    config, d, A = blank(nz=nz)
+    nmoist = 2 if qc else 1
+    dims = Dims(NX, NY, nz, nmoist=nmoist, nscalar=0)
+    vc = vertical_coordinate(nz)
+    config = DycoreConfig(nx=NX, ny=NY, nz=nz, ip=dims.ip, nmoist=nmoist, nscalar=0, cf1=vc["cf1"], cf2=vc["cf2"],
+                          cf3=vc["cf3"], cfn=vc["cfn"], cfn1=vc["cfn1"])
+    B = {n: np.zeros(dims.shape(k), dtype=np.float64) for n, k in FIELD_SPECS}
+    for n in A:
+        if A[n].shape == B[n].shape:
+            B[n][:] = A[n]
+    zw = 500.0 * np.arange(nz + 1)
+    B["phb"][:] = (config.g * zw)[None, :, None]
+    zm = np.zeros(nz + 1)
+    zm[:nz] = 0.5 * (zw[:-1] + zw[1:])
+    B["t_2"][:] = (theta_of_z(zm) - config.t0)[None, :, None]
+    B["pb"][:] = (1.0e5 * np.exp(-zm / 8000.0))[None, :, None]
+    B["moist"][0] = qv_of_z(zm)[None, :, None]
+    if qc:
+        B["moist"][1][:] = qc
+    return config, dims, B, zm
+
+
+def _n2(config, B):
+    out = run(config, B, ("compute_diff_metrics", 1, 0), ("calculate_n2", 1, 0))
+    return out["bn2"]
+
+
+def test_n2_dry_linear_theta():
+    """Dry air with theta = theta0 + Gamma*z: the centred N^2 of every interior
+    level is exactly g*Gamma/theta(z); the top level repeats the one below."""
+    G0, th0 = 0.004, 290.0
+    config, d, B, zm = _column_state(lambda z: th0 + G0 * z, lambda z: 0.0 * z)
+    bn2 = _n2(config, B)
+    kt = d.kde - 1
+    ks = np.arange(2, kt)  # Fortran levels 2..ktf-1
+    expect = config.g * G0 / (th0 + G0 * zm[ks - 1])
+    got = bn2[8, ks - 1, 9]
+    np.testing.assert_allclose(got, expect, rtol=1e-12)
+    assert bn2[8, kt - 1, 9] == bn2[8, kt - 2, 9]
+
+
+def test_n2_unsaturated_moisture_gradient():
+    """Unsaturated air with qv = q0 + beta*z under isentropic theta: the
+    virtual-temperature term 1.61*dqv/dz less the total-water term dqv/dz
+    leaves N^2 = 0.61*g*beta."""
+    q0, beta = 5.0e-4, -8.0e-8  # below saturation at every level
+    config, d, B, zm = _column_state(lambda z: 300.0 + 0.0 * z, lambda z: q0 + beta * z)
+    bn2 = _n2(config, B)
+    ks = np.arange(2, d.kde - 1)
+    np.testing.assert_allclose(bn2[8, ks - 1, 9], 0.61 * config.g * beta, rtol=1e-10)
+
+
+def test_n2_saturated_uses_equivalent_potential_temperature():
+    """Cloudy air (qc >= 1e-5 marks it saturated) with no vapour and constant
+    total water: N^2 = g/theta * d(theta_e)/dz with theta_e = theta*(1 +
+    L*qs/(cp*T)), T = theta*(p/1e5)^(R/cp) and the saturation mixing ratio
+    qs = 0.622*es/(p - es) of es = 611.2*exp(17.67*(T-273.15)/(T-29.65)) Pa
+    (Bolton 1980), centred over the two neighbouring levels."""
+    config, d, B, zm = _column_state(lambda z: 290.0 + 0.003 * z, lambda z: 0.0 * z, qc=2.0e-5)
+    bn2 = _n2(config, B)
+    L, cp, R = 2.5e6, 3.5 * 287.0, 287.0
+    th = 290.0 + 0.003 * zm
+    p = 1.0e5 * np.exp(-zm / 8000.0)
+    T = th * (p / 1.0e5) ** (R / cp)
+    es = 611.2 * np.exp(17.67 * (T - 273.15) / (T - 29.65))
+    qs = (287.0 / 461.6) * es / (p - es)
+    the = th * (1.0 + L * qs / (cp * T))
+    ks = np.arange(2, d.kde - 1)
+    expect = config.g / th[ks - 1] * (the[ks] - the[ks - 2]) / (zm[ks] - zm[ks - 2])
+    np.testing.assert_allclose(bn2[8, ks - 1, 9], expect, rtol=1e-9)

@@ -82,6 +82,10 @@ MUTATIONS = [
     ("coef_w_super_diagonal_level", "o_small.c", "A3(c2a, i, k, j)) / (mh * mfa);", "A3(c2a, i, k, j)) / (mh * mf);", 0),
     ("coef_w_top_pivot_sign", "o_small.c", "(real)1 / (btop - A3(a, i, kde, j) * gprev);", "(real)1 / (btop + A3(a, i, kde, j) * gprev);", 0),
     ("relax_weight_coefficient", "o_step.c", "(real)((real)0.1 / (real)cfg->dt)", "(real)((real)0.2 / (real)cfg->dt)", 0),
+    ("n2_virtual_coefficient", "o_diff.c", "c161 = (real)1.61;", "c161 = (real)1.0;", 0),
+    ("n2_saturated_theta_e_difference", "o_diff.c", "(thetaep1 - thetaem1)", "(thetaem1 - thetaep1)", 0),
+    ("n2_dry_theta_one_sided", "o_diff.c", "(A3(th, i, k + 1, j) - A3(th, i, k - 1, j)) / thk",
+     "(A3(th, i, k + 1, j) - A3(th, i, k, j)) / thk", 0),
     ("advect_scalar_x_divergence_sign", "o_advect.c",
      "for (int i = 2; i <= ide - 2; i++) A3(tend, i, k, j) -= A2(msftx, i, j) * rdx * (FQ(i + 1, k) - FQ(i, k));",
      "for (int i = 2; i <= ide - 2; i++) A3(tend, i, k, j) += A2(msftx, i, j) * rdx * (FQ(i + 1, k) - FQ(i, k));", 0),

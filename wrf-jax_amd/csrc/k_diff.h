@@ -182,8 +182,9 @@ __global__ void k_edge_copy(const DevCtx xv, float *f0, float *f1, int axis, int
 }
 
 // smag2d_km (module_diffusion_em.py:1502-1646) with the slope correction;
-// i [2, ide-2], j [2, jde-2], k [1, kde-1].
-__global__ void k_smag(const DevCtx xv, R3 r) {
+// i [2, ide-2], j [2, jde-2], k [1, kde-1].  With the learned closure on,
+// |D| = sqrt(def2) and the slope divisor are stored too (defmag, divisor).
+__global__ void k_smag(const DevCtx xv, R3 r, float *defmag = nullptr, float *divout = nullptr) {
   const DevCtx *__restrict__ x = &xv;
   GET_IKJ(r);
   const float *d11 = FL(defor11), *d22 = FL(defor22), *d12 = FL(defor12), *zx = FL(zx), *zy = FL(zy), *rdzw = FL(rdzw);
@@ -206,10 +207,109 @@ __global__ void k_smag(const DevCtx xv, R3 r) {
   const float lim = fmaxf(10.f / mlen, 1.0e-3f);
   const float divisor = sd > lim ? alpha * alpha : alpha;
   km = km / divisor;
+  if (defmag) {
+    A3(defmag, i, k, j) = sd;
+    A3(divout, i, k, j) = divisor;
+  }
   A3(FL(xkmh), i, k, j) = km;
   A3(FL(xkmv), i, k, j) = km;
   A3(FL(xkhh), i, k, j) = km / x->prandtl;
   A3(FL(xkhv), i, k, j) = 0.f;
+}
+
+// phy_prep's full theta, pressure and temperature at one mass point
+// (module_big_step_utilities_em.py:1823-2081, use_theta_m 0)
+struct Thermo {
+  float th, p, t;
+};
+__device__ __forceinline__ Thermo thermo_at(const DevCtx *__restrict__ x, int i, int k, int j) {
+  Thermo r;
+  r.th = A3(FL(t_2), i, k, j) + x->t0;
+  r.p = A3(FL(p), i, k, j) + A3(FL(pb), i, k, j);
+  const float rcp = (float)(287.0 / (7.0 * 287.0 / 2.0));
+  r.t = r.th * powf(r.p / 100000.0f, rcp);
+  return r;
+}
+// saturation mixing ratio of calculate_km_kh
+__device__ __forceinline__ float qvs_of(float t, float p) {
+  const float es = 1000.0f * 0.6112f * expf(17.67f * (t - 273.15f) / (t - 29.65f));
+  return (float)(287.0 / 461.6) * es / (p - es);
+}
+__device__ __forceinline__ float qtot_at(const DevCtx *__restrict__ x, int i, int k, int j) {
+  float q = A3(SP(moist, 0), i, k, j);
+  if (x->nmoist > 1) q = q + A3(SP(moist, 1), i, k, j);
+  if (x->nmoist > 3) q = q + A3(SP(moist, 3), i, k, j);
+  return q;
+}
+__device__ __forceinline__ float qc_at(const DevCtx *__restrict__ x, int i, int k, int j) {
+  return x->nmoist > 1 ? A3(SP(moist, 1), i, k, j) : 0.f;
+}
+
+// N^2 of calculate_km_kh (module_diffusion_em_fast.py:933-1005): moist where
+// saturated (qv >= qvs or qc >= 1e-5), dry otherwise; level 1 from the
+// surface values (pressure and temperature on w level 1 through mass levels
+// 1 and 2, linear in height), level ktf repeats ktf-1.  i [2, ide-2],
+// k [1, kde-1], j [2, jde-2]; one thread per point.
+__global__ void k_n2(const DevCtx xv, R3 r) {
+  const DevCtx *__restrict__ x = &xv;
+  GET_IKJ_P(r);
+  const float G = 9.81f, R_D = 287.0f, CP = (float)(7.0 * 287.0 / 2.0), XLV = 2.5e6f, RV = 461.6f, C161 = 1.61f;
+  const float *qv = SP(moist, 0), *rdz = FL(rdz), *rdzw = FL(rdzw);
+  const int ktf = x->kde - 1;
+  float val;
+  if (k >= 2) {
+    const int kk = k < ktf ? k : ktf - 1;  // level ktf repeats ktf-1
+    const Thermo a = thermo_at(x, i, kk - 1, j), b = thermo_at(x, i, kk, j), c = thermo_at(x, i, kk + 1, j);
+    const float tmpdz = 1.0f / A3(rdz, i, kk, j) + 1.0f / A3(rdz, i, kk + 1, j);
+    const float qvk = A3(qv, i, kk, j);
+    const bool sat = (qvk >= qvs_of(b.t, b.p)) || (qc_at(x, i, kk, j) >= 0.00001f);
+    const float dqt = qtot_at(x, i, kk + 1, j) - qtot_at(x, i, kk - 1, j);
+    if (sat) {
+      const float xlvqv = XLV * qvk;
+      const float coefa = (1.0f + xlvqv / R_D / b.t) / (1.0f + XLV * xlvqv / CP / RV / b.t / b.t) / b.th;
+      const float thetaep1 = c.th * (1.0f + XLV * qvs_of(c.t, c.p) / CP / c.t);
+      const float thetaem1 = a.th * (1.0f + XLV * qvs_of(a.t, a.p) / CP / a.t);
+      val = G * (coefa * (thetaep1 - thetaem1) / tmpdz - dqt / tmpdz);
+    } else {
+      val = G * ((c.th - a.th) / b.th / tmpdz + C161 * (A3(qv, i, kk + 1, j) - A3(qv, i, kk - 1, j)) / tmpdz - dqt / tmpdz);
+    }
+  } else {
+    const float g = x->g, cf1 = x->cf1, cf2 = x->cf2, cf3 = x->cf3;
+    const Thermo a = thermo_at(x, i, 1, j), b = thermo_at(x, i, 2, j), c = thermo_at(x, i, 3, j);
+    const float zw1 = (A3(FL(phb), i, 1, j) + A3(FL(ph_2), i, 1, j)) / g, zw2 = (A3(FL(phb), i, 2, j) + A3(FL(ph_2), i, 2, j)) / g;
+    const float zw3 = (A3(FL(phb), i, 3, j) + A3(FL(ph_2), i, 3, j)) / g;
+    const float z1 = 0.5f * (zw1 + zw2), z2 = 0.5f * (zw2 + zw3);
+    const float w1 = (zw1 - z2) / (z1 - z2), w2 = 1.0f - w1;
+    const float p8w = w1 * a.p + w2 * b.p, t8w = w1 * a.t + w2 * b.t;
+    const float qvs1 = qvs_of(a.t, a.p), qvs2 = qvs_of(b.t, b.p), qvs3 = qvs_of(c.t, c.p);
+    const float qv1 = A3(qv, i, 1, j), qv2 = A3(qv, i, 2, j), qv3 = A3(qv, i, 3, j);
+    float qtsfc = cf1 * qv1 + cf2 * qv2 + cf3 * qv3;
+    if (x->nmoist > 1) {
+      const float *qc = SP(moist, 1);
+      qtsfc = qtsfc + cf1 * A3(qc, i, 1, j) + cf2 * A3(qc, i, 2, j) + cf3 * A3(qc, i, 3, j);
+    }
+    if (x->nmoist > 3) {
+      const float *qi = SP(moist, 3);
+      qtsfc = qtsfc + cf1 * A3(qi, i, 1, j) + cf2 * A3(qi, i, 2, j) + cf3 * A3(qi, i, 3, j);
+    }
+    const float dqt = qtot_at(x, i, 2, j) - qtsfc;
+    const bool sat = (qv1 >= qvs1) || (qc_at(x, i, 1, j) >= 0.00001f);
+    if (sat) {
+      const float tmpdz = 1.0f / A3(rdz, i, 2, j) + 0.5f / A3(rdzw, i, 1, j);
+      const float thsfc = t8w / powf(p8w / 100000.0f, R_D / CP);
+      const float qvsfc_sat = cf1 * qvs1 + cf2 * qvs2 + cf3 * qvs3;
+      const float xlvqv = XLV * qv1;
+      const float coefa = (1.0f + xlvqv / R_D / a.t) / (1.0f + XLV * xlvqv / CP / RV / a.t / a.t) / a.th;
+      const float thetaep1 = b.th * (1.0f + XLV * qvs2 / CP / b.t);
+      const float thetaesfc = thsfc * (1.0f + XLV * qvsfc_sat / CP / t8w);
+      val = G * (coefa * (thetaep1 - thetaesfc) / tmpdz - dqt / tmpdz);
+    } else {
+      const float tmpdz = 1.0f / A3(rdzw, i, 1, j);
+      const float qvsfc = cf1 * qv1 + cf2 * qv2 + cf3 * qv3;
+      val = G * ((b.th - a.th) / a.th / tmpdz + C161 * (qv2 - qvsfc) / tmpdz - dqt / tmpdz);
+    }
+  }
+  A3(FL(bn2), i, k, j) = val;
 }
 
 // titau (module_diffusion_em.py:894-1316): stress component with rho and the

@@ -27,6 +27,7 @@
 #include "k_small.h"
 #include "k_acoustic.h"
 #include "k_species.h"
+#include "k_closure.h"
 
 // ---------------------------------------------------------------------------
 // small helper kernels local to the orchestration
@@ -177,6 +178,11 @@ struct wrfdc_ctx {
   float *spec_anti = nullptr;   // positive-definite antidiffusive face fluxes, three 3-D arrays per species
   float *hds = nullptr;         // horizontal scalar diffusion face fluxes, two 3-D arrays per diffused scalar
   double *ck = nullptr;  // kCkBlocks * CK_N partials + CK_N results
+  // learned eddy viscosity (wrfdc_set_closure): parameters, bf16 weight
+  // copies and feature normalisation; theta null = Smagorinsky
+  const float *cl_theta = nullptr;
+  const __bf16 *cl_wbf = nullptr;
+  ClosureNorm cl_norm{};
   int rk = 1, nsmall = 1, iteration = 1;
   double dt_rk = 0.0, dts_rk = 0.0, dtbc = 0.0;
   std::string err;
@@ -433,6 +439,33 @@ struct Run {
     R3 r{2, ide - 2, 1, kde - 1, 2, jde - 2};
     k_smag<<<g3(r), b3(), 0, s>>>(d, r);
   }
+  // N^2 of calculate_km_kh (read by no later stage of the km_opt 4 path
+  // itself; the learned closure takes it as an input)
+  void calculate_n2() const {
+    R3 r{2, ide - 2, 1, kde - 1, 2, jde - 2};
+    k_n2<<<g3p(r), b3(), 0, s>>>(d, r);
+  }
+  // Smagorinsky km storing |D| and the slope divisor in the scratch arrays
+  // S_T1 and S_T2 (free until the diffusion's stresses), then the learned km
+  // over the same points when the context has a network
+  void smag_for_closure() const {
+    R3 r{2, ide - 2, 1, kde - 1, 2, jde - 2};
+    k_smag<<<g3(r), b3(), 0, s>>>(d, r, Scr(S_T1), Scr(S_T2));
+  }
+  void closure_km() const {
+    R3 r{2, ide - 2, 1, kde - 1, 2, jde - 2};
+    k_closure_km<<<g3p(r), b3(), 0, s>>>(d, r, X->cl_theta, X->cl_wbf, X->cl_norm, Scr(S_T1), Scr(S_T2));
+  }
+  // training rows of the current state (features, s1, s2, divisor, km)
+  void closure_rows(const ClosureNorm &nm, __bf16 *Xr, float *T) const {
+    compute_diff_metrics();
+    BcBatch(this).add3(F(F_rdzw), 'w').add3(F(F_rdz), 'w').add3(F(F_z), 'w').add3(F(F_zx), 'e').add3(F(F_zy), 'f').flush();
+    cal_deform_and_div();
+    calculate_n2();
+    smag_for_closure();
+    R3 r{2, ide - 2, 1, kde - 1, 2, jde - 2};
+    k_closure_rows<<<g3p(r), b3(), 0, s>>>(d, r, nm, Scr(S_T1), Scr(S_T2), Xr, T);
+  }
   // two stress components (t1, t2 of one diffusion kernel) in one launch;
   // each entry: xk, defor, kind, i0, i1, j0, j1
   void titau(float *t1, float *t2, std::array<int, 7> a, std::array<int, 7> b) const {
@@ -484,11 +517,19 @@ struct Run {
     k_hdiff_w<<<g3(rw), b3(), 0, s>>>(d, t1, t2, rw);
     hdiff_scalars();
   }
-  void first_rk_step_part2() const {
+  // with_n2: also N^2 (the stage API always, the step when the learned
+  // closure reads it)
+  void first_rk_step_part2(bool with_n2) const {
     compute_diff_metrics();
     BcBatch(this).add3(F(F_rdzw), 'w').add3(F(F_rdz), 'w').add3(F(F_z), 'w').add3(F(F_zx), 'e').add3(F(F_zy), 'f').flush();
     cal_deform_and_div();
-    smag2d_km();
+    if (with_n2 || X->cl_theta) calculate_n2();
+    if (X->cl_theta) {
+      smag_for_closure();
+      closure_km();
+    } else {
+      smag2d_km();
+    }
     BcBatch(this)
         .add3(F(F_xkmh), 't').add3(F(F_xkhh), 't').add3(F(F_xkmv), 't').add3(F(F_xkhv), 't').add3(F(F_div), 't')
         .add3(F(F_defor11), 't').add3(F(F_defor22), 't').add3(F(F_defor33), 't').add3(F(F_defor12), 'd')
@@ -851,7 +892,7 @@ struct Run {
         .add3(F(F_al), 'p').add3(F(F_ph_2), 'w').flush();
     if (X->rk == 1) {
       first_rk_step_part1();
-      first_rk_step_part2();
+      first_rk_step_part2(false);
     }
     fused_tendency();
     small_step_prep();
@@ -1178,10 +1219,15 @@ int wrfdc_stage(wrfdc_ctx *x, const char *name, int rk_step, int iteration) {
     std::string n(name);
     if (n == "rk_step_prep") run.rk_step_prep();
     else if (n == "first_rk_step_part1") run.first_rk_step_part1();
-    else if (n == "first_rk_step_part2") run.first_rk_step_part2();
+    else if (n == "first_rk_step_part2") run.first_rk_step_part2(true);
     else if (n == "compute_diff_metrics") run.compute_diff_metrics();
     else if (n == "cal_deform_and_div") run.cal_deform_and_div();
     else if (n == "smag2d_km") run.smag2d_km();
+    else if (n == "calculate_n2") run.calculate_n2();
+    else if (n == "calculate_km_kh") {
+      run.calculate_n2();
+      run.smag2d_km();
+    }
     else if (n == "horizontal_diffusion_2") run.horizontal_diffusion_2();
     else if (n == "rk_tendency") run.rk_tendency();
     else if (n == "advect_u") run.advect_u();
@@ -1263,6 +1309,159 @@ int wrfdc_checksums_noalloc(const wrfdc_config *cfg, void *const *fields, double
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(nullptr, WRFDC_EHIP, std::string("checksum kernels: ") + hipGetErrorString(e));
   return WRFDC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// learned eddy viscosity
+// ---------------------------------------------------------------------------
+static_assert(sizeof(wrfdc_closure_norm) == sizeof(ClosureNorm), "closure norm layout");
+static ClosureNorm to_norm(const wrfdc_closure_norm &n) {
+  return ClosureNorm{n.d_shift, n.d_scale, n.n_shift, n.n_scale, n.dx_feature, n.kref};
+}
+static bool norm_ok(const wrfdc_closure_norm &n) { return n.d_scale > 0.f && n.n_scale > 0.f && n.kref > 0.f; }
+
+int wrfdc_set_closure(wrfdc_ctx *x, const float *theta, const void *wbf, const wrfdc_closure_norm *norm) {
+  if (!x) return set_err(nullptr, WRFDC_EINVAL, "null context");
+  if (!theta) {
+    x->cl_theta = nullptr;
+    x->cl_wbf = nullptr;
+    return WRFDC_OK;
+  }
+  if (!wbf || !norm) return set_err(x, WRFDC_EINVAL, "the network needs its bf16 weights and feature normalisation");
+  if (!norm_ok(*norm)) return set_err(x, WRFDC_EINVAL, "closure normalisation: d_scale, n_scale and kref must be > 0");
+  x->cl_theta = theta;
+  x->cl_wbf = static_cast<const __bf16 *>(wbf);
+  x->cl_norm = to_norm(*norm);
+  return WRFDC_OK;
+}
+
+long long wrfdc_closure_row_count(const wrfdc_config *cfg) {
+  if (!cfg) return -1;
+  return (long long)(cfg->nx - 2) * cfg->nz * (cfg->ny - 2);
+}
+
+int wrfdc_closure_rows(wrfdc_ctx *x, const wrfdc_closure_norm *norm, void *rows_x, float *rows_t) {
+  if (!x) return set_err(nullptr, WRFDC_EINVAL, "null context");
+  if (!norm || !rows_x || !rows_t) return set_err(x, WRFDC_EINVAL, "norm, rows_x and rows_t must not be null");
+  if (!norm_ok(*norm)) return set_err(x, WRFDC_EINVAL, "closure normalisation: d_scale, n_scale and kref must be > 0");
+  return guarded(x, [&]() {
+    Run run(x);
+    run.closure_rows(to_norm(*norm), static_cast<__bf16 *>(rows_x), rows_t);
+    return WRFDC_OK;
+  });
+}
+
+int wrfdc_subgrid_stress(wrfdc_ctx *fine, int nx_coarse, int ny_coarse, int ratio, float *rows_t) {
+  if (!fine || !rows_t) return set_err(nullptr, WRFDC_EINVAL, "fine context and rows_t must not be null");
+  if (ratio < 1 || nx_coarse * ratio != fine->c.nx || ny_coarse * ratio != fine->c.ny)
+    return set_err(fine, WRFDC_EINVAL, "the fine grid must be ratio times the coarse grid in x and y");
+  if (nx_coarse < 12 || ny_coarse < 12) return set_err(fine, WRFDC_EINVAL, "coarse grid below 12 x 12");
+  return guarded(fine, [&]() {
+    R3 rc{2, nx_coarse - 1, 1, fine->kde - 1, 2, ny_coarse - 1};
+    const dim3 g(cdiv(rc.i1 - rc.i0 + 1, 64) * cdiv(rc.k1 - rc.k0 + 1, 4) * (unsigned)(rc.j1 - rc.j0 + 1));
+    k_subgrid_stress<<<g, dim3(64, 4, 1), 0, fine->s>>>(fine->h, rc, ratio, rows_t);
+    return WRFDC_OK;
+  });
+}
+
+int wrfdc_coarse_grain(wrfdc_ctx *fine, wrfdc_ctx *coarse, int ratio) {
+  if (!fine || !coarse) return set_err(nullptr, WRFDC_EINVAL, "null context");
+  if (ratio < 1 || coarse->c.nx * ratio != fine->c.nx || coarse->c.ny * ratio != fine->c.ny || coarse->c.nz != fine->c.nz ||
+      coarse->c.nmoist != fine->c.nmoist)
+    return set_err(coarse, WRFDC_EINVAL, "coarse grid must be the fine one divided by ratio in x and y, same levels and species");
+  int rc = ensure_ready(fine);
+  if (rc != WRFDC_OK) return rc;
+  return guarded(coarse, [&]() {
+    CoarseList L;
+    memset(&L, 0, sizeof(L));
+    L.ide_c = coarse->ide;
+    L.jde_c = coarse->jde;
+    L.ip_c = coarse->ip;
+    L.kde_c = coarse->kde;
+    L.ip_f = fine->ip;
+    L.jde_f = fine->jde;
+    L.kde_f = fine->kde;
+    L.ratio = ratio;
+    auto add = [&](int f, int stag, bool two_d, int species = -1) {
+      CoarseField &c = L.f[L.n++];
+      const size_t off_c = species < 0 ? 0 : (size_t)species * coarse->n3, off_f = species < 0 ? 0 : (size_t)species * fine->n3;
+      c.dst = coarse->h.F[f] + off_c;
+      c.src = fine->h.F[f] + off_f;
+      c.stag = stag;
+      c.nk = two_d ? 1 : fine->kde;
+    };
+    add(F_u_2, 1, false);
+    add(F_v_2, 2, false);
+    for (int f : {F_w_2, F_t_2, F_ph_2, F_phb, F_p, F_pb, F_al, F_alb}) add(f, 0, false);
+    add(F_mu_2, 0, true);
+    add(F_mub, 0, true);
+    add(F_ht, 0, true);
+    for (int q = 0; q < fine->c.nmoist && L.n < kMaxCoarse; q++) add(F_moist, 0, false, q);
+    const size_t cells = (size_t)coarse->ide * coarse->jde * coarse->kde;
+    unsigned nb = (unsigned)((cells + 255) / 256);
+    if (nb > 4096) nb = 4096;
+    k_coarse_grain<<<dim3(nb, L.n), 256, 0, coarse->s>>>(L);
+    return WRFDC_OK;
+  });
+}
+
+static int mlp_err(hipError_t e, const char *what) {
+  if (e == hipSuccess) return WRFDC_OK;
+  return set_err(nullptr, WRFDC_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+constexpr int kMlpChunksPerBlock = 8;  // 256 rows per block of the training kernel
+
+long long wrfdc_mlp_work_floats(long long nrows) {
+  if (nrows < 0) return -1;
+  const long long rows_per_block = (long long)kMlpChunksPerBlock * kChunk;
+  return ((nrows + rows_per_block - 1) / rows_per_block) * (long long)(kMlpParams + 1);
+}
+
+int wrfdc_mlp_pack(const float *theta, void *wbf, void *stream) {
+  if (!theta || !wbf) return set_err(nullptr, WRFDC_EINVAL, "theta and wbf must not be null");
+  k_mlp_pack<<<16, 256, 0, static_cast<hipStream_t>(stream)>>>(theta, static_cast<__bf16 *>(wbf));
+  return mlp_err(hipGetLastError(), "k_mlp_pack");
+}
+
+static int mlp_pass(const float *theta, const void *wbf, const void *rows_x, const float *rows_t, long long nrows, float lossnorm, float kref,
+                    float *work, float *grad, float *loss, float *m, float *v, int step, float lr, void *stream) {
+  if (!theta || !wbf || !rows_x || !rows_t || !work || nrows <= 0)
+    return set_err(nullptr, WRFDC_EINVAL, "theta, wbf, rows and work must not be null and nrows > 0");
+  if (!(kref > 0.f)) return set_err(nullptr, WRFDC_EINVAL, "kref must be > 0");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const long long rows_per_block = (long long)kMlpChunksPerBlock * kChunk;
+  const long long nb = (nrows + rows_per_block - 1) / rows_per_block;
+  if (nb > (1LL << 30)) return set_err(nullptr, WRFDC_EINVAL, "too many rows");
+  k_mlp_train<true><<<(unsigned)nb, 64, 0, s>>>(theta, static_cast<const __bf16 *>(wbf), static_cast<const __bf16 *>(rows_x), rows_t, nrows, lossnorm,
+                                                 kref, kMlpChunksPerBlock, work);
+  int rc = mlp_err(hipGetLastError(), "k_mlp_train");
+  if (rc != WRFDC_OK) return rc;
+  const int adam = m != nullptr;
+  k_mlp_reduce<<<(kMlpParams + 1 + 255) / 256, 256, 0, s>>>(work, (int)nb, grad, loss, const_cast<float *>(theta), m, v,
+                                                             const_cast<__bf16 *>(static_cast<const __bf16 *>(wbf)), adam, step, lr);
+  return mlp_err(hipGetLastError(), "k_mlp_reduce");
+}
+
+int wrfdc_mlp_grad(const float *theta, const void *wbf, const void *rows_x, const float *rows_t, long long nrows, float lossnorm, float kref,
+                   float *work, float *grad, float *loss, void *stream) {
+  if (!grad) return set_err(nullptr, WRFDC_EINVAL, "grad must not be null");
+  return mlp_pass(theta, wbf, rows_x, rows_t, nrows, lossnorm, kref, work, grad, loss, nullptr, nullptr, 0, 0.f, stream);
+}
+
+int wrfdc_mlp_train_step(float *theta, void *wbf, const void *rows_x, const float *rows_t, long long nrows, float lossnorm, float kref,
+                         float *work, float *adam_m, float *adam_v, int step, float lr, float *loss, void *stream) {
+  if (!adam_m || !adam_v || step < 1) return set_err(nullptr, WRFDC_EINVAL, "Adam moments must not be null and step >= 1");
+  return mlp_pass(theta, wbf, rows_x, rows_t, nrows, lossnorm, kref, work, nullptr, loss, adam_m, adam_v, step, lr, stream);
+}
+
+int wrfdc_mlp_forward(const float *theta, const void *wbf, const void *rows_x, long long nrows, float *z, void *stream) {
+  if (!theta || !wbf || !rows_x || !z || nrows <= 0) return set_err(nullptr, WRFDC_EINVAL, "theta, wbf, rows_x and z must not be null");
+  const long long rows_per_block = (long long)kMlpChunksPerBlock * kChunk;
+  const long long nb = (nrows + rows_per_block - 1) / rows_per_block;
+  k_mlp_train<false><<<(unsigned)nb, 64, 0, static_cast<hipStream_t>(stream)>>>(theta, static_cast<const __bf16 *>(wbf),
+                                                                              static_cast<const __bf16 *>(rows_x), nullptr, nrows, 0.f, 1.f,
+                                                                              kMlpChunksPerBlock, z);
+  return mlp_err(hipGetLastError(), "k_mlp_train<forward>");
 }
 
 int wrfdc_sync(wrfdc_ctx *x) {

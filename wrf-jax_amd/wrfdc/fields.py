@@ -48,7 +48,7 @@ FIELD_SPECS: tuple[tuple[str, str], ...] = (
     ("advect_tend", "3"), ("rho", "3"), ("z", "3"), ("rdz", "3"), ("rdzw", "3"),
     ("zx", "3"), ("zy", "3"), ("div", "3"), ("defor11", "3"), ("defor22", "3"),
     ("defor33", "3"), ("defor12", "3"), ("defor13", "3"), ("defor23", "3"),
-    ("xkmh", "3"), ("xkmv", "3"), ("xkhh", "3"), ("xkhv", "3"),
+    ("xkmh", "3"), ("xkmv", "3"), ("xkhh", "3"), ("xkhv", "3"), ("bn2", "3"),
     ("mu_1", "2"), ("mu_2", "2"), ("mub", "2"), ("mut", "2"), ("muu", "2"), ("muv", "2"),
     ("muts", "2"), ("muus", "2"), ("muvs", "2"), ("muave", "2"), ("mudf", "2"),
     ("mu_tend", "2"), ("mu_tendf", "2"), ("mu_save", "2"), ("ht", "2"), ("msfux", "2"),
