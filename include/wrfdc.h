@@ -91,6 +91,25 @@ int wrfdc_checksums(wrfdc_ctx *ctx, double out[WRFDC_NCHECKSUMS]);
  * the same codes, messages via wrfdc_last_error(NULL). */
 int wrfdc_checksums_noalloc(const wrfdc_config *cfg, void *const *fields, double *out_dev, void *hip_stream);
 int wrfdc_sync(wrfdc_ctx *ctx);
+/* ---- tangent-linear and adjoint scalar transport (north-star extension: the
+ * adjoint of advect_scalar, src/wrf_jax/dyn_em/module_advect_em_fast.py:1437-1769,
+ * + rk_update_scalar, src/wrf_jax/dyn_em/module_em.py:1437-1598; no counterpart
+ * interface in the reference) ----
+ * Checkpoints: every RK stage of later wrfdc_step calls copies the dynamics its
+ * species update reads -- ru_m, rv_m, ww_m (3-D) and mu_1, mu_2 (2-D) -- into
+ * dst[5*s + 0..4] for stages s = 0 .. nstages-1 in order (nstages 0 stops).
+ * A stage of transport with such a checkpoint: q_out = (m_old q_old +
+ * dt_rk msfty [inner] A q_in) / m_new, A the advect_scalar operator of the
+ * checkpoint's fluxes (linear in q, so this is also its tangent-linear); the
+ * adjoint stage adds the transpose: qbar_old += (m_old/m_new) qbar_out,
+ * qbar_in += A^T (dt_rk msfty [inner] / m_new) qbar_out (qbar_old and
+ * qbar_in may be the same array).  3-D arrays in the registry layout. */
+int wrfdc_record_stages(wrfdc_ctx *ctx, float *const *dst, int nstages);
+int wrfdc_recorded_stages(wrfdc_ctx *ctx);
+int wrfdc_transport_stage(wrfdc_ctx *ctx, int rk_step, const float *const *stage, const float *q_in, const float *q_old, float *q_out);
+int wrfdc_transport_stage_adjoint(wrfdc_ctx *ctx, int rk_step, const float *const *stage, const float *qbar_out, float *qbar_old,
+                                  float *qbar_in);
+
 /* ---- learned eddy viscosity (north-star extension of smag_km, the km of
  * calculate_km_kh, src/wrf_jax/dyn_em/module_diffusion_em_fast.py:773-1083;
  * no counterpart interface in the reference) ----

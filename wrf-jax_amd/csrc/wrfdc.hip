@@ -28,6 +28,7 @@
 #include "k_acoustic.h"
 #include "k_species.h"
 #include "k_closure.h"
+#include "k_adjoint.h"
 
 // ---------------------------------------------------------------------------
 // small helper kernels local to the orchestration
@@ -183,6 +184,11 @@ struct wrfdc_ctx {
   const float *cl_theta = nullptr;
   const __bf16 *cl_wbf = nullptr;
   ClosureNorm cl_norm{};
+  // per-RK-stage checkpoints of the dynamics the species transport reads
+  // (wrfdc_record_stages): 5 device pointers per stage (ru_m, rv_m, ww_m,
+  // mu_1, mu_2), filled in stage order until rec_n stages are recorded
+  std::vector<float *> rec;
+  int rec_n = 0, rec_i = 0;
   int rk = 1, nsmall = 1, iteration = 1;
   double dt_rk = 0.0, dts_rk = 0.0, dtbc = 0.0;
   std::string err;
@@ -884,6 +890,35 @@ struct Run {
     }
   }
 
+  // one stage of scalar transport with recorded dynamics (tangent-linear =
+  // forward) and its adjoint; rk sets dt_rk (set_rk before)
+  void transport_stage(const float *const *st, const float *q_in, const float *q_old, float *q_out) const {
+    float *tend = Scr(S_T1);
+    R3 m = mass3();
+    fill(tend, X->n3);
+    advect_scalar(q_in, tend, st[0], st[1], st[2]);
+    k_tl_update<<<g3p(m), b3(), 0, s>>>(d, m, StageMass{st[3], st[4]}, q_old, tend, q_out, (float)X->dt_rk);
+  }
+  void transport_stage_adjoint(const float *const *st, const float *qbar_out, float *qbar_old, float *qbar_in) const {
+    float *tbar = Scr(S_T1);
+    R3 m = mass3();
+    k_ad_update<<<g3p(m), b3(), 0, s>>>(d, m, StageMass{st[3], st[4]}, qbar_out, qbar_old, tbar, (float)X->dt_rk);
+    k_ad_advect<<<g3p(m), b3(), 0, s>>>(d, m, tbar, st[0], st[1], st[2], qbar_in);
+  }
+
+  // the mass fluxes and column masses of this stage's species update
+  void record_stage() const {
+    if (X->rec_i >= X->rec_n) return;
+    float *const *dst = X->rec.data() + (size_t)5 * X->rec_i;
+    const size_t b3 = sizeof(float) * X->n3, b2 = sizeof(float) * X->n2;
+    hchk(hipMemcpyAsync(dst[0], F(F_ru_m), b3, hipMemcpyDeviceToDevice, s), "record ru_m");
+    hchk(hipMemcpyAsync(dst[1], F(F_rv_m), b3, hipMemcpyDeviceToDevice, s), "record rv_m");
+    hchk(hipMemcpyAsync(dst[2], F(F_ww_m), b3, hipMemcpyDeviceToDevice, s), "record ww_m");
+    hchk(hipMemcpyAsync(dst[3], F(F_mu_1), b2, hipMemcpyDeviceToDevice, s), "record mu_1");
+    hchk(hipMemcpyAsync(dst[4], F(F_mu_2), b2, hipMemcpyDeviceToDevice, s), "record mu_2");
+    X->rec_i++;
+  }
+
   void rk_stage() const {
     rk_step_prep();
     BcBatch(this)
@@ -907,6 +942,7 @@ struct Run {
     if (X->rk == 3)
       BcBatch(this).add3(F(F_ru_m), 'u').add3(F(F_rv_m), 'v').add3(F(F_ww_m), 'w').add2(F(F_mut), 't').add2(F(F_muts), 't').flush();
     const int nm = X->c.nmoist, nsc = X->c.nscalar;
+    record_stage();
     species_stage();
     calc_p_rho_phi();
     if (X->rk < 3) {
@@ -1309,6 +1345,57 @@ int wrfdc_checksums_noalloc(const wrfdc_config *cfg, void *const *fields, double
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(nullptr, WRFDC_EHIP, std::string("checksum kernels: ") + hipGetErrorString(e));
   return WRFDC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// tangent-linear / adjoint scalar transport
+// ---------------------------------------------------------------------------
+int wrfdc_record_stages(wrfdc_ctx *x, float *const *dst, int nstages) {
+  if (!x) return set_err(nullptr, WRFDC_EINVAL, "null context");
+  if (nstages < 0 || (nstages > 0 && !dst)) return set_err(x, WRFDC_EINVAL, "dst must hold 5 pointers per stage");
+  for (int q = 0; q < 5 * nstages; q++)
+    if (!dst[q]) return set_err(x, WRFDC_EINVAL, "null checkpoint pointer");
+  x->rec.assign(dst, dst + (size_t)5 * nstages);
+  x->rec_n = nstages;
+  x->rec_i = 0;
+  return WRFDC_OK;
+}
+
+int wrfdc_recorded_stages(wrfdc_ctx *x) { return x ? x->rec_i : -1; }
+
+static int transport_args(wrfdc_ctx *x, int rk, const float *const *stage) {
+  if (!x) return set_err(nullptr, WRFDC_EINVAL, "null context");
+  if (rk < 1 || rk > 3) return set_err(x, WRFDC_EINVAL, "rk_step must be 1, 2 or 3");
+  if (!stage) return set_err(x, WRFDC_EINVAL, "stage checkpoint is null");
+  for (int q = 0; q < 5; q++)
+    if (!stage[q]) return set_err(x, WRFDC_EINVAL, "stage checkpoint pointer is null");
+  return WRFDC_OK;
+}
+
+int wrfdc_transport_stage(wrfdc_ctx *x, int rk, const float *const *stage, const float *q_in, const float *q_old, float *q_out) {
+  int rc = transport_args(x, rk, stage);
+  if (rc != WRFDC_OK) return rc;
+  if (!q_in || !q_old || !q_out) return set_err(x, WRFDC_EINVAL, "q_in, q_old and q_out must not be null");
+  if (q_out == q_in || q_out == q_old) return set_err(x, WRFDC_EINVAL, "q_out must not alias q_in or q_old");
+  return guarded(x, [&]() {
+    Run run(x);
+    run.set_rk(rk);
+    run.transport_stage(stage, q_in, q_old, q_out);
+    return WRFDC_OK;
+  });
+}
+
+int wrfdc_transport_stage_adjoint(wrfdc_ctx *x, int rk, const float *const *stage, const float *qbar_out, float *qbar_old, float *qbar_in) {
+  int rc = transport_args(x, rk, stage);
+  if (rc != WRFDC_OK) return rc;
+  if (!qbar_out || !qbar_old || !qbar_in) return set_err(x, WRFDC_EINVAL, "adjoint arrays must not be null");
+  if (qbar_out == qbar_old || qbar_out == qbar_in) return set_err(x, WRFDC_EINVAL, "qbar_out must not alias the accumulators");
+  return guarded(x, [&]() {
+    Run run(x);
+    run.set_rk(rk);
+    run.transport_stage_adjoint(stage, qbar_out, qbar_old, qbar_in);
+    return WRFDC_OK;
+  });
 }
 
 // ---------------------------------------------------------------------------
