@@ -91,6 +91,24 @@ int wrfdc_checksums(wrfdc_ctx *ctx, double out[WRFDC_NCHECKSUMS]);
  * the same codes, messages via wrfdc_last_error(NULL). */
 int wrfdc_checksums_noalloc(const wrfdc_config *cfg, void *const *fields, double *out_dev, void *hip_stream);
 int wrfdc_sync(wrfdc_ctx *ctx);
+/* ---- batched ensemble (north-star extension; the reference advances one
+ * domain per run_direct_step, src/wrf_jax/core/direct_step.py:3425-3531) ----
+ * One RK3 step of every member context is captured once into a HIP graph:
+ * member q runs on stream branch_of[q] (NULL: q mod nbranches), members of
+ * one stream in order, streams concurrently.  wrfdc_ensemble_step replays
+ * the graph nsteps times on the ensemble's own stream, after setting each
+ * member's boundary time dtbc0 + (n+1)*dt.  The members keep their buffers
+ * and stay usable one by one (synchronise with wrfdc_ensemble_sync first);
+ * binding other buffers to a member or destroying it invalidates the
+ * ensemble. */
+typedef struct wrfdc_ensemble wrfdc_ensemble;
+int wrfdc_ensemble_create(wrfdc_ctx *const *members, int n, const int *branch_of, int nbranches, wrfdc_ensemble **out);
+int wrfdc_ensemble_step(wrfdc_ensemble *ens, int nsteps, double dtbc0);
+int wrfdc_ensemble_sync(wrfdc_ensemble *ens);
+void *wrfdc_ensemble_stream(wrfdc_ensemble *ens);
+long long wrfdc_ensemble_nodes(wrfdc_ensemble *ens);
+void wrfdc_ensemble_destroy(wrfdc_ensemble *ens);
+
 /* ---- tangent-linear and adjoint scalar transport (north-star extension: the
  * adjoint of advect_scalar, src/wrf_jax/dyn_em/module_advect_em_fast.py:1437-1769,
  * + rk_update_scalar, src/wrf_jax/dyn_em/module_em.py:1437-1598; no counterpart

@@ -197,8 +197,11 @@ struct RelaxList {
   BPtrs b[kRelaxMax], bt[kRelaxMax];
   int ibe[kRelaxMax], jbe[kRelaxMax], kt[kRelaxMax], is2d[kRelaxMax], mw[kRelaxMax];
 };
-__global__ void k_relax(const DevCtx xv, RelaxList R, float dt_bc) {
+// dt_bc_p: the time since the lateral-boundary update, read from device memory
+// (set before each step, so that a captured step reads the current value)
+__global__ void k_relax(const DevCtx xv, RelaxList R, const float *__restrict__ dt_bc_p) {
   const DevCtx *__restrict__ x = &xv;
+  const float dt_bc = *dt_bc_p;
   const int a = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = 1 + (int)blockIdx.y;
   const int rz = x->relax_zone;
@@ -287,8 +290,9 @@ struct FinalList {
   BPtrs b[kFinalMax], bt[kFinalMax];
   int ibe[kFinalMax], jbe[kFinalMax], kt[kFinalMax], is2d[kFinalMax], msfc[kFinalMax], muc[kFinalMax];
 };
-__global__ void k_spec_final(const DevCtx xv, FinalList L, float dt_bc) {
+__global__ void k_spec_final(const DevCtx xv, FinalList L, const float *__restrict__ dt_bc_p) {
   const DevCtx *__restrict__ x = &xv;
+  const float dt_bc = *dt_bc_p;
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = 1 + (int)blockIdx.y;
   const int f = (int)blockIdx.z;

@@ -144,6 +144,18 @@ __global__ void __launch_bounds__(CK_BLOCK) k_checksum_final(const double *part,
   if (threadIdx.x < CK_N) out[threadIdx.x] = red[threadIdx.x][0];
 }
 
+// boundary times of the members of an ensemble, one launch before each replay
+constexpr int kMaxMembers = 128;
+struct DtbcList {
+  float *p[kMaxMembers];
+  float v[kMaxMembers];
+  int n;
+};
+__global__ void k_put_dtbc(DtbcList L) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < L.n) *L.p[m] = L.v[m];
+}
+
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
@@ -191,6 +203,8 @@ struct wrfdc_ctx {
   int rec_n = 0, rec_i = 0;
   int rk = 1, nsmall = 1, iteration = 1;
   double dt_rk = 0.0, dts_rk = 0.0, dtbc = 0.0;
+  float *dtbc_dev = nullptr;  // dtbc as the boundary kernels read it (one float)
+  bool capturing = false;     // a step being captured into a graph leaves dtbc_dev to the replayer
   std::string err;
 };
 
@@ -354,7 +368,7 @@ struct Run {
     void flush() {
       if (n == 0) return;
       const int rz = r->X->c.relax_zone;
-      k_relax<<<dim3(cdiv(along, 64), kmax, n * 4 * (rz - 1)), 64, 0, r->s>>>(r->d, L, (float)r->X->dtbc);
+      k_relax<<<dim3(cdiv(along, 64), kmax, n * 4 * (rz - 1)), 64, 0, r->s>>>(r->d, L, r->X->dtbc_dev);
       n = 0;
       along = kmax = 1;
     }
@@ -394,7 +408,7 @@ struct Run {
     }
     void flush() {
       if (n == 0) return;
-      k_spec_final<<<dim3(cdiv(nmax, 64), kmax, n), 64, 0, r->s>>>(r->d, L, (float)r->X->dtbc);
+      k_spec_final<<<dim3(cdiv(nmax, 64), kmax, n), 64, 0, r->s>>>(r->d, L, r->X->dtbc_dev);
       n = 0;
       nmax = kmax = 1;
     }
@@ -953,7 +967,13 @@ struct Run {
     }
   }
 
+  // the boundary time the kernels of this step or stage read
+  void put_dtbc() const {
+    if (!X->capturing) k_fill<<<1, 64, 0, s>>>(X->dtbc_dev, (float)X->dtbc, 1);
+  }
+
   void step() const {
+    put_dtbc();
     if (species_in_alt()) species_to(false);
     // initialize_moist_old (module_big_step_utilities_em.py:935-975); the
     // scalars' q_old <- q of the first-stage update is done here as well
@@ -1175,6 +1195,7 @@ int wrfdc_create(const wrfdc_config *cfg, wrfdc_ctx **out) {
   try {
     hchk(hipMalloc((void **)&x->scratch, sizeof(float) * x->n3 * NSCRATCH), "hipMalloc scratch");
     hchk(hipMalloc((void **)&x->ck, sizeof(double) * (kCkBlocks * CK_N + CK_N)), "hipMalloc checksum");
+    hchk(hipMalloc((void **)&x->dtbc_dev, sizeof(float) * 64), "hipMalloc dtbc");
     hchk(hipMalloc((void **)&x->spec_alt, sizeof(float) * x->n3 * (size_t)(cfg->nmoist + cfg->nscalar)), "hipMalloc species");
     hchk(hipMalloc((void **)&x->spec_scale, sizeof(float) * x->n3 * (size_t)(cfg->nmoist + cfg->nscalar)), "hipMalloc species scale");
     if (cfg->pd_moist || cfg->pd_scalar)
@@ -1184,6 +1205,7 @@ int wrfdc_create(const wrfdc_config *cfg, wrfdc_ctx **out) {
     set_err(nullptr, WRFDC_EHIP, std::string(e.what) + ": " + hipGetErrorString(e.e));
     if (x->scratch) (void)hipFree(x->scratch);
     if (x->ck) (void)hipFree(x->ck);
+    if (x->dtbc_dev) (void)hipFree(x->dtbc_dev);
     if (x->spec_alt) (void)hipFree(x->spec_alt);
     if (x->spec_scale) (void)hipFree(x->spec_scale);
     if (x->spec_anti) (void)hipFree(x->spec_anti);
@@ -1209,6 +1231,7 @@ void wrfdc_destroy(wrfdc_ctx *x) {
   else (void)hipDeviceSynchronize();
   if (x->scratch) (void)hipFree(x->scratch);
   if (x->ck) (void)hipFree(x->ck);
+  if (x->dtbc_dev) (void)hipFree(x->dtbc_dev);
   if (x->spec_alt) (void)hipFree(x->spec_alt);
   if (x->spec_scale) (void)hipFree(x->spec_scale);
   if (x->spec_anti) (void)hipFree(x->spec_anti);
@@ -1252,6 +1275,7 @@ int wrfdc_stage(wrfdc_ctx *x, const char *name, int rk_step, int iteration) {
     run.set_rk(rk_step);
     if (iteration < 0 || iteration > x->nsmall) return set_err(x, WRFDC_EINVAL, "iteration outside [0, nsmall]");
     x->iteration = iteration;
+    run.put_dtbc();
     std::string n(name);
     if (n == "rk_step_prep") run.rk_step_prep();
     else if (n == "first_rk_step_part1") run.first_rk_step_part1();
@@ -1346,6 +1370,144 @@ int wrfdc_checksums_noalloc(const wrfdc_config *cfg, void *const *fields, double
   if (e != hipSuccess) return set_err(nullptr, WRFDC_EHIP, std::string("checksum kernels: ") + hipGetErrorString(e));
   return WRFDC_OK;
 }
+
+// ---------------------------------------------------------------------------
+// batched ensemble: one RK3 step of every member captured in one graph
+// ---------------------------------------------------------------------------
+struct wrfdc_ensemble {
+  std::vector<wrfdc_ctx *> m;
+  std::vector<int> branch;  // stream of each member
+  std::vector<hipStream_t> streams;
+  hipStream_t root = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  size_t nodes = 0;
+};
+
+static void ensemble_free(wrfdc_ensemble *e) {
+  if (!e) return;
+  if (e->root) (void)hipStreamSynchronize(e->root);
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  for (hipEvent_t v : e->join)
+    if (v) (void)hipEventDestroy(v);
+  if (e->fork) (void)hipEventDestroy(e->fork);
+  for (hipStream_t st : e->streams)
+    if (st) (void)hipStreamDestroy(st);
+  if (e->root) (void)hipStreamDestroy(e->root);
+  delete e;
+}
+
+int wrfdc_ensemble_create(wrfdc_ctx *const *members, int n, const int *branch_of, int nbranches, wrfdc_ensemble **out) {
+  if (!members || !out || n < 1 || n > kMaxMembers || nbranches < 1 || nbranches > 64)
+    return set_err(nullptr, WRFDC_EINVAL, "ensemble: 1..128 members and 1..64 streams");
+  *out = nullptr;
+  for (int q = 0; q < n; q++) {
+    if (!members[q]) return set_err(nullptr, WRFDC_EINVAL, "ensemble: null member");
+    for (int r = 0; r < q; r++)
+      if (members[r] == members[q]) return set_err(nullptr, WRFDC_EINVAL, "ensemble: a member appears twice");
+    int rc = ensure_ready(members[q]);
+    if (rc != WRFDC_OK) return set_err(nullptr, rc, "ensemble member " + std::to_string(q) + ": " + members[q]->err);
+    if (branch_of && (branch_of[q] < 0 || branch_of[q] >= nbranches))
+      return set_err(nullptr, WRFDC_EINVAL, "ensemble: stream index out of range");
+  }
+  wrfdc_ensemble *e = new wrfdc_ensemble();
+  e->m.assign(members, members + n);
+  e->branch.resize(n);
+  for (int q = 0; q < n; q++) e->branch[q] = branch_of ? branch_of[q] : q % nbranches;
+  std::vector<hipStream_t> saved(n);
+  for (int q = 0; q < n; q++) saved[q] = members[q]->s;
+  bool capturing = false;
+  try {
+    // every stream at the default priority
+    hchk(hipStreamCreateWithFlags(&e->root, hipStreamNonBlocking), "ensemble stream");
+    e->streams.resize(nbranches, nullptr);
+    e->join.resize(nbranches, nullptr);
+    for (int b = 0; b < nbranches; b++) {
+      hchk(hipStreamCreateWithFlags(&e->streams[b], hipStreamNonBlocking), "ensemble stream");
+      hchk(hipEventCreateWithFlags(&e->join[b], hipEventDisableTiming), "ensemble event");
+    }
+    hchk(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming), "ensemble event");
+    // the members' earlier work finishes before the capture's first replay reads it
+    for (int q = 0; q < n; q++)
+      if (saved[q]) hchk(hipStreamSynchronize(saved[q]), "ensemble member stream");
+    hchk(hipDeviceSynchronize(), "ensemble setup");
+    hchk(hipStreamBeginCapture(e->root, hipStreamCaptureModeThreadLocal), "begin capture");
+    capturing = true;
+    hchk(hipEventRecord(e->fork, e->root), "fork");
+    for (int b = 0; b < nbranches; b++) hchk(hipStreamWaitEvent(e->streams[b], e->fork, 0), "fork wait");
+    for (int q = 0; q < n; q++) {
+      wrfdc_ctx *x = members[q];
+      x->s = e->streams[e->branch[q]];
+      x->capturing = true;
+      Run run(x);
+      run.step();
+      x->capturing = false;
+      hchk(hipGetLastError(), "captured step");
+    }
+    for (int b = 0; b < nbranches; b++) {
+      hchk(hipEventRecord(e->join[b], e->streams[b]), "join");
+      hchk(hipStreamWaitEvent(e->root, e->join[b], 0), "join wait");
+    }
+    capturing = false;
+    hchk(hipStreamEndCapture(e->root, &e->graph), "end capture");
+    hchk(hipGraphGetNodes(e->graph, nullptr, &e->nodes), "graph nodes");
+    hchk(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0), "graph instantiate");
+  } catch (const HipError &err) {
+    if (capturing) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(e->root, &g);
+      if (g) (void)hipGraphDestroy(g);
+    }
+    for (int q = 0; q < n; q++) {
+      members[q]->s = saved[q];
+      members[q]->capturing = false;
+    }
+    ensemble_free(e);
+    return set_err(nullptr, WRFDC_EHIP, std::string("ensemble: ") + err.what + ": " + hipGetErrorString(err.e));
+  }
+  // replays run on the root stream; the members' own streams wait for it
+  for (int q = 0; q < n; q++) members[q]->s = saved[q];
+  *out = e;
+  return WRFDC_OK;
+}
+
+int wrfdc_ensemble_step(wrfdc_ensemble *e, int nsteps, double dtbc0) {
+  if (!e || nsteps < 0) return set_err(nullptr, WRFDC_EINVAL, "ensemble step: null ensemble or nsteps < 0");
+  const int n = (int)e->m.size();
+  try {
+    for (int s = 0; s < nsteps; s++) {
+      DtbcList L;
+      memset(&L, 0, sizeof(L));
+      L.n = n;
+      for (int q = 0; q < n; q++) {
+        wrfdc_ctx *x = e->m[q];
+        x->dtbc = dtbc0 + (double)(s + 1) * x->c.dt;
+        L.p[q] = x->dtbc_dev;
+        L.v[q] = (float)x->dtbc;
+      }
+      k_put_dtbc<<<(n + 63) / 64, 64, 0, e->root>>>(L);
+      hchk(hipGetLastError(), "ensemble dtbc");
+      hchk(hipGraphLaunch(e->exec, e->root), "graph launch");
+    }
+  } catch (const HipError &err) {
+    return set_err(nullptr, WRFDC_EHIP, std::string("ensemble: ") + err.what + ": " + hipGetErrorString(err.e));
+  }
+  return WRFDC_OK;
+}
+
+int wrfdc_ensemble_sync(wrfdc_ensemble *e) {
+  if (!e) return set_err(nullptr, WRFDC_EINVAL, "null ensemble");
+  hipError_t err = hipStreamSynchronize(e->root);
+  if (err != hipSuccess) return set_err(nullptr, WRFDC_EHIP, std::string("ensemble sync: ") + hipGetErrorString(err));
+  return WRFDC_OK;
+}
+
+void *wrfdc_ensemble_stream(wrfdc_ensemble *e) { return e ? (void *)e->root : nullptr; }
+long long wrfdc_ensemble_nodes(wrfdc_ensemble *e) { return e ? (long long)e->nodes : -1; }
+void wrfdc_ensemble_destroy(wrfdc_ensemble *e) { ensemble_free(e); }
 
 // ---------------------------------------------------------------------------
 // tangent-linear / adjoint scalar transport

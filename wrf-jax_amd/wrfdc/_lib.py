@@ -35,6 +35,13 @@ SIGNATURES = {
     "wrfdc_checksums": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "wrfdc_checksums_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
                                                 ctypes.c_void_p]),
+    "wrfdc_ensemble_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_void_p)]),
+    "wrfdc_ensemble_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]),
+    "wrfdc_ensemble_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "wrfdc_ensemble_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "wrfdc_ensemble_nodes": (ctypes.c_longlong, [ctypes.c_void_p]),
+    "wrfdc_ensemble_destroy": (None, [ctypes.c_void_p]),
     "wrfdc_record_stages": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "wrfdc_recorded_stages": (ctypes.c_int, [ctypes.c_void_p]),
     "wrfdc_transport_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
