@@ -15,6 +15,10 @@
  *                    = run_direct_step           src/wrf_jax/core/direct_step.py:3425-3531
  *   wrfdc_stage      one StageFn callback of strict_solve_callbacks()
  *                                                src/wrf_jax/runtime.py:16235-16301
+ *                    by the stage names solve_em calls
+ *                                                src/wrf_jax/dyn_em/solve_em.py:470-790
+ *   wrfdc_set_dtbc   _write(grid, "dtbc", dtbc + dt) at the top of solve_em
+ *                                                src/wrf_jax/dyn_em/solve_em.py:455
  *   wrfdc_checksums  assert_state_finite         src/wrf_jax/data_io.py:222-228
  *                    (plus FP64 conservation checksums: column mass, theta,
  *                    moisture, dry-air mass and total energy)
@@ -73,7 +77,29 @@ void wrfdc_destroy(wrfdc_ctx *ctx);
 int wrfdc_bind(wrfdc_ctx *ctx, const char *name, void *device_ptr);
 int wrfdc_set_stream(wrfdc_ctx *ctx, void *hip_stream);
 int wrfdc_step(wrfdc_ctx *ctx, int nsteps, double dtbc0);
+/* One stage of the step on the bound state.  name is a stage name of the
+ * reference's solve_em driver (rk_step_prep, rk_phys_bc_dry_1,
+ * first_rk_step_part1/2, rk_tendency, relax_bdy_dry, rk_addtend_dry,
+ * spec_bdy_dry, small_step_prep, calc_p_rho, calc_coef_w, advance_uv,
+ * spec_bdyupdate:u|v|t|mu_2|muts, advance_mu_t, advance_w, sumflux,
+ * spec_bdyupdate_ph, zero_grad_bdy, calc_mu_uv_1, small_step_finish,
+ * initialize_moist_old, rk_update_scalar_pd:moist_old|scalar_old,
+ * moist_scalar_advance, scalar_advance, calc_p_rho_phi[:final],
+ * rk_phys_bc_dry_2, set_phys_bc_dry_2, set_physical_bc3d:<field>[:small],
+ * set_physical_bc2d:<field>[:small], spec_bdy_final:u|v|t|ph|mu|moist|moist_qv,
+ * set_w_surface), one of its parts (advect_u/v/w, advect_scalar[_pd],
+ * rhs_ph, horizontal_pressure_gradient, pg_buoy_w, compute_diff_metrics,
+ * cal_deform_and_div, smag2d_km, calculate_n2, calculate_km_kh,
+ * horizontal_diffusion_2, dry_boundary_tendency, acoustic_substep) or a
+ * whole rk_stage / step.  rk_step 1..3 sets dt_rk, dts_rk and the number of
+ * acoustic steps as _rk_step_sizes does; iteration is the acoustic step
+ * (calc_p_rho: 0 before the loop).  The lateral boundary time is the one of
+ * wrfdc_set_dtbc (or of the last wrfdc_step).  WRFDC_EINVAL for an unknown
+ * name or a stage of an unsupported option (spec_bdyupdate:w, nested). */
 int wrfdc_stage(wrfdc_ctx *ctx, const char *name, int rk_step, int iteration);
+/* Seconds since the lateral boundary record's time, as the next wrfdc_stage
+ * calls read it (wrfdc_step sets it to dtbc0 + (n+1) dt itself). */
+int wrfdc_set_dtbc(wrfdc_ctx *ctx, double dtbc);
 int wrfdc_checksums(wrfdc_ctx *ctx, double out[WRFDC_NCHECKSUMS]);
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory or wait on the host (code under HIP graph capture, tight serving

@@ -5,6 +5,8 @@
  * src/wrf_jax/dyn_em/solve_em.py:260-800 (final boundary stages) and the
  * moist / scalar stage adapters in src/wrf_jax/runtime.py:13652-14830.
  */
+#include <stdio.h>
+
 #include "o_core.h"
 
 /* o_bc.c */
@@ -198,11 +200,12 @@ void o_first_rk_step_part2(OCtx *x) {
   o_horizontal_diffusion_2(x);
 }
 
-/* relax_bdy_dry + rk_addtend_dry + spec_bdy_dry (direct_step.py:3179-3188,
- * module_bc_em.py:627-1332). */
-void o_dry_boundary_tendency(OCtx *x) {
+/* relax_bdy_dry (module_bc_em.py:627-760): relaxation-zone tendencies of
+ * u, v, ph, theta (the last two mass-weighted) and mu into the physics
+ * tendencies. */
+void o_relax_bdy_dry(OCtx *x) {
   real *mut = FLD(mut);
-  if (x->rk_step == 1) {
+  {
     real *tmp = (real *)calloc(x->n3, sizeof(real));
     o_relax_bdytend(x, FLD(ru), FLD(ru_tendf), 'u', 0, 0, FLD(u_bxs), FLD(u_bxe), FLD(u_bys), FLD(u_bye), FLD(u_btxs), FLD(u_btxe), FLD(u_btys), FLD(u_btye));
     o_relax_bdytend(x, FLD(rv), FLD(rv_tendf), 'v', 0, 0, FLD(v_bxs), FLD(v_bxe), FLD(v_bys), FLD(v_bye), FLD(v_btxs), FLD(v_btxe), FLD(v_btys), FLD(v_btye));
@@ -221,12 +224,24 @@ void o_dry_boundary_tendency(OCtx *x) {
     o_relax_bdytend(x, FLD(mu_2), FLD(mu_tend), 'm', 1, 1, FLD(mu_bxs), FLD(mu_bxe), FLD(mu_bys), FLD(mu_bye), FLD(mu_btxs), FLD(mu_btxe), FLD(mu_btys), FLD(mu_btye));
     free(tmp);
   }
-  o_rk_addtend_dry(x);
+}
+
+/* spec_bdy_dry (module_bc_em.py:1127-1332): the specified zone's tendencies
+ * from the boundary tendencies. */
+void o_spec_bdy_dry(OCtx *x) {
   o_spec_overlay(x, FLD(ru_tend), 'u', 0, 0, FLD(u_btxs), FLD(u_btxe), FLD(u_btys), FLD(u_btye));
   o_spec_overlay(x, FLD(rv_tend), 'v', 0, 0, FLD(v_btxs), FLD(v_btxe), FLD(v_btys), FLD(v_btye));
   o_spec_overlay(x, FLD(ph_tend), 'h', 1, 0, FLD(ph_btxs), FLD(ph_btxe), FLD(ph_btys), FLD(ph_btye));
   o_spec_overlay(x, FLD(t_tend), 't', 0, 0, FLD(t_btxs), FLD(t_btxe), FLD(t_btys), FLD(t_btye));
   o_spec_overlay(x, FLD(mu_tend), 'm', 1, 1, FLD(mu_btxs), FLD(mu_btxe), FLD(mu_btys), FLD(mu_btye));
+}
+
+/* relax_bdy_dry (first stage) + rk_addtend_dry + spec_bdy_dry
+ * (direct_step.py:3179-3188). */
+void o_dry_boundary_tendency(OCtx *x) {
+  if (x->rk_step == 1) o_relax_bdy_dry(x);
+  o_rk_addtend_dry(x);
+  o_spec_bdy_dry(x);
 }
 
 /* One species of moist_scalar_advance / scalar_advance
@@ -309,14 +324,64 @@ void o_acoustic_substep(OCtx *x, int it) {
   o_bc2(x, FLD(muts), 't'); o_bc2(x, FLD(mu_2), 't'); o_bc2(x, FLD(mudf), 't');
 }
 
+/* rk_phys_bc_dry_1 (module_bc_em.py:109-216) */
+static void o_rk_phys_bc_dry_1(OCtx *x) {
+  o_bc3(x, FLD(ru), 'u'); o_bc3(x, FLD(rv), 'v'); o_bc3(x, FLD(rw), 'w'); o_bc3(x, FLD(ww), 'w');
+  o_bc3(x, FLD(php), 'w'); o_bc3(x, FLD(alt), 't'); o_bc3(x, FLD(p), 'p');
+  o_bc2(x, FLD(muu), 'u'); o_bc2(x, FLD(muv), 'v'); o_bc2(x, FLD(mut), 't');
+}
+
+/* rk_phys_bc_dry_2 (module_bc_em.py:219-318) */
+static void o_rk_phys_bc_dry_2(OCtx *x) {
+  o_bc3(x, FLD(u_2), 'u'); o_bc3(x, FLD(v_2), 'v'); o_bc3(x, FLD(w_2), 'w'); o_bc3(x, FLD(t_2), 'p');
+  o_bc3(x, FLD(ph_2), 'w'); o_bc2(x, FLD(mu_2), 't');
+}
+
+/* set_phys_bc_dry_2 (module_bc_em.py:321-445) */
+static void o_set_phys_bc_dry_2(OCtx *x) {
+  o_bc3(x, FLD(u_1), 'u'); o_bc3(x, FLD(u_2), 'u'); o_bc3(x, FLD(v_1), 'v'); o_bc3(x, FLD(v_2), 'v');
+  o_bc3(x, FLD(w_1), 'w'); o_bc3(x, FLD(w_2), 'w'); o_bc3(x, FLD(t_1), 'p'); o_bc3(x, FLD(t_2), 'p');
+  o_bc3(x, FLD(ph_1), 'w'); o_bc3(x, FLD(ph_2), 'w'); o_bc2(x, FLD(mu_1), 't'); o_bc2(x, FLD(mu_2), 't');
+}
+
+/* spec_bdy_final of one field (solve_em.py:744-760): 'u' 'v' 't' 'h' (ph)
+ * 'm' (mu) or 'q' (QV, the only species with boundary data) */
+static void o_spec_bdy_final_of(OCtx *x, char w) {
+  switch (w) {
+    case 'u': o_spec_bdy_final(x, FLD(u_2), 'u', 0, FLD(muus), FLD(msfuy), FLD(c1h), FLD(c2h), FLD(u_bxs), FLD(u_bxe), FLD(u_bys), FLD(u_bye), FLD(u_btxs), FLD(u_btxe), FLD(u_btys), FLD(u_btye)); break;
+    case 'v': o_spec_bdy_final(x, FLD(v_2), 'v', 0, FLD(muvs), FLD(msfvx), FLD(c1h), FLD(c2h), FLD(v_bxs), FLD(v_bxe), FLD(v_bys), FLD(v_bye), FLD(v_btxs), FLD(v_btxe), FLD(v_btys), FLD(v_btye)); break;
+    case 't': o_spec_bdy_final(x, FLD(t_2), 't', 0, FLD(muts), FLD(msfty), FLD(c1h), FLD(c2h), FLD(t_bxs), FLD(t_bxe), FLD(t_bys), FLD(t_bye), FLD(t_btxs), FLD(t_btxe), FLD(t_btys), FLD(t_btye)); break;
+    case 'h': o_spec_bdy_final(x, FLD(ph_2), 'h', 0, FLD(muts), FLD(msfty), FLD(c1f), FLD(c2f), FLD(ph_bxs), FLD(ph_bxe), FLD(ph_bys), FLD(ph_bye), FLD(ph_btxs), FLD(ph_btxe), FLD(ph_btys), FLD(ph_btye)); break;
+    case 'm': o_spec_bdy_final(x, FLD(mu_2), 'm', 1, FLD(muts), FLD(msfty), FLD(c1h), FLD(c2h), FLD(mu_bxs), FLD(mu_bxe), FLD(mu_bys), FLD(mu_bye), FLD(mu_btxs), FLD(mu_btxe), FLD(mu_btys), FLD(mu_btye)); break;
+    default: o_spec_bdy_final(x, SPC(moist, 0), 't', 0, FLD(muts), FLD(msfty), FLD(c1h), FLD(c2h), FLD(qv_bxs), FLD(qv_bxe), FLD(qv_bys), FLD(qv_bye), FLD(qv_btxs), FLD(qv_btxe), FLD(qv_btys), FLD(qv_btye)); break;
+  }
+}
+
+/* initialize_moist_old (module_big_step_utilities_em.py:935-975): moist_old
+ * <- moist on the mass points */
+static void o_initialize_moist_old(OCtx *x) {
+  for (int s = 0; s < x->c.nmoist; s++) {
+    real *q = SPC(moist, s), *qo = SPC(moist_old, s);
+    for (int j = 1; j <= JDE - 1; j++)
+      for (int k = 1; k <= KDE - 1; k++)
+        for (int i = 1; i <= IDE - 1; i++) A3(qo, i, k, j) = A3(q, i, k, j);
+  }
+}
+
+static void o_moist_scalar_advance(OCtx *x) {
+  for (int s = 0; s < x->c.nmoist; s++) o_species_advance(x, SPC(moist, s), SPC(moist_old, s), SPC(moist_tend, s), s == 0, 1, x->c.pd_moist);
+}
+
+static void o_scalar_advance(OCtx *x) {
+  for (int s = 0; s < x->c.nscalar; s++) o_species_advance(x, SPC(scalar, s), SPC(scalar_old, s), SPC(scalar_tend, s), 0, 0, x->c.pd_scalar);
+}
+
 /* One RK stage (direct_step.py:3120-3422). */
 void o_rk_stage(OCtx *x, int rk_step) {
   o_set_rk_stage(x, rk_step);
   o_rk_step_prep(x);
   /* rk_phys_bc_dry_1 + rho/al/ph_2 physical BC (module_bc_em.py:109-216) */
-  o_bc3(x, FLD(ru), 'u'); o_bc3(x, FLD(rv), 'v'); o_bc3(x, FLD(rw), 'w'); o_bc3(x, FLD(ww), 'w');
-  o_bc3(x, FLD(php), 'w'); o_bc3(x, FLD(alt), 't'); o_bc3(x, FLD(p), 'p');
-  o_bc2(x, FLD(muu), 'u'); o_bc2(x, FLD(muv), 'v'); o_bc2(x, FLD(mut), 't');
+  o_rk_phys_bc_dry_1(x);
   o_bc3(x, FLD(al), 'p'); o_bc3(x, FLD(ph_2), 'w');
   if (rk_step == 1) {
     o_first_rk_step_part1(x);
@@ -351,13 +416,12 @@ void o_rk_stage(OCtx *x, int rk_step) {
       o_bc3(x, SPC(scalar_old, s), 'p');
     }
   }
-  for (int s = 0; s < nm; s++) o_species_advance(x, SPC(moist, s), SPC(moist_old, s), SPC(moist_tend, s), s == 0, 1, x->c.pd_moist);
-  for (int s = 0; s < nsc; s++) o_species_advance(x, SPC(scalar, s), SPC(scalar_old, s), SPC(scalar_tend, s), 0, 0, x->c.pd_scalar);
+  o_moist_scalar_advance(x);
+  o_scalar_advance(x);
   o_calc_p_rho_phi(x);
   if (rk_step < 3) {
-    /* rk_phys_bc_dry_2 (module_bc_em.py:219-318) + moist / scalar BC */
-    o_bc3(x, FLD(u_2), 'u'); o_bc3(x, FLD(v_2), 'v'); o_bc3(x, FLD(w_2), 'w'); o_bc3(x, FLD(t_2), 'p');
-    o_bc3(x, FLD(ph_2), 'w'); o_bc2(x, FLD(mu_2), 't');
+    /* rk_phys_bc_dry_2 + moist / scalar BC */
+    o_rk_phys_bc_dry_2(x);
     for (int s = 0; s < nm; s++) o_bc3(x, SPC(moist, s), 'p');
     for (int s = 0; s < nsc; s++) o_bc3(x, SPC(scalar, s), 'p');
   }
@@ -367,28 +431,87 @@ void o_rk_stage(OCtx *x, int rk_step) {
  * final physical / specified boundary stages and set_w_surface. */
 void o_step(OCtx *x) {
   int nm = x->c.nmoist;
-  /* initialize_moist_old (module_big_step_utilities_em.py:935-975) */
-  for (int s = 0; s < nm; s++) {
-    real *q = SPC(moist, s), *qo = SPC(moist_old, s);
-    for (int j = 1; j <= JDE - 1; j++)
-      for (int k = 1; k <= KDE - 1; k++)
-        for (int i = 1; i <= IDE - 1; i++) A3(qo, i, k, j) = A3(q, i, k, j);
-    o_bc3(x, qo, 'p');
-  }
+  o_initialize_moist_old(x);
+  for (int s = 0; s < nm; s++) o_bc3(x, SPC(moist_old, s), 'p');
   for (int rk = 1; rk <= 3; rk++) o_rk_stage(x, rk);
-  /* set_phys_bc_dry_2 (module_bc_em.py:321-445) */
-  o_bc3(x, FLD(u_1), 'u'); o_bc3(x, FLD(u_2), 'u'); o_bc3(x, FLD(v_1), 'v'); o_bc3(x, FLD(v_2), 'v');
-  o_bc3(x, FLD(w_1), 'w'); o_bc3(x, FLD(w_2), 'w'); o_bc3(x, FLD(t_1), 'p'); o_bc3(x, FLD(t_2), 'p');
-  o_bc3(x, FLD(ph_1), 'w'); o_bc3(x, FLD(ph_2), 'w'); o_bc2(x, FLD(mu_1), 't'); o_bc2(x, FLD(mu_2), 't');
+  o_set_phys_bc_dry_2(x);
   o_bc3(x, SPC(moist, 0), 'p');
   /* spec_bdy_final (solve_em.py:744-760) */
-  o_spec_bdy_final(x, FLD(u_2), 'u', 0, FLD(muus), FLD(msfuy), FLD(c1h), FLD(c2h), FLD(u_bxs), FLD(u_bxe), FLD(u_bys), FLD(u_bye), FLD(u_btxs), FLD(u_btxe), FLD(u_btys), FLD(u_btye));
-  o_spec_bdy_final(x, FLD(v_2), 'v', 0, FLD(muvs), FLD(msfvx), FLD(c1h), FLD(c2h), FLD(v_bxs), FLD(v_bxe), FLD(v_bys), FLD(v_bye), FLD(v_btxs), FLD(v_btxe), FLD(v_btys), FLD(v_btye));
-  o_spec_bdy_final(x, FLD(t_2), 't', 0, FLD(muts), FLD(msfty), FLD(c1h), FLD(c2h), FLD(t_bxs), FLD(t_bxe), FLD(t_bys), FLD(t_bye), FLD(t_btxs), FLD(t_btxe), FLD(t_btys), FLD(t_btye));
-  o_spec_bdy_final(x, FLD(ph_2), 'h', 0, FLD(muts), FLD(msfty), FLD(c1f), FLD(c2f), FLD(ph_bxs), FLD(ph_bxe), FLD(ph_bys), FLD(ph_bye), FLD(ph_btxs), FLD(ph_btxe), FLD(ph_btys), FLD(ph_btye));
-  o_spec_bdy_final(x, FLD(mu_2), 'm', 1, FLD(muts), FLD(msfty), FLD(c1h), FLD(c2h), FLD(mu_bxs), FLD(mu_bxe), FLD(mu_bys), FLD(mu_bye), FLD(mu_btxs), FLD(mu_btxe), FLD(mu_btys), FLD(mu_btye));
-  o_spec_bdy_final(x, SPC(moist, 0), 't', 0, FLD(muts), FLD(msfty), FLD(c1h), FLD(c2h), FLD(qv_bxs), FLD(qv_bxe), FLD(qv_bys), FLD(qv_bye), FLD(qv_btxs), FLD(qv_btxe), FLD(qv_btys), FLD(qv_btye));
+  o_spec_bdy_final_of(x, 'u');
+  o_spec_bdy_final_of(x, 'v');
+  o_spec_bdy_final_of(x, 't');
+  o_spec_bdy_final_of(x, 'h');
+  o_spec_bdy_final_of(x, 'm');
+  o_spec_bdy_final_of(x, 'q');
   o_set_w_surface(x, FLD(w_2), FLD(u_2), FLD(v_2));
+}
+
+/* Several independent domains advanced nsteps steps each, concurrently, one
+ * OpenMP thread per domain at a time (the all-cores CPU baseline over the
+ * tiles of a workload; built without -fopenmp the loop runs serially). */
+int o_step_many(OCtx **ctxs, int n, int nsteps, double dtbc0, int threads) {
+  if (n < 0 || nsteps < 0 || threads < 1) return -1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+  for (int q = 0; q < n; q++)
+    for (int s = 0; s < nsteps; s++) {
+      o_set_dtbc(ctxs[q], dtbc0 + (double)(s + 1) * ctxs[q]->c.dt);
+      o_step(ctxs[q]);
+    }
+  return 0;
+}
+
+static int o_field_index(const char *name) {
+  for (int i = 0; i < O_NFIELDS; i++)
+    if (strcmp(o_names[i], name) == 0) return i;
+  return -1;
+}
+
+/* set_physical_bc3d:<field>[:small] / set_physical_bc2d:<field>[:small]
+ * (solve_em.py:493-676); kinds of runtime.py:7421-7455.  Returns 1 if the
+ * name is such a stage and was run, -1 if it names an unknown field, 0 if
+ * it is no boundary stage. */
+static int o_named_bc(OCtx *x, const char *name) {
+  static const char *p3 = "set_physical_bc3d:", *p2 = "set_physical_bc2d:";
+  int is3 = strncmp(name, p3, strlen(p3)) == 0, is2 = strncmp(name, p2, strlen(p2)) == 0;
+  if (!is3 && !is2) return 0;
+  char f[64];
+  snprintf(f, sizeof f, "%s", name + strlen(p3));
+  char *colon = strchr(f, ':');
+  if (colon) {
+    if (strcmp(colon, ":small") != 0) return -1;
+    *colon = 0;
+  }
+  if (is2) {
+    static const char *ok2[] = {"mu_1", "mu_2", "mudf", "muts", "mut"};
+    for (size_t i = 0; i < sizeof ok2 / sizeof ok2[0]; i++)
+      if (strcmp(f, ok2[i]) == 0) {
+        o_bc2(x, x->F[o_field_index(f)], 't');
+        return 1;
+      }
+    return -1;
+  }
+  if (!strcmp(f, "tke_1") || !strcmp(f, "tke_2")) return 1;
+  if (!strcmp(f, "moist") || !strcmp(f, "moist_old")) {
+    for (int s = 0; s < x->c.nmoist; s++) o_bc3(x, !strcmp(f, "moist") ? SPC(moist, s) : SPC(moist_old, s), 'p');
+    return 1;
+  }
+  if (!strcmp(f, "scalar") || !strcmp(f, "scalar_old")) {
+    for (int s = 0; s < x->c.nscalar; s++) o_bc3(x, !strcmp(f, "scalar") ? SPC(scalar, s) : SPC(scalar_old, s), 'p');
+    return 1;
+  }
+  if (!strcmp(f, "moist_qv")) {
+    o_bc3(x, SPC(moist, 0), 'p');
+    return 1;
+  }
+  static const struct { const char *name; char kind; } k3[] = {
+      {"rho", 'p'}, {"al", 'p'}, {"p", 'p'}, {"ph_2", 'w'}, {"t_1", 'p'}, {"t_save", 't'}, {"ru_tend", 'u'},
+      {"rv_tend", 'v'}, {"ru_m", 'u'}, {"rv_m", 'v'}, {"ww_m", 'w'}, {"h_diabatic", 't'}};
+  for (size_t i = 0; i < sizeof k3 / sizeof k3[0]; i++)
+    if (strcmp(f, k3[i].name) == 0) {
+      o_bc3(x, x->F[o_field_index(f)], k3[i].kind);
+      return 1;
+    }
+  return -1;
 }
 
 /* Individual stage entry points for stage-level parity tests. */
@@ -424,11 +547,43 @@ int o_run_stage(OCtx *x, const char *name, int rk_step, int iteration) {
   else if (IS("advance_w")) o_advance_w(x);
   else if (IS("sumflux")) o_sumflux(x);
   else if (IS("acoustic_substep")) o_acoustic_substep(x, iteration);
-  else if (IS("small_step_finish")) { o_calc_mu_uv_1(x); o_small_step_finish(x, rk_step == 3); }
+  else if (IS("small_step_finish")) o_small_step_finish(x, rk_step == 3);
+  else if (IS("calc_mu_uv_1")) o_calc_mu_uv_1(x);
+  /* the remaining stages solve_em calls by name (solve_em.py:470-790) */
+  else if (IS("initialize_moist_old")) o_initialize_moist_old(x);
+  else if (IS("rk_phys_bc_dry_1")) o_rk_phys_bc_dry_1(x);
+  else if (IS("relax_bdy_dry")) o_relax_bdy_dry(x);
+  else if (IS("rk_addtend_dry")) o_rk_addtend_dry(x);
+  else if (IS("spec_bdy_dry")) o_spec_bdy_dry(x);
+  else if (IS("spec_bdyupdate:u")) o_spec_bdyupdate(x, FLD(u_2), FLD(ru_tend), x->dts_rk, 'u', 0);
+  else if (IS("spec_bdyupdate:v")) o_spec_bdyupdate(x, FLD(v_2), FLD(rv_tend), x->dts_rk, 'v', 0);
+  else if (IS("spec_bdyupdate:t")) o_spec_bdyupdate(x, FLD(t_2), FLD(t_tend), x->dts_rk, 't', 0);
+  else if (IS("spec_bdyupdate:mu_2")) o_spec_bdyupdate(x, FLD(mu_2), FLD(mu_tend), x->dts_rk, 'm', 1);
+  else if (IS("spec_bdyupdate:muts")) o_spec_bdyupdate(x, FLD(muts), FLD(mu_tend), x->dts_rk, 'm', 1);
+  else if (IS("spec_bdyupdate_ph")) o_spec_bdyupdate_ph(x, x->dts_rk);
+  else if (IS("zero_grad_bdy")) o_zero_grad_bdy_w(x, FLD(w_2));
+  else if (IS("rk_update_scalar_pd:moist_old")) {
+    for (int s = 0; s < x->c.nmoist; s++) o_rk_update_scalar_pd(x, SPC(moist_old, s), SPC(moist_tend, s));
+  }
+  else if (IS("rk_update_scalar_pd:scalar_old")) {
+    for (int s = 0; s < x->c.nscalar; s++) o_rk_update_scalar_pd(x, SPC(scalar_old, s), SPC(scalar_tend, s));
+  }
+  else if (IS("moist_scalar_advance")) o_moist_scalar_advance(x);
+  else if (IS("scalar_advance")) o_scalar_advance(x);
+  else if (IS("calc_p_rho_phi:final")) o_calc_p_rho_phi(x);
+  else if (IS("rk_phys_bc_dry_2")) o_rk_phys_bc_dry_2(x);
+  else if (IS("set_phys_bc_dry_2")) o_set_phys_bc_dry_2(x);
+  else if (IS("spec_bdy_final:u")) o_spec_bdy_final_of(x, 'u');
+  else if (IS("spec_bdy_final:v")) o_spec_bdy_final_of(x, 'v');
+  else if (IS("spec_bdy_final:t")) o_spec_bdy_final_of(x, 't');
+  else if (IS("spec_bdy_final:ph")) o_spec_bdy_final_of(x, 'h');
+  else if (IS("spec_bdy_final:mu")) o_spec_bdy_final_of(x, 'm');
+  else if (IS("spec_bdy_final:moist") || IS("spec_bdy_final:moist_qv")) o_spec_bdy_final_of(x, 'q');
+  else if (IS("set_w_surface")) o_set_w_surface(x, FLD(w_2), FLD(u_2), FLD(v_2));
   else if (IS("calc_p_rho_phi")) o_calc_p_rho_phi(x);
   else if (IS("rk_stage")) o_rk_stage(x, rk_step);
   else if (IS("step")) o_step(x);
-  else return -1;
+  else return o_named_bc(x, name) == 1 ? 0 : -1;
 #undef IS
   return 0;
 }

@@ -64,6 +64,18 @@ STAGES = [
     ("sumflux", 2, 2), ("sumflux", 1, 1), ("small_step_finish", 2, 0), ("small_step_finish", 3, 0),
     ("calc_p_rho_phi", 2, 0), ("acoustic_substep", 1, 1), ("acoustic_substep", 2, 1), ("acoustic_substep", 2, 2),
     ("acoustic_substep", 3, 1), ("acoustic_substep", 3, 3), ("rk_stage", 1, 0), ("rk_stage", 2, 0), ("rk_stage", 3, 0),
+    # the remaining stages solve_em calls by name (src/wrf_jax/dyn_em/solve_em.py:470-790)
+    ("initialize_moist_old", 1, 0), ("rk_phys_bc_dry_1", 2, 0), ("relax_bdy_dry", 1, 0), ("rk_addtend_dry", 1, 0),
+    ("rk_addtend_dry", 3, 0), ("spec_bdy_dry", 2, 0), ("spec_bdyupdate:u", 2, 1), ("spec_bdyupdate:v", 2, 1),
+    ("spec_bdyupdate:t", 3, 2), ("spec_bdyupdate:mu_2", 2, 1), ("spec_bdyupdate:muts", 2, 1), ("spec_bdyupdate_ph", 2, 1),
+    ("zero_grad_bdy", 2, 1), ("calc_mu_uv_1", 2, 0), ("rk_update_scalar_pd:moist_old", 3, 0),
+    ("rk_update_scalar_pd:scalar_old", 3, 0), ("moist_scalar_advance", 1, 0), ("moist_scalar_advance", 2, 0),
+    ("moist_scalar_advance", 3, 0), ("scalar_advance", 1, 0), ("scalar_advance", 3, 0), ("calc_p_rho_phi:final", 3, 0),
+    ("rk_phys_bc_dry_2", 2, 0), ("set_phys_bc_dry_2", 3, 0), ("spec_bdy_final:u", 3, 0), ("spec_bdy_final:v", 3, 0),
+    ("spec_bdy_final:t", 3, 0), ("spec_bdy_final:ph", 3, 0), ("spec_bdy_final:mu", 3, 0), ("spec_bdy_final:moist", 3, 0),
+    ("set_w_surface", 3, 0), ("set_physical_bc3d:ru_tend", 2, 0), ("set_physical_bc3d:t_save", 2, 0),
+    ("set_physical_bc3d:moist", 2, 0), ("set_physical_bc3d:scalar_old", 3, 0), ("set_physical_bc3d:p:small", 2, 1),
+    ("set_physical_bc3d:ww_m", 3, 0), ("set_physical_bc2d:mudf:small", 2, 1), ("set_physical_bc2d:muts", 3, 0),
 ]
 
 

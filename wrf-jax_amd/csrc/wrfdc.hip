@@ -686,8 +686,8 @@ struct Run {
     add(F(F_mu_tend), 'm', true, true, bp(F_mu_btys, F_mu_btye, F_mu_btxs, F_mu_btxe));
     k_spec_overlay_batch<<<dim3(cdiv(nmax, 64), kmax, n), 64, 0, s>>>(d, L);
   }
-  void dry_boundary_tendency() const {
-    if (X->rk == 1) lateral_relax();
+  // rk_addtend_dry: the physics tendencies into the RK tendencies
+  void rk_addtend_dry() const {
     const int rk = X->rk;
     R3 r0{1, ide, 1, kde - 1, 1, jde - 1};
     k_addtend<<<g3p(r0), b3(), 0, s>>>(d, 0, rk, r0);
@@ -697,6 +697,11 @@ struct Run {
     k_addtend<<<g3p(r2), b3(), 0, s>>>(d, 2, rk, r2);
     R3 r3 = mass3();
     k_addtend<<<g3p(r3), b3(), 0, s>>>(d, 3, rk, r3);
+  }
+  // relax_bdy_dry (first stage) + rk_addtend_dry + spec_bdy_dry
+  void dry_boundary_tendency() const {
+    if (X->rk == 1) lateral_relax();
+    rk_addtend_dry();
     spec_overlays();
   }
   void small_step_prep() const {
@@ -751,8 +756,17 @@ struct Run {
     Z z = zd('w', true, false);
     k_zero_grad_w<<<dim3(cdiv(zone_n(z), 64), kde), 64, 0, s>>>(d);
   }
-  void small_step_finish() const {
+  // calc_mu_uv_1: the staggered column masses of muts
+  void calc_mu_uv_1() const {
     k_mu_uv<<<dim3(cdiv(ide, 64), jde), 64, 0, s>>>(d, F(F_muts), nullptr, F(F_muus), F(F_muvs), nullptr);
+  }
+  void small_step_finish() const {
+    calc_mu_uv_1();
+    recouple();
+  }
+  // small_step_finish proper (module_small_step_em.py:1119-1332), without
+  // the calc_mu_uv_1 that the RK stage runs before it
+  void recouple() const {
     R3 r = full3();
     if (X->rk == 3) k_ssf<true><<<g3(r), b3(), 0, s>>>(d, (float)X->dts_rk, (float)X->nsmall, r);
     else k_ssf<false><<<g3(r), b3(), 0, s>>>(d, (float)X->dts_rk, (float)X->nsmall, r);
@@ -933,12 +947,56 @@ struct Run {
     X->rec_i++;
   }
 
+  // ---- physical-boundary groups and spec_bdy_final of named fields --------
+  // rk_phys_bc_dry_1 (module_bc_em.py:109-216)
+  BcBatch &phys_bc_dry_1(BcBatch &b) const {
+    return b.add3(F(F_ru), 'u').add3(F(F_rv), 'v').add3(F(F_rw), 'w').add3(F(F_ww), 'w').add3(F(F_php), 'w')
+        .add3(F(F_alt), 't').add3(F(F_p), 'p').add2(F(F_muu), 'u').add2(F(F_muv), 'v').add2(F(F_mut), 't');
+  }
+  // rk_phys_bc_dry_2 (module_bc_em.py:219-318)
+  BcBatch &phys_bc_dry_2(BcBatch &b) const {
+    return b.add3(F(F_u_2), 'u').add3(F(F_v_2), 'v').add3(F(F_w_2), 'w').add3(F(F_t_2), 'p').add3(F(F_ph_2), 'w')
+        .add2(F(F_mu_2), 't');
+  }
+  // set_phys_bc_dry_2 (module_bc_em.py:321-445)
+  BcBatch &set_phys_bc_dry_2(BcBatch &b) const {
+    return b.add3(F(F_u_1), 'u').add3(F(F_u_2), 'u').add3(F(F_v_1), 'v').add3(F(F_v_2), 'v').add3(F(F_w_1), 'w')
+        .add3(F(F_w_2), 'w').add3(F(F_t_1), 'p').add3(F(F_t_2), 'p').add3(F(F_ph_1), 'w').add3(F(F_ph_2), 'w')
+        .add2(F(F_mu_1), 't').add2(F(F_mu_2), 't');
+  }
+  // spec_bdy_final of u, v, t, ph, mu or QV ('u' 'v' 't' 'h' 'm' 'q'):
+  // the specified zone from the boundary data at dtbc, mass-coupled with
+  // the stage's column masses and map factors (module_bc_em.py:1009-1124)
+  FinalBatch &spec_final(FinalBatch &b, char which) const {
+    switch (which) {
+      case 'u':
+        return b.add(F(F_u_2), 'u', false, F(F_muus), F(F_msfuy), F(F_c1h), F(F_c2h), bp(F_u_bys, F_u_bye, F_u_bxs, F_u_bxe),
+                     bp(F_u_btys, F_u_btye, F_u_btxs, F_u_btxe));
+      case 'v':
+        return b.add(F(F_v_2), 'v', false, F(F_muvs), F(F_msfvx), F(F_c1h), F(F_c2h), bp(F_v_bys, F_v_bye, F_v_bxs, F_v_bxe),
+                     bp(F_v_btys, F_v_btye, F_v_btxs, F_v_btxe));
+      case 't':
+        return b.add(F(F_t_2), 't', false, F(F_muts), F(F_msfty), F(F_c1h), F(F_c2h), bp(F_t_bys, F_t_bye, F_t_bxs, F_t_bxe),
+                     bp(F_t_btys, F_t_btye, F_t_btxs, F_t_btxe));
+      case 'h':
+        return b.add(F(F_ph_2), 'h', false, F(F_muts), F(F_msfty), F(F_c1f), F(F_c2f), bp(F_ph_bys, F_ph_bye, F_ph_bxs, F_ph_bxe),
+                     bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe));
+      case 'm':
+        return b.add(F(F_mu_2), 'm', true, F(F_muts), F(F_msfty), F(F_c1h), F(F_c2h), bp(F_mu_bys, F_mu_bye, F_mu_bxs, F_mu_bxe),
+                     bp(F_mu_btys, F_mu_btye, F_mu_btxs, F_mu_btxe));
+      default:
+        return b.add(Sp(F_moist, 0), 't', false, F(F_muts), F(F_msfty), F(F_c1h), F(F_c2h), bp(F_qv_bys, F_qv_bye, F_qv_bxs, F_qv_bxe),
+                     bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe));
+    }
+  }
+  void set_w_surface() const { k_set_w_surface<<<dim3(cdiv(ide - 1, 64), jde - 1), 64, 0, s>>>(d); }
+
   void rk_stage() const {
     rk_step_prep();
-    BcBatch(this)
-        .add3(F(F_ru), 'u').add3(F(F_rv), 'v').add3(F(F_rw), 'w').add3(F(F_ww), 'w').add3(F(F_php), 'w')
-        .add3(F(F_alt), 't').add3(F(F_p), 'p').add2(F(F_muu), 'u').add2(F(F_muv), 'v').add2(F(F_mut), 't')
-        .add3(F(F_al), 'p').add3(F(F_ph_2), 'w').flush();
+    {
+      BcBatch b(this);
+      phys_bc_dry_1(b).add3(F(F_al), 'p').add3(F(F_ph_2), 'w');
+    }
     if (X->rk == 1) {
       first_rk_step_part1();
       first_rk_step_part2(false);
@@ -961,7 +1019,7 @@ struct Run {
     calc_p_rho_phi();
     if (X->rk < 3) {
       BcBatch bc(this);
-      bc.add3(F(F_u_2), 'u').add3(F(F_v_2), 'v').add3(F(F_w_2), 'w').add3(F(F_t_2), 'p').add3(F(F_ph_2), 'w').add2(F(F_mu_2), 't');
+      phys_bc_dry_2(bc);
       for (int q = 0; q < nm; q++) bc.add3(Sp(F_moist, q), 'p');
       for (int q = 0; q < nsc; q++) bc.add3(Sp(F_scalar, q), 'p');
     }
@@ -988,26 +1046,17 @@ struct Run {
       set_rk(rk);
       rk_stage();
     }
-    BcBatch(this)
-        .add3(F(F_u_1), 'u').add3(F(F_u_2), 'u').add3(F(F_v_1), 'v').add3(F(F_v_2), 'v').add3(F(F_w_1), 'w')
-        .add3(F(F_w_2), 'w').add3(F(F_t_1), 'p').add3(F(F_t_2), 'p').add3(F(F_ph_1), 'w').add3(F(F_ph_2), 'w')
-        .add2(F(F_mu_1), 't').add2(F(F_mu_2), 't').add3(Sp(F_moist, 0), 'p').flush();
+    {
+      BcBatch b(this);
+      set_phys_bc_dry_2(b).add3(Sp(F_moist, 0), 'p');
+    }
     // the six fields read only coupling masses and map factors, none of them
     // another's result, so one launch serves all
-    FinalBatch(this)
-        .add(F(F_u_2), 'u', false, F(F_muus), F(F_msfuy), F(F_c1h), F(F_c2h), bp(F_u_bys, F_u_bye, F_u_bxs, F_u_bxe),
-             bp(F_u_btys, F_u_btye, F_u_btxs, F_u_btxe))
-        .add(F(F_v_2), 'v', false, F(F_muvs), F(F_msfvx), F(F_c1h), F(F_c2h), bp(F_v_bys, F_v_bye, F_v_bxs, F_v_bxe),
-             bp(F_v_btys, F_v_btye, F_v_btxs, F_v_btxe))
-        .add(F(F_t_2), 't', false, F(F_muts), F(F_msfty), F(F_c1h), F(F_c2h), bp(F_t_bys, F_t_bye, F_t_bxs, F_t_bxe),
-             bp(F_t_btys, F_t_btye, F_t_btxs, F_t_btxe))
-        .add(F(F_ph_2), 'h', false, F(F_muts), F(F_msfty), F(F_c1f), F(F_c2f), bp(F_ph_bys, F_ph_bye, F_ph_bxs, F_ph_bxe),
-             bp(F_ph_btys, F_ph_btye, F_ph_btxs, F_ph_btxe))
-        .add(F(F_mu_2), 'm', true, F(F_muts), F(F_msfty), F(F_c1h), F(F_c2h), bp(F_mu_bys, F_mu_bye, F_mu_bxs, F_mu_bxe),
-             bp(F_mu_btys, F_mu_btye, F_mu_btxs, F_mu_btxe))
-        .add(Sp(F_moist, 0), 't', false, F(F_muts), F(F_msfty), F(F_c1h), F(F_c2h), bp(F_qv_bys, F_qv_bye, F_qv_bxs, F_qv_bxe),
-             bp(F_qv_btys, F_qv_btye, F_qv_btxs, F_qv_btxe));
-    k_set_w_surface<<<dim3(cdiv(ide - 1, 64), jde - 1), 64, 0, s>>>(d);
+    {
+      FinalBatch fb(this);
+      for (char w : {'u', 'v', 't', 'h', 'm', 'q'}) spec_final(fb, w);
+    }
+    set_w_surface();
   }
 
   // _rk_step_sizes (solve_em.py:243-257), rk_order = 3
@@ -1267,6 +1316,153 @@ int wrfdc_step(wrfdc_ctx *x, int nsteps, double dtbc0) {
   });
 }
 
+int wrfdc_set_dtbc(wrfdc_ctx *x, double dtbc) {
+  if (!x) return WRFDC_EINVAL;
+  if (!std::isfinite(dtbc)) return set_err(x, WRFDC_EINVAL, "dtbc must be finite");
+  x->dtbc = dtbc;
+  return WRFDC_OK;
+}
+
+// ---- the stages of solve_em by the reference's callback names -------------
+// (src/wrf_jax/dyn_em/solve_em.py:470-790, src/wrf_jax/runtime.py:16235-16301)
+
+static int field_id(const std::string &name) {
+  for (int f = 0; f < NFIELDS; f++)
+    if (name == kNames[f]) return f;
+  return -1;
+}
+
+// boundary kind of set_physical_bc3d:<field> (runtime.py:7421-7443): 'u' and
+// 'v' staggered in x / y, 'w' on full levels, anything else a mass field
+static bool bc3_kind(const std::string &f, char *kind) {
+  static const struct {
+    const char *name;
+    char kind;
+  } table[] = {{"rho", 'p'},     {"al", 'p'},      {"p", 'p'},       {"ph_2", 'w'}, {"t_1", 'p'},
+               {"t_save", 't'},  {"ru_tend", 'u'}, {"rv_tend", 'v'}, {"ru_m", 'u'}, {"rv_m", 'v'},
+               {"ww_m", 'w'},    {"h_diabatic", 't'}};
+  for (auto &e : table)
+    if (f == e.name) {
+      *kind = e.kind;
+      return true;
+    }
+  return false;
+}
+
+// set_physical_bc3d:<field>[:small] and set_physical_bc2d:<field>[:small];
+// false: not such a stage.  moist / moist_old / scalar / scalar_old cover
+// every species of the array, moist_qv the first.  tke_1 / tke_2: the
+// supported closure (km_opt 4) carries no TKE, so the library holds none
+// and the stage has nothing to update.
+static bool named_bc(const Run &run, wrfdc_ctx *x, const std::string &n, std::string &err) {
+  const std::string p3 = "set_physical_bc3d:", p2 = "set_physical_bc2d:";
+  const bool is3 = n.compare(0, p3.size(), p3) == 0, is2 = n.compare(0, p2.size(), p2) == 0;
+  if (!is3 && !is2) return false;
+  std::string f = n.substr(p3.size());
+  const size_t colon = f.find(':');
+  if (colon != std::string::npos) {
+    if (f.substr(colon) != ":small") {
+      err = "unknown stage: " + n;
+      return true;
+    }
+    f = f.substr(0, colon);
+  }
+  Run::BcBatch b(&run);
+  if (is2) {
+    static const char *const ok2[] = {"mu_1", "mu_2", "mudf", "muts", "mut"};
+    for (const char *o : ok2)
+      if (f == o) {
+        b.add2(run.F(field_id(f)), 't');
+        return true;
+      }
+    err = "set_physical_bc2d: no boundary kind for field " + f;
+    return true;
+  }
+  if (f == "tke_1" || f == "tke_2") return true;
+  const int nm = x->c.nmoist, nsc = x->c.nscalar;
+  if (f == "moist" || f == "moist_old") {
+    for (int q = 0; q < nm; q++) b.add3(run.Sp(f == "moist" ? F_moist : F_moist_old, q), 'p');
+  } else if (f == "scalar" || f == "scalar_old") {
+    for (int q = 0; q < nsc; q++) b.add3(run.Sp(f == "scalar" ? F_scalar : F_scalar_old, q), 'p');
+  } else if (f == "moist_qv") {
+    b.add3(run.Sp(F_moist, 0), 'p');
+  } else {
+    char kind;
+    if (!bc3_kind(f, &kind)) {
+      err = "set_physical_bc3d: no boundary kind for field " + f;
+      return true;
+    }
+    b.add3(run.F(field_id(f)), kind);
+  }
+  return true;
+}
+
+// the stages solve_em calls by name that are not plain pieces of the step;
+// false: not one of them
+static bool solve_em_stage(const Run &run, wrfdc_ctx *x, const std::string &n, std::string &err) {
+  const int nm = x->c.nmoist, nsc = x->c.nscalar;
+  const double dts = x->dts_rk;
+  if (n == "initialize_moist_old") run.sync_old(true, false);
+  else if (n == "rk_phys_bc_dry_1") {
+    Run::BcBatch b(&run);
+    run.phys_bc_dry_1(b);
+  }
+  else if (n == "relax_bdy_dry") run.lateral_relax();
+  else if (n == "rk_addtend_dry") run.rk_addtend_dry();
+  else if (n == "spec_bdy_dry") run.spec_overlays();
+  else if (n == "spec_bdyupdate:u") run.spec_update(run.F(F_u_2), run.F(F_ru_tend), dts, 'u', false);
+  else if (n == "spec_bdyupdate:v") run.spec_update(run.F(F_v_2), run.F(F_rv_tend), dts, 'v', false);
+  else if (n == "spec_bdyupdate:t") run.spec_update(run.F(F_t_2), run.F(F_t_tend), dts, 't', false);
+  else if (n == "spec_bdyupdate:mu_2") run.spec_update(run.F(F_mu_2), run.F(F_mu_tend), dts, 'm', true);
+  else if (n == "spec_bdyupdate:muts") run.spec_update(run.F(F_muts), run.F(F_mu_tend), dts, 'm', true);
+  else if (n == "spec_bdyupdate:w") err = "spec_bdyupdate:w is the nested-domain update of w; only specified boundaries are supported";
+  else if (n == "spec_bdyupdate_ph") run.spec_bdyupdate_ph();
+  else if (n == "zero_grad_bdy") run.zero_grad_bdy_w();
+  else if (n == "calc_mu_uv_1") run.calc_mu_uv_1();
+  else if (n == "rk_update_scalar_pd:moist_old") {
+    for (int q = 0; q < nm; q++) run.update_scalar_pd(run.Sp(F_moist_old, q), run.Sp(F_moist_tend, q));
+  }
+  else if (n == "rk_update_scalar_pd:scalar_old") {
+    for (int q = 0; q < nsc; q++) run.update_scalar_pd(run.Sp(F_scalar_old, q), run.Sp(F_scalar_tend, q));
+  }
+  else if (n == "moist_scalar_advance") {
+    for (int q = 0; q < nm; q++)
+      run.species_advance(run.Sp(F_moist, q), run.Sp(F_moist_old, q), run.Sp(F_moist_tend, q), q == 0, true, x->c.pd_moist != 0);
+  }
+  else if (n == "scalar_advance") {
+    for (int q = 0; q < nsc; q++)
+      run.species_advance(run.Sp(F_scalar, q), run.Sp(F_scalar_old, q), run.Sp(F_scalar_tend, q), false, false, x->c.pd_scalar != 0);
+  }
+  else if (n == "calc_p_rho_phi:final") run.calc_p_rho_phi();
+  else if (n == "rk_phys_bc_dry_2") {
+    Run::BcBatch b(&run);
+    run.phys_bc_dry_2(b);
+  }
+  else if (n == "set_phys_bc_dry_2") {
+    Run::BcBatch b(&run);
+    run.set_phys_bc_dry_2(b);
+  }
+  else if (n.compare(0, 15, "spec_bdy_final:") == 0) {
+    const std::string f = n.substr(15);
+    char w = 0;
+    if (f == "u") w = 'u';
+    else if (f == "v") w = 'v';
+    else if (f == "t") w = 't';
+    else if (f == "ph") w = 'h';
+    else if (f == "mu") w = 'm';
+    else if (f == "moist" || f == "moist_qv") w = 'q';  // boundary data exist for QV only
+    else {
+      err = "unknown stage: " + n;
+      return true;
+    }
+    Run::FinalBatch fb(&run);
+    run.spec_final(fb, w);
+  }
+  else if (n == "set_w_surface") run.set_w_surface();
+  else return named_bc(run, x, n, err);
+  return true;
+}
+
 int wrfdc_stage(wrfdc_ctx *x, const char *name, int rk_step, int iteration) {
   if (!x || !name) return WRFDC_EINVAL;
   if (rk_step < 1 || rk_step > 3) return set_err(x, WRFDC_EINVAL, "rk_step must be 1, 2 or 3");
@@ -1321,14 +1517,18 @@ int wrfdc_stage(wrfdc_ctx *x, const char *name, int rk_step, int iteration) {
       if (iteration < 1) return set_err(x, WRFDC_EINVAL, "acoustic_substep needs iteration >= 1");
       run.acoustic_substep(iteration, true);
     }
-    else if (n == "small_step_finish") run.small_step_finish();
+    else if (n == "small_step_finish") run.recouple();
     else if (n == "calc_p_rho_phi") run.calc_p_rho_phi();
     else if (n == "rk_stage") {
       run.rk_stage();
       if (run.species_in_alt()) run.species_to(false);
     }
     else if (n == "step") run.step();
-    else return set_err(x, WRFDC_EINVAL, "unknown stage: " + n);
+    else {
+      std::string err;
+      if (!solve_em_stage(run, x, n, err)) err = "unknown stage: " + n;
+      if (!err.empty()) return set_err(x, WRFDC_EINVAL, err);
+    }
     return WRFDC_OK;
   });
 }

@@ -32,6 +32,7 @@ SIGNATURES = {
     "wrfdc_set_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "wrfdc_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]),
     "wrfdc_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
+    "wrfdc_set_dtbc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     "wrfdc_checksums": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "wrfdc_checksums_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
                                                 ctypes.c_void_p]),

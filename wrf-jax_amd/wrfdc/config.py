@@ -74,7 +74,8 @@ class DycoreConfig:
             "v_sca_adv_order": 3, "diff_opt": 2, "km_opt": 4, "damp_opt": 0, "w_damping": 0,
             "hypsometric_opt": 2, "use_theta_m": 0, "non_hydrostatic": True, "specified": True,
             "zadvect_implicit": 0, "periodic_x": False, "top_lid": False,
-            "spec_exp": 0.0, "have_bcs_moist": False, "have_bcs_scalar": False,
+            "spec_exp": 0.0, "have_bcs_moist": False, "have_bcs_scalar": False, "constant_bc": False, "nested": False,
+            "mp_physics": 0, "cu_physics": 0, "use_adaptive_time_step": False,
         }
         for key, value in self.options.items():
             if key in supported and value != supported[key]:

@@ -95,6 +95,10 @@ class Domain:
     def stage(self, name: str, rk_step: int = 1, iteration: int = 0) -> None:
         self._check(self.lib.wrfdc_stage(self.ctx, name.encode(), int(rk_step), int(iteration)), f"wrfdc_stage({name})")
 
+    def set_dtbc(self, dtbc: float) -> None:
+        """Boundary time (s since the boundary record) that stage calls read."""
+        self._check(self.lib.wrfdc_set_dtbc(self.ctx, float(dtbc)), "wrfdc_set_dtbc")
+
     def checksums(self) -> np.ndarray:
         """FP64 [sum(mu_2+mub), sum(t_2), sum(moist), n_nonfinite, dry-air mass (kg),
         total energy (J)]; raises FloatingPointError on NaN/Inf."""
