@@ -10,8 +10,9 @@ species, lateral boundaries).
 Prints one JSON line.  ``roofline`` covers the dominant kernel (algorithmic
 bytes per launch / mean launch time from HIP events on the dycore's stream);
 ``step_roofline`` the whole step against the minimal fused traffic model.
-``cpu_baseline`` times one RK3 step of the CPU oracle (one core) on the same
-case (a bounded sample: one step instead of the timed run).
+``cpu_baseline`` times the CPU oracle's OpenMP build on every host core this
+process may use (OMP_NUM_THREADS caps it), over tiles that cover the same
+workload (a bounded sample: a few steps instead of the timed run).
 """
 
 from __future__ import annotations
@@ -45,20 +46,53 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(case) -> dict:
-    """One RK3 step of the CPU oracle (FP32, one thread) on the same case."""
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.lower().startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _tile_grid(n: int) -> tuple[int, int]:
+    """px x py = n tiles, as square as n allows."""
+    px = int(n ** 0.5)
+    while n % px:
+        px -= 1
+    return n // px, px
+
+
+def cpu_baseline(workload: str, nsteps: int = 4) -> dict:
+    """The CPU oracle (FP32, OpenMP build) on all the host cores this process
+    may use: the workload's domain cut into one tile per thread, each tile a
+    domain of its own with the workload's spacing, species and vertical grid
+    (specified boundaries, no halo exchange, so the CPU is spared the
+    communication a tiled run would pay), all tiles stepped concurrently."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     from oracle import Oracle
 
-    cfg = case.config
-    o = Oracle(cfg, case.arrays)
+    from wrfdc.cases import CONFIGS, make_case
+
+    params = dict(CONFIGS[workload])
+    nx, ny, nz = params.pop("nx"), params.pop("ny"), params.pop("nz")
+    allowed = len(os.sched_getaffinity(0))
+    threads = max(1, min(allowed, int(os.environ.get("OMP_NUM_THREADS", allowed))))
+    px, py = _tile_grid(threads)
+    tiles = [make_case(nx // px, ny // py, nz, member=q, **params) for q in range(px * py)]
+    oracles = [Oracle(t.config, t.arrays, "f32_omp") for t in tiles]
     t0 = time.perf_counter()
-    o.step(1, 0.0)
+    Oracle.step_many(oracles, nsteps, 0.0, threads)
     sec = time.perf_counter() - t0
-    o.close()
-    npts = cfg.nx * cfg.ny * cfg.nz
-    return {"value": npts / sec / 1e6, "unit": "Mpoint-steps/s", "cores": 1, "kind": "port",
-            "sample": f"1 RK3 step of the CPU oracle (FP32, 1 thread) on the same {cfg.nx}x{cfg.ny}x{cfg.nz} case, {sec:.1f} s"}
+    for o in oracles:
+        o.close()
+    npts = sum(t.config.nx * t.config.ny * t.config.nz for t in tiles)
+    return {"value": npts * nsteps / sec / 1e6, "unit": "Mpoint-steps/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+            "sample": f"{nsteps} RK3 steps of the FP32 oracle (OpenMP, {threads} threads) on {px}x{py} tiles of "
+                      f"{nx // px}x{ny // py}x{nz} covering the {nx}x{ny}x{nz} workload, {sec:.1f} s"}
 
 
 def main():
@@ -167,7 +201,7 @@ def main():
         del src, dst
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(case)
+        out["cpu_baseline"] = cpu_baseline(args.workload)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

@@ -1,6 +1,7 @@
 """ctypes binding of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
 
-Loads ``oracle/_build/liboracle_f32.so`` (or the FP64 build) compiled from
+Loads ``oracle/_build/liboracle_f32.so`` (or the FP64 build, or the FP32
+build with OpenMP, ``f32_omp``, which steps several domains at once) compiled from
 ``oracle/src`` and runs the loop-form restatement of the reference dycore on
 numpy arrays laid out like the product's field registry.  Only tests/,
 ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg import this
@@ -53,6 +54,8 @@ def _lib(precision: str) -> ctypes.CDLL:
         lib.o_run_stage.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
         lib.o_run_stage.restype = ctypes.c_int
         lib.o_set_dtbc.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        lib.o_step_many.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        lib.o_step_many.restype = ctypes.c_int
         lib.o_field_name.restype = ctypes.c_char_p
         lib.o_field_name.argtypes = [ctypes.c_int]
         lib.o_field_kind.argtypes = [ctypes.c_int]
@@ -69,7 +72,7 @@ class Oracle:
 
     def __init__(self, config: DycoreConfig, arrays: dict, precision: str = "f32"):
         self.precision = precision
-        self.dtype = np.float32 if precision == "f32" else np.float64
+        self.dtype = np.float32 if precision.startswith("f32") else np.float64
         self.lib = _lib(precision)
         self.config = config
         self.dims = Dims(nx=config.nx, ny=config.ny, nz=config.nz, nmoist=config.nmoist, nscalar=config.nscalar,
@@ -110,6 +113,17 @@ class Oracle:
         for n in range(nsteps):
             self.set_dtbc(dtbc0 + (n + 1) * self.config.dt)
             self.stage("step", 3, 0)
+
+    @staticmethod
+    def step_many(oracles, nsteps: int, dtbc0: float = 0.0, threads: int = 1) -> None:
+        """Advance independent oracles of one library nsteps steps each,
+        ``threads`` at a time (the f32_omp build runs them concurrently)."""
+        lib = oracles[0].lib
+        if any(o.lib is not lib for o in oracles):
+            raise ValueError("step_many needs oracles of one build")
+        ctxs = (ctypes.c_void_p * len(oracles))(*[o.ctx for o in oracles])
+        if lib.o_step_many(ctxs, len(oracles), int(nsteps), float(dtbc0), int(threads)) != 0:
+            raise ValueError("o_step_many: bad arguments")
 
     def copy(self) -> dict:
         return {k: v.copy() for k, v in self.arrays.items()}

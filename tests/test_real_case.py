@@ -125,6 +125,33 @@ def test_namelist_parsing(tmp_path):
     assert c.start_time == "2024-06-15_12:00:00" and c.end_time == "2024-06-15_13:30:00"
 
 
+def test_namelist_grammar(tmp_path):
+    """Repeat counts, doubled quotes, '&end' terminators, text between
+    groups, comments after values and keys spread over several lines."""
+    p = tmp_path / "namelist.input"
+    p.write_text("""&domains
+ e_we = 3*101,
+ e_sn = 91 e_vert = 46 ! two assignments on one line
+ dx = 1.5D3, dy = 1500
+ title = 'it''s "quoted"' ,
+&end
+stray text outside any group
+&physics
+ mp_physics = 0, 0,
+   0 /
+""")
+    g = parse_namelist(p)
+    assert g["domains"]["e_we"] == [101, 101, 101]
+    assert g["domains"]["e_sn"] == 91 and g["domains"]["e_vert"] == 46
+    assert g["domains"]["dx"] == 1500.0 and g["domains"]["dy"] == 1500
+    assert g["domains"]["title"] == 'it\'s "quoted"'
+    assert g["physics"]["mp_physics"] == [0, 0, 0]
+    bad = tmp_path / "bad.input"
+    bad.write_text("&domains\n 7, 8\n/\n")
+    with pytest.raises(ValueError):
+        parse_namelist(bad)
+
+
 def test_case_files_round_trip(tmp_path):
     case = _case()
     export_case(tmp_path, case.config, case.arrays, interval_seconds=54)

@@ -1,19 +1,23 @@
 """``namelist.input`` -> case configuration.
 
-Mirrors src/wrf_jax/case_config.py:1-190 (``CaseConfig``, ``parse_namelist``,
-``load_case_config``): the subset of Fortran namelist syntax that WRF's
-``namelist.input`` uses (``&group`` ... ``/`` blocks, ``key = v1, v2, ...``
-assignments that may continue over lines, ``!`` comments outside quotes,
-``.true.``/``.false.``, quoted strings, integers and reals with ``d``/``e``
-exponents).  A key with one value maps to that value, with several to the
-list; every domain-indexed quantity is read at index 0 (domain 1).
+Interface of src/wrf_jax/case_config.py (``CaseConfig``, ``parse_namelist``,
+``load_case_config``).  The namelist is read by a lexer over the whole text
+and a small state machine over its tokens, which covers the Fortran namelist
+grammar WRF's ``namelist.input`` uses: ``&group`` ... ``/`` (or ``&end``)
+blocks, ``key = v1, v2, ...`` assignments whose values may run over several
+lines, ``!`` comments, logicals (``.true.``, ``.f.``, ``t`` ...), quoted
+strings (a doubled quote stands for one), integers, reals with ``e`` or ``d``
+exponents and ``r*value`` repeats.  Names are case-insensitive and stored in
+lower case.  A key with one value maps to it, with several to their list;
+domain-indexed quantities are read for domain 1 (index 0).
 """
 
 from __future__ import annotations
 
+import re
 from dataclasses import dataclass
 from pathlib import Path
-from typing import Any
+from typing import Any, Iterator
 
 
 @dataclass(frozen=True)
@@ -35,137 +39,147 @@ class CaseConfig:
     end_time: str
 
 
-def _code_part(line: str) -> str:
-    """The line up to its first ``!`` that is not inside a quoted string."""
-    quote = None
-    for pos, ch in enumerate(line):
-        if quote:
-            if ch == quote:
-                quote = None
-        elif ch in "'\"":
-            quote = ch
-        elif ch == "!":
-            return line[:pos]
-    return line
+_LEX = re.compile(
+    r"""
+      (?P<space>\s+)
+    | (?P<comment>![^\n]*)
+    | (?P<open>&(?P<gname>[A-Za-z_]\w*))
+    | (?P<close>/)
+    | (?P<key>[A-Za-z_][\w%]*(?:\(\s*\d+\s*\))?)\s*=
+    | (?P<str>'(?:[^']|'')*'|"(?:[^"]|"")*")
+    | (?P<comma>,)
+    | (?P<atom>[^\s,/!&'"=]+)
+    """,
+    re.VERBOSE,
+)
+
+_LOGICAL = {".true.": True, ".t.": True, "t": True, "true": True, ".false.": False, ".f.": False, "f": False, "false": False}
+_INT = re.compile(r"[+-]?\d+$")
+_REAL = re.compile(r"[+-]?(\d+\.?\d*|\.\d+)([ed][+-]?\d+)?$", re.IGNORECASE)
 
 
-def _tokens(text: str) -> list[str]:
-    """Comma-separated values of one assignment (commas inside quotes kept)."""
-    out, cur, quote = [], [], None
-    for ch in text:
-        if quote:
-            cur.append(ch)
-            if ch == quote:
-                quote = None
-        elif ch in "'\"":
-            quote = ch
-            cur.append(ch)
-        elif ch == ",":
-            tok = "".join(cur).strip()
-            if tok:
-                out.append(tok)
-            cur = []
+def _lex(text: str, where: str) -> Iterator[tuple[str, str]]:
+    pos = 0
+    while pos < len(text):
+        m = _LEX.match(text, pos)
+        if m is None:
+            line = text.count("\n", 0, pos) + 1
+            raise ValueError(f"{where}:{line}: cannot read namelist text {text[pos:pos + 20]!r}")
+        pos = m.end()
+        kind = m.lastgroup
+        if kind in ("space", "comment"):
+            continue
+        if kind in ("open", "gname"):
+            yield "open", m.group("gname").lower()
+        elif kind == "key":
+            yield "key", re.sub(r"\s+", "", m.group("key")).lower()
         else:
-            cur.append(ch)
-    tok = "".join(cur).strip()
-    if tok:
-        out.append(tok)
-    return out
+            yield kind, m.group(kind)
 
 
-def _value(tok: str) -> Any:
-    low = tok.lower()
-    if low in (".true.", "true", ".t."):
-        return True
-    if low in (".false.", "false", ".f."):
-        return False
-    if len(tok) >= 2 and tok[0] == tok[-1] and tok[0] in "'\"":
-        return tok[1:-1]
-    try:
-        if any(c in low for c in ".ed"):
-            return float(low.replace("d", "e"))
-        return int(tok)
-    except ValueError:
-        return tok
+def _scalar(atom: str) -> Any:
+    low = atom.lower()
+    if low in _LOGICAL:
+        return _LOGICAL[low]
+    if _INT.match(atom):
+        return int(atom)
+    if _REAL.match(atom):
+        return float(low.replace("d", "e"))
+    return atom
+
+
+def _atom_values(atom: str) -> list:
+    """One unquoted item; ``r*v`` stands for r copies of v."""
+    count, star, rest = atom.partition("*")
+    if star and count.isdigit() and rest:
+        return [_scalar(rest)] * int(count)
+    return [_scalar(atom)]
+
+
+def _unquote(tok: str) -> str:
+    q = tok[0]
+    return tok[1:-1].replace(q + q, q)
 
 
 def parse_namelist(path: str | Path) -> dict:
     """Groups (lower-case names) -> {key (lower case): value or list}."""
+    where = str(path)
     groups: dict[str, dict[str, Any]] = {}
-    group: str | None = None
-    key: str | None = None
-    parts: list[str] = []
-
-    def close() -> None:
-        nonlocal key, parts
-        if group is not None and key is not None:
-            vals = [_value(t) for t in _tokens(" ".join(parts))]
-            groups[group][key] = vals[0] if len(vals) == 1 else vals
-        key, parts = None, []
-
-    for raw in Path(path).read_text(encoding="ascii").splitlines():
-        line = _code_part(raw).strip()
-        if not line:
-            continue
-        if line.startswith("&"):
-            close()
-            group = line[1:].strip().lower()
-            groups[group] = {}
-        elif line.startswith("/"):
-            close()
-            group = None
-        elif group is None:
-            continue
-        elif "=" in line:
-            close()
-            lhs, rhs = line.split("=", 1)
-            key, parts = lhs.strip().lower(), [rhs.strip()]
-        elif key is not None:
-            parts.append(line)
-    close()
-    return groups
+    current: dict[str, Any] | None = None
+    values: list | None = None
+    for kind, tok in _lex(Path(path).read_text(encoding="ascii"), where):
+        if kind == "open":
+            if tok == "end":  # '&end' closes a group in some namelist dialects
+                current = values = None
+            else:
+                current = groups.setdefault(tok, {})
+                values = None
+        elif kind == "close":
+            current = values = None
+        elif current is None:
+            continue  # text between groups is ignored, as Fortran does
+        elif kind == "key":
+            values = current[tok] = []
+        elif values is None:
+            raise ValueError(f"{where}: value {tok!r} before any key in a namelist group")
+        elif kind == "str":
+            values.append(_unquote(tok))
+        elif kind == "atom":
+            values.extend(_atom_values(tok))
+    return {g: {k: (v[0] if len(v) == 1 else v) for k, v in kv.items()} for g, kv in groups.items()}
 
 
-def _get(groups: dict, group: str, key: str, default: Any = None, index: int = 0) -> Any:
-    """Value of ``group.key`` at domain ``index`` (the last entry when the list
-    is shorter), or ``default``."""
-    v = groups.get(group, {}).get(key, default)
-    if isinstance(v, list):
-        return v[index] if index < len(v) else v[-1]
-    return v
+def _domain1(namelist: dict, group: str, key: str, default: Any = None) -> Any:
+    """The domain-1 value of ``group.key`` (first entry of a list)."""
+    v = namelist.get(group, {}).get(key, default)
+    return v[0] if isinstance(v, list) and v else v
 
 
-def _stamp(groups: dict, prefix: str) -> str:
-    f = lambda name, default=None: int(_get(groups, "time_control", f"{prefix}_{name}", default))  # noqa: E731
-    return f"{f('year'):04d}-{f('month'):02d}-{f('day'):02d}_{f('hour', 0):02d}:{f('minute', 0):02d}:{f('second', 0):02d}"
+def _time_stamp(namelist: dict, which: str) -> str:
+    parts = [int(_domain1(namelist, "time_control", f"{which}_{unit}", 0)) for unit in
+             ("year", "month", "day", "hour", "minute", "second")]
+    return "{:04d}-{:02d}-{:02d}_{:02d}:{:02d}:{:02d}".format(*parts)
 
 
-def _history_seconds(groups: dict) -> int:
-    tc = groups.get("time_control", {})
-    for key, scale in (("history_interval_s", 1), ("history_interval_m", 60), ("history_interval_h", 3600)):
-        if tc.get(key) is not None:
-            return int(_get(groups, "time_control", key)) * scale
-    return int(_get(groups, "time_control", "history_interval", 60)) * 60
+# history output interval keys in the order WRF honours them, with seconds
+# per unit; the plain key counts minutes
+_HISTORY_KEYS = (("history_interval_s", 1), ("history_interval_m", 60), ("history_interval_h", 3600),
+                 ("history_interval", 60))
+
+
+def _history_interval(namelist: dict) -> int:
+    for key, seconds in _HISTORY_KEYS:
+        v = _domain1(namelist, "time_control", key)
+        if v is not None:
+            return int(v) * seconds
+    return 60 * 60
 
 
 def load_case_config(path: str | Path) -> CaseConfig:
-    g = parse_namelist(path)
-    tc = lambda key: int(_get(g, "time_control", key, 0))  # noqa: E731
-    run_seconds = tc("run_days") * 86400 + tc("run_hours") * 3600 + tc("run_minutes") * 60 + tc("run_seconds")
+    nl = parse_namelist(path)
+    run = {unit: int(_domain1(nl, "time_control", f"run_{unit}", 0)) for unit in ("days", "hours", "minutes", "seconds")}
+    run_seconds = ((run["days"] * 24 + run["hours"]) * 60 + run["minutes"]) * 60 + run["seconds"]
+
+    def need(group: str, key: str):
+        v = _domain1(nl, group, key)
+        if v is None:
+            raise KeyError(f"{path}: &{group} has no {key}")
+        return v
+
     return CaseConfig(
-        namelist=g,
+        namelist=nl,
         run_seconds=run_seconds,
-        time_step=float(_get(g, "domains", "time_step")),
-        boundary_interval_seconds=int(_get(g, "time_control", "interval_seconds", run_seconds)),
-        history_interval_seconds=_history_seconds(g),
-        max_dom=int(_get(g, "domains", "max_dom", 1)),
-        e_we=int(_get(g, "domains", "e_we")),
-        e_sn=int(_get(g, "domains", "e_sn")),
-        e_vert=int(_get(g, "domains", "e_vert")),
-        dx=float(_get(g, "domains", "dx")),
-        dy=float(_get(g, "domains", "dy")),
-        start_time=_stamp(g, "start"),
-        end_time=_stamp(g, "end"),
+        time_step=float(need("domains", "time_step")),
+        boundary_interval_seconds=int(_domain1(nl, "time_control", "interval_seconds", run_seconds)),
+        history_interval_seconds=_history_interval(nl),
+        max_dom=int(_domain1(nl, "domains", "max_dom", 1)),
+        e_we=int(need("domains", "e_we")),
+        e_sn=int(need("domains", "e_sn")),
+        e_vert=int(need("domains", "e_vert")),
+        dx=float(need("domains", "dx")),
+        dy=float(need("domains", "dy")),
+        start_time=_time_stamp(nl, "start"),
+        end_time=_time_stamp(nl, "end"),
     )
 
 

@@ -18,9 +18,10 @@ The files are read with ``scipy.io.netcdf_file``, which reads the NetCDF
 classic and 64-bit-offset formats (what WRF writes with ``io_form 2`` and
 classic output).  A NetCDF-4/HDF5 file needs a reader this environment does
 not have and is rejected with ``NotImplementedError`` (converting it with
-``nccopy -k 64-bit-offset`` makes it readable).  As in the reference, a
-required variable that is missing raises ``KeyError`` and a wrfbdy record
-index out of range raises ``IndexError``.
+``nccopy -k 64-bit-offset`` makes it readable).  Errors use the reference's
+exception types: a required variable that is missing raises ``KeyError``, a
+wrfbdy record index out of range ``IndexError``, a non-finite initial state
+``FloatingPointError``.  ``load_case`` opens each file once.
 """
 
 from __future__ import annotations
@@ -146,22 +147,27 @@ class _Dataset:
         return int(n)
 
     def require(self, name: str):
-        if name not in self._nc.variables:
-            raise KeyError(f"required WRF variable {name!r} is missing from {self.path}")
-        return self._nc.variables[name]
+        var = self._nc.variables.get(name)
+        if var is None:
+            raise KeyError(f"{self.path.name} has no variable {name} (variables: {', '.join(sorted(self._nc.variables)[:8])}, ...)")
+        return var
 
+    def numeric(self):
+        """(name, variable) of every numeric variable, in file order."""
+        for name, var in self._nc.variables.items():
+            if np.dtype(var.data.dtype).kind in "fiu":
+                yield name, var
 
-def _decode_times(values: Any) -> tuple:
-    a = np.asarray(values)
-    if a.dtype.kind == "S" and a.ndim == 2:
-        return tuple(b"".join(row).decode("ascii") for row in a)
-    if a.dtype.kind == "S":
-        return tuple(v.decode("ascii") for v in a.reshape(-1).tolist())
-    return tuple(str(v) for v in a.reshape(-1).tolist())
-
-
-def _numeric(var) -> bool:
-    return np.dtype(var.data.dtype).kind in "fiu"
+    def times(self) -> tuple:
+        """The Times variable as text, one string per record (trailing NUL
+        and space padding removed); () without one."""
+        var = self._nc.variables.get("Times")
+        if var is None:
+            return ()
+        a = np.asarray(var.data)
+        if a.dtype.kind == "S" and a.ndim == 2:  # (Time, DateStrLen) single characters
+            a = np.ascontiguousarray(a).view(f"S{a.shape[1]}").reshape(a.shape[0])
+        return tuple((v.decode("ascii") if isinstance(v, bytes) else str(v)).rstrip("\x00 ") for v in a.reshape(-1).tolist())
 
 
 def _memory_order(values: np.ndarray, dims: tuple) -> np.ndarray:
@@ -207,25 +213,26 @@ def _grid(ds: _Dataset, config: CaseConfig | None) -> WrfGrid:
     )
 
 
+def _state(ds: _Dataset) -> WrfState:
+    u, v, t, q = (np.array(_first_record(ds.require(n))[0]) for n in STATE_VARIABLES)
+    return WrfState(u=u, v=v, t=t, qvapor=q, time_index=0, time_seconds=0.0)
+
+
+def _input_fields(ds: _Dataset) -> dict:
+    return {name.lower(): _memory_order(*_first_record(var)) for name, var in ds.numeric()}
+
+
 def load_wrfinput(path: str | Path, config: CaseConfig | None = None) -> tuple:
     """(grid, initial U/V/T/QVAPOR state) of ``wrfinput_d01``."""
     with _Dataset(path) as ds:
-        grid = _grid(ds, config)
-        vals = [np.array(_first_record(ds.require(n))[0]) for n in STATE_VARIABLES]
-    return grid, WrfState(u=vals[0], v=vals[1], t=vals[2], qvapor=vals[3], time_index=0, time_seconds=0.0)
+        return _grid(ds, config), _state(ds)
 
 
 def load_wrfinput_fields(path: str | Path) -> dict:
     """Every numeric variable of ``wrfinput`` (record 0) in memory order, by
     lower-case name."""
-    out = {}
     with _Dataset(path) as ds:
-        for name, var in ds.variables.items():
-            if name == "Times" or not _numeric(var):
-                continue
-            values, dims = _first_record(var)
-            out[name.lower()] = _memory_order(values, dims)
-    return out
+        return _input_fields(ds)
 
 
 def load_wrfinput_attrs(path: str | Path) -> dict:
@@ -237,36 +244,37 @@ def load_wrfinput_attrs(path: str | Path) -> dict:
 # ---------------------------------------------------------------------------
 # wrfbdy
 # ---------------------------------------------------------------------------
+def _forcing(ds: _Dataset) -> BoundaryForcing:
+    pairs = [(name, np.array(var.data)) for name, var in ds.numeric()]
+    return BoundaryForcing(times=ds.times(), names=tuple(n for n, _ in pairs), arrays=tuple(a for _, a in pairs))
+
+
+def _boundary_record(ds: _Dataset, record: int) -> dict:
+    """Record ``record`` of every numeric variable in boundary order; the
+    variables without a Time axis as they are."""
+    nrec = ds.size("Time") if "Time" in ds.dims else 0
+    if not 0 <= record < max(nrec, 1):
+        raise IndexError(f"{ds.path.name} holds {nrec} boundary record(s); record {record} was requested")
+    out = {}
+    for name, var in ds.numeric():
+        dims = tuple(var.dimensions)
+        a = np.asarray(var.data)
+        if dims[:1] == ("Time",):
+            a, dims = a[record], dims[1:]
+        out[name.lower()] = _boundary_order(a, dims)
+    return out
+
+
 def load_wrfbdy(path: str | Path) -> BoundaryForcing:
     """Every numeric wrfbdy variable with all its records, in file order."""
     with _Dataset(path) as ds:
-        times = _decode_times(ds.variables["Times"].data) if "Times" in ds.variables else ()
-        names, arrays = [], []
-        for name, var in ds.variables.items():
-            if name == "Times" or not _numeric(var):
-                continue
-            names.append(name)
-            arrays.append(np.array(var.data))
-    return BoundaryForcing(times=times, names=tuple(names), arrays=tuple(arrays))
+        return _forcing(ds)
 
 
 def load_wrfbdy_fields(path: str | Path, *, time_index: int = 0) -> dict:
     """One wrfbdy record in the reference's boundary order, by lower-case name."""
-    out = {}
     with _Dataset(path) as ds:
-        for name, var in ds.variables.items():
-            if name == "Times" or not _numeric(var):
-                continue
-            dims = tuple(var.dimensions)
-            values = np.asarray(var.data)
-            if dims and dims[0] == "Time":
-                count = int(values.shape[0])
-                idx = int(time_index)
-                if idx < 0 or idx >= count:
-                    raise IndexError(f"wrfbdy time_index {idx} out of range for {name}: 0..{count - 1}")
-                values, dims = values[idx], dims[1:]
-            out[name.lower()] = _boundary_order(values, dims)
-    return out
+        return _boundary_record(ds, int(time_index))
 
 
 def wrfbdy_record_count(path: str | Path) -> int:
@@ -278,25 +286,29 @@ def wrfbdy_record_count(path: str | Path) -> int:
 # run directory
 # ---------------------------------------------------------------------------
 def load_case(case_dir: str | Path) -> CaseData:
-    """``namelist.input``, ``wrfinput_d01`` and ``wrfbdy_d01`` (record 0) of a
-    WRF run directory."""
+    """``namelist.input``, ``wrfinput_d01`` and ``wrfbdy_d01`` (record 0 merged
+    into the fields) of a WRF run directory."""
     root = Path(case_dir)
-    namelist_path, input_path, boundary_path = root / "namelist.input", root / "wrfinput_d01", root / "wrfbdy_d01"
-    config = load_case_config(namelist_path)
-    grid, state = load_wrfinput(input_path, config)
-    fields = load_wrfinput_fields(input_path)
-    attrs = load_wrfinput_attrs(input_path)
-    fields.update(load_wrfbdy_fields(boundary_path))
-    forcing = load_wrfbdy(boundary_path)
+    paths = {role: root / fname for role, fname in
+             (("namelist", "namelist.input"), ("input", "wrfinput_d01"), ("boundary", "wrfbdy_d01"))}
+    config = load_case_config(paths["namelist"])
+    with _Dataset(paths["input"]) as ds:
+        grid, state, fields, attrs = _grid(ds, config), _state(ds), _input_fields(ds), ds.attrs()
+    with _Dataset(paths["boundary"]) as ds:
+        forcing = _forcing(ds)
+        fields.update(_boundary_record(ds, 0))
     return CaseData(config=config, grid=grid, state=state, forcing=forcing, fields=fields, attrs=attrs,
-                    input_path=input_path, boundary_path=boundary_path, namelist_path=namelist_path)
+                    input_path=paths["input"], boundary_path=paths["boundary"], namelist_path=paths["namelist"])
 
 
 def assert_state_finite(state: WrfState) -> None:
-    """Raise FloatingPointError if U, V, T or QVAPOR holds NaN or infinity."""
-    for name, values in (("U", state.u), ("V", state.v), ("T", state.t), ("QVAPOR", state.qvapor)):
-        if not bool(np.all(np.isfinite(np.asarray(values)))):
-            raise FloatingPointError(f"{name} contains NaN or infinity")
+    """FloatingPointError naming each of U, V, T, QVAPOR that holds NaN or
+    infinity, with the count of such values."""
+    counts = {name: int(np.size(a) - np.count_nonzero(np.isfinite(a)))
+              for name, a in zip(STATE_VARIABLES, (state.u, state.v, state.t, state.qvapor))}
+    bad = [f"{name}: {n}" for name, n in counts.items() if n]
+    if bad:
+        raise FloatingPointError("non-finite values in the initial state (" + ", ".join(bad) + ")")
 
 
 __all__ = [
