@@ -12,10 +12,9 @@
 // Positive-definite advection on the last stage runs as two passes over all
 // PD species: the renormalisation factor of every limited cell, then the
 // tendency from the limited faces (each face scaled by the cell its
-// antidiffusive flux leaves) followed by the update.  The first pass stores
-// the antidiffusive flux of the west, south and bottom face of every limited
-// cell (and of the east / north face of the last limited column / row); the
-// second reads them back and recomputes only the upwind fluxes.
+// antidiffusive flux leaves) followed by the update.  Both passes form the
+// face fluxes from the species themselves with the same arithmetic; only the
+// factor (one value per cell) goes through memory between them.
 #pragma once
 #include "wrfdc_common.h"
 #include "k_advect.h"
@@ -27,7 +26,6 @@ struct SpecList {
   float *old[kMaxSpec];       // stage-start species (q_old)
   float *tend[kMaxSpec];      // accumulated physics / boundary tendency
   float *scale[kMaxSpec];     // PD renormalisation factors (scratch)
-  float *anti[kMaxSpec];      // PD antidiffusive face fluxes: x, y, z faces, n3 apart (scratch)
   int n;
 };
 
@@ -248,7 +246,6 @@ __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecL
   const int k1 = imin_(k + 1, kde);
   const float fm0 = A1(FL(fnm), k), fp0 = A1(FL(fnp), k), fm1 = A1(FL(fnm), k1), fp1 = A1(FL(fnp), k1);
   const unsigned ox = XI3(i, k, j);
-  const size_t n3 = x->n3;
   for (int s = 0; s < L.n; s++) {
     const BufF bq = bufv(L.in[s], nb), bo = bufv(L.old[s], nb);
     // every load of this species first: x stencil of the west face, the
@@ -285,12 +282,6 @@ __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecL
     }
     pd_vflux_sel(fz0, zw[0], zw[1], zw[2], zw[3], zo[0], zo[1], fm0, fp0, az0, lz0);
     pd_vflux_sel(fz1, zw[1], zw[2], zw[3], zw[4], zo[1], zo[2], fm1, fp1, az1, lz1);
-    float *an = L.anti[s];
-    an[o] = ax0;
-    an[n3 + o] = ay0;
-    an[2 * n3 + o] = az0;
-    if (ic == ide - 2) an[o + 1] = ax1;
-    if (j == jde - 2) an[n3 + o + sj] = ay1;
     const float ph_low = mold * zo[1] - dt * (mm * (rdx * (lx1 - lx0) + rdy * (ly1 - ly0)) + msfty * rz * (lz1 - lz0));
     const float flux_out = dt * (mm * (rdx * (fmaxf(0.f, ax1) - fminf(0.f, ax0)) + rdy * (fmaxf(0.f, ay1) - fminf(0.f, ay0))) +
                                  msfty * rz * (fminf(0.f, az1) - fmaxf(0.f, az0)));
@@ -304,66 +295,101 @@ __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecL
 
 // pass 2: tendency from the renormalised faces, then the mass-weighted update
 // (old = q_old after k_species_pd_prep, which also cleared the physics
-// tendency over the same range: it is added as the +0 it holds).  Only the interior cells
-// i [2, ide-2], j [2, jde-2] (the cells pass 1 limits: the specified zone is
-// one cell wide) take an advective tendency; their antidiffusive fluxes come
-// from pass 1.
+// tendency over the same range: it is added as the +0 it holds).  Only the
+// interior cells i [2, ide-2], j [2, jde-2] (the cells pass 1 limits: the
+// specified zone is one cell wide) take an advective tendency.  The face
+// fluxes are formed again as in pass 1 (face-sharing blocks over the mass
+// points, the same loads and arithmetic, so the same values); each
+// antidiffusive flux is scaled by the factor of the cell it leaves.
 __global__ void __launch_bounds__(256) k_species_pd_update(const DevCtx xv, SpecList L, float dt, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  GET_IKJ(r);
-  const unsigned o = XI3(i, k, j);
-  const UpdCoef c = upd_coef(x, i, k, j);
-  if (!c.inner) {
+  extern __shared__ float yal_[];  // 2 x L.n x 4 x 64: south antidiffusive / upwind flux of every species
+  GET_FACE_BLOCK(r);
+#define YA(s, w) yal_[((s) * 4 + (w)) * 64 + lane]
+#define YL(s, w) yal_[((L.n + (s)) * 4 + (w)) * 64 + lane]
+  const int kde = x->kde;
+  const unsigned nb = (unsigned)x->n3, sj = (unsigned)(kde * x->ip), sk = (unsigned)x->ip;
+  {
+    const PdFace fs = pd_face_geom(x, 1, ic, k, j, dt);
+    const unsigned os = XI3(ic, k, j);
     for (int s = 0; s < L.n; s++) {
-      const float tend = 0.f + 0.f;
-      L.out[s][o] = (c.oldm * L.old[s][o] + dt * tend) / c.newm;
+      const BufF bo = bufv(L.old[s], nb);
+      float w[6], a, l;
+      win6b(bufv(L.in[s], nb), os, sj, w);
+      const float om1 = bld(bo, os - sj), o0 = bld(bo, os);
+      pin6(w);
+      pd_hflux_br(fs, w, om1, o0, a, l);
+      YA(s, wv) = a;
+      YL(s, wv) = l;
     }
-    return;
   }
-  const float rdx = x->rdx, rdy = x->rdy, rdzw = A1(FL(rdnw), k), msftx = A2(FL(msftx), i, j);
-  const bool up_ok = (k + 1 <= x->kde - 1), dn_ok = (k - 1 >= 1);
-  const PdFace fz0 = pd_face_geom(x, 2, i, k, j, dt), fz1 = pd_face_geom(x, 2, i, k + 1, j, dt);
-  const PdFace fx0 = pd_face_geom(x, 0, i, k, j, dt), fx1 = pd_face_geom(x, 0, i + 1, k, j, dt);
-  const PdFace fy0 = pd_face_geom(x, 1, i, k, j, dt), fy1 = pd_face_geom(x, 1, i, k, j + 1, dt);
-  const unsigned sj = (unsigned)(x->kde * x->ip), sk = (unsigned)x->ip;
-  const unsigned nb = (unsigned)x->n3;
-  const size_t n3 = x->n3;
+  __syncthreads();
+  if (!jrow) return;
+  const unsigned o = XI3(ic, k, j), ox = XI3(i, k, j);
+  const UpdCoef c = upd_coef(x, ic, k, j);
+  const float rdx = x->rdx, rdy = x->rdy, rdzw = A1(FL(rdnw), k), msftx = A2(FL(msftx), ic, j);
+  const bool up_ok = (k + 1 <= kde - 1), dn_ok = (k - 1 >= 1);
+  const PdFace fx = pd_face_geom(x, 0, i, k, j, dt);
+  const PdFace fyn = (wv == 3) ? pd_face_geom(x, 1, ic, k, j + 1, dt) : PdFace{0.f, 0.f, 0.f, 0};
+  const PdFace fz0 = pd_face_geom(x, 2, ic, k, j, dt), fz1 = pd_face_geom(x, 2, ic, k + 1, j, dt);
+  const int k1 = imin_(k + 1, kde);
+  const float fm0 = A1(FL(fnm), k), fp0 = A1(FL(fnp), k), fm1 = A1(FL(fnm), k1), fp1 = A1(FL(fnp), k1);
   for (int s = 0; s < L.n; s++) {
-    // x, y and z face fluxes are separate n3-element arrays
-    const BufF bo = bufv(L.old[s], nb), bs = bufv(L.scale[s], nb), bx = bufv(L.anti[s], nb), by = bufv(L.anti[s] + n3, nb),
-               bz = bufv(L.anti[s] + 2 * n3, nb);
-    // every load of this species first (the cell below level 1 and above
-    // level kde-1 is loaded too; its values are not used)
-    const float q0 = bld(bo, o), qw = bld(bo, o - 1u), qe = bld(bo, o + 1u), qs = bld(bo, o - sj), qn = bld(bo, o + sj);
-    const float qb = bld(bo, o - sk), qt = bld(bo, o + sk);
+    const BufF bq = bufv(L.in[s], nb), bo = bufv(L.old[s], nb), bs = bufv(L.scale[s], nb);
+    float xw[6], yw[6], zw[5], zo[3];
+    win6b(bq, ox, 1u, xw);
+    const float xom = bld(bo, ox - 1u), xo0 = bld(bo, ox);
+    win5z(bq, o, sk, zw);
+#pragma unroll
+    for (int n = 0; n < 3; n++) zo[n] = bld(bo, o + (unsigned)(n - 1) * sk);
+    float yon = 0.f;
+    if (wv == 3) {
+      win6b(bq, o + sj, sj, yw);
+      yon = bld(bo, o + sj);
+      pin6(yw);
+    }
     const float s0 = bld(bs, o), sw = bld(bs, o - 1u), se = bld(bs, o + 1u), ss = bld(bs, o - sj), sn = bld(bs, o + sj);
     const float sb = bld(bs, o - sk), st = bld(bs, o + sk);
-    const float ax0 = bld(bx, o), ax1 = bld(bx, o + 1u), ay0 = bld(by, o), ay1 = bld(by, o + sj);
-    const float az0r = bld(bz, o), az1r = bld(bz, o + sk);
-    // z faces (no face below level 2 or above level kde-1)
-    const float az0 = fz0.mode != 0 ? az0r : 0.f, az1 = fz1.mode != 0 ? az1r : 0.f;
-    const float lz0 = fz0.mode != 0 ? fz0.muf * flux_up(qb, q0, fz0.cr) : 0.f;
-    const float lz1 = fz1.mode != 0 ? fz1.muf * flux_up(q0, qt, fz1.cr) : 0.f;
+    pin6(xw);
+    pin5(zw);
+    float ax0, lx0, ay1, ly1, az0, lz0, az1, lz1;
+    pd_hflux_br(fx, xw, xom, xo0, ax0, lx0);
+    const float ax1 = lane_next(ax0, lane), lx1 = lane_next(lx0, lane);
+    if (!own) continue;
+    if (!c.inner) {
+      const float tend = 0.f + 0.f;
+      elem(L.out[s], o) = (c.oldm * zo[1] + dt * tend) / c.newm;
+      continue;
+    }
+    const float ay0 = YA(s, wv), ly0 = YL(s, wv);
+    if (wv < 3) {
+      ay1 = YA(s, wv + 1);
+      ly1 = YL(s, wv + 1);
+    } else {
+      pd_hflux_br(fyn, yw, zo[1], yon, ay1, ly1);
+    }
+    pd_vflux_sel(fz0, zw[0], zw[1], zw[2], zw[3], zo[0], zo[1], fm0, fp0, az0, lz0);
+    pd_vflux_sel(fz1, zw[1], zw[2], zw[3], zw[4], zo[1], zo[2], fm1, fp1, az1, lz1);
     const float zt = az1 < 0.f ? az1 * s0 : ((az1 > 0.f && up_ok) ? az1 * st : az1);
     const float zb = az0 > 0.f ? az0 * s0 : ((az0 < 0.f && dn_ok) ? az0 * sb : az0);
     float t = 0.f;
     t += -rdzw * (zt - zb + lz1 - lz0);
     {
-      const float lx0 = fx0.muf * flux_up(qw, q0, fx0.cr), lx1 = fx1.muf * flux_up(q0, qe, fx1.cr);
       const float xe = ax1 > 0.f ? ax1 * s0 : (ax1 < 0.f ? ax1 * se : ax1);
-      const float xw = ax0 < 0.f ? ax0 * s0 : (ax0 > 0.f ? ax0 * sw : ax0);
-      t += -msftx * (rdx * (xe - xw + lx1 - lx0));
+      const float xwf = ax0 < 0.f ? ax0 * s0 : (ax0 > 0.f ? ax0 * sw : ax0);
+      t += -msftx * (rdx * (xe - xwf + lx1 - lx0));
     }
     {
-      const float ly0 = fy0.muf * flux_up(qs, q0, fy0.cr), ly1 = fy1.muf * flux_up(q0, qn, fy1.cr);
       const float yn = ay1 > 0.f ? ay1 * s0 : (ay1 < 0.f ? ay1 * sn : ay1);
       const float ys = ay0 < 0.f ? ay0 * s0 : (ay0 > 0.f ? ay0 * ss : ay0);
       t += -msftx * (rdy * (yn - ys + ly1 - ly0));
     }
     float tend = t * c.msfty;
     tend = tend + 0.f;
-    elem(L.out[s], o) = (c.oldm * q0 + dt * tend) / c.newm;
+    elem(L.out[s], o) = (c.oldm * zo[1] + dt * tend) / c.newm;
   }
+#undef YA
+#undef YL
 }
 
 // flow_dep_bdy (flow_dep_bdy.py:27-176) of several species in one launch:

@@ -188,7 +188,6 @@ struct wrfdc_ctx {
   float *scratch = nullptr;
   float *spec_alt = nullptr;    // second buffer of every species (nmoist + nscalar 3-D arrays)
   float *spec_scale = nullptr;  // positive-definite renormalisation factors, one 3-D array per species
-  float *spec_anti = nullptr;   // positive-definite antidiffusive face fluxes, three 3-D arrays per species
   float *hds = nullptr;         // horizontal scalar diffusion face fluxes, two 3-D arrays per diffused scalar
   double *ck = nullptr;  // kCkBlocks * CK_N partials + CK_N results
   // learned eddy viscosity (wrfdc_set_closure): parameters, bf16 weight
@@ -880,7 +879,6 @@ struct Run {
       L.old[L.n] = Sp(id_old, q);
       L.tend[L.n] = Sp(id_tend, q);
       L.scale[L.n] = nullptr;
-      L.anti[L.n] = nullptr;
       L.n++;
     };
     const bool pdm = (rk == 3 && X->c.pd_moist), pds = (rk == 3 && X->c.pd_scalar);
@@ -889,7 +887,6 @@ struct Run {
     if (pd.n > 0) {
       for (int q = 0; q < pd.n; q++) {
         pd.scale[q] = X->spec_scale + (size_t)q * n3;
-        pd.anti[q] = X->spec_anti + (size_t)q * 3 * n3;
       }
       k_species_pd_prep<<<g3p(m), b3(), 0, s>>>(d, pd, dt, m);
       {
@@ -898,7 +895,7 @@ struct Run {
       }
       R3 r = full3();
       k_species_pd_scale<<<dim3(face_tiles(r).count()), b3(), sizeof(float) * 2 * 256 * pd.n, s>>>(d, pd, dt, r);
-      k_species_pd_update<<<g3(m), b3(), 0, s>>>(d, pd, dt, m);
+      k_species_pd_update<<<dim3(face_tiles(m).count()), b3(), sizeof(float) * 2 * 256 * pd.n, s>>>(d, pd, dt, m);
     }
     if (plain.n > 0) k_species_update<<<dim3(face_tiles(m).count()), b3(), sizeof(float) * 256 * plain.n, s>>>(d, plain, rk, dt, m);
     // outputs become the current species
@@ -1247,8 +1244,6 @@ int wrfdc_create(const wrfdc_config *cfg, wrfdc_ctx **out) {
     hchk(hipMalloc((void **)&x->dtbc_dev, sizeof(float) * 64), "hipMalloc dtbc");
     hchk(hipMalloc((void **)&x->spec_alt, sizeof(float) * x->n3 * (size_t)(cfg->nmoist + cfg->nscalar)), "hipMalloc species");
     hchk(hipMalloc((void **)&x->spec_scale, sizeof(float) * x->n3 * (size_t)(cfg->nmoist + cfg->nscalar)), "hipMalloc species scale");
-    if (cfg->pd_moist || cfg->pd_scalar)
-      hchk(hipMalloc((void **)&x->spec_anti, sizeof(float) * x->n3 * 3 * (size_t)(cfg->nmoist + cfg->nscalar)), "hipMalloc species fluxes");
     hchk(hipMalloc((void **)&x->hds, sizeof(float) * x->n3 * 2 * (size_t)(1 + cfg->nmoist + cfg->nscalar)), "hipMalloc diffusion fluxes");
   } catch (const HipError &e) {
     set_err(nullptr, WRFDC_EHIP, std::string(e.what) + ": " + hipGetErrorString(e.e));
@@ -1257,7 +1252,6 @@ int wrfdc_create(const wrfdc_config *cfg, wrfdc_ctx **out) {
     if (x->dtbc_dev) (void)hipFree(x->dtbc_dev);
     if (x->spec_alt) (void)hipFree(x->spec_alt);
     if (x->spec_scale) (void)hipFree(x->spec_scale);
-    if (x->spec_anti) (void)hipFree(x->spec_anti);
     if (x->hds) (void)hipFree(x->hds);
     delete x;
     return WRFDC_EHIP;
@@ -1283,7 +1277,6 @@ void wrfdc_destroy(wrfdc_ctx *x) {
   if (x->dtbc_dev) (void)hipFree(x->dtbc_dev);
   if (x->spec_alt) (void)hipFree(x->spec_alt);
   if (x->spec_scale) (void)hipFree(x->spec_scale);
-  if (x->spec_anti) (void)hipFree(x->spec_anti);
   if (x->hds) (void)hipFree(x->hds);
   delete x;
 }
