@@ -7,6 +7,14 @@ u = 10 m/s + noise).  The state is resident in HBM before the timed region;
 a step is one full RK3 step (3 stages, all acoustic substeps, diffusion,
 species, lateral boundaries).
 
+Forms (``--forms``, all three by default; the headline ``value`` is plain):
+``forms.batched`` runs the 32-member ensemble of uneven domains (100^2..300^2
+columns, 45..60 levels, dx 1..9 km) with one RK3 step of every member
+captured in one HIP graph over the autotuned streams; ``forms.learned`` runs
+1000x1000x60 at dx 3 km with the bf16 MLP eddy viscosity (trained first on a
+coarse-grained 1 km run of this dycore) and the same domain with
+Smagorinsky.  Each reports Mpoint-steps/s and its ratio to plain.
+
 Prints one JSON line.  ``roofline`` covers the dominant kernel (algorithmic
 bytes per launch / mean launch time from HIP events on the dycore's stream);
 ``step_roofline`` the whole step against the minimal fused traffic model.
@@ -36,6 +44,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c1", choices=["c1", "c0"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--forms", default="plain,batched,learned",
+                    help="comma list of plain (the headline value), batched (32-member ensemble, one captured "
+                         "graph per step) and learned (1000x1000x60 with the trained closure)")
+    ap.add_argument("--form-steps", type=int, default=5, help="timed steps of the batched and learned forms")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--traffic-calibration", action="store_true",
                     help="after the run, one 256 MiB torch copy (known HBM traffic) for scaling "
@@ -93,6 +105,119 @@ def cpu_baseline(workload: str, nsteps: int = 4) -> dict:
             "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
             "sample": f"{nsteps} RK3 steps of the FP32 oracle (OpenMP, {threads} threads) on {px}x{py} tiles of "
                       f"{nx // px}x{ny // py}x{nz} covering the {nx}x{ny}x{nz} workload, {sec:.1f} s"}
+
+
+def _log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def batched_form(steps: int, plain_value: float) -> dict:
+    """32 seeded members of uneven domains (100^2..300^2 columns, 45..60
+    levels, dx 1..9 km, 6 moist species), one RK3 step of all of them
+    captured in one HIP graph over concurrent streams, the stream layout
+    chosen by the autotuner; value = sum of N_mass x steps / s."""
+    import torch
+
+    from wrfdc.device import Domain
+    from wrfdc.ensemble import Ensemble, autotune, c2_members, eager_step_time, member_case, n_mass
+
+    t0 = time.perf_counter()
+    specs = c2_members()
+    doms = []
+    for m, spec in enumerate(specs):
+        case = member_case(spec, m)
+        d = Domain(case.config, "cuda:0")
+        d.upload(case.arrays)
+        doms.append(d)
+        del case
+    nm = n_mass(doms)
+    _log(f"batched: {len(doms)} members, {nm / 1e6:.1f} M mass points, built in {time.perf_counter() - t0:.1f} s")
+    eager = eager_step_time(doms, reps=2)
+    nb, assign, table = autotune(doms, candidates=(1, 2, 4, 8), reps=2, log=lambda m: _log("batched autotune: " + m))
+    ens = Ensemble(doms, assign, nb)
+    ens.step(1, 0.0)
+    ens.synchronize()
+    t1 = time.perf_counter()
+    ens.step(steps, float(specs[0].dt))
+    ens.synchronize()
+    sec = time.perf_counter() - t1
+    bad = sum(int(d.checksums()[3]) for d in doms)  # raises on a non-finite member
+    ens.close()
+    for d in doms:
+        d.close()
+    torch.cuda.empty_cache()
+    value = nm * steps / sec / 1e6
+    _log(f"batched: {1e3 * sec / steps:.1f} ms per ensemble step, eager {1e3 * eager:.1f} ms")
+    return {"value": value, "unit": "Mpoint-steps/s", "ms_per_step": 1e3 * sec / steps, "steps": steps, "n_mass": nm,
+            "members": len(doms), "streams": nb, "eager_ms_per_step": 1e3 * eager, "graph_vs_eager": eager / (sec / steps),
+            "vs_plain": value / plain_value, "nonfinite": bad,
+            "autotune_ms": {str(b): 1e3 * t for b, t in table},
+            "workload": "32 seeded members, 100^2..300^2 x 45..60, dx 1..9 km, 6 moist species, PD on, FP32, "
+                        "one captured RK3 step per replay"}
+
+
+def learned_form(steps: int, plain_value: float, kernel_reps: int) -> dict:
+    """The learned closure at 1000x1000x60, dx 3 km (1 moist species): a
+    network trained on the coarse-grained sub-grid stresses of a 1 km run of
+    this dycore replaces smag_km; the same domain is also stepped with
+    Smagorinsky.  value = N_mass x steps / s with the network."""
+    import torch
+
+    from wrfdc.cases import make_case
+    from wrfdc.closure import FLOP_PER_POINT, Network, row_count, train_on_truth
+    from wrfdc.device import Domain
+
+    t0 = time.perf_counter()
+    net, hist = train_on_truth("cuda:0", epochs=30)
+    _log(f"learned: network trained in {time.perf_counter() - t0:.1f} s (loss {hist[0]:.4f} -> {hist[-1]:.4f})")
+    case = make_case(1000, 1000, 60, name="learned", config_index=4, dx=3000.0, dt=18.0, nmoist=1, nscalar=0,
+                     hydrometeors=False)
+    cfg = case.config
+    dom = Domain(cfg, "cuda:0")
+    dom.upload(case.arrays)
+    del case
+    nm = cfg.nx * cfg.ny * cfg.nz
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+
+    def timed(n: int, dtbc0: float) -> float:
+        torch.cuda.synchronize()
+        ev0.record(dom.stream)
+        dom.step(n, dtbc0)
+        ev1.record(dom.stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / n
+
+    dom.step(1, 0.0)
+    smag_ms = timed(steps, cfg.dt)
+    net.attach(dom)
+    dom.step(1, (steps + 1) * cfg.dt)
+    ms = timed(steps, (steps + 2) * cfg.dt)
+    ck = dom.checksums()
+    # the network's km pass (smag prep + MFMA kernel), timed live
+    dom.stage("closure_km", 1, 0)
+    torch.cuda.synchronize()
+    ev0.record(dom.stream)
+    for _ in range(kernel_reps):
+        dom.stage("closure_km", 1, 0)
+    ev1.record(dom.stream)
+    torch.cuda.synchronize()
+    k_ms = ev0.elapsed_time(ev1) / kernel_reps
+    Network.detach(dom)
+    dom.close()
+    torch.cuda.empty_cache()
+    flops = FLOP_PER_POINT * row_count(cfg)
+    tflops = flops / (k_ms * 1e-3) / 1e12
+    value = nm / (ms * 1e-3) / 1e6
+    _log(f"learned: {ms:.2f} ms per step with the network, {smag_ms:.2f} ms with Smagorinsky; km pass {k_ms:.3f} ms")
+    return {"value": value, "unit": "Mpoint-steps/s", "ms_per_step": ms, "steps": steps, "n_mass": nm,
+            "smagorinsky_ms_per_step": smag_ms, "vs_smagorinsky": smag_ms / ms, "vs_plain": value / plain_value,
+            "training_loss": [hist[0], hist[-1]], "checksums": ck.tolist(),
+            "roofline": {"bound": "mfma", "kernel": "k_closure_km (+ its k_smag prep)", "achieved": tflops,
+                         "peak": 2500.0, "unit": "TFLOP/s", "frac": tflops / 2500.0, "flop_per_launch": flops,
+                         "mean_launch_ms": k_ms, "dtype": "bf16 MFMA, FP32 accumulate"},
+            "workload": "1000x1000x60, dx 3 km, dt 18 s, 1 moist species, PD on, FP32 dycore, bf16 MLP km "
+                        "(trained on a 1 km run coarse-grained 3x)"}
 
 
 def main():
@@ -199,6 +324,12 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    out["forms"] = {}
+    if "batched" in forms:
+        out["forms"]["batched"] = batched_form(args.form_steps, out["value"])
+    if "learned" in forms:
+        out["forms"]["learned"] = learned_form(args.form_steps, out["value"], args.kernel_reps)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.workload)

@@ -89,7 +89,7 @@ int wrfdc_step(wrfdc_ctx *ctx, int nsteps, double dtbc0);
  * set_physical_bc2d:<field>[:small], spec_bdy_final:u|v|t|ph|mu|moist|moist_qv,
  * set_w_surface), one of its parts (advect_u/v/w, advect_scalar[_pd],
  * rhs_ph, horizontal_pressure_gradient, pg_buoy_w, compute_diff_metrics,
- * cal_deform_and_div, smag2d_km, calculate_n2, calculate_km_kh,
+ * cal_deform_and_div, smag2d_km, calculate_n2, calculate_km_kh, closure_km,
  * horizontal_diffusion_2, dry_boundary_tendency, acoustic_substep) or a
  * whole rk_stage / step.  rk_step 1..3 sets dt_rk, dts_rk and the number of
  * acoustic steps as _rk_step_sizes does; iteration is the acoustic step

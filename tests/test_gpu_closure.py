@@ -82,18 +82,9 @@ def test_mlp_gradient_matches_fp64(gpu):
 # data from a fine run
 # ---------------------------------------------------------------------------
 def _pair(gpu, nxc=24, nz=10, ratio=3, fine_steps=2):
-    from wrfdc.cases import make_case
-    from wrfdc.device import Domain
+    from wrfdc.closure import truth_pair
 
-    fine_case = make_case(nxc * ratio, nxc * ratio, nz, dx=3000.0 / ratio, dt=18.0 / ratio, nmoist=2, nscalar=0, hill_height=300.0)
-    coarse_case = make_case(nxc, nxc, nz, dx=3000.0, dt=18.0, nmoist=2, nscalar=0, hill_height=300.0)
-    fine = Domain(fine_case.config, gpu)
-    fine.upload(fine_case.arrays)
-    if fine_steps:
-        fine.step(fine_steps, 0.0)
-    coarse = Domain(coarse_case.config, gpu)
-    coarse.upload(coarse_case.arrays)
-    return fine, coarse
+    return truth_pair(gpu, nxc=nxc, nz=nz, ratio=ratio, fine_steps=fine_steps)
 
 
 def test_coarse_grain_block_means(gpu):
@@ -171,8 +162,9 @@ def test_subgrid_stress_and_rows(gpu):
     def2 = np.float32(0.25) * dd * dd + s2 * s2
     feat = (np.log(np.sqrt(def2).astype(np.float64) + 1e-7) - norm.d_shift) / norm.d_scale
     np.testing.assert_allclose(X[:, 13], feat.reshape(-1), rtol=2 ** -7, atol=1e-3)
-    assert np.all(X[:, 54] == np.float32(bf16_round(np.array([1.1]))[0]))
-    assert np.all(X[:, 55:] == 0.0)
+    # slots: 32*h + s (stencil point s < 27 of log|D| / N^2), 27 the spacing
+    assert np.all(X[:, 27] == np.float32(bf16_round(np.array([1.1]))[0]))
+    assert np.all(X[:, 28:32] == 0.0) and np.all(X[:, 59:] == 0.0)
     fine.close()
     coarse.close()
 
@@ -219,8 +211,8 @@ def test_closure_km_in_the_step(gpu):
             di, dk, dj = g % 3 - 1, (g // 3) % 3 - 1, g // 9 - 1
             ii, kk, jj = min(max(i + di, 2), ide - 2), min(max(k + dk, 1), kde - 1), min(max(j + dj, 2), jde - 2)
             X[r, g] = (np.log(np.float64(dmag(ii, kk, jj)) + 1e-7) - norm.d_shift) / norm.d_scale
-            X[r, 27 + g] = (np.float64(bn2[jj - 1, kk - 1, ii - 1]) - norm.n_shift) / norm.n_scale
-        X[r, 54] = norm.dx_feature
+            X[r, 32 + g] = (np.float64(bn2[jj - 1, kk - 1, ii - 1]) - norm.n_shift) / norm.n_scale
+        X[r, 27] = norm.dx_feature
     z, _ = forward(theta.astype(np.float32).astype(np.float64), X)
     km = np.zeros(len(pts))
     mlen = np.sqrt(cfg.dx * cfg.dy)

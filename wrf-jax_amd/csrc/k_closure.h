@@ -8,7 +8,12 @@
 //
 // Network (parameters theta, FP32 master copy, offsets below):
 //   x   = 55 features (+ 9 zero pads): the 3x3x3 stencils of log|D| and N^2
-//         around the point, normalised, and log(dx / 1 km)
+//         around the point, normalised, and log(dx / 1 km).  Input slot
+//         h*32 + s: stencil point s < 27 (di = s%3 - 1, dk = (s/3)%3 - 1,
+//         dj = s/9 - 1) of log|D| (h = 0) or N^2 (h = 1); slot 27 the spacing
+//         feature; the other slots zero.  Both halves share one stencil
+//         order, so a lane of the MFMA layout reads the same 8 stencil
+//         points for either half.
 //   h1  = silu(W1 x + b1)        bf16 inputs and weights, FP32 accumulate;
 //                                h1 rounded to bf16 for the next layer
 //   h2  = silu(W2 h1 + b2)       the same; h2 kept in FP32
@@ -44,8 +49,10 @@ struct ClosureNorm {
   float d_shift, d_scale, n_shift, n_scale, dx_feature, kref;
 };
 
-__device__ __forceinline__ float silu_f(float v) { return v / (1.f + expf(-v)); }
-__device__ __forceinline__ float sigmoid_f(float v) { return 1.f / (1.f + expf(-v)); }
+// SiLU and the logistic function through the hardware exp2 and reciprocal
+// (a few ulp; the network is trained and evaluated with the same forms)
+__device__ __forceinline__ float sigmoid_f(float v) { return __builtin_amdgcn_rcpf(1.f + __expf(-v)); }
+__device__ __forceinline__ float silu_f(float v) { return v * sigmoid_f(v); }
 __device__ __forceinline__ float softplus_f(float v) { return v > 20.f ? v : log1pf(expf(v)); }
 
 __device__ __forceinline__ f32x4 mma16(const __bf16 *A, int lda, const __bf16 *Bt, int ldb, f32x4 acc) {
@@ -63,17 +70,27 @@ __device__ __forceinline__ f32x4 mma16(const __bf16 *A, int lda, const __bf16 *B
 // neighbours outside it take the nearest point of it.
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// feature f of the point (i, k, j): f = 9*(dj+1) + 3*(dk+1) + (di+1) for
-// log|D| (f < 27), 27 + the same for N^2, 54 the spacing feature, 55..63 zero
-__device__ __forceinline__ float closure_feature(const DevCtx *__restrict__ x, const float *defmag, const ClosureNorm &nm, int i,
-                                                 int k, int j, int f) {
-  if (f >= kMlpFeat) return 0.f;
-  if (f == kMlpFeat - 1) return nm.dx_feature;
-  const int g = f < 27 ? f : f - 27;
+// normalised inputs of every point of the range: fd = (log(|D| + 1e-7) -
+// d_shift) / d_scale and fn = (N^2 - n_shift) / n_scale (|D| as k_smag
+// stored it)
+__global__ void k_closure_feat(const DevCtx xv, R3 r, ClosureNorm nm, const float *__restrict__ defmag, float *__restrict__ fd,
+                               float *__restrict__ fn) {
+  const DevCtx *__restrict__ x = &xv;
+  GET_IKJ_P(r);
+  A3(fd, i, k, j) = (logf(A3(defmag, i, k, j) + 1e-7f) - nm.d_shift) / nm.d_scale;
+  A3(fn, i, k, j) = (A3(FL(bn2), i, k, j) - nm.n_shift) / nm.n_scale;
+}
+
+// input slot f of the point (i, k, j) (the slot order above); stencil
+// neighbours outside the range take the nearest point of it
+__device__ __forceinline__ float closure_feature(const DevCtx *__restrict__ x, const float *fd, const float *fn, const ClosureNorm &nm,
+                                                 int i, int k, int j, int f) {
+  const int h = f >> 5, g = f & 31;
+  if (g == 27) return h == 0 ? nm.dx_feature : 0.f;
+  if (g > 27) return 0.f;
   const int di = g % 3 - 1, dk = (g / 3) % 3 - 1, dj = g / 9 - 1;
   const int ii = clampi(i + di, 2, x->ide - 2), kk = clampi(k + dk, 1, x->kde - 1), jj = clampi(j + dj, 2, x->jde - 2);
-  if (f < 27) return (logf(A3(defmag, ii, kk, jj) + 1e-7f) - nm.d_shift) / nm.d_scale;
-  return (A3(FL(bn2), ii, kk, jj) - nm.n_shift) / nm.n_scale;
+  return A3(h == 0 ? fd : fn, ii, kk, jj);
 }
 
 // ---------------------------------------------------------------------------
@@ -139,32 +156,128 @@ __device__ __forceinline__ void mlp_rows16(const __bf16 *Xs, const __bf16 *w1, c
 
 // ---------------------------------------------------------------------------
 // smag2d_km with the learned km: one wave per 64 points (64 i of one level
-// and row), its 4 tiles of 16 points run through the network.  The
-// Smagorinsky kernel has stored |D| (defmag) and the slope divisor before.
+// and row), as 4 tiles of 16 points.  The MFMA operands come straight from
+// registers: lane l feeds A row l%16 (point 16t + l%16 of tile t) with the
+// 8 k of block l/16 in each half of the 64 inputs, i.e. stencil points
+// s = 8*(l/16) .. +7 of log|D| (first MFMA) and of N^2 (second), so each
+// lane gathers exactly its 16 operands; the stencil offsets of its 8 points
+// are formed once per wave (k and j are the wave's own).  W1 and W2 (B
+// operands) are staged in LDS once per block; the first hidden layer's
+// output goes through the wave's own LDS tile (C layout to A layout), so
+// the waves synchronise only within themselves after the staging.  k_smag has stored the slope divisor and k_closure_feat the
+// normalised inputs (fd, fn: two arrays n3 apart) before.
 // Block (64, 4, 1), grid as for GET_IKJ_P over i [2, ide-2], k [1, kde-1],
 // j [2, jde-2].
 // ---------------------------------------------------------------------------
+// LDS written by some lanes of a wave, then read by others of the same
+// wave: the wave's LDS operations complete and are ordered before the reads
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ bf16x8 pack8(const float v[8]) {
+  bf16x8 r;
+#pragma unroll
+  for (int q = 0; q < 8; q++) r[q] = (__bf16)v[q];
+  return r;
+}
 __global__ void __launch_bounds__(256) k_closure_km(const DevCtx xv, R3 r, const float *__restrict__ theta, const __bf16 *__restrict__ wbf,
-                                                    ClosureNorm nm, const float *__restrict__ defmag, const float *__restrict__ divisor) {
+                                                    ClosureNorm nm, const float *__restrict__ fd, const float *__restrict__ divisor) {
   const DevCtx *__restrict__ x = &xv;
-  __shared__ __align__(16) __bf16 s_x[4][64 * kLd];
+  __shared__ __align__(16) __bf16 s_w[2][64 * kLd];
   __shared__ __align__(16) __bf16 s_h[4][16 * kLd];
   __shared__ float s_z[4][64];
   const int wv = threadIdx.y, l = threadIdx.x;
+  const int col = l & 15, grp = l >> 4, rb = grp * 4;
+  for (int q = wv * 64 + l; q < 2 * 64 * 8; q += 256) {  // 16-byte pieces: matrix, row, piece
+    const int m = q >> 9, row = (q >> 3) & 63, c = q & 7;
+    *reinterpret_cast<bf16x8 *>(&s_w[m][row * kLd + c * 8]) =
+        *reinterpret_cast<const bf16x8 *>(wbf + (m == 0 ? kWbfW1 : kWbfW2) + row * 64 + c * 8);
+  }
   const unsigned gx_ = (unsigned)(r.i1 - r.i0 + 64) / 64u;
   const unsigned gy_ = (unsigned)(r.k1 - r.k0 + 4) / 4u;
-  const int i = r.i0 + (int)((blockIdx.x % gx_) * 64u + threadIdx.x);
+  const int i0 = r.i0 + (int)((blockIdx.x % gx_) * 64u);
+  const int i = i0 + l;
   const int k = r.k0 + (int)(((blockIdx.x / gx_) % gy_) * 4u + threadIdx.y);
   const int j = r.j0 + (int)(blockIdx.x / (gx_ * gy_));
   // the whole wave takes part in the MFMA; points outside r compute on a
   // clamped copy and store nothing
   const bool own = i <= r.i1 && k <= r.k1 && j <= r.j1;
-  const int ic = own ? i : r.i0, kc = k <= r.k1 ? k : r.k0, jc = j <= r.j1 ? j : r.j0;
-  __bf16 *xs = s_x[wv];
-  for (int f = 0; f < kMlpIn; f++) xs[l * kLd + f] = (__bf16)closure_feature(x, defmag, nm, ic, kc, jc, f);
-  __syncthreads();
-  for (int t = 0; t < 4; t++)
-    mlp_rows16(xs + t * 16 * kLd, wbf + kWbfW1, wbf + kWbfW2, 64, theta, s_h[wv], nullptr, nullptr, &s_z[wv][t * 16]);
+  const int kc = k <= r.k1 ? k : r.k0, jc = j <= r.j1 ? j : r.j0;
+  // per-column biases of the C layout
+  float b1[4], b2[4], w3[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    b1[t] = theta[kOffB1 + t * 16 + col];
+    b2[t] = theta[kOffB2 + t * 16 + col];
+    w3[t] = theta[kOffW3 + t * 16 + col];
+  }
+  // this lane's 8 stencil points s = 8*grp + q: level / row part of the
+  // offset (i added per tile), the i shift, and whether the slot is a
+  // stencil point (s < 27), the spacing slot (27) or zero
+  const unsigned nb = (unsigned)(2 * x->n3);
+  const BufF bf = bufv(fd, nb);
+  int koff[8], dis[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int sp = 8 * grp + q, g = sp < 27 ? sp : 13;
+    const int di = g % 3 - 1, dk = (g / 3) % 3 - 1, dj = g / 9 - 1;
+    const int kk = clampi(kc + dk, 1, x->kde - 1), jj = clampi(jc + dj, 2, x->jde - 2);
+    koff[q] = (int)XI3(1, kk, jj) - 1;  // element offset of (i, kk, jj) is koff + i
+    dis[q] = di;
+  }
+  const unsigned n3 = (unsigned)x->n3;
+  __syncthreads();  // the staged weights
+  // B operand of output tile t2, input half h: W row t2*16 + l%16, k block l/16
+#define WB(m, t2, h) (*reinterpret_cast<const bf16x8 *>(&s_w[m][((t2) * 16 + col) * kLd + (h) * 32 + grp * 8]))
+  for (int t = 0; t < 4; t++) {
+    const int ip = imin_(i0 + 16 * t + col, r.i1);
+    float vd[8], vn[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int sp = 8 * grp + q;
+      const unsigned o = (unsigned)(koff[q] + clampi(ip + dis[q], 2, x->ide - 2));
+      const float a = bld(bf, o), b = bld(bf, o + n3);
+      vd[q] = sp < 27 ? a : (sp == 27 ? nm.dx_feature : 0.f);
+      vn[q] = sp < 27 ? b : 0.f;
+    }
+    const bf16x8 a0 = pack8(vd), a1 = pack8(vn);
+    // layer 1 -> LDS (bf16, rows = points)
+#pragma unroll
+    for (int t2 = 0; t2 < 4; t2++) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, WB(0, t2, 0), acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, WB(0, t2, 1), acc, 0, 0, 0);
+#pragma unroll
+      for (int q = 0; q < 4; q++) s_h[wv][(rb + q) * kLd + t2 * 16 + col] = (__bf16)silu_f(acc[q] + b1[t2]);
+    }
+    wave_lds_sync();
+    const bf16x8 h0 = *reinterpret_cast<const bf16x8 *>(&s_h[wv][col * kLd + grp * 8]);
+    const bf16x8 h1 = *reinterpret_cast<const bf16x8 *>(&s_h[wv][col * kLd + 32 + grp * 8]);
+    // layer 2 and the output layer
+    float zp[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t2 = 0; t2 < 4; t2++) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h0, WB(1, t2, 0), acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h1, WB(1, t2, 1), acc, 0, 0, 0);
+#pragma unroll
+      for (int q = 0; q < 4; q++) zp[q] += silu_f(acc[q] + b2[t2]) * w3[t2];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      float v = zp[q];
+      v += __shfl_xor(v, 1);
+      v += __shfl_xor(v, 2);
+      v += __shfl_xor(v, 4);
+      v += __shfl_xor(v, 8);
+      if (col == 0) s_z[wv][16 * t + rb + q] = v + theta[kOffB3];
+    }
+    // the next tile overwrites s_h (read above); s_z is read after the loop
+    wave_lds_sync();
+  }
+#undef WB
   if (!own) return;
   const float smag = A3(FL(xkmh), i, k, j);
   const float mlen = sqrtf(x->dx / A2(FL(msftx), i, j) * x->dy / A2(FL(msfty), i, j));
@@ -182,13 +295,13 @@ __global__ void __launch_bounds__(256) k_closure_km(const DevCtx xv, R3 r, const
 // smag range, row = ((j-2)*nk + (k-1))*ni + (i-2).  t1, t2 come from
 // k_subgrid_stress.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_closure_rows(const DevCtx xv, R3 r, ClosureNorm nm, const float *__restrict__ defmag,
-                                                      const float *__restrict__ divisor, __bf16 *X, float *T) {
+__global__ void __launch_bounds__(256) k_closure_rows(const DevCtx xv, R3 r, ClosureNorm nm, const float *__restrict__ fd,
+                                                      const float *__restrict__ fn, const float *__restrict__ divisor, __bf16 *X, float *T) {
   const DevCtx *__restrict__ x = &xv;
   GET_IKJ_P(r);
   const int ni = r.i1 - r.i0 + 1, nk = r.k1 - r.k0 + 1;
   const size_t row = ((size_t)(j - r.j0) * nk + (size_t)(k - r.k0)) * ni + (size_t)(i - r.i0);
-  for (int f = 0; f < kMlpIn; f++) X[row * kMlpIn + f] = (__bf16)closure_feature(x, defmag, nm, i, k, j, f);
+  for (int f = 0; f < kMlpIn; f++) X[row * kMlpIn + f] = (__bf16)closure_feature(x, fd, fn, nm, i, k, j, f);
   const float d11 = A3(FL(defor11), i, k, j), d22 = A3(FL(defor22), i, k, j);
   const float *d12 = FL(defor12);
   const float s2 = 0.25f * (A3(d12, i, k, j) + A3(d12, i, k, j + 1) + A3(d12, i + 1, k, j) + A3(d12, i + 1, k, j + 1));

@@ -467,13 +467,17 @@ struct Run {
   // Smagorinsky km storing |D| and the slope divisor in the scratch arrays
   // S_T1 and S_T2 (free until the diffusion's stresses), then the learned km
   // over the same points when the context has a network
-  void smag_for_closure() const {
+  // (and the network's normalised inputs in S_FQX, S_FQY, free outside the
+  // per-field positive-definite advection stage)
+  void smag_for_closure(const ClosureNorm &nm) const {
     R3 r{2, ide - 2, 1, kde - 1, 2, jde - 2};
     k_smag<<<g3(r), b3(), 0, s>>>(d, r, Scr(S_T1), Scr(S_T2));
+    k_closure_feat<<<g3p(r), b3(), 0, s>>>(d, r, nm, Scr(S_T1), Scr(S_FQX), Scr(S_FQY));
   }
   void closure_km() const {
     R3 r{2, ide - 2, 1, kde - 1, 2, jde - 2};
-    k_closure_km<<<g3p(r), b3(), 0, s>>>(d, r, X->cl_theta, X->cl_wbf, X->cl_norm, Scr(S_T1), Scr(S_T2));
+    static_assert(S_FQY == S_FQX + 1, "the network's two input arrays are adjacent scratch slots");
+    k_closure_km<<<g3p(r), b3(), 0, s>>>(d, r, X->cl_theta, X->cl_wbf, X->cl_norm, Scr(S_FQX), Scr(S_T2));
   }
   // training rows of the current state (features, s1, s2, divisor, km)
   void closure_rows(const ClosureNorm &nm, __bf16 *Xr, float *T) const {
@@ -481,9 +485,9 @@ struct Run {
     BcBatch(this).add3(F(F_rdzw), 'w').add3(F(F_rdz), 'w').add3(F(F_z), 'w').add3(F(F_zx), 'e').add3(F(F_zy), 'f').flush();
     cal_deform_and_div();
     calculate_n2();
-    smag_for_closure();
+    smag_for_closure(nm);
     R3 r{2, ide - 2, 1, kde - 1, 2, jde - 2};
-    k_closure_rows<<<g3p(r), b3(), 0, s>>>(d, r, nm, Scr(S_T1), Scr(S_T2), Xr, T);
+    k_closure_rows<<<g3p(r), b3(), 0, s>>>(d, r, nm, Scr(S_FQX), Scr(S_FQY), Scr(S_T2), Xr, T);
   }
   // two stress components (t1, t2 of one diffusion kernel) in one launch;
   // each entry: xk, defor, kind, i0, i1, j0, j1
@@ -544,7 +548,7 @@ struct Run {
     cal_deform_and_div();
     if (with_n2 || X->cl_theta) calculate_n2();
     if (X->cl_theta) {
-      smag_for_closure();
+      smag_for_closure(X->cl_norm);
       closure_km();
     } else {
       smag2d_km();
@@ -1476,6 +1480,12 @@ int wrfdc_stage(wrfdc_ctx *x, const char *name, int rk_step, int iteration) {
     else if (n == "calculate_km_kh") {
       run.calculate_n2();
       run.smag2d_km();
+    }
+    else if (n == "closure_km") {
+      // the network's km as the first stage computes it (needs wrfdc_set_closure)
+      if (!x->cl_theta) return set_err(x, WRFDC_EINVAL, "closure_km: no network set (wrfdc_set_closure)");
+      run.smag_for_closure(x->cl_norm);
+      run.closure_km();
     }
     else if (n == "horizontal_diffusion_2") run.horizontal_diffusion_2();
     else if (n == "rk_tendency") run.rk_tendency();

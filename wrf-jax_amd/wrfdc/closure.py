@@ -36,6 +36,9 @@ NPARAMS = 8385
 WBF_SIZE = 12288
 NFEAT = 64
 NTARGET = 6
+# input slots: h*32 + s, stencil point s < 27 (di = s%3-1, dk = (s/3)%3-1,
+# dj = s/9-1) of log|D| (h = 0) or N^2 (h = 1); slot 27 log(dx / 1 km)
+FEAT_CENTRE_D, FEAT_CENTRE_N, FEAT_DX = 13, 32 + 13, 27
 OFF = dict(W1=0, b1=4096, W2=4160, b2=8256, w3=8320, b3=8384)
 
 
@@ -119,7 +122,7 @@ def fit_norm(coarse, c_s: float = 0.25) -> ClosureNorm:
     cfg = coarse.config
     X, _ = training_rows(coarse, ClosureNorm(dx_feature=0.0, kref=1.0))
     x = X.float().cpu().numpy()
-    ld, n2 = x[:, 13].astype(np.float64), x[:, 27 + 13].astype(np.float64)
+    ld, n2 = x[:, FEAT_CENTRE_D].astype(np.float64), x[:, FEAT_CENTRE_N].astype(np.float64)
     d_shift, d_scale = float(ld.mean()), float(max(ld.std(), 1e-3))
     n_shift, n_scale = float(n2.mean()), float(max(n2.std(), 1e-8))
     kref = c_s * c_s * cfg.dx * cfg.dy * math.exp(d_shift)
@@ -240,4 +243,50 @@ class Network:
         domain._check(domain.lib.wrfdc_set_closure(domain.ctx, None, None, None), "wrfdc_set_closure")
 
 
-__all__ = ["ClosureNorm", "Network", "coarse_grain", "fit_norm", "flux_error", "init_theta", "row_count", "training_rows"]
+# ---------------------------------------------------------------------------
+# a truth run and a trained network
+# ---------------------------------------------------------------------------
+def truth_pair(device: str, nxc: int = 24, nz: int = 10, ratio: int = 3, fine_steps: int = 2, dx: float = 3000.0, dt: float = 18.0,
+               nmoist: int = 2):
+    """(fine, coarse) Domains of the seeded hill case: the fine one at
+    dx / ratio and dt / ratio (nxc * ratio columns a side), advanced
+    fine_steps steps; the coarse one at dx, dt (nxc columns a side)."""
+    from .cases import make_case
+    from .device import Domain
+
+    fine_case = make_case(nxc * ratio, nxc * ratio, nz, dx=dx / ratio, dt=dt / ratio, nmoist=nmoist, nscalar=0, hill_height=300.0)
+    coarse_case = make_case(nxc, nxc, nz, dx=dx, dt=dt, nmoist=nmoist, nscalar=0, hill_height=300.0)
+    fine = Domain(fine_case.config, device)
+    fine.upload(fine_case.arrays)
+    if fine_steps:
+        fine.step(fine_steps, 0.0)
+    coarse = Domain(coarse_case.config, device)
+    coarse.upload(coarse_case.arrays)
+    return fine, coarse
+
+
+def train_on_truth(device: str, nxc: int = 32, nz: int = 16, ratio: int = 3, fine_steps: int = 30, epochs: int = 30,
+                   batch: int = 1024, lr: float = 2e-3, dx: float = 3000.0, dt: float = 18.0, log=None):
+    """A network trained on the sub-grid stresses of a fine run of this
+    dycore (dx / ratio) coarse-grained to dx: returns (network, training
+    loss per epoch).  The fine and coarse domains are released."""
+    fine, coarse = truth_pair(device, nxc=nxc, nz=nz, ratio=ratio, fine_steps=fine_steps, dx=dx, dt=dt)
+    try:
+        coarse_grain(fine, coarse, ratio)
+        norm = fit_norm(coarse)
+        X, T = training_rows(coarse, norm, fine, ratio)
+        net = Network(norm, device, seed=0)
+        hist = net.fit(X, T, epochs=epochs, batch=batch, lr=lr, log=log)
+    finally:
+        fine.close()
+        coarse.close()
+    return net, hist
+
+
+# 2 * (55 * 64 + 64 * 64 + 64) floating-point operations per point of the
+# network's km (both hidden layers counted with their 55 / 64 real inputs)
+FLOP_PER_POINT = 2 * (55 * 64 + 64 * 64 + 64)
+
+
+__all__ = ["FEAT_CENTRE_D", "FEAT_CENTRE_N", "FEAT_DX", "ClosureNorm", "FLOP_PER_POINT", "Network", "coarse_grain", "fit_norm", "flux_error", "init_theta", "row_count", "train_on_truth",
+           "training_rows", "truth_pair"]
