@@ -1,8 +1,9 @@
 """GPU parity at the benchmark workloads and at every column-kernel shape.
 
 * the 100x100x45 plain case (6 moist species): one RK3 step against the
-  FP32 oracle (every field within relative L2 1e-5) and 100 steps (every
-  prognostic field within NRMSE 1e-3);
+  FP32 oracle (every field within relative L2 1e-5), against the FP64 oracle
+  (per prognostic field no further than the FP32 oracle is) and 100 steps
+  (every prognostic field within NRMSE 1e-3);
 * the 400x400x60 case (8 species, PD advection): one step against the
   oracle, and 100 GPU steps checked through properties that do not need the
   oracle -- a finite state, dry-air mass and total energy that drift by
@@ -51,6 +52,34 @@ def test_c0_one_step(gpu):
     got, ref = _one_step(case, gpu)
     bad = compare(got, ref, tol=REL_L2_STEP, skip=STEP_DIAGNOSTICS)
     assert not bad, f"fields above rel-L2 {REL_L2_STEP}: {bad}"
+
+
+def test_c0_error_against_fp64(gpu):
+    """The FP32 GPU step's distance from the FP64 oracle, per prognostic
+    field, at 100x100x45 after one step: no more than the FP32 oracle's own
+    (the two differ in the last bits at most), printed as the table quoted
+    for the FP32 arithmetic."""
+    from oracle import Oracle
+    from parity import rel_l2
+    from wrfdc.cases import named_case
+
+    case = named_case("c0")
+    print("[workload] c0 100x100x45, 1 step: GPU and FP32 oracle vs FP64 oracle", flush=True)
+    got, ref32 = _one_step(case, gpu)
+    o = Oracle(case.config, case.arrays, "f64")
+    o.step(1, 0.0)
+    ref64 = o.copy()
+    o.close()
+    rows = []
+    for f in PROGNOSTIC:
+        if f == "scalar" and case.config.nscalar == 0:
+            continue
+        eg, eo = rel_l2(got[f], ref64[f]), rel_l2(ref32[f], ref64[f])
+        rows.append((f, eg, eo))
+        assert eg <= 1.05 * eo + 1e-9, (f, eg, eo)
+    print("[workload] field | GPU FP32 vs FP64 | oracle FP32 vs FP64", flush=True)
+    for f, eg, eo in rows:
+        print(f"[workload] {f} | {eg:.2e} | {eo:.2e}", flush=True)
 
 
 @pytest.mark.timeout(900)
