@@ -133,7 +133,7 @@ def batched_form(steps: int, plain_value: float) -> dict:
     nm = n_mass(doms)
     _log(f"batched: {len(doms)} members, {nm / 1e6:.1f} M mass points, built in {time.perf_counter() - t0:.1f} s")
     eager = eager_step_time(doms, reps=2)
-    nb, assign, table = autotune(doms, candidates=(1, 2, 4, 8), reps=2, log=lambda m: _log("batched autotune: " + m))
+    nb, assign, table = autotune(doms, candidates=(1, 2, 4, 8, 16), reps=2, log=lambda m: _log("batched autotune: " + m))
     ens = Ensemble(doms, assign, nb)
     ens.step(1, 0.0)
     ens.synchronize()
