@@ -90,7 +90,9 @@ int wrfdc_step(wrfdc_ctx *ctx, int nsteps, double dtbc0);
  * set_w_surface), one of its parts (advect_u/v/w, advect_scalar[_pd],
  * rhs_ph, horizontal_pressure_gradient, pg_buoy_w, compute_diff_metrics,
  * cal_deform_and_div, smag2d_km, calculate_n2, calculate_km_kh, closure_km,
- * horizontal_diffusion_2, dry_boundary_tendency, acoustic_substep) or a
+ * horizontal_diffusion_2, dry_boundary_tendency, acoustic_substep,
+ * w_solve:columns | w_solve:levels: the tridiagonal w solve alone, in place
+ * on w_2, one column per lane or one column per wavefront) or a
  * whole rk_stage / step.  rk_step 1..3 sets dt_rk, dts_rk and the number of
  * acoustic steps as _rk_step_sizes does; iteration is the acoustic step
  * (calc_p_rho: 0 before the loop).  The lateral boundary time is the one of

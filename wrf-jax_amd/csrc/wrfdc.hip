@@ -29,6 +29,7 @@
 #include "k_species.h"
 #include "k_closure.h"
 #include "k_adjoint.h"
+#include "k_colsolve.h"
 
 // ---------------------------------------------------------------------------
 // small helper kernels local to the orchestration
@@ -1514,6 +1515,22 @@ int wrfdc_stage(wrfdc_ctx *x, const char *name, int rk_step, int iteration) {
         k_uv_substep<<<Run::g3(r), Run::b3(), 0, x->s>>>(x->h, dts, r);
       } else {
         launch_acoustic_col(x->h, x->s, dts, x->iteration == 1, x->iteration == x->nsmall, (float)x->nsmall, 1);
+      }
+    }
+    else if (n == "w_solve:columns" || n == "w_solve:levels") {
+      // the tridiagonal w solve alone, in place on w_2 with the current
+      // factors, in either layout (for measuring them against each other)
+      const unsigned nb = (unsigned)((x->ide - 3 + 63) / 64) * (unsigned)(x->jde - 3);
+      float *w = x->h.F[F_w_2];
+      if (n == "w_solve:columns") {
+        k_tridiag_lanes_columns<<<nb, 64, 0, x->s>>>(x->h, w, w);
+      } else {
+        if (x->kde > 64) return set_err(x, WRFDC_EINVAL, "w_solve:levels needs kde <= 64 (one level per lane)");
+        const size_t lds = tridiag_levels_lds(x->kde);
+        if (lds > 65536)
+          hchk(hipFuncSetAttribute((const void *)k_tridiag_lanes_levels, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+               "w_solve LDS");
+        k_tridiag_lanes_levels<<<nb, dim3(64, 8), lds, x->s>>>(x->h, w, w);
       }
     }
     else if (n == "acoustic_substep") {
