@@ -37,17 +37,21 @@ def step_bytes(n_mass: int, n_species: int, ns: int = 4) -> float:
 @dataclass(frozen=True)
 class KernelModel:
     name: str          # HIP kernel symbol
-    stage: str         # wrfdc_stage entry that launches exactly this kernel
+    stage: str         # wrfdc_stage entry that launches exactly this kernel ("" if none)
     rk_step: int
     iteration: int
     reads: tuple
     writes: tuple
+    per_species: int = 0  # further passes per advected species (moist + scalar)
+    per_moist: int = 0    # further passes per moist species
+    per_scalar_diffused: int = 0  # further passes per diffused scalar (theta + every species)
 
-    def passes(self) -> int:
-        return len(self.reads) + len(self.writes)
+    def passes(self, nmoist: int = 0, nscalar: int = 0) -> int:
+        return (len(self.reads) + len(self.writes) + self.per_species * (nmoist + nscalar) + self.per_moist * nmoist +
+                self.per_scalar_diffused * (1 + nmoist + nscalar))
 
     def bytes(self, cfg) -> float:
-        return 4.0 * cfg.nx * cfg.ny * cfg.nz * self.passes()
+        return 4.0 * cfg.nx * cfg.ny * cfg.nz * self.passes(cfg.nmoist, cfg.nscalar)
 
 
 KERNELS = {
@@ -76,6 +80,45 @@ KERNELS = {
                                 writes=("al", "p", "pm1")),
     "k_rhs_ph": KernelModel("k_rhs_ph", "rhs_ph", 2, 0, reads=("u_2", "v_2", "ww", "ph_2", "phb", "w_2", "ph_tend"),
                             writes=("ph_tend",)),
+    # kernels of the step's fused path (a stage name only where one launches
+    # the kernel alone); the species kernels carry every species of a stage
+    "k_species_update": KernelModel("k_species_update", "", 2, 0, reads=("ru_m", "rv_m", "ww_m"), writes=(),
+                                    per_species=4),  # q in, q_old, tendency, q out (rk 2; rk 1 reads q_old as q)
+    "k_species_pd_prep": KernelModel("k_species_pd_prep", "", 3, 0, reads=(), writes=(), per_species=4),  # q_old, tend r/w
+    "k_species_pd_scale": KernelModel("k_species_pd_scale", "", 3, 0, reads=("ru_m", "rv_m", "ww_m"), writes=(),
+                                      per_species=3),  # q in, q_old, factor out
+    "k_species_pd_update": KernelModel("k_species_pd_update", "", 3, 0, reads=("ru_m", "rv_m", "ww_m"), writes=(),
+                                       per_species=4),  # q in, q_old, factor, q out
+    "k_advect_u": KernelModel("k_advect_u", "advect_u", 2, 0, reads=("u_2", "ru", "rv", "ww", "ru_tend"), writes=("ru_tend",)),
+    "k_advect_u<true>": KernelModel("k_advect_u<true>", "", 2, 0,
+                                    reads=("u_2", "ru", "rv", "ww", "ph_2", "php", "alt", "al", "p", "pb", "v_2", "w_2", "ru_tendf",
+                                           "u_save"), writes=("ru_tend",)),
+    "k_advect_v<true>": KernelModel("k_advect_v<true>", "", 2, 0,
+                                    reads=("v_2", "ru", "rv", "ww", "ph_2", "php", "alt", "al", "p", "pb", "u_2", "w_2", "rv_tendf",
+                                           "v_save"), writes=("rv_tend",)),
+    "k_advect_w<true>": KernelModel("k_advect_w<true>", "", 2, 0,
+                                    reads=("w_2", "ru", "rv", "ww", "t_2", "t_save", "alt", "al", "p", "pb", "cqw", "u_2", "v_2",
+                                           "rw_tendf", "w_save"), writes=("rw_tend", "cqw")),
+    "k_advect_scalar<true>": KernelModel("k_advect_scalar<true>", "", 2, 0, reads=("t_2", "ru", "rv", "ww", "t_tendf", "t_save"),
+                                         writes=("t_tend",)),
+    "k_tend_ph": KernelModel("k_tend_ph", "", 2, 0, reads=("u_2", "v_2", "ww", "ph_2", "phb", "w_2", "ph_tendf", "ph_save"),
+                             writes=("ph_tend",)),
+    "k_rk_prep3": KernelModel("k_rk_prep3", "", 2, 0, reads=("u_2", "v_2", "w_2", "al", "alb", "phb", "ph_2"),
+                              writes=("ru", "rv", "rw", "alt", "php", "cqu", "cqv", "cqw"), per_moist=1),
+    "k_ssp_3d<false>": KernelModel("k_ssp_3d<false>", "", 2, 0,
+                                   reads=("pb", "p", "alt", "u_2", "v_2", "t_2", "w_2", "ph_2", "ww", "u_1", "v_1", "t_1", "w_1",
+                                          "ph_1"),
+                                   writes=("c2a", "u_save", "u_2", "v_save", "v_2", "t_save", "t_2", "w_save", "w_2", "ph_save",
+                                           "ph_2", "ww_1")),
+    "k_ssf<false>": KernelModel("k_ssf<false>", "", 2, 0,
+                                reads=("u_2", "u_save", "v_2", "v_save", "w_2", "w_save", "ph_2", "ph_save", "ww", "ww_1", "t_2",
+                                       "t_save"), writes=("u_2", "v_2", "w_2", "ph_2", "ww", "t_2")),
+    "k_calc_p_rho_phi": KernelModel("k_calc_p_rho_phi", "calc_p_rho_phi", 2, 0, reads=("ph_2", "phb", "alb", "pb", "t_2"),
+                                    writes=("al", "p"), per_moist=0),
+    "k_hdiff_s_faces": KernelModel("k_hdiff_s_faces", "", 1, 0, reads=("xkhh", "rho", "zx", "zy", "rdzw"), writes=(),
+                                   per_scalar_diffused=3),  # q in, H1 and H2 out
+    "k_hdiff_s_tend": KernelModel("k_hdiff_s_tend", "", 1, 0, reads=("zx", "zy", "rdzw"), writes=(),
+                                  per_scalar_diffused=4),  # H1, H2 in, tendency r/w
 }
 
 DOMINANT = "k_acoustic_col"

@@ -1,6 +1,6 @@
 """Text summaries of rocprofv3 result databases (SQLite) for ``profiles/``.
 
-Two reports:
+Three reports:
 
 ``kernels DB``
     per-kernel calls, total and mean duration from a ``--kernel-trace --stats``
@@ -15,8 +15,14 @@ Two reports:
     averaged over the kernel's dispatches (the format ``bytes_model.
     measured_traffic`` reads).
 
-Usage: ``python -m wrfdc.profile_summary kernels DB`` or
-``python -m wrfdc.profile_summary traffic FETCH_DB WRITE_DB``.
+``traffic_vs_model FETCH_DB WRITE_DB TRACE_DB``
+    the measured bytes per launch beside the algorithmic bytes of
+    ``bytes_model.KERNELS`` for the 400x400x60 workload, with the mean launch
+    time of a kernel trace of the same command.
+
+Usage: ``python -m wrfdc.profile_summary kernels DB``,
+``python -m wrfdc.profile_summary traffic FETCH_DB WRITE_DB`` or
+``python -m wrfdc.profile_summary traffic_vs_model FETCH_DB WRITE_DB TRACE_DB``.
 """
 
 from __future__ import annotations
@@ -92,11 +98,58 @@ def traffic(fetch_db: str, write_db: str) -> str:
     return "\n".join(out)
 
 
+def traffic_vs_model(fetch_db: str, write_db: str, trace_db: str, workload: str = "c1") -> str:
+    """Per kernel of bytes_model.KERNELS: mean launch time (kernel trace),
+    measured HBM bytes per launch (the two counter runs, calibrated as in
+    ``traffic``), the algorithmic bytes of the model, their ratio and the
+    bandwidth each implies."""
+    from .bytes_model import KERNELS
+    from .cases import CONFIGS
+
+    c = CONFIGS[workload]
+    nmoist, nscalar = c["nmoist"], c["nscalar"]
+    npts = c["nx"] * c["ny"] * c["nz"]
+    meas = {}
+    for line in traffic(fetch_db, write_db).splitlines():
+        parts = [p.strip() for p in line.strip().strip("|").split("|")]
+        if len(parts) == 5 and parts[3].replace(".", "").isdigit():
+            meas[parts[0]] = float(parts[3])
+    cur = sqlite3.connect(trace_db).cursor()
+    dur = {_short(n): avg for n, calls, tot, avg in cur.execute("select name, total_calls, total_duration, average from top_kernels")}
+    out = [f"Workload {workload}: {c['nx']}x{c['ny']}x{c['nz']}, {nmoist} moist + {nscalar} scalar species "
+           f"(algorithmic bytes: 4 B x N_mass per distinct 3-D field pass, bytes_model.KERNELS)", "",
+           "| kernel | mean us | traffic MB | algorithmic MB | traffic / alg | GB/s (traffic) | GB/s (alg) |",
+           "|---|---:|---:|---:|---:|---:|---:|"]
+    rows = []
+    def match(name, names):
+        # a model entry without template arguments stands for the one
+        # instance of that kernel that has no entry of its own
+        if name in names:
+            return name
+        if "<" in name:
+            return None
+        return next((k for k in names if k.startswith(name + "<") and k not in KERNELS), None)
+
+    for name, km in KERNELS.items():
+        key, dk = match(name, meas), match(name, dur)
+        if key is None or dk is None:
+            continue
+        alg = 4.0 * npts * km.passes(nmoist, nscalar)
+        us = dur[dk]
+        rows.append((us, name, meas[key], alg))
+    for us, name, tr, alg in sorted(rows, reverse=True):
+        out.append(f"| {name} | {us:.1f} | {tr / 1e6:.0f} | {alg / 1e6:.0f} | {tr / alg:.2f} | {tr / (us * 1e3):.0f} | "
+                   f"{alg / (us * 1e3):.0f} |")
+    return "\n".join(out)
+
+
 def main(argv):
     if len(argv) >= 2 and argv[0] == "kernels":
         print(kernels(argv[1]))
     elif len(argv) >= 3 and argv[0] == "traffic":
         print(traffic(argv[1], argv[2]))
+    elif len(argv) >= 4 and argv[0] == "traffic_vs_model":
+        print(traffic_vs_model(argv[1], argv[2], argv[3]))
     else:
         print(__doc__)
         return 2
