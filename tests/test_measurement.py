@@ -18,7 +18,12 @@ def test_step_model_matches_documented_workload():
 
 def test_measured_traffic_reads_rows_with_template_arguments(tmp_path):
     p = tmp_path / "traffic_x.md"
-    p.write_text("| kernel | fetch MiB | write MiB | traffic bytes | dispatches |\n"
+    # the traffic-vs-model table comes first in the profile and has more
+    # columns: its rows must not be taken for the raw counter rows
+    p.write_text("| kernel | mean us | traffic MB | algorithmic MB | traffic / alg | GB/s (traffic) | GB/s (alg) |\n"
+                 "|---|---:|---:|---:|---:|---:|---:|\n"
+                 "| k_acoustic_col | 437.6 | 1791 | 1382 | 1.30 | 4093 | 3159 |\n\n"
+                 "| kernel | fetch MiB | write MiB | traffic bytes | dispatches |\n"
                  "|---|---:|---:|---:|---:|\n"
                  "| k_acoustic_col<8> | 1.0 | 2.0 | 3.5e+09 | 9 |\n"
                  "| k_uv_substep | 1.0 | 2.0 | 5.0e+08 | 7 |\n")

@@ -4,6 +4,7 @@
 // module_big_step_utilities_em_fast.py and module_em.py.
 #pragma once
 #include "wrfdc_common.h"
+#include "k_f64math.h"
 
 __global__ void k_fill(float *p, float v, size_t n) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -573,16 +574,18 @@ __global__ void k_calc_p_rho_phi(const DevCtx xv, R3 r) {
   const unsigned o = XI3(i, k, j), o1 = o + (unsigned)x->ip;
   const float ph1 = elem(ph, o1), ph0 = elem(ph, o), phb1 = elem(phb, o1), phb0 = elem(phb, o), albv = elem(alb, o);
   const float qv = elem(SP(moist, 0), o), tv = elem(t, o), pbv = elem(pb, o);
-  // log and pow rounded once to FP32 from an FP64 evaluation: the correctly
-  // rounded FP32 values in all but vanishingly rare cases.  p and al are
-  // small differences from the base state (p = p0*temp^cpovcv - pb), which
-  // magnify an ulp of the total about a thousandfold: the device library's
-  // FP32 logf / powf (within 1-2 ulp) put p and al 4.5e-5 and 2.8e-5 (relative
-  // L2) away from the correctly rounded result after one call.
-  const float lg = (float)log((double)(pfd / pfu));
+  // log and pow rounded once to FP32 from an FP64 evaluation (f64_log,
+  // f64_pow: the same FP32 values as the C library's FP64 log and pow for
+  // every FP32 input in [1, 2), and in [1/16, 2) with exponent 1.4f).
+  // p and al are small differences from the base state (p =
+  // p0*temp^cpovcv - pb), which magnify an ulp of the total about a
+  // thousandfold: the device library's FP32 logf / powf (within 1-2 ulp) put
+  // p and al 4.5e-5 and 2.8e-5 (relative L2) away from the correctly rounded
+  // result after one call.
+  const float lg = (float)f64_log((double)(pfd / pfu));
   const float alv = (ph1 - ph0 + phb1 - phb0) / phm / lg - albv;
   elem(FL(al), o) = alv;
   const float qvf = 1.f + x->rvovrd * qv;
   const float temp = (x->r_d * (x->t0 + tv) * qvf) / (x->p0 * (alv + albv));
-  elem(FL(p), o) = (float)pow((double)temp, (double)x->cpovcv) * x->p0 - pbv;
+  elem(FL(p), o) = (float)f64_pow((double)temp, (double)x->cpovcv) * x->p0 - pbv;
 }

@@ -27,6 +27,11 @@ struct SpecList {
   float *tend[kMaxSpec];      // accumulated physics / boundary tendency
   float *scale[kMaxSpec];     // PD renormalisation factors (scratch)
   int n;
+  // species whose last-stage physics update of q_old (rk_update_scalar_pd)
+  // the second stage's update also does, with the last stage's dt: bit s
+  // set for species s (k_species_update at rk = 2 only)
+  unsigned prep_mask;
+  float prep_dt;
 };
 
 // mass-weighted RK update of one point (rk_update_scalar)
@@ -52,6 +57,14 @@ __device__ __forceinline__ UpdCoef upd_coef(const DevCtx *__restrict__ x, int i,
 // copy q_old (equal to q there), so old = in.  Face-sharing blocks (block
 // (64, 4), 1-D grid of face_tiles(r).count() blocks): the south faces of all
 // species go to LDS first, one barrier, then each species' update.
+//
+// At rk = 2 the species of L.prep_mask also get the last stage's
+// rk_update_scalar_pd here (k_species_pd_prep's arithmetic, over the same
+// points): q_old and the tendency it reads are not written again before the
+// last stage, and c.oldm is that kernel's column mass (c1h (mu_1 + mub) +
+// c2h), so the last stage finds the same q_old and a cleared tendency.  Each
+// thread reads q_old and the tendency at its own point only, so updating them
+// in place is race-free.
 __global__ void __launch_bounds__(256) k_species_update(const DevCtx xv, SpecList L, int rk, float dt, R3 r) {
   const DevCtx *__restrict__ x = &xv;
   extern __shared__ float ys_[];  // L.n x 4 x 64: south face flux of every species
@@ -116,7 +129,13 @@ __global__ void __launch_bounds__(256) k_species_update(const DevCtx xv, SpecLis
     float tend = c.inner ? adv * c.msfty : 0.f;
     tend = tend + tnd;
     const float out = (c.oldm * old + dt * tend) / c.newm;
-    if (own) elem(L.out[s], o) = out;
+    if (own) {
+      elem(L.out[s], o) = out;
+      if (rk == 2 && ((L.prep_mask >> s) & 1u)) {
+        elem(L.old[s], o) = (c.oldm * old + L.prep_dt * tnd) / c.oldm;
+        elem(L.tend[s], o) = 0.f;
+      }
+    }
   }
 #undef YS
 }

@@ -232,6 +232,10 @@ struct Run {
   int ide, jde, kde;
   float *user_moist, *user_scalar, *alt_moist, *alt_scalar;
   mutable bool old_synced = false;  // q_old == q on mass points for every species
+  // within step(): the second stage's species update also does the last
+  // stage's rk_update_scalar_pd of the PD species (pd_prepped records that it
+  // did); a stage called on its own keeps the reference's order
+  mutable bool fold_pd_prep = false, pd_prepped = false;
   explicit Run(wrfdc_ctx *x)
       : X(x), d(x->h), s(x->s), ide(x->ide), jde(x->jde), kde(x->kde), user_moist(x->h.F[F_moist]), user_scalar(x->h.F[F_scalar]),
         alt_moist(x->spec_alt), alt_scalar(x->spec_alt + x->n3 * (size_t)x->c.nmoist) {}
@@ -889,11 +893,20 @@ struct Run {
     const bool pdm = (rk == 3 && X->c.pd_moist), pds = (rk == 3 && X->c.pd_scalar);
     for (int q = 0; q < nm; q++) add(pdm ? pd : plain, F_moist, F_moist_old, F_moist_tend, out_alt ? alt_moist : user_moist, q);
     for (int q = 0; q < nsc; q++) add(pds ? pd : plain, F_scalar, F_scalar_old, F_scalar_tend, out_alt ? alt_scalar : user_scalar, q);
+    if (rk == 2 && fold_pd_prep) {
+      // the species of the last stage's PD list, in the list's order
+      for (int q = 0; q < nm; q++)
+        if (X->c.pd_moist) plain.prep_mask |= 1u << q;
+      for (int q = 0; q < nsc; q++)
+        if (X->c.pd_scalar) plain.prep_mask |= 1u << (nm + q);
+      plain.prep_dt = (float)X->c.dt;  // the last stage's dt_rk
+      pd_prepped = plain.prep_mask != 0;
+    }
     if (pd.n > 0) {
       for (int q = 0; q < pd.n; q++) {
         pd.scale[q] = X->spec_scale + (size_t)q * n3;
       }
-      k_species_pd_prep<<<g3p(m), b3(), 0, s>>>(d, pd, dt, m);
+      if (!pd_prepped) k_species_pd_prep<<<g3p(m), b3(), 0, s>>>(d, pd, dt, m);
       {
         BcBatch bc(this);
         for (int q = 0; q < pd.n; q++) bc.add3(pd.old[q], 'p');
@@ -1044,6 +1057,11 @@ struct Run {
       for (int q = 0; q < X->c.nmoist; q++) bc.add3(Sp(F_moist_old, q), 'p');
     }
     old_synced = true;
+    struct FoldScope {  // cleared however the stages end
+      const Run *r;
+      explicit FoldScope(const Run *run) : r(run) { r->fold_pd_prep = true, r->pd_prepped = false; }
+      ~FoldScope() { r->fold_pd_prep = r->pd_prepped = false; }
+    } fold(this);
     for (int rk = 1; rk <= 3; rk++) {
       set_rk(rk);
       rk_stage();

@@ -136,8 +136,11 @@ def measured_traffic(kernel: str, workload: str, path: str | None = None):
         path = os.path.join(repo, "profiles", f"traffic_{workload}.md")
     if not os.path.exists(path):
         return None
-    # the kernel symbol may carry template arguments (k_acoustic_col<8>)
-    pat = re.compile(rf"^\|\s*{re.escape(kernel)}(<[^>|]*>)?\s*\|\s*([0-9.eE+]+)\s*\|\s*([0-9.eE+]+)\s*\|\s*([0-9.eE+]+)\s*\|")
+    # a row of the raw counter table: kernel | fetch MiB | write MiB |
+    # traffic bytes | dispatches (the symbol may carry template arguments,
+    # k_acoustic_col<8>); rows of the other tables have more columns
+    num = r"\s*([0-9.eE+]+)\s*\|"
+    pat = re.compile(rf"^\|\s*{re.escape(kernel)}(<[^>|]*>)?\s*\|{num}{num}{num}{num}$")
     with open(path) as fh:
         for line in fh:
             m = pat.match(line.strip())
