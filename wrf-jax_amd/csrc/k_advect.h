@@ -38,38 +38,45 @@ __device__ __forceinline__ float lane_next(float v, int lane) {
 // top wave also forms its own north face), one level k.  Each x face flux is
 // evaluated once and each y face 5/4 times instead of twice; every thread
 // still owns one point, so the grid keeps one wave per 63 points.
-// Tile order: i fastest, then kFaceLG consecutive levels, then rows, then
-// groups of kFaceLG levels, XCD-contiguous: the neighbour rows of a y stencil and the
+// Tile order: i fastest, then LG consecutive levels, then rows, then groups
+// of LG levels, XCD-contiguous: the neighbour rows of a y stencil and the
 // neighbour levels of a z stencil are read by the same XCD shortly before.
+// LG is 4 for most kernels; the PD species passes and advect_w take 8, which
+// measured faster for them (each kernel timed at 400x400x60 with LG 2, 4, 8
+// and 16; the other kernels are fastest at 4).
 #ifndef WRFDC_FACE_LG
 #define WRFDC_FACE_LG 4
 #endif
 constexpr unsigned kFaceLG = WRFDC_FACE_LG;  // levels per level group of the tile order
+constexpr unsigned kFaceLGPd = 8;            // k_species_pd_scale, k_species_pd_update
+constexpr unsigned kFaceLGW = 8;             // k_advect_w
 struct FaceTiles {
-  unsigned ni, nj, nkc;  // tiles along i, row groups, level groups
-  __host__ __device__ unsigned count() const { return ni * nj * nkc * kFaceLG; }
+  unsigned ni, nj, nkc, lg;  // tiles along i, row groups, level groups, levels per group
+  __host__ __device__ unsigned count() const { return ni * nj * nkc * lg; }
 };
-__host__ __device__ inline FaceTiles face_tiles(const R3 &r) {
-  return FaceTiles{(unsigned)(r.i1 - r.i0 + 63) / 63u, (unsigned)(r.j1 - r.j0 + 4) / 4u, (unsigned)(r.k1 - r.k0 + kFaceLG) / kFaceLG};
+__host__ __device__ inline FaceTiles face_tiles(const R3 &r, unsigned lg = kFaceLG) {
+  return FaceTiles{(unsigned)(r.i1 - r.i0 + 63) / 63u, (unsigned)(r.j1 - r.j0 + 4) / 4u, (unsigned)(r.k1 - r.k0 + (int)lg) / lg, lg};
 }
 // thread -> lane, wave (row in the block), column i, level k, row j and the
 // block's first row j0; own: the lane holds a point of r; ic: i clamped into
 // r (in-range loads for every lane).  Blocks past the last level return as a
 // whole; waves past the last row (jrow false) still take part in the block's
-// barrier and form their south face.
-#define GET_FACE_BLOCK(r)                                                                \
-  const FaceTiles T_ = face_tiles(r);                                                    \
+// barrier and form their south face.  The launch's grid is
+// face_tiles(r, LG).count() blocks with the kernel's LG.
+#define GET_FACE_BLOCK_LG(r, LG)                                                         \
+  const FaceTiles T_ = face_tiles(r, LG);                                                \
   const unsigned t_ = xcd_tile(blockIdx.x, gridDim.x);                                   \
   const int lane = (int)threadIdx.x, wv = (int)wave_y();                                 \
   const unsigned u_ = t_ / T_.ni;                                                        \
   const int i = (r).i0 + (int)(t_ % T_.ni) * 63 + lane;                                  \
-  const int k = (r).k0 + (int)((u_ / kFaceLG) / T_.nj) * (int)kFaceLG + (int)(u_ % kFaceLG); \
-  const int j0 = (r).j0 + (int)((u_ / kFaceLG) % T_.nj) * 4;                             \
+  const int k = (r).k0 + (int)((u_ / (LG)) / T_.nj) * (int)(LG) + (int)(u_ % (LG));      \
+  const int j0 = (r).j0 + (int)((u_ / (LG)) % T_.nj) * 4;                                \
   const int j = j0 + wv;                                                                 \
   if (k > (r).k1) return; /* whole block */                                              \
   const bool jrow = (j <= (r).j1);                                                       \
   const bool own = (lane < 63 && i <= (r).i1 && jrow);                                   \
   const int ic = i <= (r).i1 ? i : (r).i1
+#define GET_FACE_BLOCK(r) GET_FACE_BLOCK_LG(r, kFaceLG)
 
 // ---------------- face fluxes from loaded stencils ---------------------------
 // Every kernel below issues all loads of a point first (buffer loads,
@@ -420,7 +427,7 @@ __global__ void __launch_bounds__(256) k_advect_w(const DevCtx xv, const float *
                                                   const float *rom, R3 r, int rk) {
   const DevCtx *__restrict__ x = &xv;
   __shared__ float ys[4][64];
-  GET_FACE_BLOCK(r);
+  GET_FACE_BLOCK_LG(r, kFaceLGW);
   const int ide = x->ide, jde = x->jde, kde = x->kde;
   const unsigned nb = (unsigned)x->n3, sj = (unsigned)(kde * x->ip), sk = (unsigned)x->ip;
   const BufF bq = bufv(w, nb), bv = bufv(rv, nb);

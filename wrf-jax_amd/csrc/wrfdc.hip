@@ -574,7 +574,7 @@ struct Run {
   }
   void advect_w() const {
     R3 r{1, ide - 1, 2, kde, 1, jde - 1};
-    k_advect_w<false><<<dim3(face_tiles(r).count()), b3(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
+    k_advect_w<false><<<dim3(face_tiles(r, kFaceLGW).count()), b3(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
   }
   void advect_scalar(const float *q, float *tend, const float *ru, const float *rv, const float *rom) const {
     R3 m = mass3();
@@ -650,7 +650,7 @@ struct Run {
     const R3 rv{1, ide - 1, 1, kde - 1, 1, jde};
     k_advect_v<true><<<dim3(face_tiles(rv).count()), b3(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), rv, rk);
     const R3 rw{1, ide - 1, 1, kde, 1, jde - 1};
-    k_advect_w<true><<<dim3(face_tiles(rw).count()), b3(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), rw, rk);
+    k_advect_w<true><<<dim3(face_tiles(rw, kFaceLGW).count()), b3(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), rw, rk);
     const R3 rq{1, ide - 1, kde - 1, kde - 1, 1, jde - 1};  // levels [2, kde-2]: k_advect_w<true>
     k_cqw_inv<<<g3p(rq), b3(), 0, s>>>(d, rq);
     const R3 m = mass3();
@@ -912,8 +912,8 @@ struct Run {
         for (int q = 0; q < pd.n; q++) bc.add3(pd.old[q], 'p');
       }
       R3 r = full3();
-      k_species_pd_scale<<<dim3(face_tiles(r).count()), b3(), sizeof(float) * 2 * 256 * pd.n, s>>>(d, pd, dt, r);
-      k_species_pd_update<<<dim3(face_tiles(m).count()), b3(), sizeof(float) * 2 * 256 * pd.n, s>>>(d, pd, dt, m);
+      k_species_pd_scale<<<dim3(face_tiles(r, kFaceLGPd).count()), b3(), sizeof(float) * 2 * 256 * pd.n, s>>>(d, pd, dt, r);
+      k_species_pd_update<<<dim3(face_tiles(m, kFaceLGPd).count()), b3(), sizeof(float) * 2 * 256 * pd.n, s>>>(d, pd, dt, m);
     }
     if (plain.n > 0) k_species_update<<<dim3(face_tiles(m).count()), b3(), sizeof(float) * 256 * plain.n, s>>>(d, plain, rk, dt, m);
     // outputs become the current species
