@@ -82,8 +82,11 @@ KERNELS = {
                             writes=("ph_tend",)),
     # kernels of the step's fused path (a stage name only where one launches
     # the kernel alone); the species kernels carry every species of a stage
+    # q in, q_old, tendency, q out; at rk 2 also q_old' and the cleared
+    # tendency of every PD species (the last stage's rk_update_scalar_pd):
+    # 5 per species is the mean of the step's two launches with all species PD
     "k_species_update": KernelModel("k_species_update", "", 2, 0, reads=("ru_m", "rv_m", "ww_m"), writes=(),
-                                    per_species=4),  # q in, q_old, tendency, q out (rk 2; rk 1 reads q_old as q)
+                                    per_species=5),
     "k_species_pd_prep": KernelModel("k_species_pd_prep", "", 3, 0, reads=(), writes=(), per_species=4),  # q_old, tend r/w
     "k_species_pd_scale": KernelModel("k_species_pd_scale", "", 3, 0, reads=("ru_m", "rv_m", "ww_m"), writes=(),
                                       per_species=3),  # q in, q_old, factor out

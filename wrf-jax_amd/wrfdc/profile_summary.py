@@ -1,6 +1,6 @@
 """Text summaries of rocprofv3 result databases (SQLite) for ``profiles/``.
 
-Three reports:
+Four reports:
 
 ``kernels DB``
     per-kernel calls, total and mean duration from a ``--kernel-trace --stats``
@@ -20,9 +20,20 @@ Three reports:
     ``bytes_model.KERNELS`` for the 400x400x60 workload, with the mean launch
     time of a kernel trace of the same command.
 
+``issue COUNTER_DB TRACE_DB``
+    instruction issue per kernel from one ``--pmc`` run of ``SQ_WAVES
+    SQ_INSTS_VALU SQ_INSTS_VMEM_RD SQ_INSTS_SALU SQ_INSTS_LDS
+    SQ_ACTIVE_INST_VALU GRBM_GUI_ACTIVE``: VALU and vector-memory
+    instructions per wave, the engine clock over the dispatch
+    (GRBM_GUI_ACTIVE cycles of one XCD / the trace's mean duration) and the share of the
+    dispatch the SIMDs spend issuing VALU work: SQ_INSTS_VALU x 4 cycles per
+    wave64 instruction, and SQ_ACTIVE_INST_VALU (quad-cycles), over 1024
+    SIMDs x GRBM_GUI_ACTIVE.
+
 Usage: ``python -m wrfdc.profile_summary kernels DB``,
-``python -m wrfdc.profile_summary traffic FETCH_DB WRITE_DB`` or
-``python -m wrfdc.profile_summary traffic_vs_model FETCH_DB WRITE_DB TRACE_DB``.
+``python -m wrfdc.profile_summary traffic FETCH_DB WRITE_DB``,
+``python -m wrfdc.profile_summary traffic_vs_model FETCH_DB WRITE_DB TRACE_DB`` or
+``python -m wrfdc.profile_summary issue COUNTER_DB TRACE_DB``.
 """
 
 from __future__ import annotations
@@ -143,6 +154,44 @@ def traffic_vs_model(fetch_db: str, write_db: str, trace_db: str, workload: str 
     return "\n".join(out)
 
 
+N_SIMD = 1024  # MI355X: 256 compute units x 4 SIMDs
+N_XCD = 8  # GRBM_GUI_ACTIVE comes summed over the 8 XCDs, each counting the dispatch's cycles
+
+
+def issue(counter_db: str, trace_db: str) -> str:
+    cur = sqlite3.connect(counter_db).cursor()
+    per = collections.defaultdict(lambda: collections.defaultdict(float))
+    name = {}
+    for kname, cname, v, disp in cur.execute("select kernel_name, counter_name, value, dispatch_id from counters_collection"):
+        d = per[disp]
+        d[cname] += v
+        name[disp] = _short(kname)
+    tcur = sqlite3.connect(trace_db).cursor()
+    dur = {_short(n): avg for n, avg in tcur.execute("select name, average from top_kernels")}
+    agg = collections.defaultdict(lambda: collections.defaultdict(float))
+    cnt = collections.Counter()
+    for disp, d in per.items():
+        for c, v in d.items():
+            agg[name[disp]][c] += v
+        cnt[name[disp]] += 1
+    out = ["| kernel | dispatches | mean us | clock GHz | VALU / wave | VMEM rd / wave | VALU issue % | VALU active % |",
+           "|---|---:|---:|---:|---:|---:|---:|---:|"]
+    rows = []
+    for k, a in agg.items():
+        n = cnt[k]
+        waves = a["SQ_WAVES"] / n
+        cyc = a["GRBM_GUI_ACTIVE"] / n / N_XCD
+        if waves <= 0 or cyc <= 0 or k not in dur:
+            continue
+        us = dur[k]
+        valu = a["SQ_INSTS_VALU"] / n
+        rows.append((us * n, k, n, us, cyc / (us * 1e3), valu / waves, a["SQ_INSTS_VMEM_RD"] / n / waves,
+                     100.0 * valu * 4.0 / (N_SIMD * cyc), 100.0 * a["SQ_ACTIVE_INST_VALU"] / n * 4.0 / (N_SIMD * cyc)))
+    for _, k, n, us, ghz, vw, mw, vi, va in sorted(rows, reverse=True):
+        out.append(f"| {k} | {n} | {us:.1f} | {ghz:.2f} | {vw:.0f} | {mw:.0f} | {vi:.0f} | {va:.0f} |")
+    return "\n".join(out)
+
+
 def main(argv):
     if len(argv) >= 2 and argv[0] == "kernels":
         print(kernels(argv[1]))
@@ -150,6 +199,8 @@ def main(argv):
         print(traffic(argv[1], argv[2]))
     elif len(argv) >= 4 and argv[0] == "traffic_vs_model":
         print(traffic_vs_model(argv[1], argv[2], argv[3]))
+    elif len(argv) >= 3 and argv[0] == "issue":
+        print(issue(argv[1], argv[2]))
     else:
         print(__doc__)
         return 2
