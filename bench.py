@@ -46,7 +46,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--forms", default="plain,batched,learned",
                     help="comma list of plain (the headline value), batched (32-member ensemble, one captured "
-                         "graph per step) and learned (1000x1000x60 with the trained closure)")
+                         "graph per step), learned (1000x1000x60 with the trained closure) and shard (one GPU's "
+                         "share of the plain workload split 8 ways; not in the default list)")
     ap.add_argument("--form-steps", type=int, default=5, help="timed steps of the batched and learned forms")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--traffic-calibration", action="store_true",
@@ -154,6 +155,51 @@ def batched_form(steps: int, plain_value: float) -> dict:
             "autotune_ms": {str(b): 1e3 * t for b, t in table},
             "workload": "32 seeded members, 100^2..300^2 x 45..60, dx 1..9 km, 6 moist species, PD on, FP32, "
                         "one captured RK3 step per replay"}
+
+
+def shard_form(steps: int, plain_value: float, warmup: int = 2) -> dict:
+    """One GPU's share of the 400x400x60 workload if it were split over 8
+    GPUs (design only: no exchange is run): the same case on a 1 x 8 row slab
+    (400 x 50 owned rows + 3 halo rows each side) and on a 2 x 4 tile
+    (200 x 100 owned + 3 each side), each a domain of its own with the
+    workload's species and spacing.  value = owned mass points x steps / s;
+    ``computed_value`` counts the halo points too."""
+    import torch
+
+    from wrfdc.cases import named_case
+    from wrfdc.device import Domain
+
+    halo = 3
+    out = {}
+    for name, (ox, oy) in (("slab_1x8", (400, 50)), ("tile_2x4", (200, 100))):
+        nx, ny = ox + (2 * halo if ox < 400 else 0), oy + 2 * halo
+        case = named_case("c1", nx=nx, ny=ny)
+        dom = Domain(case.config, "cuda:0")
+        dom.upload(case.arrays)
+        del case
+        for w in range(warmup):
+            dom.step(1, w * dom.config.dt)
+        dom.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(dom.stream)
+        dom.step(steps, warmup * dom.config.dt)
+        ev1.record(dom.stream)
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / steps
+        bad = int(dom.checksums()[3])
+        dom.close()
+        nz = 60
+        owned, computed = ox * oy * nz, nx * ny * nz
+        out[name] = {"domain": f"{nx}x{ny}x{nz}", "owned": f"{ox}x{oy}x{nz}", "ms_per_step": ms,
+                     "value": owned / (ms * 1e-3) / 1e6, "computed_value": computed / (ms * 1e-3) / 1e6,
+                     "vs_plain_per_point": computed / (ms * 1e-3) / 1e6 / plain_value, "nonfinite": bad}
+        _log(f"shard {name}: {nx}x{ny}x{nz}, {ms:.2f} ms per step")
+    torch.cuda.empty_cache()
+    out["unit"] = "Mpoint-steps/s"
+    out["steps"] = steps
+    out["workload"] = "one GPU's share of 400x400x60 split 8 ways (no exchange run), 8 species, PD on, FP32"
+    return out
 
 
 def learned_form(steps: int, plain_value: float, kernel_reps: int) -> dict:
@@ -330,6 +376,8 @@ def main():
         out["forms"]["batched"] = batched_form(args.form_steps, out["value"])
     if "learned" in forms:
         out["forms"]["learned"] = learned_form(args.form_steps, out["value"], args.kernel_reps)
+    if "shard" in forms:
+        out["forms"]["shard"] = shard_form(args.steps, out["value"])
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.workload)
