@@ -100,6 +100,49 @@ def test_stage_parity(stepped_state, domain_for, name, rk, it):
 SHAPES = [(28, 24, 12), (13, 17, 7), (40, 33, 20)]
 
 
+def _with_map_factors(case):
+    """The case with smooth non-unit map factors (a conformal-projection-like
+    field, 0.95..1.05), each staggering sampled at its own points and
+    msfvx_inv = 1 / msfvx in FP32: every division by a map factor then has a
+    divisor other than 1."""
+    A = case.arrays
+    jde, ip = A["msftx"].shape
+    cfg = case.config
+    nx, ny = cfg.nx, cfg.ny
+
+    def field(di, dj):
+        ii = np.arange(ip, dtype=np.float64)[None, :] + di
+        jj = np.arange(jde, dtype=np.float64)[:, None] + dj
+        return (1.0 + 0.05 * np.sin(2.0 * np.pi * ii / nx) * np.cos(np.pi * jj / ny) + 0.01 * jj / ny).astype(np.float32)
+
+    for key, (di, dj) in {"msftx": (0.0, 0.0), "msfty": (0.0, 0.0), "msfux": (-0.5, 0.0), "msfuy": (-0.5, 0.0),
+                          "msfvx": (0.0, -0.5), "msfvy": (0.0, -0.5)}.items():
+        A[key][...] = field(di, dj) * (1.0 if key.endswith("x") else np.float32(1.002))
+    A["msfvx_inv"][...] = np.float32(1.0) / A["msfvx"]
+    return case
+
+
+def test_one_step_parity_map_factors(domain_for):
+    """One RK3 step with non-unit map factors against the oracle: the kernels
+    divide by map factors through per-column reciprocals (div_rn), the
+    oracle with IEEE divisions; every field must agree."""
+    from oracle import Oracle
+
+    print("[step parity] 40x33x20 with map factors 0.95..1.05", flush=True)
+    case = _with_map_factors(_case(40, 33, 20, nmoist=6, nscalar=2, hydrometeors=True, hill_height=500.0))
+    dom = domain_for(case.config)
+    o = Oracle(case.config, case.arrays)
+    o.step(1, 0.0)
+    dom.upload(case.arrays)
+    dom.step(1, 0.0)
+    got = dom.download()
+    diff = {n: int(np.count_nonzero(np.asarray(got[n]) != np.asarray(o.arrays[n]))) for n in ("u_2", "v_2", "w_2", "t_2", "ph_2", "p", "al")}
+    print(f"[step parity] differing elements: {diff}", flush=True)
+    bad = compare(got, o.arrays, tol=REL_L2_STEP, skip=STEP_DIAGNOSTICS)
+    o.close()
+    assert not bad, f"fields above rel-L2 {REL_L2_STEP} after one RK3 step: {bad}"
+
+
 @pytest.mark.parametrize("shape", SHAPES, ids=[f"{a}x{b}x{c}" for a, b, c in SHAPES])
 def test_one_step_parity(domain_for, shape):
     from oracle import Oracle
