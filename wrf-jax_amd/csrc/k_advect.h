@@ -47,7 +47,13 @@ __device__ __forceinline__ float lane_next(float v, int lane) {
 #ifndef WRFDC_FACE_LG
 #define WRFDC_FACE_LG 4
 #endif
+#ifndef WRFDC_FACE_ROWS
+#define WRFDC_FACE_ROWS 4
+#endif
 constexpr unsigned kFaceLG = WRFDC_FACE_LG;  // levels per level group of the tile order
+// rows (waves) per face block: 4 measured fastest for every face-block kernel
+// at 400x400x60 (2 rows: step +8 %; 8 rows: step +2 %, k_species_pd_update +11 %)
+constexpr int kFaceRows = WRFDC_FACE_ROWS;
 constexpr unsigned kFaceLGPd = 8;            // k_species_pd_scale, k_species_pd_update
 constexpr unsigned kFaceLGW = 8;             // k_advect_w
 struct FaceTiles {
@@ -55,7 +61,8 @@ struct FaceTiles {
   __host__ __device__ unsigned count() const { return ni * nj * nkc * lg; }
 };
 __host__ __device__ inline FaceTiles face_tiles(const R3 &r, unsigned lg = kFaceLG) {
-  return FaceTiles{(unsigned)(r.i1 - r.i0 + 63) / 63u, (unsigned)(r.j1 - r.j0 + 4) / 4u, (unsigned)(r.k1 - r.k0 + (int)lg) / lg, lg};
+  return FaceTiles{(unsigned)(r.i1 - r.i0 + 63) / 63u, (unsigned)(r.j1 - r.j0 + kFaceRows) / (unsigned)kFaceRows,
+                   (unsigned)(r.k1 - r.k0 + (int)lg) / lg, lg};
 }
 // thread -> lane, wave (row in the block), column i, level k, row j and the
 // block's first row j0; own: the lane holds a point of r; ic: i clamped into
@@ -70,7 +77,7 @@ __host__ __device__ inline FaceTiles face_tiles(const R3 &r, unsigned lg = kFace
   const unsigned u_ = t_ / T_.ni;                                                        \
   const int i = (r).i0 + (int)(t_ % T_.ni) * 63 + lane;                                  \
   const int k = (r).k0 + (int)((u_ / (LG)) / T_.nj) * (int)(LG) + (int)(u_ % (LG));      \
-  const int j0 = (r).j0 + (int)((u_ / (LG)) % T_.nj) * 4;                                \
+  const int j0 = (r).j0 + (int)((u_ / (LG)) % T_.nj) * kFaceRows;                        \
   const int j = j0 + wv;                                                                 \
   if (k > (r).k1) return; /* whole block */                                              \
   const bool jrow = (j <= (r).j1);                                                       \
@@ -205,10 +212,10 @@ __device__ __forceinline__ float rvw_sel(const DevCtx *__restrict__ x, float r0,
 // TEND: the theta tendency of the RK step, from zero (the cleared array) plus
 // rk_addtend_dry, written once.
 template <bool TEND>
-__global__ void __launch_bounds__(256) k_advect_scalar(const DevCtx xv, const float *q, float *tend, const float *ru, const float *rv,
+__global__ void __launch_bounds__(64 * kFaceRows) k_advect_scalar(const DevCtx xv, const float *q, float *tend, const float *ru, const float *rv,
                                                        const float *rom, R3 r, int rk) {
   const DevCtx *__restrict__ x = &xv;
-  __shared__ float ys[4][64];
+  __shared__ float ys[kFaceRows][64];
   GET_FACE_BLOCK(r);
   const int ide = x->ide, jde = x->jde, kde = x->kde;
   const unsigned nb = (unsigned)x->n3, sj = (unsigned)(kde * x->ip), sk = (unsigned)x->ip;
@@ -235,7 +242,7 @@ __global__ void __launch_bounds__(256) k_advect_scalar(const DevCtx xv, const fl
   win5z(bq, o, sk, wz);
   const float uw = bld(bu, ox), wt = bld(bw, o + sk), wb = bld(bw, o);
   float vn = 0.f;
-  if (wv == 3) {
+  if (wv == kFaceRows - 1) {
     win6b(bq, o + sj, sj, wn);
     vn = bld(bv, o + sj);
   }
@@ -245,13 +252,13 @@ __global__ void __launch_bounds__(256) k_advect_scalar(const DevCtx xv, const fl
   const float t0 = TEND ? 0.f : elem(tend, o);
   pin6(wx);
   pin5(wz);
-  if (wv == 3) pin6(wn);
+  if (wv == kFaceRows - 1) pin6(wn);
   const float fw0 = hflux_br(wx, uw, i, ide);
   const float fw = (i >= 2 && i <= ide - 1) ? fw0 : 0.f;
   const float fe = lane_next(fw, lane);
   if (!own) return;
   float fn;
-  if (wv < 3) fn = ys[wv + 1][lane];
+  if (wv < kFaceRows - 1) fn = ys[wv + 1][lane];
   else fn = (j + 1 <= jde - 1) ? hflux_br(wn, vn, j + 1, jde) : 0.f;
   float t = t0;
   if (j >= 2 && j <= jde - 2) t -= msf * x->rdy * (fn - fs);
@@ -271,10 +278,10 @@ __global__ void __launch_bounds__(256) k_advect_scalar(const DevCtx xv, const fl
 // a column clamped into [2, ide-1], so their loads stay in the field) and
 // used where they apply.
 template <bool TEND>
-__global__ void __launch_bounds__(256) k_advect_u(const DevCtx xv, const float *u, float *tend, const float *ru, const float *rv,
+__global__ void __launch_bounds__(64 * kFaceRows) k_advect_u(const DevCtx xv, const float *u, float *tend, const float *ru, const float *rv,
                                                   const float *rom, R3 r, int rk) {
   const DevCtx *__restrict__ x = &xv;
-  __shared__ float ys[4][64];
+  __shared__ float ys[kFaceRows][64];
   GET_FACE_BLOCK(r);
   const int ide = x->ide, jde = x->jde, kde = x->kde;
   const unsigned nb = (unsigned)x->n3, sj = (unsigned)(kde * x->ip), sk = (unsigned)x->ip;
@@ -299,7 +306,7 @@ __global__ void __launch_bounds__(256) k_advect_u(const DevCtx xv, const float *
   const float rus = bld(bu, ox) + bld(bu, ox - 1u);
   const float romt = bld(bw, o + sk) + bld(bw, o + sk - 1u), romb = bld(bw, o) + bld(bw, o - 1u);
   float rvn = 0.f;
-  if (wv == 3) {
+  if (wv == kFaceRows - 1) {
     win6b(bq, o + sj, sj, wn);
     rvn = bld(bv, o + sj) + bld(bv, o + sj - 1u);
   }
@@ -322,7 +329,7 @@ __global__ void __launch_bounds__(256) k_advect_u(const DevCtx xv, const float *
   const float fe = lane_next(fw, lane);
   if (!own) return;
   float fn;
-  if (wv < 3) fn = ys[wv + 1][lane];
+  if (wv < kFaceRows - 1) fn = ys[wv + 1][lane];
   else fn = (adv && j + 1 <= jde - 1) ? fy_u_sel(wn, rvn, j + 1, jde, j + 1 >= 4 && j + 1 <= jde - 3) : 0.f;
   float t = t0;
   if (adv) {
@@ -346,10 +353,10 @@ __global__ void __launch_bounds__(256) k_advect_u(const DevCtx xv, const float *
 // gradient, Coriolis/curvature at j [2, jde-1], then rk_addtend_dry); the
 // pressure-gradient and Coriolis terms at a row clamped into [2, jde-1].
 template <bool TEND>
-__global__ void __launch_bounds__(256) k_advect_v(const DevCtx xv, const float *v, float *tend, const float *ru, const float *rv,
+__global__ void __launch_bounds__(64 * kFaceRows) k_advect_v(const DevCtx xv, const float *v, float *tend, const float *ru, const float *rv,
                                                   const float *rom, R3 r, int rk) {
   const DevCtx *__restrict__ x = &xv;
-  __shared__ float ys[4][64];
+  __shared__ float ys[kFaceRows][64];
   GET_FACE_BLOCK(r);
   const int ide = x->ide, jde = x->jde, kde = x->kde;
   const unsigned nb = (unsigned)x->n3, sj = (unsigned)(kde * x->ip), sk = (unsigned)x->ip;
@@ -374,7 +381,7 @@ __global__ void __launch_bounds__(256) k_advect_v(const DevCtx xv, const float *
   const float rus = bld(bu, ox) + bld(bu, ox - sj);
   const float romt = bld(bw, o + sk) + bld(bw, o + sk - sj), romb = bld(bw, o) + bld(bw, o - sj);
   float rvn = 0.f;
-  if (wv == 3) {
+  if (wv == kFaceRows - 1) {
     win6b(bq, o + sj, sj, wn);
     rvn = bld(bv, o + sj) + bld(bv, o);
   }
@@ -398,7 +405,7 @@ __global__ void __launch_bounds__(256) k_advect_v(const DevCtx xv, const float *
   const float fe = lane_next(fw, lane);
   if (!own) return;
   float fn;
-  if (wv < 3) fn = ys[wv + 1][lane];
+  if (wv < kFaceRows - 1) fn = ys[wv + 1][lane];
   else fn = (j + 1 >= 2 && j + 1 <= jde) ? fy_v_sel(wn, rvn, j + 1, jde, j + 1 >= 4 && j + 1 <= jde - 2) : 0.f;
   float t = t0;
   if (adv) {
@@ -423,10 +430,10 @@ __global__ void __launch_bounds__(256) k_advect_v(const DevCtx xv, const float *
 // the top, Coriolis/curvature at k [2, kde]; then rk_addtend_dry.  Buoyancy
 // and Coriolis terms at a level clamped into the range they apply to.
 template <bool TEND>
-__global__ void __launch_bounds__(256) k_advect_w(const DevCtx xv, const float *w, float *tend, const float *ru, const float *rv,
+__global__ void __launch_bounds__(64 * kFaceRows) k_advect_w(const DevCtx xv, const float *w, float *tend, const float *ru, const float *rv,
                                                   const float *rom, R3 r, int rk) {
   const DevCtx *__restrict__ x = &xv;
-  __shared__ float ys[4][64];
+  __shared__ float ys[kFaceRows][64];
   GET_FACE_BLOCK_LG(r, kFaceLGW);
   const int ide = x->ide, jde = x->jde, kde = x->kde;
   const unsigned nb = (unsigned)x->n3, sj = (unsigned)(kde * x->ip), sk = (unsigned)x->ip;
@@ -451,7 +458,7 @@ __global__ void __launch_bounds__(256) k_advect_w(const DevCtx xv, const float *
   const float uw = rvw_sel(x, bld(bu, ox), bld(bu, ox - sk), bld(bu, ox - 2u * sk), k);
   const float rp = bld(bw, o + sk), r0 = bld(bw, o), rm = bld(bw, o - sk);
   float vn = 0.f;
-  if (wv == 3) {
+  if (wv == kFaceRows - 1) {
     win6b(bq, o + sj, sj, wn);
     vn = rvw_sel(x, bld(bv, o + sj), bld(bv, o + sj - sk), bld(bv, o + sj - 2u * sk), k);
   }
@@ -473,7 +480,7 @@ __global__ void __launch_bounds__(256) k_advect_w(const DevCtx xv, const float *
   const float fe = lane_next(fw, lane);
   if (!own) return;
   float fn;
-  if (wv < 3) fn = ys[wv + 1][lane];
+  if (wv < kFaceRows - 1) fn = ys[wv + 1][lane];
   else fn = (adv && j + 1 <= jde - 1) ? hflux_sel(wn, vn, j + 1, jde, j + 1 >= 4 && j + 1 <= jde - 3) : 0.f;
   float t = t0;
   if (adv) {

@@ -55,7 +55,7 @@ __device__ __forceinline__ UpdCoef upd_coef(const DevCtx *__restrict__ x, int i,
 // plain (non-PD) stages: advection with the time-averaged mass fluxes and
 // the update, all species of the list.  At rk = 1 `in` is the stage-start
 // copy q_old (equal to q there), so old = in.  Face-sharing blocks (block
-// (64, 4), 1-D grid of face_tiles(r).count() blocks): the south faces of all
+// (64, kFaceRows), 1-D grid of face_tiles(r).count() blocks): the south faces of all
 // species go to LDS first, one barrier, then each species' update.
 //
 // At rk = 2 the species of L.prep_mask also get the last stage's
@@ -65,11 +65,11 @@ __device__ __forceinline__ UpdCoef upd_coef(const DevCtx *__restrict__ x, int i,
 // c2h), so the last stage finds the same q_old and a cleared tendency.  Each
 // thread reads q_old and the tendency at its own point only, so updating them
 // in place is race-free.
-__global__ void __launch_bounds__(256) k_species_update(const DevCtx xv, SpecList L, int rk, float dt, R3 r) {
+__global__ void __launch_bounds__(64 * kFaceRows) k_species_update(const DevCtx xv, SpecList L, int rk, float dt, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  extern __shared__ float ys_[];  // L.n x 4 x 64: south face flux of every species
+  extern __shared__ float ys_[];  // L.n x kFaceRows x 64: south face flux of every species
   GET_FACE_BLOCK(r);
-#define YS(s, w) ys_[((s) * 4 + (w)) * 64 + lane]
+#define YS(s, w) ys_[((s) * kFaceRows + (w)) * 64 + lane]
   const int ide = x->ide, jde = x->jde, kde = x->kde;
   const float *ru = FL(ru_m), *rv = FL(rv_m), *rom = FL(ww_m);
   const unsigned nb = (unsigned)x->n3, sj = (unsigned)(kde * x->ip), sk = (unsigned)x->ip;
@@ -89,7 +89,7 @@ __global__ void __launch_bounds__(256) k_species_update(const DevCtx xv, SpecLis
   __syncthreads();
   if (!jrow) return;
   const bool xface = (i >= 2 && i <= ide - 1);
-  const bool yface_n = (wv == 3 && j + 1 <= jde - 1);
+  const bool yface_n = (wv == kFaceRows - 1 && j + 1 <= jde - 1);
   const bool doy = (j >= 2 && j <= jde - 2), dox = (ic >= 2 && ic <= ide - 2);
   const float uw = xface ? A3(ru, i, k, j) : 0.f;
   const float vn = yface_n ? A3(rv, ic, k, j + 1) : 0.f;
@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(256) k_species_update(const DevCtx xv, SpecLis
     win5z(bq, o, sk, zw);
     const float tnd = elem(L.tend[s], o);
     const float old = elem(rk == 1 ? L.in[s] : L.old[s], o);
-    if (wv == 3) {
+    if (wv == kFaceRows - 1) {
       win6b(bq, o + sj, sj, yw);
       pin6(yw);
     }
@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(256) k_species_update(const DevCtx xv, SpecLis
     const float fe = lane_next(fw, lane);
     const float fs = YS(s, wv);
     float fn;
-    if (wv < 3) fn = YS(s, wv + 1);
+    if (wv < kFaceRows - 1) fn = YS(s, wv + 1);
     else fn = yface_n ? hflux_br(yw, vn, j + 1, jde) : 0.f;
     float adv = 0.f;
     if (doy) adv -= msftx * x->rdy * (fn - fs);
@@ -227,12 +227,12 @@ __device__ __forceinline__ void pd_vflux_sel(const PdFace &f, float qm2, float q
 // Face-sharing blocks as k_species_update: the west face comes from this
 // lane, the east face from the next lane, the south face from this wave and
 // the north face from the wave above (LDS).
-__global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecList L, float dt, R3 r) {
+__global__ void __launch_bounds__(64 * kFaceRows) k_species_pd_scale(const DevCtx xv, SpecList L, float dt, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  extern __shared__ float yal_[];  // 2 x L.n x 4 x 64: south antidiffusive / upwind flux of every species
+  extern __shared__ float yal_[];  // 2 x L.n x kFaceRows x 64: south antidiffusive / upwind flux of every species
   GET_FACE_BLOCK_LG(r, kFaceLGPd);
-#define YA(s, w) yal_[((s) * 4 + (w)) * 64 + lane]
-#define YL(s, w) yal_[((L.n + (s)) * 4 + (w)) * 64 + lane]
+#define YA(s, w) yal_[((s) * kFaceRows + (w)) * 64 + lane]
+#define YL(s, w) yal_[((L.n + (s)) * kFaceRows + (w)) * 64 + lane]
   const int ide = x->ide, jde = x->jde, kde = x->kde;
   const unsigned nb = (unsigned)x->n3, sj = (unsigned)(kde * x->ip), sk = (unsigned)x->ip;
   {
@@ -260,7 +260,7 @@ __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecL
   const float mm = A2(FL(msftx), ic, j) * A2(FL(msfty), ic, j), msfty = A2(FL(msfty), ic, j);
   const float mold = (A1(c1, k) * A2(FL(mub), ic, j) + A1(c2, k)) + (A1(c1, k) * A2(FL(mu_1), ic, j));
   const PdFace fx = pd_face_geom(x, 0, i, k, j, dt);
-  const PdFace fyn = (wv == 3) ? pd_face_geom(x, 1, ic, k, j + 1, dt) : PdFace{0.f, 0.f, 0.f, 0};
+  const PdFace fyn = (wv == kFaceRows - 1) ? pd_face_geom(x, 1, ic, k, j + 1, dt) : PdFace{0.f, 0.f, 0.f, 0};
   const PdFace fz0 = pd_face_geom(x, 2, ic, k, j, dt), fz1 = pd_face_geom(x, 2, ic, k + 1, j, dt);
   const int k1 = imin_(k + 1, kde);
   const float fm0 = A1(FL(fnm), k), fp0 = A1(FL(fnp), k), fm1 = A1(FL(fnm), k1), fp1 = A1(FL(fnp), k1);
@@ -277,7 +277,7 @@ __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecL
 #pragma unroll
     for (int n = 0; n < 3; n++) zo[n] = bld(bo, o + (unsigned)(n - 1) * sk);
     float yon = 0.f;
-    if (wv == 3) {
+    if (wv == kFaceRows - 1) {
       win6b(bq, o + sj, sj, yw);
       yon = bld(bo, o + sj);
       pin6(yw);
@@ -293,7 +293,7 @@ __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecL
       continue;
     }
     const float ay0 = YA(s, wv), ly0 = YL(s, wv);
-    if (wv < 3) {
+    if (wv < kFaceRows - 1) {
       ay1 = YA(s, wv + 1);
       ly1 = YL(s, wv + 1);
     } else {
@@ -320,12 +320,12 @@ __global__ void __launch_bounds__(256) k_species_pd_scale(const DevCtx xv, SpecL
 // fluxes are formed again as in pass 1 (face-sharing blocks over the mass
 // points, the same loads and arithmetic, so the same values); each
 // antidiffusive flux is scaled by the factor of the cell it leaves.
-__global__ void __launch_bounds__(256) k_species_pd_update(const DevCtx xv, SpecList L, float dt, R3 r) {
+__global__ void __launch_bounds__(64 * kFaceRows) k_species_pd_update(const DevCtx xv, SpecList L, float dt, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  extern __shared__ float yal_[];  // 2 x L.n x 4 x 64: south antidiffusive / upwind flux of every species
+  extern __shared__ float yal_[];  // 2 x L.n x kFaceRows x 64: south antidiffusive / upwind flux of every species
   GET_FACE_BLOCK_LG(r, kFaceLGPd);
-#define YA(s, w) yal_[((s) * 4 + (w)) * 64 + lane]
-#define YL(s, w) yal_[((L.n + (s)) * 4 + (w)) * 64 + lane]
+#define YA(s, w) yal_[((s) * kFaceRows + (w)) * 64 + lane]
+#define YL(s, w) yal_[((L.n + (s)) * kFaceRows + (w)) * 64 + lane]
   const int kde = x->kde;
   const unsigned nb = (unsigned)x->n3, sj = (unsigned)(kde * x->ip), sk = (unsigned)x->ip;
   {
@@ -349,7 +349,7 @@ __global__ void __launch_bounds__(256) k_species_pd_update(const DevCtx xv, Spec
   const float rdx = x->rdx, rdy = x->rdy, rdzw = A1(FL(rdnw), k), msftx = A2(FL(msftx), ic, j);
   const bool up_ok = (k + 1 <= kde - 1), dn_ok = (k - 1 >= 1);
   const PdFace fx = pd_face_geom(x, 0, i, k, j, dt);
-  const PdFace fyn = (wv == 3) ? pd_face_geom(x, 1, ic, k, j + 1, dt) : PdFace{0.f, 0.f, 0.f, 0};
+  const PdFace fyn = (wv == kFaceRows - 1) ? pd_face_geom(x, 1, ic, k, j + 1, dt) : PdFace{0.f, 0.f, 0.f, 0};
   const PdFace fz0 = pd_face_geom(x, 2, ic, k, j, dt), fz1 = pd_face_geom(x, 2, ic, k + 1, j, dt);
   const int k1 = imin_(k + 1, kde);
   const float fm0 = A1(FL(fnm), k), fp0 = A1(FL(fnp), k), fm1 = A1(FL(fnm), k1), fp1 = A1(FL(fnp), k1);
@@ -362,7 +362,7 @@ __global__ void __launch_bounds__(256) k_species_pd_update(const DevCtx xv, Spec
 #pragma unroll
     for (int n = 0; n < 3; n++) zo[n] = bld(bo, o + (unsigned)(n - 1) * sk);
     float yon = 0.f;
-    if (wv == 3) {
+    if (wv == kFaceRows - 1) {
       win6b(bq, o + sj, sj, yw);
       yon = bld(bo, o + sj);
       pin6(yw);
@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(256) k_species_pd_update(const DevCtx xv, Spec
       continue;
     }
     const float ay0 = YA(s, wv), ly0 = YL(s, wv);
-    if (wv < 3) {
+    if (wv < kFaceRows - 1) {
       ay1 = YA(s, wv + 1);
       ly1 = YL(s, wv + 1);
     } else {

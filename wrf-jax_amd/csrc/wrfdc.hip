@@ -253,6 +253,7 @@ struct Run {
   static dim3 g3p(const R3 &r) { return dim3(cdiv(r.i1 - r.i0 + 1, 64) * cdiv(r.k1 - r.k0 + 1, 4) * (unsigned)(r.j1 - r.j0 + 1)); }
   static dim3 g2(const R2 &r) { return dim3(cdiv(r.i1 - r.i0 + 1, 64) * (unsigned)(r.j1 - r.j0 + 1)); }
   static dim3 b3() { return dim3(64, 4, 1); }
+  static dim3 bf() { return dim3(64, kFaceRows, 1); }  // face-sharing blocks
 
   void fill(float *p, size_t n, float v = 0.f) const {
     unsigned nb = (unsigned)((n + 255) / 256);
@@ -566,19 +567,19 @@ struct Run {
   }
   void advect_u() const {
     R3 r{2, ide - 1, 1, kde - 1, 1, jde - 1};
-    k_advect_u<false><<<dim3(face_tiles(r).count()), b3(), 0, s>>>(d, F(F_u_2), F(F_ru_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
+    k_advect_u<false><<<dim3(face_tiles(r).count()), bf(), 0, s>>>(d, F(F_u_2), F(F_ru_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
   }
   void advect_v() const {
     R3 r{1, ide - 1, 1, kde - 1, 2, jde - 1};
-    k_advect_v<false><<<dim3(face_tiles(r).count()), b3(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
+    k_advect_v<false><<<dim3(face_tiles(r).count()), bf(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
   }
   void advect_w() const {
     R3 r{1, ide - 1, 2, kde, 1, jde - 1};
-    k_advect_w<false><<<dim3(face_tiles(r, kFaceLGW).count()), b3(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
+    k_advect_w<false><<<dim3(face_tiles(r, kFaceLGW).count()), bf(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
   }
   void advect_scalar(const float *q, float *tend, const float *ru, const float *rv, const float *rom) const {
     R3 m = mass3();
-    k_advect_scalar<false><<<dim3(face_tiles(m).count()), b3(), 0, s>>>(d, q, tend, ru, rv, rom, m, 0);
+    k_advect_scalar<false><<<dim3(face_tiles(m).count()), bf(), 0, s>>>(d, q, tend, ru, rv, rom, m, 0);
   }
   void advect_scalar_pd(const float *q, const float *q_old, float *tend, double dt) const {
     R3 r = full3();
@@ -646,15 +647,15 @@ struct Run {
     clear_tendencies(rk == 1);
     if (rk == 1) lateral_relax();
     const R3 ru{1, ide, 1, kde - 1, 1, jde - 1};
-    k_advect_u<true><<<dim3(face_tiles(ru).count()), b3(), 0, s>>>(d, F(F_u_2), F(F_ru_tend), F(F_ru), F(F_rv), F(F_ww), ru, rk);
+    k_advect_u<true><<<dim3(face_tiles(ru).count()), bf(), 0, s>>>(d, F(F_u_2), F(F_ru_tend), F(F_ru), F(F_rv), F(F_ww), ru, rk);
     const R3 rv{1, ide - 1, 1, kde - 1, 1, jde};
-    k_advect_v<true><<<dim3(face_tiles(rv).count()), b3(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), rv, rk);
+    k_advect_v<true><<<dim3(face_tiles(rv).count()), bf(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), rv, rk);
     const R3 rw{1, ide - 1, 1, kde, 1, jde - 1};
-    k_advect_w<true><<<dim3(face_tiles(rw, kFaceLGW).count()), b3(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), rw, rk);
+    k_advect_w<true><<<dim3(face_tiles(rw, kFaceLGW).count()), bf(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), rw, rk);
     const R3 rq{1, ide - 1, kde - 1, kde - 1, 1, jde - 1};  // levels [2, kde-2]: k_advect_w<true>
     k_cqw_inv<<<g3p(rq), b3(), 0, s>>>(d, rq);
     const R3 m = mass3();
-    k_advect_scalar<true><<<dim3(face_tiles(m).count()), b3(), 0, s>>>(d, F(F_t_2), F(F_t_tend), F(F_ru), F(F_rv), F(F_ww), m, rk);
+    k_advect_scalar<true><<<dim3(face_tiles(m).count()), bf(), 0, s>>>(d, F(F_t_2), F(F_t_tend), F(F_ru), F(F_rv), F(F_ww), m, rk);
     k_tend_ph<<<g3(rw), b3(), 0, s>>>(d, rw, rk);
     spec_overlays();
   }
@@ -912,10 +913,10 @@ struct Run {
         for (int q = 0; q < pd.n; q++) bc.add3(pd.old[q], 'p');
       }
       R3 r = full3();
-      k_species_pd_scale<<<dim3(face_tiles(r, kFaceLGPd).count()), b3(), sizeof(float) * 2 * 256 * pd.n, s>>>(d, pd, dt, r);
-      k_species_pd_update<<<dim3(face_tiles(m, kFaceLGPd).count()), b3(), sizeof(float) * 2 * 256 * pd.n, s>>>(d, pd, dt, m);
+      k_species_pd_scale<<<dim3(face_tiles(r, kFaceLGPd).count()), bf(), sizeof(float) * 2 * 64 * kFaceRows * pd.n, s>>>(d, pd, dt, r);
+      k_species_pd_update<<<dim3(face_tiles(m, kFaceLGPd).count()), bf(), sizeof(float) * 2 * 64 * kFaceRows * pd.n, s>>>(d, pd, dt, m);
     }
-    if (plain.n > 0) k_species_update<<<dim3(face_tiles(m).count()), b3(), sizeof(float) * 256 * plain.n, s>>>(d, plain, rk, dt, m);
+    if (plain.n > 0) k_species_update<<<dim3(face_tiles(m).count()), bf(), sizeof(float) * 64 * kFaceRows * plain.n, s>>>(d, plain, rk, dt, m);
     // outputs become the current species
     use_species(out_alt);
     if (rk == 3) {
