@@ -188,8 +188,8 @@ def shard_form(steps: int, plain_value: float, warmup: int = 2) -> dict:
         torch.cuda.synchronize()
         ms = ev0.elapsed_time(ev1) / steps
         bad = int(dom.checksums()[3])
+        nz = dom.config.nz
         dom.close()
-        nz = 60
         owned, computed = ox * oy * nz, nx * ny * nz
         out[name] = {"domain": f"{nx}x{ny}x{nz}", "owned": f"{ox}x{oy}x{nz}", "ms_per_step": ms,
                      "value": owned / (ms * 1e-3) / 1e6, "computed_value": computed / (ms * 1e-3) / 1e6,
