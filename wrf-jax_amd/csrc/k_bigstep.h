@@ -384,7 +384,6 @@ __device__ __forceinline__ float buoy_w_term(const DevCtx *__restrict__ x, int i
   const float mid = gm * (cq1 * A1(FL(rdn), kc) * (pk - pkm) - (A1(c1f, k) * A2(mu, i, j)) - cq2 * (A1(c1f, k) * A2(mub, i, j) + A1(c2f, k)));
   return k == kde ? top : mid;
 }
-__device__ __forceinline__ float buoy_w(const DevCtx *__restrict__ x, int i, int k, int j, float rwt) { return rwt + buoy_w_term(x, i, k, j); }
 // cqw := 1 / (1 + cqw) (the second half of pg_buoy_w): levels [2, kde-1] in
 // the stage API; in the fused step only level kde-1 (the w kernel's top
 // level reads it, so the fused w kernel replaces levels [2, kde-2] itself)
