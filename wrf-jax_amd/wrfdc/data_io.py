@@ -74,9 +74,6 @@ class BoundaryForcing:
     names: tuple
     arrays: tuple
 
-    def record_count(self) -> int:
-        return len(self.times) if self.times else (int(self.arrays[0].shape[0]) if self.arrays else 0)
-
 
 @dataclass(frozen=True)
 class CaseData:
