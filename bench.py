@@ -78,7 +78,7 @@ def _tile_grid(n: int) -> tuple[int, int]:
     return n // px, px
 
 
-def cpu_baseline(workload: str, nsteps: int = 4) -> dict:
+def cpu_baseline(workload: str, nsteps: int = 12) -> dict:
     """The CPU oracle (FP32, OpenMP build) on all the host cores this process
     may use: the workload's domain cut into one tile per thread, each tile a
     domain of its own with the workload's spacing, species and vertical grid
