@@ -100,6 +100,33 @@ def test_stage_parity(stepped_state, domain_for, name, rk, it):
 SHAPES = [(28, 24, 12), (13, 17, 7), (40, 33, 20)]
 
 
+SPECIES_MIXES = [(1, 0, 1), (1, 0, 0), (3, 1, 0), (2, 3, 1)]
+
+
+@pytest.mark.parametrize("nmoist,nscalar,pd", SPECIES_MIXES, ids=[f"m{a}s{b}pd{c}" for a, b, c in SPECIES_MIXES])
+def test_one_step_parity_species_mixes(domain_for, nmoist, nscalar, pd):
+    """One RK3 step at 28x24x12 with other species counts and the
+    positive-definite advection on or off: the species kernels' lists, the
+    second stage's fold of the last stage's q_old update (PD only) and the
+    diffusion of every scalar follow the configuration."""
+    import dataclasses
+
+    from oracle import Oracle
+
+    print(f"[step parity] 28x24x12, {nmoist} moist + {nscalar} scalars, PD {pd}", flush=True)
+    case = _case(28, 24, 12, nmoist=nmoist, nscalar=nscalar, hydrometeors=nmoist > 1, hill_height=500.0)
+    cfg = dataclasses.replace(case.config, pd_moist=pd, pd_scalar=pd)
+    dom = domain_for(cfg)
+    o = Oracle(cfg, case.arrays)
+    o.step(1, 0.0)
+    dom.upload(case.arrays)
+    dom.step(1, 0.0)
+    got = dom.download()
+    bad = compare(got, o.arrays, tol=REL_L2_STEP, skip=STEP_DIAGNOSTICS)
+    o.close()
+    assert not bad, f"fields above rel-L2 {REL_L2_STEP} after one RK3 step: {bad}"
+
+
 def _with_map_factors(case):
     """The case with smooth non-unit map factors (a conformal-projection-like
     field, 0.95..1.05), each staggering sampled at its own points and
