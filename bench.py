@@ -116,7 +116,7 @@ def batched_form(steps: int, plain_value: float) -> dict:
     """32 seeded members of uneven domains (100^2..300^2 columns, 45..60
     levels, dx 1..9 km, 6 moist species), one RK3 step of all of them
     captured in one HIP graph over concurrent streams, the stream layout
-    chosen by the autotuner; value = sum of N_mass x steps / s."""
+    and the face-block tile shape chosen by the autotuner; value = sum of N_mass x steps / s."""
     import torch
 
     from wrfdc.device import Domain
@@ -134,7 +134,9 @@ def batched_form(steps: int, plain_value: float) -> dict:
     nm = n_mass(doms)
     _log(f"batched: {len(doms)} members, {nm / 1e6:.1f} M mass points, built in {time.perf_counter() - t0:.1f} s")
     eager = eager_step_time(doms, reps=2)
-    nb, assign, table = autotune(doms, candidates=(1, 2, 4, 8, 16), reps=2, log=lambda m: _log("batched autotune: " + m))
+    nb, assign, table, tiles = autotune(doms, candidates=(1, 2, 4, 8, 16), reps=2,
+                                        log=lambda m: _log("batched autotune: " + m))
+    face_levels = min(tiles, key=lambda e: e[1])[0]
     ens = Ensemble(doms, assign, nb)
     ens.step(1, 0.0)
     ens.synchronize()
@@ -152,7 +154,8 @@ def batched_form(steps: int, plain_value: float) -> dict:
     return {"value": value, "unit": "Mpoint-steps/s", "ms_per_step": 1e3 * sec / steps, "steps": steps, "n_mass": nm,
             "members": len(doms), "streams": nb, "eager_ms_per_step": 1e3 * eager, "graph_vs_eager": eager / (sec / steps),
             "vs_plain": value / plain_value, "nonfinite": bad,
-            "autotune_ms": {str(b): 1e3 * t for b, t in table},
+            "face_levels": face_levels, "autotune_ms": {str(b): 1e3 * t for b, t in table},
+            "autotune_face_levels_ms": {str(lg): 1e3 * t for lg, t in tiles},
             "workload": "32 seeded members, 100^2..300^2 x 45..60, dx 1..9 km, 6 moist species, PD on, FP32, "
                         "one captured RK3 step per replay"}
 

@@ -103,6 +103,15 @@ int wrfdc_stage(wrfdc_ctx *ctx, const char *name, int rk_step, int iteration);
  * calls read it (wrfdc_step sets it to dtbc0 + (n+1) dt itself). */
 int wrfdc_set_dtbc(wrfdc_ctx *ctx, double dtbc);
 int wrfdc_checksums(wrfdc_ctx *ctx, double out[WRFDC_NCHECKSUMS]);
+/* No counterpart in the reference (the batched ensemble's autotuner uses
+ * it): a launch-shape setting of the context.  "face_levels": levels per
+ * level group in the tile order of the face-sharing stencil kernels
+ * (advection, species), 1..32, or 0 for the per-kernel defaults (4, and 8
+ * for advect_w and the positive-definite species passes).  Results do not
+ * depend on it; it takes effect at the next wrfdc_step / wrfdc_stage (and is
+ * baked into a captured ensemble graph).  WRFDC_EINVAL for another name or
+ * value. */
+int wrfdc_set_tile(wrfdc_ctx *ctx, const char *name, int value);
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory or wait on the host (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc): the checksums of wrfdc_checksums

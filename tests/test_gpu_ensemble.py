@@ -5,7 +5,7 @@ uneven members, csrc/wrfdc.hip wrfdc_ensemble_*).
   does (bit-identical fields), also with time-varying lateral forcing, where
   each replay must see the current boundary time;
 * every member matches the FP32 oracle after the replays (relative L2 1e-5);
-* the autotuner returns a valid layout.
+* the autotuner returns a valid layout and tile shape.
 """
 
 import numpy as np
@@ -80,11 +80,13 @@ def test_graph_replay_matches_members_stepped_alone(gpu):
 def test_autotune_returns_a_layout(gpu):
     from wrfdc.ensemble import Ensemble, autotune
 
-    print("[ensemble] autotune over 1, 2, 4 streams", flush=True)
+    print("[ensemble] autotune over 1, 2, 4 streams and face_levels 0, 2, 16", flush=True)
     doms = _domains(_cases(), gpu)
-    nb, assign, table = autotune(doms, candidates=(1, 2, 4), reps=2, log=lambda s: print("[ensemble]", s, flush=True))
+    nb, assign, table, tiles = autotune(doms, candidates=(1, 2, 4), reps=2, face_levels=(0, 2, 16),
+                                        log=lambda s: print("[ensemble]", s, flush=True))
     assert nb in (1, 2, 4) and len(assign) == len(doms) and all(0 <= b < nb for b in assign)
     assert len(table) == 3 and all(t > 0 for _, t in table)
+    assert [lg for lg, _ in tiles] == [0, 2, 16] and all(t > 0 for _, t in tiles)
     ens = Ensemble(doms, assign, nb)
     ens.step(1)
     ens.synchronize()

@@ -127,6 +127,36 @@ def test_one_step_parity_species_mixes(domain_for, nmoist, nscalar, pd):
     assert not bad, f"fields above rel-L2 {REL_L2_STEP} after one RK3 step: {bad}"
 
 
+@pytest.mark.parametrize("levels", [1, 3, 16, 32])
+def test_face_levels_do_not_change_results(gpu, levels):
+    """The face-block kernels' level-group size (wrfdc_set_tile) is a launch
+    shape only: two RK3 steps at 30x26x13 (13 levels: ragged for every
+    group size above 1) give bit-identical fields to the default shapes, and
+    out-of-range settings are refused."""
+    from wrfdc.device import Domain
+
+    print(f"[tile] face_levels={levels} vs default, 30x26x13, 2 steps", flush=True)
+    case = _case(30, 26, 13, nmoist=3, nscalar=1, hydrometeors=True, hill_height=500.0)
+    ref, dom = Domain(case.config, gpu), Domain(case.config, gpu)
+    try:
+        for d in (ref, dom):
+            d.upload(case.arrays)
+        dom.set_tile("face_levels", levels)
+        ref.step(2, 0.0)
+        dom.step(2, 0.0)
+        a, b = ref.download(), dom.download()
+        diff = [n for n in a if not np.array_equal(a[n].view(np.uint32), b[n].view(np.uint32))]
+        assert not diff, f"fields that differ with face_levels={levels}: {diff}"
+        for bad in (-1, 33):
+            with pytest.raises(ValueError, match="face_levels"):
+                dom.set_tile("face_levels", bad)
+        with pytest.raises(ValueError, match="unknown tile setting"):
+            dom.set_tile("no_such_setting", 4)
+    finally:
+        ref.close()
+        dom.close()
+
+
 def _with_map_factors(case):
     """The case with smooth non-unit map factors (a conformal-projection-like
     field, 0.95..1.05), each staggering sampled at its own points and

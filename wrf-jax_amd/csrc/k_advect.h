@@ -41,21 +41,22 @@ __device__ __forceinline__ float lane_next(float v, int lane) {
 // Tile order: i fastest, then LG consecutive levels, then rows, then groups
 // of LG levels, XCD-contiguous: the neighbour rows of a y stencil and the
 // neighbour levels of a z stencil are read by the same XCD shortly before.
-// LG is 4 for most kernels; the PD species passes and advect_w take 8, which
-// measured faster for them (each kernel timed at 400x400x60 with LG 2, 4, 8
-// and 16; the other kernels are fastest at 4).
+// LG is a launch setting of the domain (DevCtx::face_lg, wrfdc_set_tile):
+// by default 4 for most kernels and 8 for the PD species passes and
+// advect_w, which measured fastest at 400x400x60 (each kernel timed with LG
+// 2, 4, 8 and 16); the ensemble's autotuner tries others per layout.
 #ifndef WRFDC_FACE_LG
 #define WRFDC_FACE_LG 4
 #endif
 #ifndef WRFDC_FACE_ROWS
 #define WRFDC_FACE_ROWS 4
 #endif
-constexpr unsigned kFaceLG = WRFDC_FACE_LG;  // levels per level group of the tile order
+constexpr unsigned kFaceLG = WRFDC_FACE_LG;  // default levels per level group of the tile order
 // rows (waves) per face block: 4 measured fastest for every face-block kernel
 // at 400x400x60 (2 rows: step +8 %; 8 rows: step +2 %, k_species_pd_update +11 %)
 constexpr int kFaceRows = WRFDC_FACE_ROWS;
-constexpr unsigned kFaceLGPd = 8;            // k_species_pd_scale, k_species_pd_update
-constexpr unsigned kFaceLGW = 8;             // k_advect_w
+constexpr unsigned kFaceLGPd = 8;            // default for k_species_pd_scale, k_species_pd_update
+constexpr unsigned kFaceLGW = 8;             // default for k_advect_w
 struct FaceTiles {
   unsigned ni, nj, nkc, lg;  // tiles along i, row groups, level groups, levels per group
   __host__ __device__ unsigned count() const { return ni * nj * nkc * lg; }
@@ -83,7 +84,7 @@ __host__ __device__ inline FaceTiles face_tiles(const R3 &r, unsigned lg = kFace
   const bool jrow = (j <= (r).j1);                                                       \
   const bool own = (lane < 63 && i <= (r).i1 && jrow);                                   \
   const int ic = i <= (r).i1 ? i : (r).i1
-#define GET_FACE_BLOCK(r) GET_FACE_BLOCK_LG(r, kFaceLG)
+#define GET_FACE_BLOCK(r) GET_FACE_BLOCK_LG(r, (unsigned)x->face_lg[0])
 
 // ---------------- face fluxes from loaded stencils ---------------------------
 // Every kernel below issues all loads of a point first (buffer loads,
@@ -434,7 +435,7 @@ __global__ void __launch_bounds__(64 * kFaceRows) k_advect_w(const DevCtx xv, co
                                                   const float *rom, R3 r, int rk) {
   const DevCtx *__restrict__ x = &xv;
   __shared__ float ys[kFaceRows][64];
-  GET_FACE_BLOCK_LG(r, kFaceLGW);
+  GET_FACE_BLOCK_LG(r, (unsigned)x->face_lg[1]);
   const int ide = x->ide, jde = x->jde, kde = x->kde;
   const unsigned nb = (unsigned)x->n3, sj = (unsigned)(kde * x->ip), sk = (unsigned)x->ip;
   const BufF bq = bufv(w, nb), bv = bufv(rv, nb);

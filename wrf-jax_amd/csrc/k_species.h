@@ -230,7 +230,7 @@ __device__ __forceinline__ void pd_vflux_sel(const PdFace &f, float qm2, float q
 __global__ void __launch_bounds__(64 * kFaceRows) k_species_pd_scale(const DevCtx xv, SpecList L, float dt, R3 r) {
   const DevCtx *__restrict__ x = &xv;
   extern __shared__ float yal_[];  // 2 x L.n x kFaceRows x 64: south antidiffusive / upwind flux of every species
-  GET_FACE_BLOCK_LG(r, kFaceLGPd);
+  GET_FACE_BLOCK_LG(r, (unsigned)x->face_lg[2]);
 #define YA(s, w) yal_[((s) * kFaceRows + (w)) * 64 + lane]
 #define YL(s, w) yal_[((L.n + (s)) * kFaceRows + (w)) * 64 + lane]
   const int ide = x->ide, jde = x->jde, kde = x->kde;
@@ -323,7 +323,7 @@ __global__ void __launch_bounds__(64 * kFaceRows) k_species_pd_scale(const DevCt
 __global__ void __launch_bounds__(64 * kFaceRows) k_species_pd_update(const DevCtx xv, SpecList L, float dt, R3 r) {
   const DevCtx *__restrict__ x = &xv;
   extern __shared__ float yal_[];  // 2 x L.n x kFaceRows x 64: south antidiffusive / upwind flux of every species
-  GET_FACE_BLOCK_LG(r, kFaceLGPd);
+  GET_FACE_BLOCK_LG(r, (unsigned)x->face_lg[2]);
 #define YA(s, w) yal_[((s) * kFaceRows + (w)) * 64 + lane]
 #define YL(s, w) yal_[((L.n + (s)) * kFaceRows + (w)) * 64 + lane]
   const int kde = x->kde;

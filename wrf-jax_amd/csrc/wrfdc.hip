@@ -567,19 +567,19 @@ struct Run {
   }
   void advect_u() const {
     R3 r{2, ide - 1, 1, kde - 1, 1, jde - 1};
-    k_advect_u<false><<<dim3(face_tiles(r).count()), bf(), 0, s>>>(d, F(F_u_2), F(F_ru_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
+    k_advect_u<false><<<dim3(face_tiles(r, d.face_lg[0]).count()), bf(), 0, s>>>(d, F(F_u_2), F(F_ru_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
   }
   void advect_v() const {
     R3 r{1, ide - 1, 1, kde - 1, 2, jde - 1};
-    k_advect_v<false><<<dim3(face_tiles(r).count()), bf(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
+    k_advect_v<false><<<dim3(face_tiles(r, d.face_lg[0]).count()), bf(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
   }
   void advect_w() const {
     R3 r{1, ide - 1, 2, kde, 1, jde - 1};
-    k_advect_w<false><<<dim3(face_tiles(r, kFaceLGW).count()), bf(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
+    k_advect_w<false><<<dim3(face_tiles(r, d.face_lg[1]).count()), bf(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
   }
   void advect_scalar(const float *q, float *tend, const float *ru, const float *rv, const float *rom) const {
     R3 m = mass3();
-    k_advect_scalar<false><<<dim3(face_tiles(m).count()), bf(), 0, s>>>(d, q, tend, ru, rv, rom, m, 0);
+    k_advect_scalar<false><<<dim3(face_tiles(m, d.face_lg[0]).count()), bf(), 0, s>>>(d, q, tend, ru, rv, rom, m, 0);
   }
   void advect_scalar_pd(const float *q, const float *q_old, float *tend, double dt) const {
     R3 r = full3();
@@ -647,15 +647,15 @@ struct Run {
     clear_tendencies(rk == 1);
     if (rk == 1) lateral_relax();
     const R3 ru{1, ide, 1, kde - 1, 1, jde - 1};
-    k_advect_u<true><<<dim3(face_tiles(ru).count()), bf(), 0, s>>>(d, F(F_u_2), F(F_ru_tend), F(F_ru), F(F_rv), F(F_ww), ru, rk);
+    k_advect_u<true><<<dim3(face_tiles(ru, d.face_lg[0]).count()), bf(), 0, s>>>(d, F(F_u_2), F(F_ru_tend), F(F_ru), F(F_rv), F(F_ww), ru, rk);
     const R3 rv{1, ide - 1, 1, kde - 1, 1, jde};
-    k_advect_v<true><<<dim3(face_tiles(rv).count()), bf(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), rv, rk);
+    k_advect_v<true><<<dim3(face_tiles(rv, d.face_lg[0]).count()), bf(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), rv, rk);
     const R3 rw{1, ide - 1, 1, kde, 1, jde - 1};
-    k_advect_w<true><<<dim3(face_tiles(rw, kFaceLGW).count()), bf(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), rw, rk);
+    k_advect_w<true><<<dim3(face_tiles(rw, d.face_lg[1]).count()), bf(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), rw, rk);
     const R3 rq{1, ide - 1, kde - 1, kde - 1, 1, jde - 1};  // levels [2, kde-2]: k_advect_w<true>
     k_cqw_inv<<<g3p(rq), b3(), 0, s>>>(d, rq);
     const R3 m = mass3();
-    k_advect_scalar<true><<<dim3(face_tiles(m).count()), bf(), 0, s>>>(d, F(F_t_2), F(F_t_tend), F(F_ru), F(F_rv), F(F_ww), m, rk);
+    k_advect_scalar<true><<<dim3(face_tiles(m, d.face_lg[0]).count()), bf(), 0, s>>>(d, F(F_t_2), F(F_t_tend), F(F_ru), F(F_rv), F(F_ww), m, rk);
     k_tend_ph<<<g3(rw), b3(), 0, s>>>(d, rw, rk);
     spec_overlays();
   }
@@ -913,10 +913,10 @@ struct Run {
         for (int q = 0; q < pd.n; q++) bc.add3(pd.old[q], 'p');
       }
       R3 r = full3();
-      k_species_pd_scale<<<dim3(face_tiles(r, kFaceLGPd).count()), bf(), sizeof(float) * 2 * 64 * kFaceRows * pd.n, s>>>(d, pd, dt, r);
-      k_species_pd_update<<<dim3(face_tiles(m, kFaceLGPd).count()), bf(), sizeof(float) * 2 * 64 * kFaceRows * pd.n, s>>>(d, pd, dt, m);
+      k_species_pd_scale<<<dim3(face_tiles(r, d.face_lg[2]).count()), bf(), sizeof(float) * 2 * 64 * kFaceRows * pd.n, s>>>(d, pd, dt, r);
+      k_species_pd_update<<<dim3(face_tiles(m, d.face_lg[2]).count()), bf(), sizeof(float) * 2 * 64 * kFaceRows * pd.n, s>>>(d, pd, dt, m);
     }
-    if (plain.n > 0) k_species_update<<<dim3(face_tiles(m).count()), bf(), sizeof(float) * 64 * kFaceRows * plain.n, s>>>(d, plain, rk, dt, m);
+    if (plain.n > 0) k_species_update<<<dim3(face_tiles(m, d.face_lg[0]).count()), bf(), sizeof(float) * 64 * kFaceRows * plain.n, s>>>(d, plain, rk, dt, m);
     // outputs become the current species
     use_species(out_alt);
     if (rk == 3) {
@@ -1201,6 +1201,9 @@ void fill_devctx(const wrfdc_config &c, DevCtx &h) {
   h.cfn = (float)cfg->cfn;
   h.cfn1 = (float)cfg->cfn1;
   h.reradius = (float)cfg->reradius;
+  h.face_lg[0] = (int)kFaceLG;
+  h.face_lg[1] = (int)kFaceLGW;
+  h.face_lg[2] = (int)kFaceLGPd;
   // lbc_fcx_gcx (boundary/lbc_fcx_gcx.py:10-56), spec_exp = 0, in FP32
   for (int b = cfg->spec_zone + 1; b <= cfg->spec_zone + cfg->relax_zone && b <= cfg->spec_bdy_width; b++) {
     const float dist = (float)(cfg->spec_zone + cfg->relax_zone - b);
@@ -1337,6 +1340,17 @@ int wrfdc_set_dtbc(wrfdc_ctx *x, double dtbc) {
   if (!x) return WRFDC_EINVAL;
   if (!std::isfinite(dtbc)) return set_err(x, WRFDC_EINVAL, "dtbc must be finite");
   x->dtbc = dtbc;
+  return WRFDC_OK;
+}
+
+int wrfdc_set_tile(wrfdc_ctx *x, const char *name, int value) {
+  if (!x || !name) return WRFDC_EINVAL;
+  if (std::string(name) != "face_levels")
+    return set_err(x, WRFDC_EINVAL, std::string("unknown tile setting: ") + name + " (known: face_levels)");
+  if (value < 0 || value > 32) return set_err(x, WRFDC_EINVAL, "face_levels must be 0 (the defaults) or 1..32");
+  x->h.face_lg[0] = value ? value : (int)kFaceLG;
+  x->h.face_lg[1] = value ? value : (int)kFaceLGW;
+  x->h.face_lg[2] = value ? value : (int)kFaceLGPd;
   return WRFDC_OK;
 }
 

@@ -69,6 +69,10 @@ struct DevCtx {
   float rdx, rdy, dx, dy, dt, g, t0, p0, ptop, r_d, cpovcv, rvovrd, epssm, smdiv, emdiv, c_s, prandtl;
   float cf1, cf2, cf3, cfn, cfn1, reradius;
   float fcx[16], gcx[16];
+  // levels per level group of the face-block tile order (wrfdc_set_tile):
+  // [0] advection of u, v, theta and the plain species update, [1] advection
+  // of w, [2] the positive-definite species passes
+  int face_lg[3];
   float *F[NFIELDS];
   float *S[NSCRATCH];
 };

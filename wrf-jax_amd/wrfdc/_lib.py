@@ -33,6 +33,7 @@ SIGNATURES = {
     "wrfdc_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]),
     "wrfdc_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
     "wrfdc_set_dtbc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "wrfdc_set_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
     "wrfdc_checksums": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "wrfdc_checksums_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
                                                 ctypes.c_void_p]),

@@ -95,6 +95,12 @@ class Domain:
     def stage(self, name: str, rk_step: int = 1, iteration: int = 0) -> None:
         self._check(self.lib.wrfdc_stage(self.ctx, name.encode(), int(rk_step), int(iteration)), f"wrfdc_stage({name})")
 
+    def set_tile(self, name: str, value: int) -> None:
+        """A launch-shape setting (wrfdc_set_tile): "face_levels" = levels per
+        level group of the face-block kernels, 0 for the per-kernel defaults.
+        Results do not depend on it."""
+        self._check(self.lib.wrfdc_set_tile(self.ctx, name.encode(), int(value)), f"wrfdc_set_tile({name})")
+
     def set_dtbc(self, dtbc: float) -> None:
         """Boundary time (s since the boundary record) that stage calls read."""
         self._check(self.lib.wrfdc_set_dtbc(self.ctx, float(dtbc)), "wrfdc_set_dtbc")

@@ -5,10 +5,14 @@ species) and the HIP graph in which one RK3 step of every member was
 captured (``wrfdc_ensemble_create``): members on the same stream run one
 after another, the streams concurrently, and each replay
 (``wrfdc_ensemble_step``) launches the whole ensemble's step at once.
-``autotune`` picks how many streams to use and which member goes on which:
-it times each member's step on its own, assigns members longest first to
-the least loaded stream for each candidate stream count, times the
-captured ensemble step of each layout and keeps the fastest.
+``autotune`` picks how many streams to use, which member goes on which and
+the tile shape of the face-block stencil kernels: it times each member's
+step on its own, assigns members longest first to the least loaded stream
+for each candidate stream count, times the captured ensemble step of each
+layout and keeps the fastest; then, on that layout, it times each candidate
+level-group size of the face-block tiles (``Domain.set_tile("face_levels",
+n)``, 0 = the per-kernel defaults) and leaves the members set to the
+fastest.  Neither choice changes any result bit.
 
 ``c2_members`` draws the seeded members of the batched workload: 32
 domains of 100^2..300^2 columns and 45..60 levels, dx 1..9 km, dt = 6 s per
@@ -148,8 +152,10 @@ def graph_step_time(ens: Ensemble, reps: int = 5) -> float:
     return (time.perf_counter() - t0) / reps
 
 
-def autotune(domains, candidates=(1, 2, 4, 8), reps: int = 3, log=None):
-    """(best stream count, member -> stream, table of (streams, s/step))."""
+def autotune(domains, candidates=(1, 2, 4, 8), reps: int = 3, log=None, face_levels=(0, 2, 8, 16)):
+    """(best stream count, member -> stream, table of (streams, s/step),
+    table of (face_levels, s/step)); the members are left set to the best
+    face_levels, so an Ensemble captured afterwards uses it."""
     times = member_times(domains, reps)
     table = []
     best = None
@@ -166,7 +172,22 @@ def autotune(domains, candidates=(1, 2, 4, 8), reps: int = 3, log=None):
             log(f"{nb} stream(s): {1e3 * t:.2f} ms per ensemble step")
         if best is None or t < best[1]:
             best = (nb, t, assign)
-    return best[0], best[2], table
+    nb, assign = best[0], best[2]
+    tiles = []
+    for lg in face_levels:
+        for d in domains:
+            d.set_tile("face_levels", lg)
+        ens = Ensemble(domains, assign, nb)
+        t = graph_step_time(ens, reps)
+        ens.close()
+        tiles.append((lg, t))
+        if log:
+            log(f"{nb} stream(s), face_levels {lg}: {1e3 * t:.2f} ms per ensemble step")
+    if tiles:
+        lg = min(tiles, key=lambda e: e[1])[0]
+        for d in domains:
+            d.set_tile("face_levels", lg)
+    return nb, assign, table, tiles
 
 
 __all__ = ["Ensemble", "MemberSpec", "assign_longest_first", "autotune", "c2_members", "eager_step_time", "graph_step_time",
