@@ -195,15 +195,17 @@ __global__ void __launch_bounds__(256) k_closure_km(const DevCtx xv, R3 r, const
     *reinterpret_cast<bf16x8 *>(&s_w[m][row * kLd + c * 8]) =
         *reinterpret_cast<const bf16x8 *>(wbf + (m == 0 ? kWbfW1 : kWbfW2) + row * 64 + c * 8);
   }
-  const unsigned gx_ = (unsigned)(r.i1 - r.i0 + 64) / 64u;
+  // the tiles of GET_IKJ_P: 64-aligned in i from ib <= r.i0
+  const int ib = 1 + ((r.i0 - 1) & ~63);
+  const unsigned gx_ = (unsigned)(r.i1 - ib + 64) / 64u;
   const unsigned gy_ = (unsigned)(r.k1 - r.k0 + 4) / 4u;
-  const int i0 = r.i0 + (int)((blockIdx.x % gx_) * 64u);
+  const int i0 = ib + (int)((blockIdx.x % gx_) * 64u);
   const int i = i0 + l;
   const int k = r.k0 + (int)(((blockIdx.x / gx_) % gy_) * 4u + threadIdx.y);
   const int j = r.j0 + (int)(blockIdx.x / (gx_ * gy_));
   // the whole wave takes part in the MFMA; points outside r compute on a
   // clamped copy and store nothing
-  const bool own = i <= r.i1 && k <= r.k1 && j <= r.j1;
+  const bool own = i >= r.i0 && i <= r.i1 && k <= r.k1 && j <= r.j1;
   const int kc = k <= r.k1 ? k : r.k0, jc = j <= r.j1 ? j : r.j0;
   // per-column biases of the C layout
   float b1[4], b2[4], w3[4];
@@ -232,7 +234,7 @@ __global__ void __launch_bounds__(256) k_closure_km(const DevCtx xv, R3 r, const
   // B operand of output tile t2, input half h: W row t2*16 + l%16, k block l/16
 #define WB(m, t2, h) (*reinterpret_cast<const bf16x8 *>(&s_w[m][((t2) * 16 + col) * kLd + (h) * 32 + grp * 8]))
   for (int t = 0; t < 4; t++) {
-    const int ip = imin_(i0 + 16 * t + col, r.i1);
+    const int ip = imin_(imax_(i0 + 16 * t + col, r.i0), r.i1);
     float vd[8], vn[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) {

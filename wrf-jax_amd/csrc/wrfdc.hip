@@ -249,8 +249,10 @@ struct Run {
   R2 mass2() const { return R2{1, ide - 1, 1, jde - 1}; }
   // 1-D grids decoded by GET_IKJ (stencil kernels), GET_IKJ_P (pointwise
   // kernels) and GET_IJ
-  static dim3 g3(const R3 &r) { return dim3(cdiv(r.i1 - r.i0 + 1, 64) * cdiv(r.k1 - r.k0 + 1, 4) * 4u * cdiv(r.j1 - r.j0 + 1, 4)); }
-  static dim3 g3p(const R3 &r) { return dim3(cdiv(r.i1 - r.i0 + 1, 64) * cdiv(r.k1 - r.k0 + 1, 4) * (unsigned)(r.j1 - r.j0 + 1)); }
+  // first i of the 64-aligned tiles of GET_IKJ / GET_IKJ_P
+  static int ib(const R3 &r) { return 1 + ((r.i0 - 1) & ~63); }
+  static dim3 g3(const R3 &r) { return dim3(cdiv(r.i1 - ib(r) + 1, 64) * cdiv(r.k1 - r.k0 + 1, 4) * 4u * cdiv(r.j1 - r.j0 + 1, 4)); }
+  static dim3 g3p(const R3 &r) { return dim3(cdiv(r.i1 - ib(r) + 1, 64) * cdiv(r.k1 - r.k0 + 1, 4) * (unsigned)(r.j1 - r.j0 + 1)); }
   static dim3 g2(const R2 &r) { return dim3(cdiv(r.i1 - r.i0 + 1, 64) * (unsigned)(r.j1 - r.j0 + 1)); }
   static dim3 b3() { return dim3(64, 4, 1); }
   static dim3 bf() { return dim3(64, kFaceRows, 1); }  // face-sharing blocks
@@ -1860,8 +1862,7 @@ int wrfdc_subgrid_stress(wrfdc_ctx *fine, int nx_coarse, int ny_coarse, int rati
   if (nx_coarse < 12 || ny_coarse < 12) return set_err(fine, WRFDC_EINVAL, "coarse grid below 12 x 12");
   return guarded(fine, [&]() {
     R3 rc{2, nx_coarse - 1, 1, fine->kde - 1, 2, ny_coarse - 1};
-    const dim3 g(cdiv(rc.i1 - rc.i0 + 1, 64) * cdiv(rc.k1 - rc.k0 + 1, 4) * (unsigned)(rc.j1 - rc.j0 + 1));
-    k_subgrid_stress<<<g, dim3(64, 4, 1), 0, fine->s>>>(fine->h, rc, ratio, rows_t);
+    k_subgrid_stress<<<Run::g3p(rc), Run::b3(), 0, fine->s>>>(fine->h, rc, ratio, rows_t);
     return WRFDC_OK;
   });
 }
