@@ -86,42 +86,6 @@ __host__ __device__ inline FaceTiles face_tiles(const R3 &r, unsigned lg = kFace
   const int ic = i <= (r).i1 ? i : (r).i1
 #define GET_FACE_BLOCK(r) GET_FACE_BLOCK_LG(r, (unsigned)x->face_lg[0])
 
-// 64-aligned face blocks: the 64 lanes of a wave own the points ts .. ts+63
-// of a row, ts = 1 + 64 n (lanes before r.i0 stand in for r.i0 and own
-// nothing), so every wave load of an operand at the lane's own column covers
-// two cache lines instead of three.  The east face of the last lane, the
-// west face of the next tile's first point ie = ts + 64, is formed once per
-// block before the block's barrier by lanes 0 .. kFaceRows-1 of wave 0, lane
-// n for row je = j0 + n (xe_lane), and read by lane 63 from LDS.  The launch's
-// grid is face_tiles_a(r, LG).count() blocks.
-__host__ __device__ inline FaceTiles face_tiles_a(const R3 &r, unsigned lg) {
-  const int ib = 1 + ((r.i0 - 1) & ~63);
-  return FaceTiles{(unsigned)(r.i1 - ib + 64) / 64u, (unsigned)(r.j1 - r.j0 + kFaceRows) / (unsigned)kFaceRows,
-                   (unsigned)(r.k1 - r.k0 + (int)lg) / lg, lg};
-}
-#define GET_FACE_BLOCK_A(r, LG)                                                          \
-  const FaceTiles T_ = face_tiles_a(r, LG);                                              \
-  const unsigned t_ = xcd_tile(blockIdx.x, gridDim.x);                                   \
-  const int lane = (int)threadIdx.x, wv = (int)wave_y();                                 \
-  const unsigned u_ = t_ / T_.ni;                                                        \
-  const int ts = 1 + (((r).i0 - 1) & ~63) + (int)(t_ % T_.ni) * 64;                      \
-  const int i = imax_(ts + lane, (r).i0);                                                \
-  const int k = (r).k0 + (int)((u_ / (LG)) / T_.nj) * (int)(LG) + (int)(u_ % (LG));      \
-  const int j0 = (r).j0 + (int)((u_ / (LG)) % T_.nj) * kFaceRows;                        \
-  const int j = j0 + wv;                                                                 \
-  if (k > (r).k1) return; /* whole block */                                              \
-  const bool jrow = (j <= (r).j1);                                                       \
-  const bool own = (ts + lane >= (r).i0 && i <= (r).i1 && jrow);                         \
-  const int ic = i <= (r).i1 ? i : (r).i1;                                               \
-  const int ie = ts + 64, je = j0 + lane;                                                \
-  const bool xe_lane = (wv == 0 && lane < kFaceRows && je <= (r).j1 && ts + 63 <= (r).i1)
-// east face of a lane: the next lane's west face, for lane 63 the block's
-// extra face of its row
-__device__ __forceinline__ float east_face(float fw, int lane, float xe_row) {
-  const float n = lane_next(fw, lane);
-  return lane == 63 ? xe_row : n;
-}
-
 // ---------------- face fluxes from loaded stencils ---------------------------
 // Every kernel below issues all loads of a point first (buffer loads,
 // unclamped: the slots a boundary-degraded order does not use may fall
@@ -394,8 +358,7 @@ __global__ void __launch_bounds__(64 * kFaceRows) k_advect_v(const DevCtx xv, co
                                                   const float *rom, R3 r, int rk) {
   const DevCtx *__restrict__ x = &xv;
   __shared__ float ys[kFaceRows][64];
-  __shared__ float xe[kFaceRows];
-  GET_FACE_BLOCK_A(r, (unsigned)x->face_lg[0]);
+  GET_FACE_BLOCK(r);
   const int ide = x->ide, jde = x->jde, kde = x->kde;
   const unsigned nb = (unsigned)x->n3, sj = (unsigned)(kde * x->ip), sk = (unsigned)x->ip;
   const BufF bq = bufv(v, nb), bv = bufv(rv, nb);
@@ -403,21 +366,10 @@ __global__ void __launch_bounds__(64 * kFaceRows) k_advect_v(const DevCtx xv, co
   const unsigned o = XI3(ic, k, j);
   float fs;
   {
-    // the xe lanes also form the x face at ie of row je, as the lanes form
-    // theirs below, its loads issued with the south face's
-    const bool xf = xe_lane && (!TEND || (je >= 2 && je <= jde - 1)) && ie >= 2 && ie <= ide - 1;
-    const unsigned oe = xf ? XI3(ie, k, je) : 0u;
-    float w[6], we[6], rue = 0.f;
+    float w[6];
     win6b(bq, o, sj, w);
-    const float vsum = bld(bv, o) + bld(bv, o - sj);
-    if (xe_lane) {
-      const BufF bu = bufv(ru, nb);
-      win6b(bq, oe, 1u, we);
-      rue = bld(bu, oe) + bld(bu, oe - sj);
-    }
-    const float f = fy_v_sel(w, vsum, j, jde, j >= 4 && j <= jde - 2);
+    const float f = fy_v_sel(w, bld(bv, o) + bld(bv, o - sj), j, jde, j >= 4 && j <= jde - 2);
     fs = (j >= 2 && j <= jde) ? f : 0.f;
-    if (xe_lane) xe[lane] = xf ? fx_v_sel(we, rue, ie, ide, false) : 0.f;
   }
   ys[wv][lane] = fs;
   __syncthreads();
@@ -449,9 +401,9 @@ __global__ void __launch_bounds__(64 * kFaceRows) k_advect_v(const DevCtx xv, co
     mvx = msx;
     mvxi = A2(FL(msfvx_inv), ic, j);
   }
-  const float fw0 = fx_v_sel(wx, rus, i, ide, ts >= 4 && ts + 63 <= ide - 3);
+  const float fw0 = fx_v_sel(wx, rus, i, ide, i - lane >= 4 && i - lane + 63 <= ide - 3);
   const float fw = (adv && i >= 2 && i <= ide - 1) ? fw0 : 0.f;
-  const float fe = east_face(fw, lane, xe[wv]);
+  const float fe = lane_next(fw, lane);
   if (!own) return;
   float fn;
   if (wv < kFaceRows - 1) fn = ys[wv + 1][lane];

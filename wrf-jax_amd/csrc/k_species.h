@@ -67,34 +67,23 @@ __device__ __forceinline__ UpdCoef upd_coef(const DevCtx *__restrict__ x, int i,
 // in place is race-free.
 __global__ void __launch_bounds__(64 * kFaceRows) k_species_update(const DevCtx xv, SpecList L, int rk, float dt, R3 r) {
   const DevCtx *__restrict__ x = &xv;
-  // L.n x kFaceRows x 64: south face flux of every species, then
-  // L.n x kFaceRows: the x face at ie of every row
-  extern __shared__ float ys_[];
-  GET_FACE_BLOCK_A(r, (unsigned)x->face_lg[0]);
+  extern __shared__ float ys_[];  // L.n x kFaceRows x 64: south face flux of every species
+  GET_FACE_BLOCK(r);
 #define YS(s, w) ys_[((s) * kFaceRows + (w)) * 64 + lane]
-#define XE(s, w) ys_[(L.n * kFaceRows * 64) + (s) * kFaceRows + (w)]
   const int ide = x->ide, jde = x->jde, kde = x->kde;
   const float *ru = FL(ru_m), *rv = FL(rv_m), *rom = FL(ww_m);
   const unsigned nb = (unsigned)x->n3, sj = (unsigned)(kde * x->ip), sk = (unsigned)x->ip;
   {
     // rows past the range still form their south face (the row below reads
     // it); its window may fall outside the field, the face is then 0
-    // the xe lanes also form the x face at ie of row je, as the lanes form
-    // theirs below, its loads issued with the south face's
     const bool yf = (j >= 2 && j <= jde - 1);
     const float vs = yf ? A3(rv, ic, k, j) : 0.f;
     const unsigned os = XI3(ic, k, j);
-    const bool xf = xe_lane && ie >= 2 && ie <= ide - 1;
-    const float ue = xf ? A3(ru, ie, k, je) : 0.f;
-    const unsigned oe = xf ? XI3(ie, k, je) : 0u;
     for (int s = 0; s < L.n; s++) {
-      const BufF bq = bufv(L.in[s], nb);
-      float w[6], we[6];
-      win6b(bq, os, sj, w);
-      if (xe_lane) win6b(bq, oe, 1u, we);
+      float w[6];
+      win6b(bufv(L.in[s], nb), os, sj, w);
       pin6(w);
       YS(s, wv) = yf ? hflux_br(w, vs, j, jde) : 0.f;
-      if (xe_lane) XE(s, lane) = xf ? hflux_br(we, ue, ie, ide) : 0.f;
     }
   }
   __syncthreads();
@@ -128,7 +117,7 @@ __global__ void __launch_bounds__(64 * kFaceRows) k_species_update(const DevCtx 
     pin5(zw);
     // fluxes
     const float fw = xface ? hflux_br(xw, uw, i, ide) : 0.f;
-    const float fe = east_face(fw, lane, XE(s, wv));
+    const float fe = lane_next(fw, lane);
     const float fs = YS(s, wv);
     float fn;
     if (wv < kFaceRows - 1) fn = YS(s, wv + 1);
@@ -149,7 +138,6 @@ __global__ void __launch_bounds__(64 * kFaceRows) k_species_update(const DevCtx 
     }
   }
 #undef YS
-#undef XE
 }
 
 // rk_update_scalar_pd (module_em.py:1601-1669) for every species of the list:

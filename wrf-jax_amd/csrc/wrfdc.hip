@@ -573,7 +573,7 @@ struct Run {
   }
   void advect_v() const {
     R3 r{1, ide - 1, 1, kde - 1, 2, jde - 1};
-    k_advect_v<false><<<dim3(face_tiles_a(r, d.face_lg[0]).count()), bf(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
+    k_advect_v<false><<<dim3(face_tiles(r, d.face_lg[0]).count()), bf(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), r, 0);
   }
   void advect_w() const {
     R3 r{1, ide - 1, 2, kde, 1, jde - 1};
@@ -651,7 +651,7 @@ struct Run {
     const R3 ru{1, ide, 1, kde - 1, 1, jde - 1};
     k_advect_u<true><<<dim3(face_tiles(ru, d.face_lg[0]).count()), bf(), 0, s>>>(d, F(F_u_2), F(F_ru_tend), F(F_ru), F(F_rv), F(F_ww), ru, rk);
     const R3 rv{1, ide - 1, 1, kde - 1, 1, jde};
-    k_advect_v<true><<<dim3(face_tiles_a(rv, d.face_lg[0]).count()), bf(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), rv, rk);
+    k_advect_v<true><<<dim3(face_tiles(rv, d.face_lg[0]).count()), bf(), 0, s>>>(d, F(F_v_2), F(F_rv_tend), F(F_ru), F(F_rv), F(F_ww), rv, rk);
     const R3 rw{1, ide - 1, 1, kde, 1, jde - 1};
     k_advect_w<true><<<dim3(face_tiles(rw, d.face_lg[1]).count()), bf(), 0, s>>>(d, F(F_w_2), F(F_rw_tend), F(F_ru), F(F_rv), F(F_ww), rw, rk);
     const R3 rq{1, ide - 1, kde - 1, kde - 1, 1, jde - 1};  // levels [2, kde-2]: k_advect_w<true>
@@ -918,7 +918,7 @@ struct Run {
       k_species_pd_scale<<<dim3(face_tiles(r, d.face_lg[2]).count()), bf(), sizeof(float) * 2 * 64 * kFaceRows * pd.n, s>>>(d, pd, dt, r);
       k_species_pd_update<<<dim3(face_tiles(m, d.face_lg[2]).count()), bf(), sizeof(float) * 2 * 64 * kFaceRows * pd.n, s>>>(d, pd, dt, m);
     }
-    if (plain.n > 0) k_species_update<<<dim3(face_tiles_a(m, d.face_lg[0]).count()), bf(), sizeof(float) * 65 * kFaceRows * plain.n, s>>>(d, plain, rk, dt, m);
+    if (plain.n > 0) k_species_update<<<dim3(face_tiles(m, d.face_lg[0]).count()), bf(), sizeof(float) * 64 * kFaceRows * plain.n, s>>>(d, plain, rk, dt, m);
     // outputs become the current species
     use_species(out_alt);
     if (rk == 3) {
