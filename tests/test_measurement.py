@@ -59,6 +59,31 @@ def test_traffic_summary_scales_by_the_calibration_copy(tmp_path):
     assert cells[4] == "2"
 
 
+def test_mfma_report_busy_share(tmp_path):
+    c, t = str(tmp_path / "c.db"), str(tmp_path / "t.db")
+    con = sqlite3.connect(c)
+    con.execute("create table counters_collection (kernel_name text, counter_name text, value real, dispatch_id int)")
+    rows = []
+    for disp in (1, 2):  # two dispatches, GRBM summed over the 8 XCDs
+        rows += [("kM(x)", "SQ_WAVES", 1000.0, disp), ("kM(x)", "SQ_INSTS_MFMA", 64000.0, disp),
+                 ("kM(x)", "SQ_INSTS_VALU_TRANS_F32", 266000.0, disp), ("kM(x)", "GRBM_GUI_ACTIVE", 8 * 2000.0, disp),
+                 ("kM(x)", "SQ_VALU_MFMA_BUSY_CYCLES", 0.25 * 1024 * 2000.0, disp),
+                 ("kV(x)", "SQ_WAVES", 10.0, 3), ("kV(x)", "SQ_INSTS_MFMA", 0.0, 3), ("kV(x)", "GRBM_GUI_ACTIVE", 80.0, 3)]
+    con.executemany("insert into counters_collection values (?, ?, ?, ?)", rows)
+    con.commit()
+    con.close()
+    con = sqlite3.connect(t)
+    con.execute("create table top_kernels (name text, total_calls int, total_duration real, average real)")
+    con.execute("insert into top_kernels values ('kM(x)', 2, 2000.0, 1000.0)")
+    con.commit()
+    con.close()
+    text = profile_summary.mfma(c, t)
+    rows = [ln for ln in text.splitlines() if ln.startswith("| kM") or ln.startswith("| kV")]
+    assert len(rows) == 1  # the kernel without MFMA instructions is left out
+    cells = [x.strip() for x in rows[0].strip("|").split("|")]
+    assert cells[:6] == ["kM", "2", "1000.0", "64", "266", "25.0"]
+
+
 def test_bench_refuses_more_than_one_gpu():
     res = subprocess.run([sys.executable, os.path.join(REPO, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert res.returncode == 2

@@ -1,6 +1,6 @@
 """Text summaries of rocprofv3 result databases (SQLite) for ``profiles/``.
 
-Four reports:
+Five reports:
 
 ``kernels DB``
     per-kernel calls, total and mean duration from a ``--kernel-trace --stats``
@@ -30,10 +30,16 @@ Four reports:
     wave64 instruction, and SQ_ACTIVE_INST_VALU (quad-cycles), over 1024
     SIMDs x GRBM_GUI_ACTIVE.
 
+``mfma COUNTER_DB TRACE_DB``
+    matrix-core use of the kernels that issue MFMA instructions, from one
+    ``--pmc`` run of ``SQ_WAVES SQ_INSTS_MFMA SQ_VALU_MFMA_BUSY_CYCLES
+    SQ_INSTS_VALU_TRANS_F32 GRBM_GUI_ACTIVE``.
+
 Usage: ``python -m wrfdc.profile_summary kernels DB``,
 ``python -m wrfdc.profile_summary traffic FETCH_DB WRITE_DB``,
-``python -m wrfdc.profile_summary traffic_vs_model FETCH_DB WRITE_DB TRACE_DB`` or
-``python -m wrfdc.profile_summary issue COUNTER_DB TRACE_DB``.
+``python -m wrfdc.profile_summary traffic_vs_model FETCH_DB WRITE_DB TRACE_DB``,
+``python -m wrfdc.profile_summary issue COUNTER_DB TRACE_DB`` or
+``python -m wrfdc.profile_summary mfma COUNTER_DB TRACE_DB``.
 """
 
 from __future__ import annotations
@@ -192,6 +198,43 @@ def issue(counter_db: str, trace_db: str) -> str:
     return "\n".join(out)
 
 
+def mfma(counter_db: str, trace_db: str) -> str:
+    """Matrix-core use of the kernels that issue MFMA instructions, from one
+    counter pass of SQ_WAVES, SQ_INSTS_MFMA, SQ_VALU_MFMA_BUSY_CYCLES,
+    SQ_INSTS_VALU_TRANS_F32 and GRBM_GUI_ACTIVE: MFMA and transcendental
+    instructions per wave and the share of SIMD cycles the MFMA unit is busy
+    (busy cycles summed over the SIMDs / (1024 SIMDs x one XCD's cycles))."""
+    cur = sqlite3.connect(counter_db).cursor()
+    per = collections.defaultdict(lambda: collections.defaultdict(float))
+    name = {}
+    for kname, cname, v, disp in cur.execute("select kernel_name, counter_name, value, dispatch_id from counters_collection"):
+        per[disp][cname] += v
+        name[disp] = _short(kname)
+    tcur = sqlite3.connect(trace_db).cursor()
+    dur = {_short(n): avg for n, avg in tcur.execute("select name, average from top_kernels")}
+    agg = collections.defaultdict(lambda: collections.defaultdict(float))
+    cnt = collections.Counter()
+    for disp, d in per.items():
+        for c, v in d.items():
+            agg[name[disp]][c] += v
+        cnt[name[disp]] += 1
+    out = ["| kernel | dispatches | mean us | MFMA / wave | transcendental / wave | MFMA busy % |",
+           "|---|---:|---:|---:|---:|---:|"]
+    rows = []
+    for k, a in agg.items():
+        n = cnt[k]
+        waves = a["SQ_WAVES"] / n
+        cyc = a["GRBM_GUI_ACTIVE"] / n / N_XCD
+        if a["SQ_INSTS_MFMA"] <= 0 or waves <= 0 or cyc <= 0:
+            continue
+        us = dur.get(k, float("nan"))
+        rows.append((us * n, k, n, us, a["SQ_INSTS_MFMA"] / n / waves, a["SQ_INSTS_VALU_TRANS_F32"] / n / waves,
+                     100.0 * a["SQ_VALU_MFMA_BUSY_CYCLES"] / n / (N_SIMD * cyc)))
+    for _, k, n, us, mw, tw, busy in sorted(rows, reverse=True):
+        out.append(f"| {k} | {n} | {us:.1f} | {mw:.0f} | {tw:.0f} | {busy:.1f} |")
+    return "\n".join(out)
+
+
 def main(argv):
     if len(argv) >= 2 and argv[0] == "kernels":
         print(kernels(argv[1]))
@@ -201,6 +244,8 @@ def main(argv):
         print(traffic_vs_model(argv[1], argv[2], argv[3]))
     elif len(argv) >= 3 and argv[0] == "issue":
         print(issue(argv[1], argv[2]))
+    elif len(argv) >= 3 and argv[0] == "mfma":
+        print(mfma(argv[1], argv[2]))
     else:
         print(__doc__)
         return 2
