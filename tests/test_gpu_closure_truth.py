@@ -18,7 +18,7 @@ def test_closure_against_truth_run(gpu):
     from wrfdc.closure_truth import run
 
     print("[closure_truth] truth 180x180x20 at 1 km, model 60x60x20 at 3 km", flush=True)
-    res = run(gpu, nxc=60, nz=20, spinup_steps=20, heldout_steps=20, judge_steps=5, epochs=8, batch=1024,
+    res = run(gpu, nxc=60, nz=20, spinup_steps=20, heldout_steps=20, compare_steps=5, epochs=8, batch=1024,
               log=lambda m: print("[closure_truth]", m, flush=True))
     loss = res["training_loss"]
     assert loss[-1] < loss[0]
@@ -26,6 +26,6 @@ def test_closure_against_truth_run(gpu):
     assert all(math.isfinite(v) for v in fe.values())
     assert fe["network"] < fe["smagorinsky"], fe
     assert res["nonfinite"] == 0
-    for n, e in res["judge_rel_l2"].items():
+    for n, e in res["truth_rel_l2"].items():
         assert math.isfinite(e["network"]) and math.isfinite(e["smagorinsky"]), n
         assert e["network"] < 10.0 * e["smagorinsky"] + 1e-6, (n, e)

@@ -13,7 +13,7 @@ that evaluates or trains it live in ``csrc/k_closure.h`` behind the C ABI
   the Smagorinsky km);
 * ``Network``: parameters in device memory, Adam training on minibatches
   (the minibatch gather is torch indexing, the rest HIP kernels);
-* ``flux_error``: the judge of a diffusivity, sum((t + K s)^2) / sum(t^2) over
+* ``flux_error``: the score of a diffusivity, sum((t + K s)^2) / sum(t^2) over
   rows, i.e. how much of the fine run's sub-grid stress the eddy-viscosity
   model K*s leaves unexplained (0 perfect, 1 = no model).
 

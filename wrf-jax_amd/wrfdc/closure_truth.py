@@ -13,9 +13,9 @@ about 196 GB of HBM with one species, built on the GPU by
    rows, the fraction of the sub-grid stress each diffusivity leaves
    unexplained (``closure.flux_error``: 0 perfect, 1 = no model) for the
    network, Smagorinsky and no model;
-4. the plain high-resolution run as the judge: two model-grid domains start
+4. the plain high-resolution run as the reference: two model-grid domains start
    from the coarse-grained truth, one with the network's km, one with
-   Smagorinsky; after ``judge_steps`` model steps (the truth advancing
+   Smagorinsky; after ``compare_steps`` model steps (the truth advancing
    ``ratio`` times as many) each is compared with the coarse-grained truth
    (relative L2 of u, v, w, theta over the interior, and of their change
    over the window).
@@ -39,7 +39,7 @@ def _rel(a, b) -> float:
 
 
 def run(device: str = "cuda:0", nxc: int = 100, nz: int = 30, ratio: int = 3, dx: float = 3000.0, dt: float = 18.0,
-        spinup_steps: int = 30, heldout_steps: int = 30, judge_steps: int = 10, epochs: int = 2, batch: int = 4096,
+        spinup_steps: int = 30, heldout_steps: int = 30, compare_steps: int = 10, epochs: int = 2, batch: int = 4096,
         lr: float = 2e-3, border: int = 10, log=None) -> dict:
     import torch
 
@@ -91,7 +91,7 @@ def run(device: str = "cuda:0", nxc: int = 100, nz: int = 30, ratio: int = 3, dx
     del Xb, Tb, tb
     torch.cuda.empty_cache()
 
-    # the truth as the judge of two model runs
+    # two model runs compared with the truth
     smag = device_case(nxc, nxc, nz, device=device, dx=dx, dt=dt, nmoist=1, nscalar=0, config_index=4, member=2)
     to_model_grid(coarse)
     to_model_grid(smag)
@@ -100,25 +100,25 @@ def run(device: str = "cuda:0", nxc: int = 100, nz: int = 30, ratio: int = 3, dx
     start = {n: a[inner] for n, a in coarse.download(names).items()}
     net.attach(coarse)
     dtbc0 = (spinup_steps + heldout_steps) * fine.config.dt
-    coarse.step(judge_steps, dtbc0)
-    smag.step(judge_steps, dtbc0)
+    coarse.step(compare_steps, dtbc0)
+    smag.step(compare_steps, dtbc0)
     got_net = {n: a[inner] for n, a in coarse.download(names).items()}
     got_smag = {n: a[inner] for n, a in smag.download(names).items()}
     ck_net, ck_smag = coarse.checksums(), smag.checksums()
-    fine.step(judge_steps * ratio, dtbc0)
+    fine.step(compare_steps * ratio, dtbc0)
     to_model_grid(smag)
     truth = {n: a[inner] for n, a in smag.download(names).items()}
-    judge = {}
+    vs_truth = {}
     for n in names:
-        judge[n] = {
+        vs_truth[n] = {
             "network": _rel(got_net[n], truth[n]), "smagorinsky": _rel(got_smag[n], truth[n]),
             "network_change": _rel(got_net[n] - start[n], truth[n] - start[n]),
             "smagorinsky_change": _rel(got_smag[n] - start[n], truth[n] - start[n]),
         }
-    out["judge_steps"] = judge_steps
-    out["judge_rel_l2"] = judge
+    out["compare_steps"] = compare_steps
+    out["truth_rel_l2"] = vs_truth
     out["nonfinite"] = int(ck_net[3] + ck_smag[3])
-    say(f"judge after {judge_steps} model steps: {json.dumps(judge)}")
+    say(f"against the truth after {compare_steps} model steps: {json.dumps(vs_truth)}")
     Network.detach(coarse)
     for d in (coarse, smag, fine):
         d.close()
@@ -132,11 +132,11 @@ def main(argv=None):
     ap.add_argument("--nz", type=int, default=60)
     ap.add_argument("--spinup-steps", type=int, default=30)
     ap.add_argument("--heldout-steps", type=int, default=30)
-    ap.add_argument("--judge-steps", type=int, default=10)
+    ap.add_argument("--compare-steps", type=int, default=10)
     ap.add_argument("--epochs", type=int, default=2)
     args = ap.parse_args(argv)
     res = run(nxc=args.nxc, nz=args.nz, spinup_steps=args.spinup_steps, heldout_steps=args.heldout_steps,
-              judge_steps=args.judge_steps, epochs=args.epochs, log=lambda m: print("[closure_truth]", m, flush=True))
+              compare_steps=args.compare_steps, epochs=args.epochs, log=lambda m: print("[closure_truth]", m, flush=True))
     print(json.dumps(res))
 
 
