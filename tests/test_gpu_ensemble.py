@@ -4,7 +4,8 @@ uneven members, csrc/wrfdc.hip wrfdc_ensemble_*).
 * replaying the graph advances every member exactly as stepping it alone
   does (bit-identical fields), also with time-varying lateral forcing, where
   each replay must see the current boundary time;
-* every member matches the FP32 oracle after the replays (relative L2 1e-5);
+* every member matches the FP32 oracle after the replays (relative L2 1e-5),
+  also the smallest members of the 32-member batched workload;
 * the autotuner returns a valid layout and tile shape.
 """
 
@@ -75,6 +76,38 @@ def test_graph_replay_matches_members_stepped_alone(gpu):
     ens.close()
     for d in A + B:
         d.close()
+
+
+def test_workload_members_match_oracle(gpu):
+    """The three smallest members of the 32-member batched workload
+    (104^2..116^2 columns, 47..58 levels, 6 moist species, PD on, their own
+    dx and dt) advanced two steps by one captured ensemble graph on two
+    streams: every member within relative L2 1e-5 of the FP32 oracle."""
+    from oracle import Oracle
+    from wrfdc.device import Domain
+    from wrfdc.ensemble import Ensemble, c2_members, member_case
+
+    specs = c2_members()
+    pick = sorted(range(len(specs)), key=lambda q: specs[q].nx * specs[q].ny * specs[q].nz)[:3]
+    print(f"[ensemble] workload members {pick}, 2 streams, 2 replays vs the oracle", flush=True)
+    cases = [member_case(specs[q], q) for q in pick]
+    doms = []
+    for c in cases:
+        d = Domain(c.config, gpu)
+        d.upload(c.arrays)
+        doms.append(d)
+    ens = Ensemble(doms, branch_of=[0, 1, 0], nbranches=2)
+    ens.step(2, 0.0)
+    ens.synchronize()
+    ens.close()
+    for q, d, c in zip(pick, doms, cases):
+        got = d.download()
+        d.close()
+        o = Oracle(c.config, c.arrays)
+        o.step(2, 0.0)
+        bad = compare(got, o.arrays, tol=REL_L2_STEP, skip=STEP_DIAGNOSTICS)
+        o.close()
+        assert not bad, (q, bad)
 
 
 def test_autotune_returns_a_layout(gpu):
