@@ -264,7 +264,8 @@ def learned_form(steps: int, plain_value: float, kernel_reps: int) -> dict:
             "training_loss": [hist[0], hist[-1]], "checksums": ck.tolist(),
             "roofline": {"bound": "mfma", "kernel": "k_closure_km (+ its k_smag prep)", "achieved": tflops,
                          "peak": 2500.0, "unit": "TFLOP/s", "frac": tflops / 2500.0, "flop_per_launch": flops,
-                         "mean_launch_ms": k_ms, "dtype": "bf16 MFMA, FP32 accumulate"},
+                         "mean_launch_ms": k_ms, "dtype": "bf16 MFMA, FP32 accumulate",
+                         "limited_by": "VALU issue (activations, bias, bf16 packing around the MFMA products)"},
             "workload": "1000x1000x60, dx 3 km, dt 18 s, 1 moist species, PD on, FP32 dycore, bf16 MLP km "
                         "(trained on a 1 km run coarse-grained 3x)"}
 
