@@ -97,7 +97,8 @@ def test_stage_parity(stepped_state, domain_for, name, rk, it):
     assert not bad, f"{name}: fields above rel-L2 {REL_L2_STEP}: {bad}"
 
 
-SHAPES = [(28, 24, 12), (13, 17, 7), (40, 33, 20)]
+# 65 and 129 columns: the i ranges end on, or one past, a 64-point tile edge
+SHAPES = [(28, 24, 12), (13, 17, 7), (40, 33, 20), (65, 21, 8), (129, 13, 7)]
 
 
 SPECIES_MIXES = [(1, 0, 1), (1, 0, 0), (3, 1, 0), (2, 3, 1)]
